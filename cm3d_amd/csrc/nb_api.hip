@@ -1,0 +1,551 @@
+// nb_api.hip -- C ABI entry points (include/cm3d_nb.h): context set-up,
+// list build, rebuild trigger, force evaluation and energy readback.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "nb_internal.h"
+
+static thread_local char g_err[512];
+
+void nb_set_error(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+extern "C" const char *nb_last_error(void) { return g_err; }
+
+#define HCHK(x)                                                        \
+  do {                                                                 \
+    hipError_t _e = (x);                                               \
+    if (_e != hipSuccess) {                                            \
+      nb_set_error("%s: %s", #x, hipGetErrorString(_e));               \
+      return NB_ERR_HIP;                                               \
+    }                                                                  \
+  } while (0)
+
+template <typename T>
+static int dalloc(T **p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+  if (e != hipSuccess) {
+    nb_set_error("hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
+    return NB_ERR_HIP;
+  }
+  return NB_OK;
+}
+
+template <typename T>
+static int upload(T **p, const T *src, size_t count) {
+  int rc = dalloc(p, count);
+  if (rc) return rc;
+  if (count && src) HCHK(hipMemcpy(*p, src, count * sizeof(T), hipMemcpyHostToDevice));
+  return NB_OK;
+}
+
+#define RC(x)                      \
+  do {                             \
+    int _r = (x);                  \
+    if (_r != NB_OK) return _r;    \
+  } while (0)
+
+// gethinv9 (utils/md_mathlib.c:30-50): cofactor inverse, same operation order.
+static Box make_box(const double h[9]) {
+  Box b;
+  for (int k = 0; k < 9; k++) b.h[k] = h[k];
+  double rvol = 1. / (h[0] * (h[4] * h[8] - h[5] * h[7]) + h[1] * (h[5] * h[6] - h[3] * h[8]) +
+                      h[2] * (h[3] * h[7] - h[4] * h[6]));
+  b.hi[0] = (h[4] * h[8] - h[5] * h[7]) * rvol;
+  b.hi[1] = (h[2] * h[7] - h[1] * h[8]) * rvol;
+  b.hi[2] = (h[1] * h[5] - h[2] * h[4]) * rvol;
+  b.hi[3] = (h[5] * h[6] - h[3] * h[8]) * rvol;
+  b.hi[4] = (h[0] * h[8] - h[2] * h[6]) * rvol;
+  b.hi[5] = (h[2] * h[3] - h[0] * h[5]) * rvol;
+  b.hi[6] = (h[3] * h[7] - h[4] * h[6]) * rvol;
+  b.hi[7] = (h[1] * h[6] - h[0] * h[7]) * rvol;
+  b.hi[8] = (h[0] * h[4] - h[1] * h[3]) * rvol;
+  return b;
+}
+
+static double det9(const double h[9]) {
+  return h[0] * (h[4] * h[8] - h[5] * h[7]) + h[1] * (h[5] * h[6] - h[3] * h[8]) +
+         h[2] * (h[3] * h[7] - h[4] * h[6]);
+}
+
+extern "C" void nb_destroy(nb_ctx *c) {
+  if (!c) return;
+  hipSetDevice(c->dev);
+  void *ptrs[] = {c->d_itype, c->d_map, c->d_ip, c->d_vtab3, c->d_ctab3, c->d_vtab3f,
+                  c->d_ctab3f, c->d_qch, c->d_xoff, c->d_xlist, c->d_eoff, c->d_elist,
+                  c->d_qidx, c->d_seg, c->d_kabc, c->d_pos, c->d_posb, c->d_fs, c->d_fl,
+                  c->d_perm, c->d_iperm, c->d_cell, c->d_ccount, c->d_cstart, c->d_cfill,
+                  c->d_tmp, c->d_rec64, c->d_stype, c->d_nbr, c->d_nnb, c->d_flags,
+                  c->d_chg, c->d_stp, c->d_spart, c->d_kd, c->d_epair, c->d_eew, c->d_eec, c->d_eout,
+                  c->d_fxp, c->d_recf};
+  for (void *p : ptrs)
+    if (p) hipFree(p);
+  if (c->h_flags) hipHostFree(c->h_flags);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+size_t nb_kdata_bytes();
+
+extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
+  if (!p || !out) { nb_set_error("nb_create: null argument"); return NB_ERR_ARG; }
+  *out = nullptr;
+  const int n = p->natoms;
+  if (n <= 0 || n >= NB_MAX_ATOMS || p->ntypes <= 0 || p->ninter < 0 || p->ntable < 4 ||
+      !p->itype || !p->map || !p->qch) {
+    nb_set_error("nb_create: bad sizes or missing arrays");
+    return NB_ERR_ARG;
+  }
+  if (p->iperd != 3) {
+    nb_set_error("nb_create: only iperd==3 (3-D periodic) is supported");
+    return NB_ERR_UNSUPPORTED;
+  }
+  if (p->precision != NB_PREC_FP64 && p->precision != NB_PREC_MIXED) {
+    nb_set_error("nb_create: unknown precision %d", p->precision);
+    return NB_ERR_ARG;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    nb_set_error("nb_create: no HIP device");
+    return NB_ERR_NODEVICE;
+  }
+  if (p->device < 0 || p->device >= ndev) {
+    nb_set_error("nb_create: device %d out of range", p->device);
+    return NB_ERR_ARG;
+  }
+  // host-side validation of the type map and interaction indices
+  for (int i = 0; i < n; i++)
+    if (p->itype[i] < 0 || p->itype[i] >= p->ntypes) {
+      nb_set_error("nb_create: itype[%d]=%d out of range", i, p->itype[i]);
+      return NB_ERR_ARG;
+    }
+  for (int k = 0; k < p->ntypes * p->ntypes; k++)
+    if (p->map[k] < -1 || p->map[k] >= p->ninter) {
+      nb_set_error("nb_create: map entry %d out of range", p->map[k]);
+      return NB_ERR_ARG;
+    }
+  nb_ctx *c = new nb_ctx();
+  memset(c, 0, sizeof(*c));
+  c->dev = p->device;
+  if (hipSetDevice(c->dev) != hipSuccess) { delete c; nb_set_error("hipSetDevice failed"); return NB_ERR_HIP; }
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    nb_set_error("hipStreamCreate failed");
+    return NB_ERR_HIP;
+  }
+  int rc = NB_OK;
+  auto fail = [&](int r) { nb_destroy(c); return r; };
+  c->natoms = n; c->ntypes = p->ntypes; c->ninter = p->ninter; c->ntable = p->ntable;
+  c->iperd = p->iperd; c->prec = p->precision; c->skin = p->skin;
+  c->kappa = p->kappa; c->ewald_self = p->ewald_self; c->ewald_bkg = p->ewald_bkg;
+  c->dx2coul = p->dx2coul; c->rmins_coul2 = p->rmins_coul2; c->rmaxs_coul2 = p->rmaxs_coul2;
+  c->rminl_coul2 = p->rminl_coul2; c->rmaxl_coul2 = p->rmaxl_coul2;
+  c->rmaxs_skin2_e = p->rmaxs_skin2_e; c->rmaxl_skin2_e = p->rmaxl_skin2_e;
+
+  const int nt = p->ntable, ni = std::max(p->ninter, 1);
+  // interleaved tables {dV/r, switch, V} per node
+  std::vector<double> vt3((size_t)ni * nt * 3, 0.), ct3((size_t)nt * 3, 0.);
+  std::vector<InterPar> ip(ni);
+  memset(ip.data(), 0, sizeof(InterPar) * ni);
+  for (int k = 0; k < p->ninter; k++) {
+    for (int t = 0; t < nt; t++) {
+      vt3[((size_t)k * nt + t) * 3 + 0] = p->dvtab[(size_t)k * nt + t];
+      vt3[((size_t)k * nt + t) * 3 + 1] = p->switchs[(size_t)k * nt + t];
+      vt3[((size_t)k * nt + t) * 3 + 2] = p->vtab[(size_t)k * nt + t];
+    }
+    ip[k].rmins2 = p->rmins_cut2[k]; ip[k].rmaxs2 = p->rmaxs_cut2[k];
+    ip[k].rminl2 = p->rminl_cut2[k]; ip[k].rmaxl2 = p->rmaxl_cut2[k];
+    ip[k].dx2 = p->dx2tab[k]; ip[k].rmaxs_skin2 = p->rmaxs_skin2[k];
+    ip[k].rmaxl_skin2 = p->rmaxl_skin2[k]; ip[k].vlast = p->vtab[(size_t)k * nt + nt - 4];
+  }
+  int ncharge = 0;
+  std::vector<int> qidx;
+  for (int i = 0; i < n; i++)
+    if (p->qch[i] != 0.) { qidx.push_back(i); ncharge++; }
+  c->ncharge = ncharge;
+  if (ncharge > 0) {
+    if (!p->coultab || !p->dcoultab || !p->switchse) {
+      nb_set_error("nb_create: charges present but coulomb tables missing");
+      return fail(NB_ERR_ARG);
+    }
+    for (int t = 0; t < nt; t++) {
+      ct3[(size_t)t * 3 + 0] = p->dcoultab[t];
+      ct3[(size_t)t * 3 + 1] = p->switchse[t];
+      ct3[(size_t)t * 3 + 2] = p->coultab[t];
+    }
+    c->clast = p->coultab[nt - 4];
+  }
+  // list cutoff: the longest (r+skin) of any interaction / the coulomb list
+  double r2 = 0.;
+  for (int k = 0; k < p->ninter; k++) r2 = std::max(r2, p->rmaxl_skin2[k]);
+  if (ncharge > 0) r2 = std::max(r2, p->rmaxl_skin2_e);
+  c->rlist = sqrt(r2);
+
+  // exclusions, both directions: partner | bit30 (vdW) | bit31 (coulomb)
+  std::vector<std::vector<unsigned>> ex(n);
+  auto add_ex = [&](int a, int b, unsigned bit) {
+    for (auto &v : ex[a])
+      if ((int)(v & 0x3FFFFFFFu) == b) { v |= bit; return; }
+    ex[a].push_back((unsigned)b | bit);
+  };
+  {
+    size_t off = 0;
+    for (int i = 0; i < n; i++) {
+      int m = p->nexclude ? p->nexclude[i] : 0;
+      for (int k = 0; k < m; k++) {
+        int j = p->exclude[off + k];
+        if (j < 0 || j >= n) { nb_set_error("nb_create: exclusion out of range"); return fail(NB_ERR_ARG); }
+        add_ex(i, j, 1u << 30);
+        add_ex(j, i, 1u << 30);
+      }
+      off += m;
+    }
+    off = 0;
+    for (int i = 0; i < n; i++) {
+      int m = p->cnexclude ? p->cnexclude[i] : 0;
+      for (int k = 0; k < m; k++) {
+        int j = p->cexclude[off + k];
+        if (j < 0 || j >= n) { nb_set_error("nb_create: exclusion out of range"); return fail(NB_ERR_ARG); }
+        add_ex(i, j, 1u << 31);
+        add_ex(j, i, 1u << 31);
+      }
+      off += m;
+    }
+  }
+  std::vector<int> xoff(n + 1, 0), xlist;
+  for (int i = 0; i < n; i++) {
+    std::sort(ex[i].begin(), ex[i].end(), [](unsigned a, unsigned b) {
+      return (a & 0x3FFFFFFFu) < (b & 0x3FFFFFFFu);
+    });
+    for (unsigned v : ex[i]) xlist.push_back((int)v);
+    xoff[i + 1] = (int)xlist.size();
+  }
+  // ecorr partners, both directions
+  c->necorr = p->necorr;
+  std::vector<std::vector<int>> ec(n);
+  for (int k = 0; k < p->necorr; k++) {
+    int a = p->iecorr[k], b = p->jecorr[k];
+    if (a < 0 || a >= n || b < 0 || b >= n) { nb_set_error("nb_create: ecorr pair out of range"); return fail(NB_ERR_ARG); }
+    ec[a].push_back(b);
+    ec[b].push_back(a);
+  }
+  std::vector<int> eoff(n + 1, 0), elist;
+  for (int i = 0; i < n; i++) {
+    for (int v : ec[i]) elist.push_back(v);
+    eoff[i + 1] = (int)elist.size();
+  }
+  // Ewald k-vectors and segments (runs of consecutive kc in one (ka,kb) row)
+  c->ktot = (ncharge > 0) ? p->ktot : 0;
+  std::vector<EwSeg> seg;
+  std::vector<int> kabc(3 * (size_t)std::max(c->ktot, 1), 0);
+  for (int k = 0; k < c->ktot; k++) {
+    kabc[3 * k] = p->ka[k]; kabc[3 * k + 1] = p->kb[k]; kabc[3 * k + 2] = p->kc[k];
+    bool cont = !seg.empty() && seg.back().ka == p->ka[k] && seg.back().kb == p->kb[k] &&
+                seg.back().kc0 + seg.back().cnt == p->kc[k] && seg.back().cnt < NB_EW_SEG;
+    if (cont) {
+      seg.back().cnt++;
+    } else {
+      EwSeg s = {k, 1, p->ka[k], p->kb[k], p->kc[k], 0};
+      seg.push_back(s);
+    }
+  }
+  c->nseg = (int)seg.size();
+  c->nchunk = std::max(1, (ncharge + 4095) / 4096);
+
+  // device uploads
+  if ((rc = upload(&c->d_itype, p->itype, n))) return fail(rc);
+  if ((rc = upload(&c->d_map, p->map, (size_t)p->ntypes * p->ntypes))) return fail(rc);
+  if ((rc = upload(&c->d_ip, ip.data(), ni))) return fail(rc);
+  if ((rc = upload(&c->d_vtab3, vt3.data(), vt3.size()))) return fail(rc);
+  if ((rc = upload(&c->d_ctab3, ct3.data(), ct3.size()))) return fail(rc);
+  if ((rc = upload(&c->d_qch, p->qch, n))) return fail(rc);
+  if ((rc = upload(&c->d_xoff, xoff.data(), n + 1))) return fail(rc);
+  if ((rc = upload(&c->d_xlist, xlist.data(), xlist.size()))) return fail(rc);
+  if ((rc = upload(&c->d_eoff, eoff.data(), n + 1))) return fail(rc);
+  if ((rc = upload(&c->d_elist, elist.data(), elist.size()))) return fail(rc);
+  if ((rc = upload(&c->d_qidx, qidx.data(), qidx.size()))) return fail(rc);
+  if ((rc = upload(&c->d_seg, seg.data(), seg.size()))) return fail(rc);
+  if ((rc = upload(&c->d_kabc, kabc.data(), kabc.size()))) return fail(rc);
+  if ((rc = dalloc(&c->d_pos, 3 * (size_t)n))) return fail(rc);
+  if ((rc = dalloc(&c->d_posb, 3 * (size_t)n))) return fail(rc);
+  if ((rc = dalloc(&c->d_fs, 3 * (size_t)n))) return fail(rc);
+  if ((rc = dalloc(&c->d_fl, 3 * (size_t)n))) return fail(rc);
+  if ((rc = dalloc(&c->d_perm, n))) return fail(rc);
+  if ((rc = dalloc(&c->d_iperm, n))) return fail(rc);
+  if ((rc = dalloc(&c->d_cell, n))) return fail(rc);
+  if ((rc = dalloc(&c->d_tmp, n))) return fail(rc);
+  if ((rc = dalloc(&c->d_rec64, 4 * (size_t)n))) return fail(rc);
+  if ((rc = dalloc(&c->d_stype, n))) return fail(rc);
+  if ((rc = dalloc(&c->d_nnb, n))) return fail(rc);
+  if ((rc = dalloc(&c->d_flags, 8))) return fail(rc);
+  if ((rc = dalloc(&c->d_chg, 4 * (size_t)std::max(ncharge, 1)))) return fail(rc);
+  if ((rc = dalloc(&c->d_stp, 2 * (size_t)std::max(ncharge, 1)))) return fail(rc);
+  if ((rc = dalloc(&c->d_spart, 2 * (size_t)c->nchunk * std::max(c->ktot, 1)))) return fail(rc);
+  if ((rc = dalloc((char **)&c->d_kd, nb_kdata_bytes() * std::max(c->ktot, 1)))) return fail(rc);
+  c->npair_blocks = std::min((n + 3) / 4, 2048);
+  c->new_blocks = std::max(1, (c->ktot + 255) / 256);
+  c->necorr_blocks = (n + 255) / 256;
+  if ((rc = dalloc(&c->d_epair, (size_t)c->npair_blocks * NB_NE_PAIR))) return fail(rc);
+  if ((rc = dalloc(&c->d_eec, (size_t)c->necorr_blocks * NB_NE_ECORR))) return fail(rc);
+  if ((rc = dalloc(&c->d_eew, (size_t)c->new_blocks * NB_NE_EW))) return fail(rc);
+  if ((rc = dalloc(&c->d_eout, 64))) return fail(rc);
+  if (hipHostMalloc((void **)&c->h_flags, 16 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+    nb_set_error("hipHostMalloc failed");
+    return fail(NB_ERR_HIP);
+  }
+  // tables in LDS when they fit the 64 KiB a workgroup may claim
+  size_t tab_bytes = (vt3.size() + ct3.size()) * sizeof(double);
+  c->ctab_off = (int)vt3.size();
+  c->tab_lds_bytes = tab_bytes <= 60 * 1024 ? (int)tab_bytes : 0;
+  // neighbour capacity from the density, grown on overflow at build time
+  {
+    double vol_per_atom = 1e300;
+    (void)vol_per_atom;
+    c->cap = 0;
+  }
+  HCHK(hipMemset(c->d_eout, 0, 64 * sizeof(double)));
+  *out = c;
+  return NB_OK;
+}
+
+// Grid of cells no narrower than rlist in any direction: the perpendicular
+// width of the box along a is V/|b x c| (box vectors are hmat's columns).
+static void cell_grid(nb_ctx *c, const Box &b, int *nx, int *ny, int *nz) {
+  const double *h = b.h;
+  double a[3] = {h[0], h[3], h[6]}, bb[3] = {h[1], h[4], h[7]}, cc[3] = {h[2], h[5], h[8]};
+  auto cross = [](const double *u, const double *v, double *w) {
+    w[0] = u[1] * v[2] - u[2] * v[1];
+    w[1] = u[2] * v[0] - u[0] * v[2];
+    w[2] = u[0] * v[1] - u[1] * v[0];
+  };
+  double w[3], vol = fabs(det9(h));
+  double widths[3];
+  cross(bb, cc, w); widths[0] = vol / sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+  cross(cc, a, w);  widths[1] = vol / sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+  cross(a, bb, w);  widths[2] = vol / sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+  int ncs[3];
+  for (int d = 0; d < 3; d++) {
+    double r = c->rlist > 0 ? c->rlist * (1. + 1e-9) : widths[d];
+    int m = (int)floor(widths[d] / r);
+    if (m > 64) m = 64;
+    ncs[d] = m >= 3 ? m : 1;
+  }
+  *nx = ncs[0]; *ny = ncs[1]; *nz = ncs[2];
+}
+
+static int ensure_capacity(nb_ctx *c, const Box &b, int want_cap) {
+  if (want_cap <= c->cap && c->d_nbr) return NB_OK;
+  if (c->d_nbr) { hipFree(c->d_nbr); c->d_nbr = nullptr; }
+  c->cap = (want_cap + 63) & ~63;
+  return dalloc(&c->d_nbr, (size_t)c->cap * c->natoms);
+}
+
+static int build_on(nb_ctx *c, const double hmat[9], hipStream_t s) {
+  Box b = make_box(hmat);
+  int nx, ny, nz;
+  cell_grid(c, b, &nx, &ny, &nz);
+  c->ncx = nx; c->ncy = ny; c->ncz = nz;
+  c->ncells = nx * ny * nz;
+  if (c->ncells > c->cells_alloc) {
+    if (c->d_ccount) hipFree(c->d_ccount);
+    if (c->d_cstart) hipFree(c->d_cstart);
+    if (c->d_cfill) hipFree(c->d_cfill);
+    c->d_ccount = c->d_cstart = c->d_cfill = nullptr;
+    RC(dalloc(&c->d_ccount, c->ncells));
+    RC(dalloc(&c->d_cstart, c->ncells));
+    RC(dalloc(&c->d_cfill, c->ncells));
+    c->cells_alloc = c->ncells;
+  }
+  if (c->cap == 0) {
+    // expected full-list length from the mean density, with a margin
+    double vol = fabs(det9(hmat));
+    double nexp = 4.18879 * c->rlist * c->rlist * c->rlist * c->natoms / vol;
+    int want = (int)(1.6 * nexp) + 64;
+    if (want > c->natoms) want = c->natoms + 63;
+    RC(ensure_capacity(c, b, want));
+  }
+  for (int attempt = 0; attempt < 3; attempt++) {
+    RC(nb_launch_build(c, b, s));
+    HCHK(hipMemcpyAsync(c->h_flags, c->d_flags, 8 * sizeof(int), hipMemcpyDeviceToHost, s));
+    HCHK(hipStreamSynchronize(s));
+    if (c->h_flags[0] <= c->cap) {
+      c->box_built = b;
+      c->have_lists = 1;
+      return NB_OK;
+    }
+    RC(ensure_capacity(c, b, (int)(c->h_flags[0] * 1.25) + 64));
+  }
+  nb_set_error("neighbour list capacity could not be satisfied");
+  return NB_ERR_CAPACITY;
+}
+
+extern "C" int nb_build_lists(nb_ctx *c, const double *pxyz, const double hmat[9]) {
+  if (!c || !pxyz || !hmat) { nb_set_error("nb_build_lists: null argument"); return NB_ERR_ARG; }
+  HCHK(hipSetDevice(c->dev));
+  HCHK(hipMemcpyAsync(c->d_pos, pxyz, sizeof(double) * 3 * c->natoms, hipMemcpyHostToDevice,
+                      c->stream));
+  return build_on(c, hmat, c->stream);
+}
+
+extern "C" int nb_build_lists_device(nb_ctx *c, const double *d_pxyz, const double hmat[9],
+                                     void *stream) {
+  if (!c || !d_pxyz || !hmat) { nb_set_error("nb_build_lists_device: null argument"); return NB_ERR_ARG; }
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  HCHK(hipSetDevice(c->dev));
+  if (d_pxyz != c->d_pos)
+    HCHK(hipMemcpyAsync(c->d_pos, d_pxyz, sizeof(double) * 3 * c->natoms,
+                        hipMemcpyDeviceToDevice, s));
+  return build_on(c, hmat, s);
+}
+
+extern "C" int nb_need_rebuild(nb_ctx *c, const double *pxyz, int *flag) {
+  if (!c || !pxyz || !flag) { nb_set_error("nb_need_rebuild: null argument"); return NB_ERR_ARG; }
+  if (!c->have_lists) { *flag = 1; return NB_OK; }
+  HCHK(hipSetDevice(c->dev));
+  // positions go to the scratch force buffer so the build copy stays intact
+  HCHK(hipMemcpyAsync(c->d_fs, pxyz, sizeof(double) * 3 * c->natoms, hipMemcpyHostToDevice,
+                      c->stream));
+  RC(nb_launch_rebuild_check(c, c->d_fs, c->stream));
+  HCHK(hipMemcpyAsync(c->h_flags + 1, c->d_flags + 1, sizeof(int), hipMemcpyDeviceToHost,
+                      c->stream));
+  HCHK(hipStreamSynchronize(c->stream));
+  *flag = c->h_flags[1] ? 1 : 0;
+  return NB_OK;
+}
+
+static void fill_energy(nb_ctx *c, int iflg, const double *E, const double hmat[9],
+                        nb_energy *e) {
+  memset(e, 0, sizeof(*e));
+  e->nclamp = (int)(E[NB_NE_PAIR - 1] * .5 + .25);
+  if (iflg != 1) return;
+  const double hf = .5;
+  e->spotl = hf * E[0]; e->spothl = hf * E[1]; e->sprsl = hf * E[2];
+  e->sprslt[0] = hf * E[3]; e->sprslt[1] = hf * E[4]; e->sprslt[2] = hf * E[5];
+  e->sprslt[4] = hf * E[6]; e->sprslt[5] = hf * E[7]; e->sprslt[8] = hf * E[8];
+  e->sprslt[3] = e->sprslt[1]; e->sprslt[6] = e->sprslt[2]; e->sprslt[7] = e->sprslt[5];
+  e->spote = hf * E[9]; e->spothe = hf * E[10]; e->sprse = hf * E[11];
+  e->sprset[0] = hf * E[12]; e->sprset[1] = hf * E[13]; e->sprset[2] = hf * E[14];
+  e->sprset[4] = hf * E[15]; e->sprset[5] = hf * E[16]; e->sprset[8] = hf * E[17];
+  e->sprset[3] = e->sprset[1]; e->sprset[6] = e->sprset[2]; e->sprset[7] = e->sprset[5];
+  if (c->iperd == 3 && c->necorr > 0) {
+    e->pot_ecorr = hf * E[20];
+    e->prs_ecorr = hf * E[21];
+    for (int k = 0; k < 9; k++) e->prs_ecorr_tensor[k] = hf * E[22 + k];
+  }
+  if (c->iperd == 3 && c->ktot > 0) {
+    const double *W = E + 32;
+    double bk = c->ewald_bkg / det9(hmat);
+    e->vk_ewald = W[0] + bk + c->ewald_self;
+    e->vp_ewald = W[1] + 3. * (W[0] + bk);
+    for (int k = 0; k < 9; k++) e->vp_ewald_tensor[k] = W[2 + k];
+    e->vp_ewald_tensor[0] += W[0] + bk;
+    e->vp_ewald_tensor[4] += W[0] + bk;
+    e->vp_ewald_tensor[8] += W[0] + bk;
+  } else if (c->iperd == 3) {
+    // cm3d still runs fk_ewald with no k vectors: background + self only
+    double bk = c->ewald_bkg / det9(hmat);
+    e->vk_ewald = bk + c->ewald_self;
+    e->vp_ewald = 3. * bk;
+    e->vp_ewald_tensor[0] = e->vp_ewald_tensor[4] = e->vp_ewald_tensor[8] = bk;
+  }
+}
+
+static int force_on(nb_ctx *c, int iflg, const double hmat[9], double *d_fs, double *d_fl,
+                    nb_energy *e, hipStream_t s) {
+  if (iflg != 0 && iflg != 1) { nb_set_error("nb_force: iflg must be 0 or 1"); return NB_ERR_ARG; }
+  if (!c->have_lists) { nb_set_error("nb_force: lists not built (call nb_build_lists)"); return NB_ERR_ARG; }
+  Box b = make_box(hmat);
+  RC(nb_launch_force(c, iflg, b, d_fs, d_fl, s));
+  if (e) {
+    double E[64];
+    HCHK(hipMemcpyAsync(E, c->d_eout, sizeof(E), hipMemcpyDeviceToHost, s));
+    HCHK(hipStreamSynchronize(s));
+    fill_energy(c, iflg, E, hmat, e);
+  }
+  return NB_OK;
+}
+
+extern "C" int nb_force(nb_ctx *c, int iflg, const double *pxyz, const double hmat[9],
+                        double *fs, double *fl, nb_energy *e) {
+  if (!c || !pxyz || !hmat || !fs || (iflg == 1 && !fl)) {
+    nb_set_error("nb_force: null argument");
+    return NB_ERR_ARG;
+  }
+  HCHK(hipSetDevice(c->dev));
+  const size_t bytes = sizeof(double) * 3 * c->natoms;
+  HCHK(hipMemcpyAsync(c->d_pos, pxyz, bytes, hipMemcpyHostToDevice, c->stream));
+  RC(force_on(c, iflg, hmat, c->d_fs, iflg ? c->d_fl : nullptr, nullptr, c->stream));
+  HCHK(hipMemcpyAsync(fs, c->d_fs, bytes, hipMemcpyDeviceToHost, c->stream));
+  if (iflg == 1) HCHK(hipMemcpyAsync(fl, c->d_fl, bytes, hipMemcpyDeviceToHost, c->stream));
+  if (e) {
+    double E[64];
+    HCHK(hipMemcpyAsync(E, c->d_eout, sizeof(E), hipMemcpyDeviceToHost, c->stream));
+    HCHK(hipStreamSynchronize(c->stream));
+    fill_energy(c, iflg, E, hmat, e);
+  } else {
+    HCHK(hipStreamSynchronize(c->stream));
+  }
+  return NB_OK;
+}
+
+extern "C" int nb_force_device(nb_ctx *c, int iflg, const double *d_pxyz, const double hmat[9],
+                               double *d_fs, double *d_fl, nb_energy *e, void *stream) {
+  if (!c || !d_pxyz || !hmat || (iflg == 1 && !d_fl) || (iflg == 0 && !d_fs)) {
+    nb_set_error("nb_force_device: null argument");
+    return NB_ERR_ARG;
+  }
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  HCHK(hipSetDevice(c->dev));
+  if (d_pxyz != c->d_pos)
+    HCHK(hipMemcpyAsync(c->d_pos, d_pxyz, sizeof(double) * 3 * c->natoms,
+                        hipMemcpyDeviceToDevice, s));
+  return force_on(c, iflg, hmat, d_fs, iflg ? d_fl : nullptr, e, s);
+}
+
+extern "C" int nb_list_counts(nb_ctx *c, int counts[4]) {
+  if (!c || !counts) { nb_set_error("nb_list_counts: null argument"); return NB_ERR_ARG; }
+  if (!c->have_lists) { nb_set_error("nb_list_counts: lists not built"); return NB_ERR_ARG; }
+  counts[0] = c->h_flags[2]; counts[1] = c->h_flags[3];
+  counts[2] = c->h_flags[4]; counts[3] = c->h_flags[5];
+  return NB_OK;
+}
+
+extern "C" int nb_get_pairs(nb_ctx *c, int which, int *ip, int *jp, int cap, int *nout) {
+  if (!c || !nout || which < 0 || which > 3) { nb_set_error("nb_get_pairs: bad argument"); return NB_ERR_ARG; }
+  if (!c->have_lists) { nb_set_error("nb_get_pairs: lists not built"); return NB_ERR_ARG; }
+  HCHK(hipSetDevice(c->dev));
+  const int n = c->natoms;
+  std::vector<uint32_t> nbr((size_t)n * c->cap);
+  std::vector<int> nnb(n), perm(n);
+  HCHK(hipStreamSynchronize(c->stream));
+  HCHK(hipMemcpy(nbr.data(), c->d_nbr, nbr.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  HCHK(hipMemcpy(nnb.data(), c->d_nnb, n * sizeof(int), hipMemcpyDeviceToHost));
+  HCHK(hipMemcpy(perm.data(), c->d_perm, n * sizeof(int), hipMemcpyDeviceToHost));
+  static const unsigned bit[4] = {NB_FLAG_VS, NB_FLAG_VT, NB_FLAG_ES, NB_FLAG_ET};
+  std::vector<std::pair<int, int>> pr;
+  for (int s = 0; s < n; s++) {
+    int a = perm[s];
+    for (int k = 0; k < nnb[s]; k++) {
+      uint32_t e = nbr[(size_t)s * c->cap + k];
+      if (!((e >> 28) & bit[which])) continue;
+      int b = perm[e & NB_IDX_MASK];
+      if (a < b) pr.emplace_back(a, b);
+    }
+  }
+  std::sort(pr.begin(), pr.end());
+  *nout = (int)pr.size();
+  if ((int)pr.size() > cap) { nb_set_error("nb_get_pairs: capacity %d < %zu", cap, pr.size()); return NB_ERR_CAPACITY; }
+  for (size_t k = 0; k < pr.size(); k++) { ip[k] = pr[k].first; jp[k] = pr[k].second; }
+  return NB_OK;
+}

@@ -1,0 +1,133 @@
+// nb_internal.h -- device-side data layout and helpers of the gfx950
+// nonbonded engine (not part of the C ABI).
+//
+// HBM layout per context (N atoms, Q charges, K k-vectors, cap = list slots
+// per atom):
+//   pos      3N f64   caller's positions, cm3d SoA (x[N] y[N] z[N])
+//   posb     3N f64   positions at the last list build (rebuild trigger)
+//   fs, fl   3N f64   outputs, cm3d SoA, before rmass
+//   perm/iperm N i32  cell-sorted order <-> atom index
+//   rec64    N x {x,y,z,q} f64 in sorted order, refreshed every evaluation
+//   stype    N i32    atom type in sorted order
+//   nbr      N*cap u32 full (both directions) neighbor lists in sorted
+//                     order: low 28 bits the sorted index j, high 4 bits
+//                     the list flags (vdW t, vdW s, coul te, coul se)
+//   nnb      N i32    list lengths
+//   chg      Q x {2pi s_a, 2pi s_b, 2pi s_c, q sqrt(4pi)} f64, stp Q x {cos,
+//                     sin}(2pi s_c): per-evaluation Ewald charge records
+//   kd       K x {kx, ky, kz, 2 preg Re S, 2 preg Im S} f64
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/cm3d_nb.h"
+
+#define NB_WAVE 64
+#define NB_FLAG_VT 1u  // vdW t list  (rmaxl+skin)
+#define NB_FLAG_VS 2u  // vdW s list  (rmaxs+skin)
+#define NB_FLAG_ET 4u  // coulomb te  (rcute_max+skin)
+#define NB_FLAG_ES 8u  // coulomb se  (rcute_resp+skin)
+#define NB_IDX_MASK 0x0FFFFFFFu
+#define NB_MAX_ATOMS (1 << 28)
+#define NB_EW_SEG 16     // k vectors per Ewald segment (one c-axis run)
+#define NB_NE_PAIR 19    // pair energy/virial accumulators + clamp count
+#define NB_NE_ECORR 11   // ecorr energy/virial accumulators
+#define NB_NE_EW 11      // Ewald accumulators
+
+struct InterPar {   // one vdW interaction (cm3d INTER arrays at index in)
+  double rmins2, rmaxs2, rminl2, rmaxl2, dx2, rmaxs_skin2, rmaxl_skin2, vlast;
+};
+
+struct EwSeg {      // run of <= NB_EW_SEG consecutive kc in one (ka,kb) row
+  int k0, cnt, ka, kb, kc0, pad;
+};
+
+// cm3d's -DLINUX anint: (int)(x +- 0.5), truncating (md_macros.h:31-52).
+__device__ __forceinline__ int nb_anint(double a) {
+  return (int)(a + (a >= 0 ? .5 : -.5));
+}
+
+// Minimum image in cm3d's operation order (md_period.new.c:39-51).  The
+// library is compiled with -ffp-contract=off so no FMA is formed here and
+// the result is bitwise the reference's.
+__device__ __forceinline__ void nb_minimg(double &dx, double &dy, double &dz,
+                                          const double *h, const double *hi) {
+  double sx = dx * hi[0] + dy * hi[1] + dz * hi[2];
+  double sy = dx * hi[3] + dy * hi[4] + dz * hi[5];
+  double sz = dx * hi[6] + dy * hi[7] + dz * hi[8];
+  sx -= nb_anint(sx);
+  sy -= nb_anint(sy);
+  sz -= nb_anint(sz);
+  dx = sx * h[0] + sy * h[1] + sz * h[2];
+  dy = sx * h[3] + sy * h[4] + sz * h[5];
+  dz = sx * h[6] + sy * h[7] + sz * h[8];
+}
+
+// 3-point quadratic interpolation (md_fnopol.c:146) on an interleaved table
+// t[3*k + c]: c = component (0 force, 1 switch, 2 energy).
+__device__ __forceinline__ double nb_interp3(const double *t, int k, int c,
+                                             double p) {
+  double fm1 = t[3 * (k - 1) + c], f0 = t[3 * k + c], f1 = t[3 * (k + 1) + c];
+  return f0 + .5 * p * (f1 - fm1 + p * (fm1 - 2. * f0 + f1));
+}
+
+template <typename T>
+__device__ __forceinline__ T nb_wave_sum(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, NB_WAVE);
+  return v;
+}
+
+struct Box {
+  double h[9], hi[9];
+};
+
+struct nb_ctx {
+  int dev;
+  hipStream_t stream;
+  int natoms, ntypes, ninter, ntable, iperd, prec;
+  int ncharge, ktot, nseg, necorr;
+  double skin, kappa, ewald_self, ewald_bkg;
+  double dx2coul, rmins_coul2, rmaxs_coul2, rminl_coul2, rmaxl_coul2;
+  double rmaxs_skin2_e, rmaxl_skin2_e, rlist, clast;
+  // parameters on device
+  int *d_itype, *d_map;
+  InterPar *d_ip;
+  double *d_vtab3, *d_ctab3;
+  float *d_vtab3f, *d_ctab3f;
+  double *d_qch;
+  int *d_xoff, *d_xlist;       // exclusions, both directions, flag bits
+  int *d_eoff, *d_elist;       // ecorr partners, both directions
+  int *d_qidx;                 // charge -> atom
+  EwSeg *d_seg;
+  int *d_kabc;                 // ka,kb,kc per k (3K)
+  // state
+  double *d_pos, *d_posb, *d_fs, *d_fl;
+  int *d_perm, *d_iperm, *d_cell, *d_ccount, *d_cstart, *d_cfill, *d_tmp;
+  double *d_rec64;
+  int *d_stype;
+  uint32_t *d_nbr;
+  int *d_nnb;
+  int cap;
+  int *d_flags;                // [0] overflow max count, [1] rebuild flag,
+                               // [2..5] half-pair counts s,t,se,te, [6] clamps
+  int ncx, ncy, ncz, ncells, cells_alloc;
+  double *d_chg, *d_stp, *d_spart, *d_kd;
+  int nchunk;
+  double *d_epair, *d_eew, *d_eec, *d_eout;
+  int npair_blocks, new_blocks, necorr_blocks;
+  long long *d_fxp;            // fixed-point force accumulators (mixed)
+  float4 *d_recf;              // mixed: {sx, sy, sz (u32 bits), q}
+  Box box_built;
+  int have_lists;
+  int tab_lds_bytes, ctab_off;
+  // host mirrors
+  int *h_flags;
+};
+
+// kernel launchers (nb_kernels.hip)
+int nb_launch_build(nb_ctx *c, const Box &b, hipStream_t s);
+int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs,
+                    double *d_fl, hipStream_t s);
+int nb_launch_rebuild_check(nb_ctx *c, const double *d_pos, hipStream_t s);
+void nb_set_error(const char *fmt, ...);

@@ -1,0 +1,718 @@
+// nb_kernels.hip -- gfx950 kernels of the nonbonded path.
+//
+//   list build   k_cell_assign -> k_cell_scan -> k_cell_scatter ->
+//                k_cell_order -> k_pack -> k_build_list
+//                (cm3d get_ngbr / get_ngbr_e, energy/control/ngbr_list.c:297,
+//                 elec_ngbr.c:220, set_link.c:203)
+//   pair forces  k_pair64 / k_pair_mixed (force_npol md_fnopol.c:46, force_e
+//                finter_e.c:42, ecorr md_ewald.c:385)
+//   reciprocal   k_ew_charges -> k_ew_sfac -> k_ew_kreduce -> k_ew_force
+//                (fk_ewald md_ewald.c:185)
+//   reductions   k_reduce_rows (the getsvirter / getsvirter_e / getvewald sums)
+//
+// Every kernel is launched on the context's stream; none allocates.
+#include <stdarg.h>
+#include <stdio.h>
+
+#include "nb_internal.h"
+
+static __device__ __forceinline__ int lane_id() { return threadIdx.x & (NB_WAVE - 1); }
+
+static __device__ __forceinline__ int mbcnt(unsigned long long m) {
+  return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
+// ---------------------------------------------------------------------------
+// Cell sort.  Cells tile the box in fractional coordinates with widths no
+// smaller than the longest list cutoff, so every pair within it under cm3d's
+// minimum image lies in the 27 (or fewer, for thin boxes) neighbouring cells.
+// Binning follows get_lnklist (set_link.c:218-236): wrap s -= anint(s), then
+// (int)((s+.5)*n) clamped.
+// ---------------------------------------------------------------------------
+__global__ void k_cell_assign(const double *__restrict__ pos, int n, Box b,
+                              int ncx, int ncy, int ncz, int *__restrict__ cell,
+                              int *__restrict__ count) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double x = pos[i], y = pos[n + i], z = pos[2 * n + i];
+  const double *hi = b.hi;
+  double sx = x * hi[0] + y * hi[1] + z * hi[2];
+  double sy = x * hi[3] + y * hi[4] + z * hi[5];
+  double sz = x * hi[6] + y * hi[7] + z * hi[8];
+  sx -= nb_anint(sx);
+  sy -= nb_anint(sy);
+  sz -= nb_anint(sz);
+  int a = (int)((sx + .5) * ncx), c = (int)((sy + .5) * ncy), e = (int)((sz + .5) * ncz);
+  a = a < 0 ? 0 : (a > ncx - 1 ? ncx - 1 : a);
+  c = c < 0 ? 0 : (c > ncy - 1 ? ncy - 1 : c);
+  e = e < 0 ? 0 : (e > ncz - 1 ? ncz - 1 : e);
+  int id = (a * ncy + c) * ncz + e;
+  cell[i] = id;
+  atomicAdd(&count[id], 1);
+}
+
+// Exclusive scan of the cell counts in one workgroup of 1024 threads.
+__global__ __launch_bounds__(1024) void k_cell_scan(const int *__restrict__ count,
+                                                    int ncells,
+                                                    int *__restrict__ start,
+                                                    int *__restrict__ fill) {
+  __shared__ int wsum[16];
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < ncells; base += 1024) {
+    int i = base + threadIdx.x;
+    int v = i < ncells ? count[i] : 0;
+    // inclusive scan within the wave
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < NB_WAVE; o <<= 1) {
+      int y = __shfl_up(x, o, NB_WAVE);
+      if (lane_id() >= o) x += y;
+    }
+    int w = threadIdx.x >> 6;
+    if (lane_id() == NB_WAVE - 1) wsum[w] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int acc = 0;
+      for (int k = 0; k < 16; k++) { int t = wsum[k]; wsum[k] = acc; acc += t; }
+    }
+    __syncthreads();
+    int excl = carry + wsum[w] + x - v;
+    if (i < ncells) { start[i] = excl; fill[i] = excl; }
+    __syncthreads();
+    if (threadIdx.x == 1023) carry = excl + v;
+    __syncthreads();
+  }
+}
+
+__global__ void k_cell_scatter(const int *__restrict__ cell, int n,
+                               int *__restrict__ fill, int *__restrict__ tmp) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int slot = atomicAdd(&fill[cell[i]], 1);
+  tmp[slot] = i;
+}
+
+// Order each cell's atoms by atom index (the scatter's atomics leave them in
+// arbitrary order): every member's rank is the number of smaller indices in
+// the cell.  Makes the sorted order, and so every list, deterministic.
+#define NB_CELL_LDS 4096
+__global__ __launch_bounds__(256) void k_cell_order(
+    const int *__restrict__ count, const int *__restrict__ start,
+    const int *__restrict__ tmp, int *__restrict__ perm, int *__restrict__ iperm,
+    int *__restrict__ scell) {
+  __shared__ int seg[NB_CELL_LDS];
+  int c = blockIdx.x, n = count[c], st = start[c];
+  const int *src = tmp + st;
+  if (n <= NB_CELL_LDS) {
+    for (int t = threadIdx.x; t < n; t += blockDim.x) seg[t] = src[t];
+    __syncthreads();
+    src = seg;
+  }
+  for (int t = threadIdx.x; t < n; t += blockDim.x) {
+    int a = src[t], r = 0;
+    for (int u = 0; u < n; u++) r += src[u] < a;
+    perm[st + r] = a;
+    iperm[a] = st + r;
+    scell[st + r] = c;
+  }
+}
+
+// Sorted-order records refreshed at every evaluation: {x,y,z,q} in FP64.
+__global__ void k_pack64(const double *__restrict__ pos, const double *__restrict__ q,
+                         const int *__restrict__ itype, const int *__restrict__ perm,
+                         int n, double4 *__restrict__ rec, int *__restrict__ stype) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  int a = perm[s];
+  rec[s] = make_double4(pos[a], pos[n + a], pos[2 * n + a], q[a]);
+  if (stype) stype[s] = itype[a];
+}
+
+// ---------------------------------------------------------------------------
+// Verlet list build, one wavefront per (sorted) atom.  Each wave walks the
+// neighbouring cells 64 candidates at a time, applies cm3d's filters in FP64
+// exactly as ngbr_list / ngbr_liste do (null map, exclusion, minimum image,
+// strict r^2 < (rc+skin)^2, s nested in t), and compacts the survivors with a
+// ballot + mbcnt prefix into the atom's slots, so the order is the candidate
+// order.  Both directions of every pair are stored.
+// ---------------------------------------------------------------------------
+struct ListArgs {
+  int n, ntypes, cap, ncx, ncy, ncz;
+  const double4 *rec;
+  const int *stype, *perm, *scell, *cstart, *ccount;
+  const int *xoff, *xlist, *map;
+  const InterPar *ip;
+  double rmaxs_skin2_e, rmaxl_skin2_e;
+  Box b;
+  uint32_t *nbr;
+  int *nnb, *flags;
+};
+
+static __device__ __forceinline__ int wrapc(int v, int n) {
+  return v < 0 ? v + n : (v >= n ? v - n : v);
+}
+
+__global__ __launch_bounds__(256) void k_build_list(ListArgs A) {
+  const int lane = lane_id();
+  const int s = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (s >= A.n) return;
+  const double4 ri = A.rec[s];
+  const int ti = A.stype[s], ai = A.perm[s];
+  const int c = A.scell[s];
+  const int cz = c % A.ncz, cy = (c / A.ncz) % A.ncy, cx = c / (A.ncy * A.ncz);
+  const int x0 = A.xoff[ai], x1 = A.xoff[ai + 1];
+  const bool qi = ri.w != 0.0;
+  const int rx = A.ncx >= 3 ? 1 : 0, ry = A.ncy >= 3 ? 1 : 0, rz = A.ncz >= 3 ? 1 : 0;
+  int count = 0, hs = 0, ht = 0, hse = 0, hte = 0;
+  uint32_t *out = A.nbr + (size_t)s * A.cap;
+  for (int ox = -rx; ox <= rx; ox++)
+    for (int oy = -ry; oy <= ry; oy++)
+      for (int oz = -rz; oz <= rz; oz++) {
+        int cc = (wrapc(cx + ox, A.ncx) * A.ncy + wrapc(cy + oy, A.ncy)) * A.ncz +
+                 wrapc(cz + oz, A.ncz);
+        int st = A.cstart[cc], nn = A.ccount[cc];
+        for (int b0 = 0; b0 < nn; b0 += NB_WAVE) {
+          int t = b0 + lane, s2 = st + t, aj = -1;
+          unsigned fl = 0;
+          if (t < nn && s2 != s) {
+            double4 rj = A.rec[s2];
+            int tj = A.stype[s2];
+            aj = A.perm[s2];
+            double dx = ri.x - rj.x, dy = ri.y - rj.y, dz = ri.z - rj.z;
+            nb_minimg(dx, dy, dz, A.b.h, A.b.hi);
+            double d2 = dx * dx + dy * dy + dz * dz;
+            unsigned ex = 0;
+            for (int e = x0; e < x1; e++) {
+              unsigned v = (unsigned)A.xlist[e];
+              if ((int)(v & 0x3FFFFFFFu) == aj) ex |= v >> 30;
+            }
+            int in = A.map[ti * A.ntypes + tj];
+            if (in >= 0 && !(ex & 1u)) {
+              const InterPar &P = A.ip[in];
+              if (d2 < P.rmaxl_skin2) {
+                fl |= NB_FLAG_VT;
+                if (d2 < P.rmaxs_skin2) fl |= NB_FLAG_VS;
+              }
+            }
+            if (qi && rj.w != 0.0 && !(ex & 2u)) {
+              if (d2 < A.rmaxl_skin2_e) {
+                fl |= NB_FLAG_ET;
+                if (d2 < A.rmaxs_skin2_e) fl |= NB_FLAG_ES;
+              }
+            }
+          }
+          unsigned long long m = __ballot(fl != 0);
+          if (fl) {
+            int slot = count + mbcnt(m);
+            if (slot < A.cap) out[slot] = (uint32_t)s2 | (fl << 28);
+            if (ai < aj) {
+              hs += (fl & NB_FLAG_VS) != 0;
+              ht += (fl & NB_FLAG_VT) != 0;
+              hse += (fl & NB_FLAG_ES) != 0;
+              hte += (fl & NB_FLAG_ET) != 0;
+            }
+          }
+          count += __popcll(m);
+        }
+      }
+  hs = nb_wave_sum(hs);
+  ht = nb_wave_sum(ht);
+  hse = nb_wave_sum(hse);
+  hte = nb_wave_sum(hte);
+  if (lane == 0) {
+    A.nnb[s] = count < A.cap ? count : A.cap;
+    if (count > A.cap) atomicMax(&A.flags[0], count);
+    if (hs) atomicAdd(&A.flags[2], hs);
+    if (ht) atomicAdd(&A.flags[3], ht);
+    if (hse) atomicAdd(&A.flags[4], hse);
+    if (hte) atomicAdd(&A.flags[5], hte);
+  }
+}
+
+// nrefsh criterion (md_finter.c:61-73): any |r - r_build|^2 > skin^2/4.
+__global__ void k_rebuild_check(const double *__restrict__ pos,
+                                const double *__restrict__ posb, int n,
+                                double skin2q, int *__restrict__ flag) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double x = posb[i] - pos[i], y = posb[n + i] - pos[n + i], z = posb[2 * n + i] - pos[2 * n + i];
+  if (x * x + y * y + z * z > skin2q) atomicOr(flag, 1);
+}
+
+// ---------------------------------------------------------------------------
+// Pair forces, FP64.  One wavefront per sorted atom i walks i's full list,
+// 64 neighbours at a time; each lane evaluates cm3d's table kernels for its
+// pair with cm3d's arithmetic (so each pair force is bitwise the reference's)
+// and accumulates F_i; the wave then reduces F_i with shuffles.  Energies and
+// virials count every pair twice and are halved at the end.  The tables
+// (interleaved {dV/r, switch, V} per r^2 node) are staged in LDS when they
+// fit.  Excluded charged pairs get the ecorr term here too.
+// ---------------------------------------------------------------------------
+struct PairArgs {
+  int n, ntypes, ninter, ntable, cap, tab_lds, ctab_off;
+  const uint32_t *nbr;
+  const int *nnb, *stype, *perm, *map;
+  const double4 *rec;
+  const InterPar *ip;
+  const double *vtab3, *ctab3;
+  double rmins_coul2, dx2coul, rmaxl_coul2, clast;
+  Box b;
+  double *fl, *fs;
+  double *epart;
+};
+
+// accumulator slots of the pair kernel (NB_NE_PAIR = 19 with the clamp count)
+enum {
+  E_SPOTL = 0, E_SPOTHL, E_SPRSL, E_SL0, E_SL1, E_SL2, E_SL4, E_SL5, E_SL8,
+  E_SPOTE, E_SPOTHE, E_SPRSE, E_SE0, E_SE1, E_SE2, E_SE4, E_SE5, E_SE8, E_CLAMP
+};
+
+// Sum NACC per-thread accumulators over a 256-thread block (wave shuffles,
+// then the 4 wave totals in LDS, fixed order) and store them to out[0..NACC).
+template <int NACC>
+static __device__ __forceinline__ void block_reduce_store(const double *acc, double *out) {
+  __shared__ double red[4][NACC];
+  const int w = threadIdx.x >> 6, lane = lane_id();
+#pragma unroll
+  for (int k = 0; k < NACC; k++) {
+    double v = nb_wave_sum(acc[k]);
+    if (lane == 0) red[w][k] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < NACC) {
+    double v = red[0][threadIdx.x] + red[1][threadIdx.x];
+    v += red[2][threadIdx.x] + red[3][threadIdx.x];
+    out[threadIdx.x] = v;
+  }
+}
+
+// IFLG 1: fl = full force, fs = switched force (when WANT_FS), energies and
+// virials; IFLG 0: fs = short-range force over the s lists, no energies
+// (cm3d computes them only at the long level).
+template <int IFLG, bool WANT_FS>
+__global__ __launch_bounds__(256, 2) void k_pair64(PairArgs A) {
+  extern __shared__ double tabs[];
+  const double *vt = A.vtab3, *ct = A.ctab3;
+  if (A.tab_lds) {
+    int nv = A.ninter * A.ntable * 3, nc = A.ntable * 3;
+    for (int t = threadIdx.x; t < nv; t += blockDim.x) tabs[t] = A.vtab3[t];
+    for (int t = threadIdx.x; t < nc; t += blockDim.x) tabs[A.ctab_off + t] = A.ctab3[t];
+    __syncthreads();
+    vt = tabs;
+    ct = tabs + A.ctab_off;
+  }
+  double acc[NB_NE_PAIR];
+#pragma unroll
+  for (int k = 0; k < NB_NE_PAIR; k++) acc[k] = 0.;
+  const int lane = lane_id();
+  const int nw = gridDim.x * 4;
+  constexpr unsigned fv = IFLG ? NB_FLAG_VT : NB_FLAG_VS;
+  constexpr unsigned fe = IFLG ? NB_FLAG_ET : NB_FLAG_ES;
+  const int ntm4 = A.ntable - 4;
+  for (int s = blockIdx.x * 4 + (threadIdx.x >> 6); s < A.n; s += nw) {
+    const double4 ri = A.rec[s];
+    const int ti = A.stype[s];
+    const int nb = A.nnb[s];
+    const uint32_t *lst = A.nbr + (size_t)s * A.cap;
+    double fx = 0., fy = 0., fz = 0., sx = 0., sy = 0., sz = 0.;
+    for (int k = lane; k < nb; k += NB_WAVE) {
+      uint32_t e = lst[k];
+      unsigned fl = e >> 28;
+      if (!(fl & (fv | fe))) continue;
+      int j = (int)(e & NB_IDX_MASK);
+      double4 rj = A.rec[j];
+      double dx = ri.x - rj.x, dy = ri.y - rj.y, dz = ri.z - rj.z;
+      nb_minimg(dx, dy, dz, A.b.h, A.b.hi);
+      double d2 = dx * dx + dy * dy + dz * dz;
+      if (fl & fv) {
+        int in = A.map[ti * A.ntypes + A.stype[j]];
+        const InterPar &P = A.ip[in];
+        double rmax2 = IFLG ? P.rmaxl2 : P.rmaxs2;
+        if (d2 < rmax2) {
+          double p = (d2 - P.rmins2) * P.dx2;
+          int kk = (int)p;
+          if (kk < 1) { acc[E_CLAMP] += 1.; kk = 1; p = 1.; }
+          p = p - (double)kk;
+          const double *T = vt + (size_t)in * A.ntable * 3;
+          double f = nb_interp3(T, kk, 0, p);
+          double swv = 1.;
+          if ((WANT_FS || !IFLG) && d2 > P.rminl2) swv = nb_interp3(T, kk, 1, p);
+          if (IFLG) {
+            acc[E_SPRSL] -= d2 * f;
+            acc[E_SL0] -= dx * dx * f;
+            acc[E_SL1] -= dy * dx * f;
+            acc[E_SL2] -= dz * dx * f;
+            acc[E_SL4] -= dy * dy * f;
+            acc[E_SL5] -= dz * dy * f;
+            acc[E_SL8] -= dz * dz * f;
+            double gx = dx * f, gy = dy * f, gz = dz * f;
+            fx -= gx; fy -= gy; fz -= gz;
+            if (WANT_FS) { sx -= gx * swv; sy -= gy * swv; sz -= gz * swv; }
+            double v = nb_interp3(T, kk, 2, p);
+            acc[E_SPOTL] += v;
+            acc[E_SPOTHL] += v - T[3 * ntm4 + 2];
+          } else {
+            f *= swv;
+            sx -= dx * f; sy -= dy * f; sz -= dz * f;
+          }
+        }
+      }
+      if ((fl & fe) && d2 < A.rmaxl_coul2) {
+        double p = (d2 - A.rmins_coul2) * A.dx2coul;
+        int kk = (int)p;
+        if (kk < 1) { acc[E_CLAMP] += 1.; kk = 1; p = 1; }
+        p = p - (double)kk;
+        double qij = ri.w * rj.w;
+        double fcc = qij * nb_interp3(ct, kk, 0, p);
+        double gx = dx * fcc, gy = dy * fcc, gz = dz * fcc;
+        if (IFLG) {
+          acc[E_SPRSE] -= d2 * fcc;
+          acc[E_SE0] -= dx * dx * fcc;
+          acc[E_SE1] -= dy * dx * fcc;
+          acc[E_SE2] -= dz * dx * fcc;
+          acc[E_SE4] -= dy * dy * fcc;
+          acc[E_SE5] -= dz * dy * fcc;
+          acc[E_SE8] -= dz * dz * fcc;
+          fx -= gx; fy -= gy; fz -= gz;
+          double v = nb_interp3(ct, kk, 2, p);
+          acc[E_SPOTHE] += qij * (v - A.clast);
+          acc[E_SPOTE] += qij * v;
+        }
+        if (WANT_FS || !IFLG) { sx -= gx; sy -= gy; sz -= gz; }
+      }
+    }
+    const int ai = A.perm[s];
+    if (IFLG) { fx = nb_wave_sum(fx); fy = nb_wave_sum(fy); fz = nb_wave_sum(fz); }
+    if (WANT_FS || !IFLG) { sx = nb_wave_sum(sx); sy = nb_wave_sum(sy); sz = nb_wave_sum(sz); }
+    if (lane == 0) {
+      const int n = A.n;
+      if (IFLG) { A.fl[ai] = fx; A.fl[n + ai] = fy; A.fl[2 * n + ai] = fz; }
+      if (WANT_FS || !IFLG) { A.fs[ai] = sx; A.fs[n + ai] = sy; A.fs[2 * n + ai] = sz; }
+    }
+  }
+  block_reduce_store<NB_NE_PAIR>(acc, A.epart + (size_t)blockIdx.x * NB_NE_PAIR);
+}
+
+// ecorr (md_ewald.c:385-453): one thread per atom over its excluded charged
+// partners (both directions, so energies and virials are halved later), on
+// the caller's unwrapped positions with no minimum image.  Adds to fl.
+__global__ __launch_bounds__(256) void k_ecorr(const int *__restrict__ eoff,
+                                               const int *__restrict__ elist,
+                                               const double *__restrict__ pos,
+                                               const double *__restrict__ q, int n,
+                                               double kappa, double *__restrict__ fl,
+                                               double *__restrict__ epart) {
+  double acc[NB_NE_ECORR];
+#pragma unroll
+  for (int k = 0; k < NB_NE_ECORR; k++) acc[k] = 0.;
+  int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a < n) {
+    const double spi = 1. / sqrt(M_PI), ka2 = kappa * kappa;
+    double fx = 0., fy = 0., fz = 0.;
+    for (int e = eoff[a]; e < eoff[a + 1]; e++) {
+      int b = elist[e];
+      double dx = pos[a] - pos[b], dy = pos[n + a] - pos[n + b], dz = pos[2 * n + a] - pos[2 * n + b];
+      double d2 = dx * dx + dy * dy + dz * dz, d = sqrt(d2);
+      double qr = q[a] * q[b] / d, er = erf(d * kappa);
+      acc[0] += -qr * er;
+      double dv = -qr * (-2. * kappa * exp(-ka2 * d2) * spi + er / d);
+      acc[1] += d * dv;
+      dv /= d;
+      acc[2] += dv * dx * dx; acc[3] += dv * dx * dy; acc[4] += dv * dx * dz;
+      acc[5] += dv * dy * dx; acc[6] += dv * dy * dy; acc[7] += dv * dy * dz;
+      acc[8] += dv * dz * dx; acc[9] += dv * dz * dy; acc[10] += dv * dz * dz;
+      fx += dx * dv; fy += dy * dv; fz += dz * dv;
+    }
+    if (eoff[a + 1] > eoff[a]) { fl[a] += fx; fl[n + a] += fy; fl[2 * n + a] += fz; }
+  }
+  block_reduce_store<NB_NE_ECORR>(acc, epart + (size_t)blockIdx.x * NB_NE_ECORR);
+}
+
+// ---------------------------------------------------------------------------
+// Reciprocal-space Ewald, FP64 throughout.
+// k_ew_charges: per charge, fractional coordinates s = h^-1 r wrapped to
+//   [-1/2,1/2) (exact for e^{2 pi i k.s} with integer k), stored as 2 pi s,
+//   the charge times sqrt(4 pi) (md_ewald.c:220-224), and e^{2 pi i s_c}, the
+//   c-axis step of cm3d's recursion (md_ewald.c:227-232, 273-278).
+// k_ew_sfac: one workgroup per (segment, charge chunk); each thread starts a
+//   segment with one sincos and walks its <= 16 k vectors by the recursion,
+//   accumulating q e^{i k.r} over its charges; a block reduction writes the
+//   chunk's partial S(k).
+// k_ew_kreduce: per k, sum the chunk partials in fixed order, form preg,
+//   prep, |S|^2 and the energy / virial terms (md_ewald.c:247-302).
+// k_ew_force: per charge, walk all segments (staged in LDS) and add
+//   q k (Re S' sin - Im S' cos) to fl (md_ewald.c:303-323).
+// ---------------------------------------------------------------------------
+__global__ void k_ew_charges(const double *__restrict__ pos, const double *__restrict__ q,
+                             const int *__restrict__ qidx, int nq, int n, Box b,
+                             double4 *__restrict__ chg, double2 *__restrict__ stp) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nq) return;
+  int a = qidx[c];
+  double x = pos[a], y = pos[n + a], z = pos[2 * n + a];
+  const double *hi = b.hi;
+  double sa = x * hi[0] + y * hi[1] + z * hi[2];
+  double sb = x * hi[3] + y * hi[4] + z * hi[5];
+  double sc = x * hi[6] + y * hi[7] + z * hi[8];
+  sa -= rint(sa);
+  sb -= rint(sb);
+  sc -= rint(sc);
+  const double tpi = 2. * M_PI;
+  chg[c] = make_double4(tpi * sa, tpi * sb, tpi * sc, q[a] * sqrt(4.0 * M_PI));
+  double sn, cs;
+  sincos(tpi * sc, &sn, &cs);
+  stp[c] = make_double2(cs, sn);
+}
+
+__global__ __launch_bounds__(256) void k_ew_sfac(const EwSeg *__restrict__ seg,
+                                                 const double4 *__restrict__ chg,
+                                                 const double2 *__restrict__ stp,
+                                                 int nq, int chunk, int ktot,
+                                                 double2 *__restrict__ part) {
+  const EwSeg sg = seg[blockIdx.x];
+  const int c0 = blockIdx.y * chunk;
+  const int c1 = min(nq, c0 + chunk);
+  double re[NB_EW_SEG], im[NB_EW_SEG];
+#pragma unroll
+  for (int m = 0; m < NB_EW_SEG; m++) { re[m] = 0.; im[m] = 0.; }
+  for (int c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
+    double4 r = chg[c];
+    double2 st = stp[c];
+    double th = sg.ka * r.x + sg.kb * r.y + sg.kc0 * r.z;
+    double sn, cs;
+    sincos(th, &sn, &cs);
+#pragma unroll
+    for (int m = 0; m < NB_EW_SEG; m++) {
+      if (m < sg.cnt) {
+        re[m] += r.w * cs;
+        im[m] += r.w * sn;
+        double c2 = cs * st.x - sn * st.y;
+        sn = sn * st.x + cs * st.y;
+        cs = c2;
+      }
+    }
+  }
+  __shared__ double red[4][2 * NB_EW_SEG];
+  const int w = threadIdx.x >> 6, lane = lane_id();
+#pragma unroll
+  for (int m = 0; m < NB_EW_SEG; m++) {
+    double a = nb_wave_sum(re[m]), bb = nb_wave_sum(im[m]);
+    if (lane == 0) { red[w][2 * m] = a; red[w][2 * m + 1] = bb; }
+  }
+  __syncthreads();
+  if (threadIdx.x < sg.cnt) {
+    int m = threadIdx.x;
+    double a = 0., bb = 0.;
+    for (int k = 0; k < (int)(blockDim.x >> 6); k++) { a += red[k][2 * m]; bb += red[k][2 * m + 1]; }
+    part[(size_t)blockIdx.y * ktot + sg.k0 + m] = make_double2(a, bb);
+  }
+}
+
+struct KData { double kx, ky, kz, cr, ci; };
+
+__global__ __launch_bounds__(256) void k_ew_kreduce(const int *__restrict__ kabc, int ktot,
+                                                    const double2 *__restrict__ part,
+                                                    int nchunk, Box b, double rvol,
+                                                    double falp2, KData *__restrict__ kd,
+                                                    double *__restrict__ epart) {
+  double acc[NB_NE_EW];
+#pragma unroll
+  for (int k = 0; k < NB_NE_EW; k++) acc[k] = 0.;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < ktot) {
+    double sr = 0., si = 0.;
+    for (int c = 0; c < nchunk; c++) {
+      double2 p = part[(size_t)c * ktot + i];
+      sr += p.x;
+      si += p.y;
+    }
+    const double tpi = 2. * M_PI, *hi = b.hi;
+    double six = tpi * kabc[3 * i], siy = tpi * kabc[3 * i + 1], siz = tpi * kabc[3 * i + 2];
+    double aka = six * hi[0] + siy * hi[3] + siz * hi[6];
+    double akb = six * hi[1] + siy * hi[4] + siz * hi[7];
+    double akc = six * hi[2] + siy * hi[5] + siz * hi[8];
+    double g2 = aka * aka + akb * akb + akc * akc;
+    double preg = exp(-g2 * falp2) * rvol / g2;
+    double prep = -2. * preg * (g2 * falp2 + 1.) / g2;
+    double sm = sr * sr + si * si;
+    acc[0] = preg * sm;
+    acc[1] = prep * sm * g2;
+    acc[2] = prep * sm * aka * aka; acc[3] = prep * sm * akb * aka; acc[4] = prep * sm * akc * aka;
+    acc[5] = prep * sm * aka * akb; acc[6] = prep * sm * akb * akb; acc[7] = prep * sm * akc * akb;
+    acc[8] = prep * sm * aka * akc; acc[9] = prep * sm * akb * akc; acc[10] = prep * sm * akc * akc;
+    KData d;
+    d.kx = aka; d.ky = akb; d.kz = akc;
+    d.cr = sr * 2. * preg;
+    d.ci = si * 2. * preg;
+    kd[i] = d;
+  }
+  block_reduce_store<NB_NE_EW>(acc, epart + (size_t)blockIdx.x * NB_NE_EW);
+}
+
+#define NB_EW_TILE 64   // segments per LDS tile in k_ew_force
+__global__ __launch_bounds__(256) void k_ew_force(const EwSeg *__restrict__ seg, int nseg,
+                                                  const KData *__restrict__ kd,
+                                                  const double4 *__restrict__ chg,
+                                                  const double2 *__restrict__ stp,
+                                                  const int *__restrict__ qidx, int nq,
+                                                  int n, double *__restrict__ fl) {
+  __shared__ EwSeg ss[NB_EW_TILE];
+  __shared__ KData sk[NB_EW_TILE * NB_EW_SEG];
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  double4 r = make_double4(0., 0., 0., 0.);
+  double2 st = make_double2(1., 0.);
+  if (c < nq) { r = chg[c]; st = stp[c]; }
+  double fx = 0., fy = 0., fz = 0.;
+  for (int t0 = 0; t0 < nseg; t0 += NB_EW_TILE) {
+    int nt = min(NB_EW_TILE, nseg - t0);
+    __syncthreads();
+    for (int t = threadIdx.x; t < nt; t += blockDim.x) ss[t] = seg[t0 + t];
+    __syncthreads();
+    int kbase = ss[0].k0, kend = ss[nt - 1].k0 + ss[nt - 1].cnt;
+    for (int t = threadIdx.x; t < kend - kbase; t += blockDim.x) sk[t] = kd[kbase + t];
+    __syncthreads();
+    for (int g = 0; g < nt; g++) {
+      const EwSeg sg = ss[g];
+      double th = sg.ka * r.x + sg.kb * r.y + sg.kc0 * r.z;
+      double sn, cs;
+      sincos(th, &sn, &cs);
+      const KData *K = sk + (sg.k0 - kbase);
+      for (int m = 0; m < sg.cnt; m++) {
+        double tt = K[m].cr * sn - K[m].ci * cs;
+        fx += K[m].kx * tt;
+        fy += K[m].ky * tt;
+        fz += K[m].kz * tt;
+        double c2 = cs * st.x - sn * st.y;
+        sn = sn * st.x + cs * st.y;
+        cs = c2;
+      }
+    }
+  }
+  if (c < nq) {
+    int a = qidx[c];
+    fl[a] += r.w * fx;
+    fl[n + a] += r.w * fy;
+    fl[2 * n + a] += r.w * fz;
+  }
+}
+
+// Column sums of a [rows][COLS] partial table in one 256-thread block, with
+// a fixed assignment of rows to threads and a fixed tree (deterministic).
+template <int COLS>
+__global__ __launch_bounds__(256) void k_reduce_rows(const double *__restrict__ part,
+                                                     int rows, double *__restrict__ out) {
+  double acc[COLS];
+#pragma unroll
+  for (int k = 0; k < COLS; k++) acc[k] = 0.;
+  for (int r = threadIdx.x; r < rows; r += blockDim.x) {
+#pragma unroll
+    for (int k = 0; k < COLS; k++) acc[k] += part[(size_t)r * COLS + k];
+  }
+  block_reduce_store<COLS>(acc, out);
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+#define LCHK(x)                                                        \
+  do {                                                                 \
+    hipError_t _e = (x);                                               \
+    if (_e != hipSuccess) {                                            \
+      nb_set_error("%s: %s", #x, hipGetErrorString(_e));               \
+      return NB_ERR_HIP;                                               \
+    }                                                                  \
+  } while (0)
+
+static int blocks(int n, int t) { return (n + t - 1) / t; }
+
+int nb_launch_build(nb_ctx *c, const Box &b, hipStream_t s) {
+  const int n = c->natoms;
+  LCHK(hipMemsetAsync(c->d_ccount, 0, sizeof(int) * c->ncells, s));
+  LCHK(hipMemsetAsync(c->d_flags, 0, sizeof(int) * 8, s));
+  k_cell_assign<<<blocks(n, 256), 256, 0, s>>>(c->d_pos, n, b, c->ncx, c->ncy, c->ncz,
+                                                c->d_cell, c->d_ccount);
+  k_cell_scan<<<1, 1024, 0, s>>>(c->d_ccount, c->ncells, c->d_cstart, c->d_cfill);
+  k_cell_scatter<<<blocks(n, 256), 256, 0, s>>>(c->d_cell, n, c->d_cfill, c->d_tmp);
+  // scell reuses d_cell: the sorted cell id of each sorted slot
+  k_cell_order<<<c->ncells, 256, 0, s>>>(c->d_ccount, c->d_cstart, c->d_tmp, c->d_perm,
+                                         c->d_iperm, c->d_cell);
+  k_pack64<<<blocks(n, 256), 256, 0, s>>>(c->d_pos, c->d_qch, c->d_itype, c->d_perm, n,
+                                          (double4 *)c->d_rec64, c->d_stype);
+  ListArgs A;
+  A.n = n; A.ntypes = c->ntypes; A.cap = c->cap;
+  A.ncx = c->ncx; A.ncy = c->ncy; A.ncz = c->ncz;
+  A.rec = (const double4 *)c->d_rec64; A.stype = c->d_stype; A.perm = c->d_perm;
+  A.scell = c->d_cell; A.cstart = c->d_cstart; A.ccount = c->d_ccount;
+  A.xoff = c->d_xoff; A.xlist = c->d_xlist; A.map = c->d_map; A.ip = c->d_ip;
+  A.rmaxs_skin2_e = c->rmaxs_skin2_e; A.rmaxl_skin2_e = c->rmaxl_skin2_e;
+  A.b = b; A.nbr = c->d_nbr; A.nnb = c->d_nnb; A.flags = c->d_flags;
+  k_build_list<<<blocks(n, 4), 256, 0, s>>>(A);
+  LCHK(hipGetLastError());
+  LCHK(hipMemcpyAsync(c->d_posb, c->d_pos, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, s));
+  return NB_OK;
+}
+
+int nb_launch_rebuild_check(nb_ctx *c, const double *d_pos, hipStream_t s) {
+  LCHK(hipMemsetAsync(c->d_flags + 1, 0, sizeof(int), s));
+  k_rebuild_check<<<blocks(c->natoms, 256), 256, 0, s>>>(d_pos, c->d_posb, c->natoms,
+                                                         .25 * c->skin * c->skin, c->d_flags + 1);
+  LCHK(hipGetLastError());
+  return NB_OK;
+}
+
+int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_fl,
+                    hipStream_t s) {
+  const int n = c->natoms;
+  k_pack64<<<blocks(n, 256), 256, 0, s>>>(c->d_pos, c->d_qch, c->d_itype, c->d_perm, n,
+                                          (double4 *)c->d_rec64, nullptr);
+  PairArgs P;
+  P.n = n; P.ntypes = c->ntypes; P.ninter = c->ninter; P.ntable = c->ntable;
+  P.cap = c->cap; P.tab_lds = c->tab_lds_bytes > 0; P.ctab_off = c->ctab_off;
+  P.nbr = c->d_nbr; P.nnb = c->d_nnb; P.stype = c->d_stype; P.perm = c->d_perm;
+  P.map = c->d_map; P.rec = (const double4 *)c->d_rec64; P.ip = c->d_ip;
+  P.vtab3 = c->d_vtab3; P.ctab3 = c->d_ctab3;
+  P.rmins_coul2 = c->rmins_coul2; P.dx2coul = c->dx2coul; P.rmaxl_coul2 = c->rmaxl_coul2;
+  P.clast = c->clast;
+  P.b = b; P.fl = d_fl; P.fs = d_fs; P.epart = c->d_epair;
+  const int g = c->npair_blocks, lds = c->tab_lds_bytes;
+  if (iflg == 0)
+    k_pair64<0, true><<<g, 256, lds, s>>>(P);
+  else if (d_fs)
+    k_pair64<1, true><<<g, 256, lds, s>>>(P);
+  else
+    k_pair64<1, false><<<g, 256, lds, s>>>(P);
+  LCHK(hipGetLastError());
+  k_reduce_rows<NB_NE_PAIR><<<1, 256, 0, s>>>(c->d_epair, g, c->d_eout);
+  if (iflg == 1 && c->iperd == 3 && c->necorr > 0) {
+    k_ecorr<<<c->necorr_blocks, 256, 0, s>>>(c->d_eoff, c->d_elist, c->d_pos, c->d_qch, n,
+                                             c->kappa, d_fl, c->d_eec);
+    k_reduce_rows<NB_NE_ECORR><<<1, 256, 0, s>>>(c->d_eec, c->necorr_blocks, c->d_eout + 20);
+  }
+  if (iflg == 1 && c->iperd == 3 && c->ktot > 0 && c->ncharge > 0) {
+    const int nq = c->ncharge;
+    k_ew_charges<<<blocks(nq, 256), 256, 0, s>>>(c->d_pos, c->d_qch, c->d_qidx, nq, n, b,
+                                                 (double4 *)c->d_chg, (double2 *)c->d_stp);
+    int chunk = (nq + c->nchunk - 1) / c->nchunk;
+    dim3 gs(c->nseg, c->nchunk);
+    k_ew_sfac<<<gs, 256, 0, s>>>(c->d_seg, (const double4 *)c->d_chg, (const double2 *)c->d_stp,
+                                 nq, chunk, c->ktot, (double2 *)c->d_spart);
+    double det = b.h[0] * (b.h[4] * b.h[8] - b.h[5] * b.h[7]) +
+                 b.h[1] * (b.h[5] * b.h[6] - b.h[3] * b.h[8]) +
+                 b.h[2] * (b.h[3] * b.h[7] - b.h[4] * b.h[6]);
+    double falp2 = 1. / (4. * c->kappa * c->kappa);
+    k_ew_kreduce<<<c->new_blocks, 256, 0, s>>>(c->d_kabc, c->ktot, (const double2 *)c->d_spart,
+                                               c->nchunk, b, 1. / det, falp2,
+                                               (KData *)c->d_kd, c->d_eew);
+    k_reduce_rows<NB_NE_EW><<<1, 256, 0, s>>>(c->d_eew, c->new_blocks, c->d_eout + 32);
+    k_ew_force<<<blocks(nq, 256), 256, 0, s>>>(c->d_seg, c->nseg, (const KData *)c->d_kd,
+                                               (const double4 *)c->d_chg, (const double2 *)c->d_stp,
+                                               c->d_qidx, nq, n, d_fl);
+    LCHK(hipGetLastError());
+  }
+  return NB_OK;
+}
+
+size_t nb_kdata_bytes() { return sizeof(KData); }

@@ -1,0 +1,289 @@
+"""ctypes binding of libcm3d_nb.so (the C ABI in include/cm3d_nb.h).
+
+This is the host-side mirror of cm3d's inter-force interface for Python
+callers (tests, bench, the serving loop): ``NBContext`` wraps one device
+context and exposes the cm3d entry points it replaces --
+
+    build_lists   get_ngbr + get_ngbr_e  (energy/control/ngbr_list.c:297)
+    need_rebuild  nrefsh's criterion     (energy/control/md_finter.c:43-73)
+    force         force(...,iflg) minus rmass (energy/control/md_fcntr.c:44)
+
+A *system* is a dict of numpy arrays keyed like cm3d's INTER / COUL /
+SIMPARMS fields (``natoms``, ``itype``, ``map``, ``vtab`` ...), as built by
+:mod:`cm3d_amd.systems`.  The library must be built (``python -m
+cm3d_amd.build``); there is no fallback: a missing library raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcm3d_nb.so")
+
+NB_OK = 0
+NB_PREC_FP64 = 0
+NB_PREC_MIXED = 1
+PRECISIONS = {"fp64": NB_PREC_FP64, "mixed": NB_PREC_MIXED}
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class NBParams(C.Structure):
+    _fields_ = [
+        ("natoms", C.c_int), ("ntypes", C.c_int), ("ninter", C.c_int),
+        ("ntable", C.c_int), ("iperd", C.c_int),
+        ("itype", _ip), ("map", _ip),
+        ("vtab", _dp), ("dvtab", _dp), ("switchs", _dp),
+        ("rmins_cut2", _dp), ("rmaxs_cut2", _dp), ("rminl_cut2", _dp),
+        ("rmaxl_cut2", _dp), ("dx2tab", _dp), ("rmaxs_skin2", _dp),
+        ("rmaxl_skin2", _dp), ("skin", C.c_double),
+        ("nexclude", _ip), ("exclude", _ip),
+        ("qch", _dp),
+        ("coultab", _dp), ("dcoultab", _dp), ("switchse", _dp),
+        ("dx2coul", C.c_double), ("rmins_coul2", C.c_double),
+        ("rmaxs_coul2", C.c_double), ("rminl_coul2", C.c_double),
+        ("rmaxl_coul2", C.c_double), ("rmaxs_skin2_e", C.c_double),
+        ("rmaxl_skin2_e", C.c_double),
+        ("cnexclude", _ip), ("cexclude", _ip),
+        ("necorr", C.c_int), ("iecorr", _ip), ("jecorr", _ip),
+        ("ktot", C.c_int), ("ka", _ip), ("kb", _ip), ("kc", _ip),
+        ("kappa", C.c_double), ("ewald_self", C.c_double),
+        ("ewald_bkg", C.c_double),
+        ("precision", C.c_int), ("device", C.c_int),
+    ]
+
+
+class NBEnergy(C.Structure):
+    _fields_ = [("spotl", C.c_double), ("spothl", C.c_double),
+                ("sprsl", C.c_double), ("sprslt", C.c_double * 9),
+                ("spote", C.c_double), ("spothe", C.c_double),
+                ("sprse", C.c_double), ("sprset", C.c_double * 9),
+                ("vk_ewald", C.c_double), ("vp_ewald", C.c_double),
+                ("vp_ewald_tensor", C.c_double * 9),
+                ("pot_ecorr", C.c_double), ("prs_ecorr", C.c_double),
+                ("prs_ecorr_tensor", C.c_double * 9), ("nclamp", C.c_int)]
+
+    def as_dict(self):
+        out = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            out[name] = np.array(v[:]) if hasattr(v, "__len__") else v
+        return out
+
+
+# every symbol include/cm3d_nb.h declares
+EXPORTS = ["nb_create", "nb_destroy", "nb_last_error", "nb_build_lists",
+           "nb_need_rebuild", "nb_force", "nb_force_device",
+           "nb_build_lists_device", "nb_list_counts", "nb_get_pairs",
+           "nb_table_pair", "nb_table_coul", "nb_kvectors", "nb_ewald_consts"]
+
+_LIB = None
+
+
+class NBError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libcm3d_nb.so; raises if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise NBError("libcm3d_nb.so is not built (python -m cm3d_amd.build)")
+    L = C.CDLL(LIB_PATH)
+    L.nb_create.argtypes = [C.POINTER(NBParams), C.POINTER(C.c_void_p)]
+    L.nb_destroy.argtypes = [C.c_void_p]
+    L.nb_destroy.restype = None
+    L.nb_last_error.restype = C.c_char_p
+    L.nb_build_lists.argtypes = [C.c_void_p, _dp, _dp]
+    L.nb_need_rebuild.argtypes = [C.c_void_p, _dp, _ip]
+    L.nb_force.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp,
+                           C.POINTER(NBEnergy)]
+    L.nb_force_device.argtypes = [C.c_void_p, C.c_int, C.c_void_p, _dp,
+                                  C.c_void_p, C.c_void_p, C.POINTER(NBEnergy),
+                                  C.c_void_p]
+    L.nb_build_lists_device.argtypes = [C.c_void_p, C.c_void_p, _dp, C.c_void_p]
+    L.nb_list_counts.argtypes = [C.c_void_p, _ip]
+    L.nb_get_pairs.argtypes = [C.c_void_p, C.c_int, _ip, _ip, C.c_int, _ip]
+    L.nb_table_pair.argtypes = [C.c_int, _dp, C.c_double, C.c_double,
+                                C.c_double, C.c_double, C.c_double, C.c_double,
+                                C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+    L.nb_table_coul.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double,
+                                C.c_double, C.c_double, C.c_double, C.c_double,
+                                C.c_int, _dp, _dp, _dp, _dp, _dp]
+    L.nb_kvectors.argtypes = [C.c_int, _ip, _ip, _ip]
+    L.nb_ewald_consts.argtypes = [C.c_int, _dp, C.c_double, _dp, _dp]
+    _LIB = L
+    return L
+
+
+def check(rc, what="nb call"):
+    if rc != NB_OK:
+        msg = lib().nb_last_error().decode(errors="replace")
+        raise NBError(f"{what} failed ({rc}): {msg}")
+
+
+def _d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(_ip)
+
+
+def _scalar(sysd, key, default=0):
+    v = sysd.get(key, default)
+    return np.asarray(v).reshape(-1)[0].item() if np.size(v) else default
+
+
+def make_params(sysd, precision="fp64", device=0):
+    """NBParams from a system dict; returns (params, keep-alive list)."""
+    p = NBParams()
+    keep = []
+    for name, typ in NBParams._fields_:
+        if name in ("precision", "device"):
+            continue
+        if typ is C.c_int:
+            setattr(p, name, int(_scalar(sysd, name, 0)))
+        elif typ is C.c_double:
+            setattr(p, name, float(_scalar(sysd, name, 0.0)))
+        elif name in sysd and np.size(sysd[name]) > 0:
+            arr, ptr = (_d if typ is _dp else _i)(sysd[name])
+            keep.append(arr)
+            setattr(p, name, ptr)
+    p.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    p.device = int(device)
+    return p, keep
+
+
+class NBContext:
+    """One device context of the nonbonded engine (a cm3d system)."""
+
+    def __init__(self, sysd, precision="fp64", device=0):
+        self.natoms = int(_scalar(sysd, "natoms"))
+        params, keep = make_params(sysd, precision, device)
+        h = C.c_void_p()
+        check(lib().nb_create(C.byref(params), C.byref(h)), "nb_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().nb_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def build_lists(self, pos, hmat):
+        p, pp = _d(pos)
+        h, hp = _d(hmat)
+        check(lib().nb_build_lists(self._h, pp, hp), "nb_build_lists")
+
+    def need_rebuild(self, pos):
+        p, pp = _d(pos)
+        f = C.c_int(0)
+        check(lib().nb_need_rebuild(self._h, pp, C.byref(f)), "nb_need_rebuild")
+        return bool(f.value)
+
+    def force(self, pos, hmat, iflg=1):
+        """(fs, fl, energy dict) -- cm3d's force(...,iflg) before rmass."""
+        p, pp = _d(pos)
+        h, hp = _d(hmat)
+        n3 = 3 * self.natoms
+        fs = np.zeros(n3)
+        fl = np.zeros(n3)
+        e = NBEnergy()
+        check(lib().nb_force(self._h, int(iflg), pp, hp, fs.ctypes.data_as(_dp),
+                             fl.ctypes.data_as(_dp), C.byref(e)), "nb_force")
+        return fs, fl, e.as_dict()
+
+    def force_device(self, iflg, d_pos, hmat, d_fs, d_fl, stream=None, energy=False):
+        """Device-pointer variant (ints from torch.Tensor.data_ptr())."""
+        h, hp = _d(hmat)
+        e = NBEnergy() if energy else None
+        check(lib().nb_force_device(self._h, int(iflg), C.c_void_p(d_pos), hp,
+                                    C.c_void_p(d_fs) if d_fs else None,
+                                    C.c_void_p(d_fl) if d_fl else None,
+                                    C.byref(e) if e is not None else None,
+                                    C.c_void_p(stream) if stream else None),
+              "nb_force_device")
+        return e.as_dict() if e is not None else None
+
+    def build_lists_device(self, d_pos, hmat, stream=None):
+        h, hp = _d(hmat)
+        check(lib().nb_build_lists_device(self._h, C.c_void_p(d_pos), hp,
+                                          C.c_void_p(stream) if stream else None),
+              "nb_build_lists_device")
+
+    def list_counts(self):
+        """Half-pair counts as cm3d reports them: nprs, nprt, nprse, nprte."""
+        c = (C.c_int * 4)()
+        check(lib().nb_list_counts(self._h, c), "nb_list_counts")
+        return dict(nprs=c[0], nprt=c[1], nprse=c[2], nprte=c[3])
+
+    def get_pairs(self, which):
+        """Sorted (i<j) pair arrays of list 's', 't', 'se' or 'te'."""
+        w = {"s": 0, "t": 1, "se": 2, "te": 3}[which]
+        n = C.c_int(0)
+        lib().nb_get_pairs(self._h, w, None, None, 0, C.byref(n))
+        ip = np.zeros(max(n.value, 1), np.int32)
+        jp = np.zeros(max(n.value, 1), np.int32)
+        check(lib().nb_get_pairs(self._h, w, ip.ctypes.data_as(_ip),
+                                 jp.ctypes.data_as(_ip), len(ip), C.byref(n)),
+              "nb_get_pairs")
+        return ip[:n.value].copy(), jp[:n.value].copy()
+
+
+# ---------------------------------------------------------------- set-up
+POT_LJ, POT_WILLIAMS = 0, 1
+
+
+def table_pair(pot, prm, rmin, rmax, rres, rheal, skin, ntable, scaleeps=1.0):
+    """get_ter_parm's table for one interaction (host)."""
+    prm_a, pp = _d(prm)
+    v = np.zeros(ntable); dv = np.zeros(ntable); sw = np.zeros(ntable)
+    cut = np.zeros(7); dx2 = np.zeros(1); vl = np.zeros(1); wl = np.zeros(1)
+    check(lib().nb_table_pair(int(pot), pp, rmin, rmax, rres, rheal, skin, scaleeps,
+                              int(ntable), *[a.ctypes.data_as(_dp) for a in
+                                             (v, dv, sw, cut, dx2, vl, wl)]),
+          "nb_table_pair")
+    return dict(vtab=v, dvtab=dv, switchs=sw, cut2=cut, dx2tab=dx2[0],
+                vlrc=vl[0], wlrc=wl[0])
+
+
+def table_coul(ntable, iperd, alpha, rmin, rresp, rmax, rheal, skin, ishift=0):
+    """set_charges' erfc(a r)/r (or 1/r) table (host)."""
+    t = np.zeros(ntable); dt = np.zeros(ntable); sw = np.zeros(ntable)
+    cut = np.zeros(6); dx2 = np.zeros(1)
+    check(lib().nb_table_coul(int(ntable), int(iperd), alpha, rmin, rresp, rmax,
+                              rheal, skin, int(ishift),
+                              *[a.ctypes.data_as(_dp) for a in (t, dt, sw, cut, dx2)]),
+          "nb_table_coul")
+    return dict(coultab=t, dcoultab=dt, switchse=sw, cut2=cut, dx2coul=dx2[0])
+
+
+def kvectors(kmax):
+    n = lib().nb_kvectors(int(kmax), None, None, None)
+    ka = np.zeros(n, np.int32); kb = np.zeros(n, np.int32); kc = np.zeros(n, np.int32)
+    lib().nb_kvectors(int(kmax), *[a.ctypes.data_as(_ip) for a in (ka, kb, kc)])
+    return ka, kb, kc
+
+
+def ewald_consts(qch, alpha):
+    q, qp = _d(qch)
+    s = np.zeros(1); b = np.zeros(1)
+    check(lib().nb_ewald_consts(len(q), qp, alpha, s.ctypes.data_as(_dp),
+                                b.ctypes.data_as(_dp)), "nb_ewald_consts")
+    return s[0], b[0]

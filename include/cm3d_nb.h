@@ -1,0 +1,157 @@
+/*
+ * cm3d_nb.h -- C ABI of the MI355X (gfx950) nonbonded force engine.
+ *
+ * Drop-in for cm3d's intermolecular force path.  Every array argument has
+ * cm3d's own layout: positions and forces are 3N SoA doubles (x[N] y[N] z[N],
+ * the px|py|pz and fxl|fyl|fzl blocks of utils/md_util.new.c:113-142),
+ * tables are [ninter][ntable] doubles uniform in r^2, the type map is the
+ * row-major ntypes x ntypes int matrix with -1 for null pairs.  Units are
+ * cm3d's: Angstrom, Kelvin, charges in sqrt(K*A).  Forces are returned
+ * *before* cm3d's rmass division (energy/control/md_fcntr.c:74-75).
+ *
+ * Entry points and the cm3d routine each replaces (reference tree paths):
+ *   nb_create        search_ter_base + set_charges + set_exclude outputs
+ *                    (input/md_searchbase.c:34, energy/control/elec_ngbr.c:36,
+ *                     energy/control/md_setexclude.c:38) -> device context
+ *   nb_build_lists   get_ngbr + get_ngbr_e   (energy/control/ngbr_list.c:297,
+ *                                             energy/control/elec_ngbr.c:220)
+ *   nb_need_rebuild  nrefsh's displacement criterion
+ *                                            (energy/control/md_finter.c:43-73)
+ *   nb_force         force(...,iflg) for iflg 0/1 minus rmass: force_ter +
+ *                    force_ter_e (+ fk_ewald + ecorr when iflg==1, iperd==3)
+ *                    (energy/control/md_fcntr.c:44-76)
+ *   nb_energy fields the statics read by getvireal(...,1): getsvirter,
+ *                    getsvirter_e, getvewald (energy/inter/md_fnopol.c:386,
+ *                    energy/inter/finter_e.c:410, energy/inter/md_ewald.c:364)
+ *   nb_table_pair / nb_table_coul / nb_kvectors / nb_ewald_consts
+ *                    set-up helpers restating get_ter_parm
+ *                    (input/md_getparm.c:41), set_charges
+ *                    (energy/control/elec_ngbr.c:36) and ewald_setup
+ *                    (energy/inter/md_ewald.c:48) for callers without
+ *                    cm3d's input layer.
+ *
+ * Errors: every entry point returns NB_OK (0) or a negative NB_ERR_* code
+ * and never exits; nb_last_error() gives the message.  cm3d's md_error maps
+ * to a negative return, md_warning (a pair below the table minimum, which
+ * cm3d clamps, md_fnopol.c:133-138) to nb_energy.nclamp > 0.
+ *
+ * Threading: one context is used by one host thread at a time (cm3d's own
+ * path is non-reentrant through file statics; here all that state lives in
+ * the context, so distinct contexts are independent).
+ */
+#ifndef CM3D_NB_H
+#define CM3D_NB_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NB_OK 0
+#define NB_ERR_ARG (-1)
+#define NB_ERR_HIP (-2)
+#define NB_ERR_CAPACITY (-3)
+#define NB_ERR_NODEVICE (-4)
+#define NB_ERR_UNSUPPORTED (-5)
+
+/* Arithmetic of the pair kernels.  NB_PREC_FP64 follows cm3d's FP64
+ * arithmetic pair by pair; NB_PREC_MIXED evaluates pairs in FP32 from
+ * 32-bit fixed-point fractional coordinates and accumulates forces in 64-bit
+ * fixed point.  Ewald sums, energies and virials are FP64 in both. */
+#define NB_PREC_FP64 0
+#define NB_PREC_MIXED 1
+
+typedef struct nb_ctx nb_ctx;
+
+/* Everything cm3d's set-up leaves in INTER / COUL / SIMPARMS for the path
+ * (include/md_typedef.h:59-69,120-131,255-268).  Host pointers, read during
+ * nb_create only. */
+typedef struct nb_params {
+  int natoms, ntypes, ninter, ntable, iperd;
+  const int *itype;                  /* [natoms]                         */
+  const int *map;                    /* [ntypes*ntypes], -1 = null       */
+  const double *vtab, *dvtab, *switchs;   /* [ninter*ntable]             */
+  const double *rmins_cut2, *rmaxs_cut2, *rminl_cut2, *rmaxl_cut2;
+  const double *dx2tab, *rmaxs_skin2, *rmaxl_skin2;  /* [ninter]         */
+  double skin;                       /* inter->skin_ter                  */
+  const int *nexclude, *exclude;     /* vdW exclusions: per atom i the   */
+                                     /* sorted partners j>i, flattened   */
+  const double *qch;                 /* [natoms], sqrt(K A)              */
+  const double *coultab, *dcoultab, *switchse;       /* [ntable]         */
+  double dx2coul, rmins_coul2, rmaxs_coul2, rminl_coul2, rmaxl_coul2;
+  double rmaxs_skin2_e, rmaxl_skin2_e;
+  const int *cnexclude, *cexclude;   /* coulomb exclusions, same layout  */
+  int necorr;
+  const int *iecorr, *jecorr;        /* [necorr] excluded charged pairs  */
+  int ktot;                          /* half-space k vectors             */
+  const int *ka, *kb, *kc;           /* [ktot], cm3d enumeration order   */
+  double kappa, ewald_self, ewald_bkg;
+  int precision;                     /* NB_PREC_*                        */
+  int device;                        /* HIP device ordinal               */
+} nb_params;
+
+/* By-products of force(...,1) that getvireal(...,1) consumes. */
+typedef struct nb_energy {
+  double spotl, spothl, sprsl, sprslt[9];  /* getsvirter               */
+  double spote, spothe, sprse, sprset[9];  /* getsvirter_e             */
+  double vk_ewald, vp_ewald, vp_ewald_tensor[9];
+  double pot_ecorr, prs_ecorr, prs_ecorr_tensor[9];
+  int nclamp;                              /* pairs below table min    */
+} nb_energy;
+
+int nb_create(const nb_params *p, nb_ctx **ctx);
+void nb_destroy(nb_ctx *ctx);
+const char *nb_last_error(void);
+
+/* get_ngbr + get_ngbr_e on host positions; records the build positions. */
+int nb_build_lists(nb_ctx *ctx, const double *pxyz, const double hmat[9]);
+
+/* nrefsh criterion: *flag = 1 if any atom moved more than skin/2 since the
+ * last build. */
+int nb_need_rebuild(nb_ctx *ctx, const double *pxyz, int *flag);
+
+/* force(...,iflg) minus rmass.  iflg 1: fl gets the full intermolecular
+ * force (vdW + real-space coulomb + Ewald + ecorr), fs the RESPA-switched
+ * part; iflg 0: fs gets the short-range force, fl is untouched.  fs/fl are
+ * overwritten (cm3d zeroes them first, md_fcntr.c:55-63).  e may be NULL. */
+int nb_force(nb_ctx *ctx, int iflg, const double *pxyz, const double hmat[9],
+             double *fs, double *fl, nb_energy *e);
+
+/* Same on device-resident arrays (3N doubles in HBM) on the given HIP stream
+ * (NULL = the context's stream).  fs may be NULL to skip the switched force. */
+int nb_force_device(nb_ctx *ctx, int iflg, const double *d_pxyz,
+                    const double hmat[9], double *d_fs, double *d_fl,
+                    nb_energy *e, void *stream);
+int nb_build_lists_device(nb_ctx *ctx, const double *d_pxyz,
+                          const double hmat[9], void *stream);
+
+/* Pair counts of the current lists as cm3d counts them (half pairs):
+ * counts[0..3] = nprs, nprt, nprse, nprte. */
+int nb_list_counts(nb_ctx *ctx, int counts[4]);
+
+/* Copy one list as half pairs (i<j) in ascending (i,j) order.
+ * which: 0 = s (vdW short), 1 = t (vdW), 2 = se, 3 = te.  Fails with
+ * NB_ERR_CAPACITY (and sets *n) when cap is too small. */
+int nb_get_pairs(nb_ctx *ctx, int which, int *ip, int *jp, int cap, int *n);
+
+/* ---- set-up helpers (host only) ---- */
+#define NB_POT_LJ 0
+#define NB_POT_WILLIAMS 1
+/* prm: LJ {eps, sig}; Williams {awill, bwill, c6, c8, c10}.  cut2 out:
+ * rmins, rmaxs, rminl, rmaxl, rmaxs+skin, rmaxl+skin, rminl-skin squared. */
+int nb_table_pair(int pot, const double *prm, double rmin, double rmax,
+                  double rres, double rheal, double skin, double scaleeps,
+                  int ntable, double *vtab, double *dvtab, double *switchs,
+                  double cut2[7], double *dx2tab, double *vlrc, double *wlrc);
+/* cut2 out: rmins, rmaxs, rminl, rmaxl, rmaxs+skin, rmaxl+skin squared. */
+int nb_table_coul(int ntable, int iperd, double alpha, double rcute_min,
+                  double rcute_resp, double rcute_max, double rheal,
+                  double skin, int ishift, double *coultab, double *dcoultab,
+                  double *switchse, double cut2[6], double *dx2coul);
+/* Returns ktot; arrays may be NULL to count. */
+int nb_kvectors(int kmax, int *ka, int *kb, int *kc);
+int nb_ewald_consts(int natoms, const double *qch, double alpha,
+                    double *self, double *bkg);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

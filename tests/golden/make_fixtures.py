@@ -1,0 +1,126 @@
+"""Generate the golden fixtures under tests/golden/ with the compiled cm3d
+reference (oracle/_ref/cm3d_ref_dump, built by `make -C oracle ref`).
+
+Run in the build container only (it reads the reference tree):
+    python tests/golden/make_fixtures.py
+
+Each fixture is an .npz of one cm3d run: every input the nonbonded path reads
+(positions, box, tables, cutoffs, exclusions, charges, k-vectors) exactly as
+cm3d's own set-up built them, and every output the path produces (the
+Verlet lists, fxl/fxs before rmass, the getvireal by-products), plus a short
+trajectory of cm3d's own RESPA step.  Large lists are stored as their length
+and SHA-256 digests (ordered list, and sorted (min,max) pair set).
+Cases:
+  ref_lj2, ref_lj256   the reference's test/lj decks (2 and 256 LJ atoms)
+  ref_abutane          the reference's test/abutane deck (Williams, erfc
+                       real space, Ewald kmax 15, 64 ecorr pairs)
+  ar864                the 864-atom argon FCC workload
+  water512             512 flexible waters (rc 10, alpha 0.35, kmax 9, RESPA 4)
+  mix600               600-atom LJ + charge binary mixture (3 interactions)
+"""
+import hashlib
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, HERE)
+
+from cm3d_amd import systems as S  # noqa: E402
+from decks import write_deck  # noqa: E402
+from refdump import read_dump  # noqa: E402
+
+REF = "/root/reference"
+DUMP = os.path.join(ROOT, "oracle", "_ref", "cm3d_ref_dump")
+LISTS = ("inpt", "jnpt", "inps", "jnps", "inpte", "jnpte", "inpse", "jnpse")
+FULL_LIST_LIMIT = 120000
+
+
+def pair_digest(i, j):
+    a = np.minimum(i, j).astype(np.int64)
+    b = np.maximum(i, j).astype(np.int64)
+    key = np.sort(a * (1 << 32) + b)
+    return hashlib.sha256(key.tobytes()).hexdigest()
+
+
+def order_digest(i, j):
+    return hashlib.sha256(np.stack([i, j], 1).astype(np.int32).tobytes()).hexdigest()
+
+
+def run_ref(workdir, deck, nsteps):
+    out = os.path.join(workdir, "dump.bin")
+    r = subprocess.run([DUMP, deck, out, str(nsteps)], cwd=workdir,
+                       capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(r.stdout[-2000:] + r.stderr[-2000:])
+    return read_dump(out)
+
+
+def save(name, d, extra=None):
+    out = {}
+    n_list = sum(len(d.get(k, [])) for k in ("inpt", "inpte"))
+    for k, v in d.items():
+        if k in LISTS and n_list > FULL_LIST_LIMIT:
+            continue
+        out[k] = v
+    for t, (a, b) in (("t", ("inpt", "jnpt")), ("s", ("inps", "jnps")),
+                      ("te", ("inpte", "jnpte")), ("se", ("inpse", "jnpse"))):
+        i = d.get(a, np.zeros(0, np.int32))
+        j = d.get(b, np.zeros(0, np.int32))
+        out[f"npr_{t}"] = np.int64(len(i))
+        out[f"digest_set_{t}"] = np.array(pair_digest(i, j))
+        out[f"digest_ord_{t}"] = np.array(order_digest(i, j))
+    if extra:
+        out.update(extra)
+    path = os.path.join(HERE, f"{name}.npz")
+    np.savez_compressed(path, **out)
+    print(f"{name}: natoms={int(d['natoms'][0])} nprt={len(d.get('inpt', []))} "
+          f"nprte={len(d.get('inpte', []))} -> {os.path.getsize(path) // 1024} KiB")
+
+
+def ref_case(name, subdir, deck, nsteps):
+    with tempfile.TemporaryDirectory() as tmp:
+        w = os.path.join(tmp, "w")
+        shutil.copytree(os.path.join(REF, "test", subdir), w)
+        d = run_ref(w, os.path.join(w, deck), nsteps)
+    save(name, d)
+
+
+def synth_case(name, sysd, kind, nsteps, **kw):
+    with tempfile.TemporaryDirectory() as tmp:
+        deck = write_deck(tmp, sysd, kind, nstep=nsteps, **kw)
+        d = run_ref(tmp, deck, nsteps)
+    # the synthetic positions must round-trip the deck exactly
+    assert np.array_equal(d["pos"], sysd["pos"]), name
+    save(name, d)
+
+
+def coul_keys(s):
+    rc = float(np.sqrt(s["rmaxl_coul2"]))
+    return [rf"\alp_ewald{{{float(s['kappa'])!r}}}", rf"\kmax{{{int(s['kmax'])}}}",
+            rf"\rcute_max{{{rc!r}}}", rf"\rcute_resp{{{rc!r}}}",
+            rf"\rcute_min{{{float(np.sqrt(s['rmins_coul2']))!r}}}"]
+
+
+def main():
+    if not os.path.exists(DUMP):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
+    ref_case("ref_lj2", "lj", "lj.input_ljnve.2", 5)
+    ref_case("ref_lj256", "lj", "lj.input_ljnve", 3)
+    ref_case("ref_abutane", "abutane", "abutane.input", 2)
+    synth_case("ar864", S.lj_fcc(), "lj", 3, dt=2.0, temp=120.0)
+    w = S.water(nside=8)
+    synth_case("water512", w, "water", 2, dt=2.0, nres_intra=4, temp=300.0,
+               extra=coul_keys(w))
+    m = S.mixture(600)
+    synth_case("mix600", m, "mixture", 2, dt=2.0, temp=300.0, extra=coul_keys(m))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,76 @@
+"""The C-ABI library loads, exports every symbol include/cm3d_nb.h declares,
+and its host set-up helpers rebuild cm3d's own tables.  CPU only (no kernel
+is launched)."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, load_fixture
+from cm3d_amd import nb, systems as S
+from oracle import oracle as O
+
+
+def test_header_symbols_exported():
+    hdr = open(os.path.join(ROOT, "include", "cm3d_nb.h")).read()
+    declared = set(re.findall(r"^\s*(?:int|void|const char \*)\s*\*?\s*(nb_\w+)\s*\(", hdr, re.M))
+    assert declared >= {"nb_create", "nb_force", "nb_build_lists"}
+    L = nb.lib()
+    for name in sorted(declared):
+        assert hasattr(L, name), name
+    assert declared == set(nb.EXPORTS)
+
+
+@pytest.mark.parametrize("name", ["ar864", "water512", "mix600"])
+def test_tables_equal_cm3d_setup(name):
+    """nb_table_pair / nb_table_coul / nb_kvectors rebuild, bit for bit, what
+    cm3d's input layer built from the equivalent deck (the fixture)."""
+    d = load_fixture(name)
+    builder = {"ar864": lambda: S.lj_fcc(), "water512": lambda: S.water(nside=8),
+               "mix600": lambda: S.mixture(600)}[name]
+    s = builder()
+    for k in ("vtab", "dvtab", "switchs", "rmins_cut2", "rmaxs_cut2", "rminl_cut2",
+              "rmaxl_cut2", "rmaxs_skin2", "rmaxl_skin2", "dx2tab", "itype", "map",
+              "nexclude", "exclude"):
+        assert np.array_equal(np.asarray(s[k]).ravel(), d[k].ravel()), k
+    assert np.array_equal(s["qch"], d["qch"])
+    assert s["vlrc"] == d["vlrc"][0] and s["wlrc"] == d["wlrc"][0]
+    if int(d["ktot"][0]) > 0:
+        for k in ("coultab", "dcoultab", "switchse"):
+            assert np.array_equal(s[k], d[k]), k
+        for k in ("dx2coul", "rmins_coul2", "rmaxl_coul2", "rmaxs_skin2_e", "rmaxl_skin2_e",
+                  "ewald_self", "ewald_bkg"):
+            assert float(s[k]) == float(d[k][0]), k
+        for k in ("ka", "kb", "kc", "kup", "cnexclude", "cexclude", "iecorr", "jecorr"):
+            ref = d.get(k, np.zeros(0, np.int32))
+            assert np.array_equal(np.asarray(s[k]), ref), k
+
+
+def test_tables_equal_oracle_restatement():
+    """The product's and the oracle's restatements agree bit for bit (LJ and
+    Williams pair tables, the erfc table, the k sphere)."""
+    for pot, prm in ((nb.POT_LJ, [77.99954795, 3.166]),
+                     (nb.POT_WILLIAMS, [42084530., 3.60, 285830.6, 0.0, 0.0])):
+        a = nb.table_pair(pot, prm, 1.0, 9.0, 7.0, 0.5, 1.0, 800)
+        b = O.table_pair(pot, prm, 1.0, 9.0, 7.0, 0.5, 1.0, 800)
+        for k in ("vtab", "dvtab", "switchs", "cut2"):
+            assert np.array_equal(a[k], b[k]), k
+        assert a["dx2tab"] == b["dx2tab"] and a["vlrc"] == b["vlrc"] and a["wlrc"] == b["wlrc"]
+    a = nb.table_coul(600, 3, 0.3, 0.5, 8.0, 12.0, 0.5, 1.0)
+    b = O.table_coul(600, 3, 0.3, 0.5, 8.0, 12.0, 0.5, 1.0)
+    for k in ("coultab", "dcoultab", "switchse", "cut2"):
+        assert np.array_equal(a[k], b[k]), k
+    for kmax in (0, 1, 7, 9, 15):
+        ka, kb, kc = nb.kvectors(kmax)
+        oa, ob, oc, _ = O.kvectors(kmax)
+        assert np.array_equal(ka, oa) and np.array_equal(kb, ob) and np.array_equal(kc, oc)
+
+
+def test_create_without_gpu_fails_loudly():
+    """No device: nb_create returns an error (never a CPU fallback)."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(nb.NBError):
+        nb.NBContext(S.lj_fcc(ncells=3))
