@@ -92,6 +92,10 @@ extern "C" void nb_destroy(nb_ctx *c) {
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (c->h_flags) hipHostFree(c->h_flags);
+  if (c->tev) {
+    for (int k = 0; k < NB_TIME_REC * NB_TIME_EV; k++) hipEventDestroy(c->tev[k]);
+    delete[] c->tev;
+  }
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
 }
@@ -308,6 +312,13 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
   size_t tab_bytes = (vt3.size() + ct3.size()) * sizeof(double);
   c->ctab_off = (int)vt3.size();
   c->tab_lds_bytes = tab_bytes <= 60 * 1024 ? (int)tab_bytes : 0;
+  c->tab_lds_bytes_f = tab_bytes / 2 <= 60 * 1024 ? (int)(tab_bytes / 2) : 0;
+  if (c->prec == NB_PREC_MIXED) {
+    std::vector<float> vf(vt3.begin(), vt3.end()), cf(ct3.begin(), ct3.end());
+    if ((rc = upload(&c->d_vtab3f, vf.data(), vf.size()))) return fail(rc);
+    if ((rc = upload(&c->d_ctab3f, cf.data(), cf.size()))) return fail(rc);
+    if ((rc = dalloc(&c->d_recf, (size_t)n))) return fail(rc);
+  }
   // neighbour capacity from the density, grown on overflow at build time
   {
     double vol_per_atom = 1e300;
@@ -315,6 +326,7 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
     c->cap = 0;
   }
   HCHK(hipMemset(c->d_eout, 0, 64 * sizeof(double)));
+  HCHK(hipMemset(c->d_flags, 0, 8 * sizeof(int)));
   *out = c;
   return NB_OK;
 }
@@ -351,7 +363,7 @@ static int ensure_capacity(nb_ctx *c, const Box &b, int want_cap) {
   return dalloc(&c->d_nbr, (size_t)c->cap * c->natoms);
 }
 
-static int build_on(nb_ctx *c, const double hmat[9], hipStream_t s) {
+int nb_build_now(nb_ctx *c, const double hmat[9], hipStream_t s) {
   Box b = make_box(hmat);
   int nx, ny, nz;
   cell_grid(c, b, &nx, &ny, &nz);
@@ -395,7 +407,7 @@ extern "C" int nb_build_lists(nb_ctx *c, const double *pxyz, const double hmat[9
   HCHK(hipSetDevice(c->dev));
   HCHK(hipMemcpyAsync(c->d_pos, pxyz, sizeof(double) * 3 * c->natoms, hipMemcpyHostToDevice,
                       c->stream));
-  return build_on(c, hmat, c->stream);
+  return nb_build_now(c, hmat, c->stream);
 }
 
 extern "C" int nb_build_lists_device(nb_ctx *c, const double *d_pxyz, const double hmat[9],
@@ -406,7 +418,7 @@ extern "C" int nb_build_lists_device(nb_ctx *c, const double *d_pxyz, const doub
   if (d_pxyz != c->d_pos)
     HCHK(hipMemcpyAsync(c->d_pos, d_pxyz, sizeof(double) * 3 * c->natoms,
                         hipMemcpyDeviceToDevice, s));
-  return build_on(c, hmat, s);
+  return nb_build_now(c, hmat, s);
 }
 
 extern "C" int nb_need_rebuild(nb_ctx *c, const double *pxyz, int *flag) {
@@ -547,5 +559,44 @@ extern "C" int nb_get_pairs(nb_ctx *c, int which, int *ip, int *jp, int cap, int
   *nout = (int)pr.size();
   if ((int)pr.size() > cap) { nb_set_error("nb_get_pairs: capacity %d < %zu", cap, pr.size()); return NB_ERR_CAPACITY; }
   for (size_t k = 0; k < pr.size(); k++) { ip[k] = pr[k].first; jp[k] = pr[k].second; }
+  return NB_OK;
+}
+
+void nb_time_mark(nb_ctx *c, int ev, hipStream_t s) {
+  if (!c->timing || c->trec >= NB_TIME_REC) return;
+  hipEventRecord(c->tev[c->trec * NB_TIME_EV + ev], s);
+}
+
+extern "C" int nb_timing(nb_ctx *c, int mode, double out[6]) {
+  if (!c) { nb_set_error("nb_timing: null context"); return NB_ERR_ARG; }
+  HCHK(hipSetDevice(c->dev));
+  if (mode == 1) {  // start / restart recording
+    if (!c->tev) {
+      c->tev = new hipEvent_t[NB_TIME_REC * NB_TIME_EV];
+      for (int k = 0; k < NB_TIME_REC * NB_TIME_EV; k++) HCHK(hipEventCreate(&c->tev[k]));
+    }
+    c->timing = 1;
+    c->trec = 0;
+    return NB_OK;
+  }
+  if (mode == 0) { c->timing = 0; return NB_OK; }
+  // mode 2: averages over the recorded force calls, milliseconds:
+  // out = {calls, pack, pair kernel, ecorr, ewald (all four kernels), total}
+  if (!out || !c->tev) { nb_set_error("nb_timing: nothing recorded"); return NB_ERR_ARG; }
+  int nrec = c->trec < NB_TIME_REC ? c->trec : NB_TIME_REC;
+  for (int k = 0; k < 6; k++) out[k] = 0.;
+  out[0] = nrec;
+  for (int r = 0; r < nrec; r++) {
+    hipEvent_t *e = c->tev + r * NB_TIME_EV;
+    HCHK(hipEventSynchronize(e[5]));
+    float ms;
+    HCHK(hipEventElapsedTime(&ms, e[0], e[1])); out[1] += ms;
+    HCHK(hipEventElapsedTime(&ms, e[1], e[2])); out[2] += ms;
+    HCHK(hipEventElapsedTime(&ms, e[2], e[3])); out[3] += ms;
+    HCHK(hipEventElapsedTime(&ms, e[3], e[4])); out[4] += ms;
+    HCHK(hipEventElapsedTime(&ms, e[0], e[5])); out[5] += ms;
+  }
+  if (nrec > 0)
+    for (int k = 1; k < 6; k++) out[k] /= nrec;
   return NB_OK;
 }

@@ -110,7 +110,8 @@ struct nb_ctx {
   int *d_nnb;
   int cap;
   int *d_flags;                // [0] overflow max count, [1] rebuild flag,
-                               // [2..5] half-pair counts s,t,se,te, [6] clamps
+                               // [2..5] half-pair counts s,t,se,te,
+                               // [6] builds done by the conditional path
   int ncx, ncy, ncz, ncells, cells_alloc;
   double *d_chg, *d_stp, *d_spart, *d_kd;
   int nchunk;
@@ -120,14 +121,24 @@ struct nb_ctx {
   float4 *d_recf;              // mixed: {sx, sy, sz (u32 bits), q}
   Box box_built;
   int have_lists;
-  int tab_lds_bytes, ctab_off;
+  int tab_lds_bytes, tab_lds_bytes_f, ctab_off;
   // host mirrors
   int *h_flags;
+  // optional per-phase timing: a ring of event records, one per force call
+  int timing, trec;
+  hipEvent_t *tev;            // NB_TIME_REC x NB_TIME_EV events
 };
+#define NB_TIME_REC 2048
+#define NB_TIME_EV 6             // start | pack | pair | ecorr | ewald | end
 
 // kernel launchers (nb_kernels.hip)
 int nb_launch_build(nb_ctx *c, const Box &b, hipStream_t s);
+int nb_launch_build_cond(nb_ctx *c, const Box &b, hipStream_t s);
+int nb_build_now(nb_ctx *c, const double hmat[9], hipStream_t s);
 int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs,
                     double *d_fl, hipStream_t s);
 int nb_launch_rebuild_check(nb_ctx *c, const double *d_pos, hipStream_t s);
+void nb_time_mark(nb_ctx *c, int ev, hipStream_t s);
+int nb_launch_pair_mixed(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_fl,
+                         hipStream_t s);
 void nb_set_error(const char *fmt, ...);

@@ -30,9 +30,28 @@ static __device__ __forceinline__ int mbcnt(unsigned long long m) {
 // Binning follows get_lnklist (set_link.c:218-236): wrap s -= anint(s), then
 // (int)((s+.5)*n) clamped.
 // ---------------------------------------------------------------------------
+// Every list-build kernel takes `cond`: when non-null and *cond == 0 it
+// returns at once, so a device-side rebuild test can gate a captured build.
+#define NB_SKIP(cond) if ((cond) && *(cond) == 0) return
+
+__global__ void k_zero_cond(int *__restrict__ p, int n, const int *__restrict__ cond) {
+  NB_SKIP(cond);
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+__global__ void k_copy_cond(double *__restrict__ dst, const double *__restrict__ src, int n,
+                            const int *__restrict__ cond, int *__restrict__ counter) {
+  NB_SKIP(cond);
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+  if (i == 0 && counter) atomicAdd(counter, 1);
+}
+
 __global__ void k_cell_assign(const double *__restrict__ pos, int n, Box b,
                               int ncx, int ncy, int ncz, int *__restrict__ cell,
-                              int *__restrict__ count) {
+                              int *__restrict__ count, const int *__restrict__ cond) {
+  NB_SKIP(cond);
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double x = pos[i], y = pos[n + i], z = pos[2 * n + i];
@@ -56,7 +75,9 @@ __global__ void k_cell_assign(const double *__restrict__ pos, int n, Box b,
 __global__ __launch_bounds__(1024) void k_cell_scan(const int *__restrict__ count,
                                                     int ncells,
                                                     int *__restrict__ start,
-                                                    int *__restrict__ fill) {
+                                                    int *__restrict__ fill,
+                                                    const int *__restrict__ cond) {
+  NB_SKIP(cond);
   __shared__ int wsum[16];
   __shared__ int carry;
   if (threadIdx.x == 0) carry = 0;
@@ -88,7 +109,9 @@ __global__ __launch_bounds__(1024) void k_cell_scan(const int *__restrict__ coun
 }
 
 __global__ void k_cell_scatter(const int *__restrict__ cell, int n,
-                               int *__restrict__ fill, int *__restrict__ tmp) {
+                               int *__restrict__ fill, int *__restrict__ tmp,
+                               const int *__restrict__ cond) {
+  NB_SKIP(cond);
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   int slot = atomicAdd(&fill[cell[i]], 1);
@@ -102,7 +125,8 @@ __global__ void k_cell_scatter(const int *__restrict__ cell, int n,
 __global__ __launch_bounds__(256) void k_cell_order(
     const int *__restrict__ count, const int *__restrict__ start,
     const int *__restrict__ tmp, int *__restrict__ perm, int *__restrict__ iperm,
-    int *__restrict__ scell) {
+    int *__restrict__ scell, const int *__restrict__ cond) {
+  NB_SKIP(cond);
   __shared__ int seg[NB_CELL_LDS];
   int c = blockIdx.x, n = count[c], st = start[c];
   const int *src = tmp + st;
@@ -123,7 +147,9 @@ __global__ __launch_bounds__(256) void k_cell_order(
 // Sorted-order records refreshed at every evaluation: {x,y,z,q} in FP64.
 __global__ void k_pack64(const double *__restrict__ pos, const double *__restrict__ q,
                          const int *__restrict__ itype, const int *__restrict__ perm,
-                         int n, double4 *__restrict__ rec, int *__restrict__ stype) {
+                         int n, double4 *__restrict__ rec, int *__restrict__ stype,
+                         const int *__restrict__ cond) {
+  NB_SKIP(cond);
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   int a = perm[s];
@@ -149,6 +175,7 @@ struct ListArgs {
   Box b;
   uint32_t *nbr;
   int *nnb, *flags;
+  const int *cond;
 };
 
 static __device__ __forceinline__ int wrapc(int v, int n) {
@@ -156,6 +183,7 @@ static __device__ __forceinline__ int wrapc(int v, int n) {
 }
 
 __global__ __launch_bounds__(256) void k_build_list(ListArgs A) {
+  NB_SKIP(A.cond);
   const int lane = lane_id();
   const int s = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (s >= A.n) return;
@@ -628,19 +656,23 @@ __global__ __launch_bounds__(256) void k_reduce_rows(const double *__restrict__ 
 
 static int blocks(int n, int t) { return (n + t - 1) / t; }
 
-int nb_launch_build(nb_ctx *c, const Box &b, hipStream_t s) {
+// The list build.  cond == nullptr: unconditional; otherwise every kernel
+// checks *cond (the device rebuild flag) and does nothing when it is clear.
+// No memset nodes: zeroing is done by kernels so the sequence can be captured.
+static int launch_build_impl(nb_ctx *c, const Box &b, hipStream_t s, const int *cond) {
   const int n = c->natoms;
-  LCHK(hipMemsetAsync(c->d_ccount, 0, sizeof(int) * c->ncells, s));
-  LCHK(hipMemsetAsync(c->d_flags, 0, sizeof(int) * 8, s));
+  k_zero_cond<<<blocks(c->ncells, 256), 256, 0, s>>>(c->d_ccount, c->ncells, cond);
+  k_zero_cond<<<1, 64, 0, s>>>(c->d_flags + 2, 4, cond);
+  k_zero_cond<<<1, 64, 0, s>>>(c->d_flags, 1, cond);
   k_cell_assign<<<blocks(n, 256), 256, 0, s>>>(c->d_pos, n, b, c->ncx, c->ncy, c->ncz,
-                                                c->d_cell, c->d_ccount);
-  k_cell_scan<<<1, 1024, 0, s>>>(c->d_ccount, c->ncells, c->d_cstart, c->d_cfill);
-  k_cell_scatter<<<blocks(n, 256), 256, 0, s>>>(c->d_cell, n, c->d_cfill, c->d_tmp);
+                                                c->d_cell, c->d_ccount, cond);
+  k_cell_scan<<<1, 1024, 0, s>>>(c->d_ccount, c->ncells, c->d_cstart, c->d_cfill, cond);
+  k_cell_scatter<<<blocks(n, 256), 256, 0, s>>>(c->d_cell, n, c->d_cfill, c->d_tmp, cond);
   // scell reuses d_cell: the sorted cell id of each sorted slot
   k_cell_order<<<c->ncells, 256, 0, s>>>(c->d_ccount, c->d_cstart, c->d_tmp, c->d_perm,
-                                         c->d_iperm, c->d_cell);
+                                         c->d_iperm, c->d_cell, cond);
   k_pack64<<<blocks(n, 256), 256, 0, s>>>(c->d_pos, c->d_qch, c->d_itype, c->d_perm, n,
-                                          (double4 *)c->d_rec64, c->d_stype);
+                                          (double4 *)c->d_rec64, c->d_stype, cond);
   ListArgs A;
   A.n = n; A.ntypes = c->ntypes; A.cap = c->cap;
   A.ncx = c->ncx; A.ncy = c->ncy; A.ncz = c->ncz;
@@ -648,15 +680,24 @@ int nb_launch_build(nb_ctx *c, const Box &b, hipStream_t s) {
   A.scell = c->d_cell; A.cstart = c->d_cstart; A.ccount = c->d_ccount;
   A.xoff = c->d_xoff; A.xlist = c->d_xlist; A.map = c->d_map; A.ip = c->d_ip;
   A.rmaxs_skin2_e = c->rmaxs_skin2_e; A.rmaxl_skin2_e = c->rmaxl_skin2_e;
-  A.b = b; A.nbr = c->d_nbr; A.nnb = c->d_nnb; A.flags = c->d_flags;
+  A.b = b; A.nbr = c->d_nbr; A.nnb = c->d_nnb; A.flags = c->d_flags; A.cond = cond;
   k_build_list<<<blocks(n, 4), 256, 0, s>>>(A);
+  k_copy_cond<<<blocks(3 * n, 256), 256, 0, s>>>(c->d_posb, c->d_pos, 3 * n, cond,
+                                                 c->d_flags + 6);
   LCHK(hipGetLastError());
-  LCHK(hipMemcpyAsync(c->d_posb, c->d_pos, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, s));
   return NB_OK;
 }
 
+int nb_launch_build(nb_ctx *c, const Box &b, hipStream_t s) {
+  return launch_build_impl(c, b, s, nullptr);
+}
+
+int nb_launch_build_cond(nb_ctx *c, const Box &b, hipStream_t s) {
+  return launch_build_impl(c, b, s, c->d_flags + 1);
+}
+
 int nb_launch_rebuild_check(nb_ctx *c, const double *d_pos, hipStream_t s) {
-  LCHK(hipMemsetAsync(c->d_flags + 1, 0, sizeof(int), s));
+  k_zero_cond<<<1, 64, 0, s>>>(c->d_flags + 1, 1, nullptr);
   k_rebuild_check<<<blocks(c->natoms, 256), 256, 0, s>>>(d_pos, c->d_posb, c->natoms,
                                                          .25 * c->skin * c->skin, c->d_flags + 1);
   LCHK(hipGetLastError());
@@ -666,31 +707,41 @@ int nb_launch_rebuild_check(nb_ctx *c, const double *d_pos, hipStream_t s) {
 int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_fl,
                     hipStream_t s) {
   const int n = c->natoms;
-  k_pack64<<<blocks(n, 256), 256, 0, s>>>(c->d_pos, c->d_qch, c->d_itype, c->d_perm, n,
-                                          (double4 *)c->d_rec64, nullptr);
-  PairArgs P;
-  P.n = n; P.ntypes = c->ntypes; P.ninter = c->ninter; P.ntable = c->ntable;
-  P.cap = c->cap; P.tab_lds = c->tab_lds_bytes > 0; P.ctab_off = c->ctab_off;
-  P.nbr = c->d_nbr; P.nnb = c->d_nnb; P.stype = c->d_stype; P.perm = c->d_perm;
-  P.map = c->d_map; P.rec = (const double4 *)c->d_rec64; P.ip = c->d_ip;
-  P.vtab3 = c->d_vtab3; P.ctab3 = c->d_ctab3;
-  P.rmins_coul2 = c->rmins_coul2; P.dx2coul = c->dx2coul; P.rmaxl_coul2 = c->rmaxl_coul2;
-  P.clast = c->clast;
-  P.b = b; P.fl = d_fl; P.fs = d_fs; P.epart = c->d_epair;
-  const int g = c->npair_blocks, lds = c->tab_lds_bytes;
-  if (iflg == 0)
-    k_pair64<0, true><<<g, 256, lds, s>>>(P);
-  else if (d_fs)
-    k_pair64<1, true><<<g, 256, lds, s>>>(P);
-  else
-    k_pair64<1, false><<<g, 256, lds, s>>>(P);
-  LCHK(hipGetLastError());
-  k_reduce_rows<NB_NE_PAIR><<<1, 256, 0, s>>>(c->d_epair, g, c->d_eout);
+  nb_time_mark(c, 0, s);
+  if (c->prec == NB_PREC_MIXED) {
+    nb_time_mark(c, 1, s);
+    int rc = nb_launch_pair_mixed(c, iflg, b, d_fs, d_fl, s);
+    if (rc) return rc;
+  } else {
+    k_pack64<<<blocks(n, 256), 256, 0, s>>>(c->d_pos, c->d_qch, c->d_itype, c->d_perm, n,
+                                            (double4 *)c->d_rec64, nullptr, nullptr);
+    nb_time_mark(c, 1, s);
+    PairArgs P;
+    P.n = n; P.ntypes = c->ntypes; P.ninter = c->ninter; P.ntable = c->ntable;
+    P.cap = c->cap; P.tab_lds = c->tab_lds_bytes > 0; P.ctab_off = c->ctab_off;
+    P.nbr = c->d_nbr; P.nnb = c->d_nnb; P.stype = c->d_stype; P.perm = c->d_perm;
+    P.map = c->d_map; P.rec = (const double4 *)c->d_rec64; P.ip = c->d_ip;
+    P.vtab3 = c->d_vtab3; P.ctab3 = c->d_ctab3;
+    P.rmins_coul2 = c->rmins_coul2; P.dx2coul = c->dx2coul; P.rmaxl_coul2 = c->rmaxl_coul2;
+    P.clast = c->clast;
+    P.b = b; P.fl = d_fl; P.fs = d_fs; P.epart = c->d_epair;
+    const int g = c->npair_blocks, lds = c->tab_lds_bytes;
+    if (iflg == 0)
+      k_pair64<0, true><<<g, 256, lds, s>>>(P);
+    else if (d_fs)
+      k_pair64<1, true><<<g, 256, lds, s>>>(P);
+    else
+      k_pair64<1, false><<<g, 256, lds, s>>>(P);
+    LCHK(hipGetLastError());
+  }
+  nb_time_mark(c, 2, s);
+  k_reduce_rows<NB_NE_PAIR><<<1, 256, 0, s>>>(c->d_epair, c->npair_blocks, c->d_eout);
   if (iflg == 1 && c->iperd == 3 && c->necorr > 0) {
     k_ecorr<<<c->necorr_blocks, 256, 0, s>>>(c->d_eoff, c->d_elist, c->d_pos, c->d_qch, n,
                                              c->kappa, d_fl, c->d_eec);
     k_reduce_rows<NB_NE_ECORR><<<1, 256, 0, s>>>(c->d_eec, c->necorr_blocks, c->d_eout + 20);
   }
+  nb_time_mark(c, 3, s);
   if (iflg == 1 && c->iperd == 3 && c->ktot > 0 && c->ncharge > 0) {
     const int nq = c->ncharge;
     k_ew_charges<<<blocks(nq, 256), 256, 0, s>>>(c->d_pos, c->d_qch, c->d_qidx, nq, n, b,
@@ -712,6 +763,9 @@ int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_f
                                                c->d_qidx, nq, n, d_fl);
     LCHK(hipGetLastError());
   }
+  nb_time_mark(c, 4, s);
+  nb_time_mark(c, 5, s);
+  if (c->timing) c->trec++;
   return NB_OK;
 }
 

@@ -1,0 +1,495 @@
+// nb_md.hip -- device-resident velocity-Verlet / RESPA step around the
+// nonbonded engine: cm3d's step() (integrate/md_step.c:62-159) for the
+// nres_lrf = 1 case every configuration uses, with harmonic bonds and bends
+// (energy/intra/md_fbond.c:76-128, md_fbend.c:126-260) on the inner levels.
+//
+// Per outer step, all on one stream with no host round trip:
+//   k_md_inner    one thread per molecule: v += dter*al; then nres_intra times
+//                 { v += dta2*aa; r += dta2*v; r += dta2*v; aa = bonded(r)/m;
+//                   v += dta2*aa } -- molecules never share bonded terms, so
+//                 the whole inner RESPA loop runs in one launch with the
+//                 molecule's atoms in LDS
+//   rebuild test  nrefsh's |r - r_build| > skin/2 on device (flag in HBM)
+//   list build    the cell sort + list kernels, each returning at once when
+//                 the flag is clear (so the step is graph-capturable)
+//   force(...,1)  the pair, ecorr and Ewald kernels
+//   k_md_kick     v += dter * fl / m
+// Positions and velocities stay FP64 (cm3d's integrator arithmetic, no FMA).
+// A step can be captured once into a hipGraph and replayed.
+#include <string.h>
+
+#include <vector>
+
+#include "nb_internal.h"
+
+#define MD_MAXA 8          // atoms per molecule held in LDS by k_md_inner
+#define MD_TPB 64          // molecules per workgroup
+
+struct nb_md {
+  nb_ctx *c;
+  int n, nmol, nbond, nbend, ninnra;
+  double dt, dter, dta2;
+  double *d_vel, *d_rm, *d_aa, *d_amass;
+  int *d_moff, *d_boff, *d_aoff;
+  int *d_bi, *d_bj, *d_ai, *d_aj, *d_ak;
+  double *d_bk, *d_br0, *d_afk, *d_ath0;
+  double *d_ep, *d_eo;
+  int ep_blocks;
+  Box box;
+  double hmat[9];
+  hipGraph_t graph;
+  hipGraphExec_t gexec;
+  int have_graph;
+  int nrebuild_host;
+};
+
+// Harmonic bonds then bends of one molecule, in cm3d's term order, on the
+// molecule's atoms in LDS (p: [MD_MAXA][3] of this thread); f is zeroed here.
+static __device__ void mol_bonded(const double (*p)[3], double (*f)[3], int na, int a0,
+                                  int b0, int b1, int e0, int e1, const int *bi, const int *bj,
+                                  const double *bk, const double *br0, const int *ai,
+                                  const int *aj, const int *ak, const double *afk,
+                                  const double *ath0) {
+  for (int a = 0; a < na; a++) f[a][0] = f[a][1] = f[a][2] = 0.;
+  for (int b = b0; b < b1; b++) {
+    int i = bi[b] - a0, j = bj[b] - a0;
+    double dx = p[i][0] - p[j][0], dy = p[i][1] - p[j][1], dz = p[i][2] - p[j][2];
+    double r = sqrt(dx * dx + dy * dy + dz * dz), dr = r - br0[b];
+    double fr = bk[b] * dr / r;
+    dx *= fr; dy *= fr; dz *= fr;
+    f[i][0] -= dx; f[i][1] -= dy; f[i][2] -= dz;
+    f[j][0] += dx; f[j][1] += dy; f[j][2] += dz;
+  }
+  for (int e = e0; e < e1; e++) {
+    int i = ai[e] - a0, j = aj[e] - a0, k = ak[e] - a0;
+    double ax = p[i][0] - p[j][0], ay = p[i][1] - p[j][1], az = p[i][2] - p[j][2];
+    double bx = p[k][0] - p[j][0], by = p[k][1] - p[j][1], bz = p[k][2] - p[j][2];
+    double al = ax * ax + ay * ay + az * az, bl = bx * bx + by * by + bz * bz;
+    double ab = ax * bx + ay * by + az * bz, r11 = 1. / sqrt(al * bl);
+    double c = ab * r11, dudc;
+    if (fabs(c) >= 1.) {
+      dudc = 0. * -1.;
+    } else {
+      double th = acos(c), dth = th - ath0[e];
+      double dudth = afk[e] * dth, dthdcos = -1. / sqrt(1. - c * c);
+      dudc = dudth * dthdcos;
+    }
+    double d1x = c * ax / al - bx * r11, d1y = c * ay / al - by * r11, d1z = c * az / al - bz * r11;
+    double d3x = c * bx / bl - ax * r11, d3y = c * by / bl - ay * r11, d3z = c * bz / bl - az * r11;
+    double d2x = -d1x - d3x, d2y = -d1y - d3y, d2z = -d1z - d3z;
+    f[i][0] += dudc * d1x; f[i][1] += dudc * d1y; f[i][2] += dudc * d1z;
+    f[j][0] += dudc * d2x; f[j][1] += dudc * d2y; f[j][2] += dudc * d2z;
+    f[k][0] += dudc * d3x; f[k][1] += dudc * d3y; f[k][2] += dudc * d3z;
+  }
+}
+
+struct InnerArgs {
+  int n, nmol, ninnra, kick_first;
+  double dter, dta2;
+  double *pos, *vel, *aa;
+  const double *fl, *rm;
+  const int *moff, *boff, *aoff, *bi, *bj, *ai, *aj, *ak;
+  const double *bk, *br0, *afk, *ath0;
+};
+
+__global__ __launch_bounds__(MD_TPB) void k_md_inner(InnerArgs A) {
+  __shared__ double sp[MD_TPB][MD_MAXA][3], sv[MD_TPB][MD_MAXA][3], sa[MD_TPB][MD_MAXA][3];
+  int m = blockIdx.x * MD_TPB + threadIdx.x;
+  if (m >= A.nmol) return;
+  const int n = A.n, a0 = A.moff[m], na = A.moff[m + 1] - a0;
+  double (*p)[3] = sp[threadIdx.x], (*v)[3] = sv[threadIdx.x], (*aa)[3] = sa[threadIdx.x];
+  for (int a = 0; a < na; a++) {
+    int g = a0 + a;
+    double rm = A.rm[g];
+    for (int d = 0; d < 3; d++) {
+      p[a][d] = A.pos[d * n + g];
+      v[a][d] = A.vel[d * n + g];
+      aa[a][d] = A.aa[d * n + g];
+      if (A.kick_first) v[a][d] += A.dter * (A.fl[d * n + g] * rm);
+    }
+  }
+  const int b0 = A.boff[m], b1 = A.boff[m + 1], e0 = A.aoff[m], e1 = A.aoff[m + 1];
+  const bool bonded = (b1 > b0) || (e1 > e0);
+  const double dta2 = A.dta2;
+  for (int it = 0; it < A.ninnra; it++) {
+    for (int a = 0; a < na; a++)
+      for (int d = 0; d < 3; d++) {
+        v[a][d] += dta2 * aa[a][d];
+        p[a][d] += dta2 * v[a][d];
+        p[a][d] += dta2 * v[a][d];
+      }
+    if (bonded) {
+      mol_bonded(p, aa, na, a0, b0, b1, e0, e1, A.bi, A.bj, A.bk, A.br0, A.ai, A.aj, A.ak,
+                 A.afk, A.ath0);
+      for (int a = 0; a < na; a++) {
+        double rm = A.rm[a0 + a];
+        for (int d = 0; d < 3; d++) aa[a][d] *= rm;
+      }
+    }
+    for (int a = 0; a < na; a++)
+      for (int d = 0; d < 3; d++) v[a][d] += dta2 * aa[a][d];
+  }
+  for (int a = 0; a < na; a++) {
+    int g = a0 + a;
+    for (int d = 0; d < 3; d++) {
+      A.pos[d * n + g] = p[a][d];
+      A.vel[d * n + g] = v[a][d];
+      A.aa[d * n + g] = aa[a][d];
+    }
+  }
+}
+
+// cm3d's force(...,2) at step_init: bonded accelerations at the current
+// positions (md_step.c:48, md_fcntr.c:79-105 with rmass).
+__global__ __launch_bounds__(MD_TPB) void k_md_bonded(InnerArgs A) {
+  __shared__ double sp[MD_TPB][MD_MAXA][3], sa[MD_TPB][MD_MAXA][3];
+  int m = blockIdx.x * MD_TPB + threadIdx.x;
+  if (m >= A.nmol) return;
+  const int n = A.n, a0 = A.moff[m], na = A.moff[m + 1] - a0;
+  double (*p)[3] = sp[threadIdx.x], (*aa)[3] = sa[threadIdx.x];
+  for (int a = 0; a < na; a++)
+    for (int d = 0; d < 3; d++) p[a][d] = A.pos[d * n + a0 + a];
+  mol_bonded(p, aa, na, a0, A.boff[m], A.boff[m + 1], A.aoff[m], A.aoff[m + 1], A.bi, A.bj,
+             A.bk, A.br0, A.ai, A.aj, A.ak, A.afk, A.ath0);
+  for (int a = 0; a < na; a++) {
+    double rm = A.rm[a0 + a];
+    for (int d = 0; d < 3; d++) A.aa[d * n + a0 + a] = aa[a][d] * rm;
+  }
+}
+
+__global__ void k_md_kick(double *__restrict__ vel, const double *__restrict__ fl,
+                          const double *__restrict__ rm, int n, double dter) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 3 * n) return;
+  vel[i] += dter * (fl[i] * rm[i % n]);
+}
+
+// Kinetic, bond and bend energies (getvbond / getvbend harmonic cases,
+// md_fbond.c:165-173, md_fbend.c:318-338; p2m trace, md_fcntr.c:338-356).
+__global__ __launch_bounds__(256) void k_md_energy(InnerArgs A, const double *__restrict__ amass,
+                                                   double *__restrict__ epart) {
+  double acc[3] = {0., 0., 0.};
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m < A.nmol) {
+    const int n = A.n, a0 = A.moff[m], a1 = A.moff[m + 1];
+    for (int g = a0; g < a1; g++)
+      for (int d = 0; d < 3; d++) acc[0] += amass[g] * (A.vel[d * n + g] * A.vel[d * n + g]);
+    for (int b = A.boff[m]; b < A.boff[m + 1]; b++) {
+      int i = A.bi[b], j = A.bj[b];
+      double dx = A.pos[i] - A.pos[j], dy = A.pos[n + i] - A.pos[n + j];
+      double dz = A.pos[2 * n + i] - A.pos[2 * n + j];
+      double dr = sqrt(dx * dx + dy * dy + dz * dz) - A.br0[b];
+      acc[1] += .5 * A.bk[b] * dr * dr;
+    }
+    for (int e = A.aoff[m]; e < A.aoff[m + 1]; e++) {
+      int i = A.ai[e], j = A.aj[e], k = A.ak[e];
+      double ax = A.pos[i] - A.pos[j], ay = A.pos[n + i] - A.pos[n + j], az = A.pos[2 * n + i] - A.pos[2 * n + j];
+      double bx = A.pos[k] - A.pos[j], by = A.pos[n + k] - A.pos[n + j], bz = A.pos[2 * n + k] - A.pos[2 * n + j];
+      double c = (ax * bx + ay * by + az * bz) / sqrt((ax * ax + ay * ay + az * az) * (bx * bx + by * by + bz * bz));
+      c = c < 1. ? c : 1.;
+      c = c > -1. ? c : -1.;
+      double th = fabs(c) >= 1. ? 1. : acos(c), dth = th - A.ath0[e];
+      acc[2] += .5 * A.afk[e] * dth * dth;
+    }
+  }
+  __shared__ double red[4][3];
+  int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int k = 0; k < 3; k++) {
+    double v = nb_wave_sum(acc[k]);
+    if (lane == 0) red[w][k] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3)
+    epart[blockIdx.x * 3 + threadIdx.x] =
+        red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+}
+
+// ---------------------------------------------------------------------------
+static thread_local char g_dummy;  // keeps this TU's statics distinct
+
+#define MCHK(x)                                                        \
+  do {                                                                 \
+    hipError_t _e = (x);                                               \
+    if (_e != hipSuccess) {                                            \
+      nb_set_error("%s: %s", #x, hipGetErrorString(_e));               \
+      return NB_ERR_HIP;                                               \
+    }                                                                  \
+  } while (0)
+
+template <typename T>
+static int mup(T **p, const T *src, size_t cnt) {
+  if (cnt == 0) cnt = 1;
+  MCHK(hipMalloc((void **)p, cnt * sizeof(T)));
+  if (src) MCHK(hipMemcpy(*p, src, cnt * sizeof(T), hipMemcpyHostToDevice));
+  return NB_OK;
+}
+
+static InnerArgs inner_args(nb_md *md) {
+  InnerArgs A;
+  A.n = md->n; A.nmol = md->nmol; A.ninnra = md->ninnra; A.kick_first = 1;
+  A.dter = md->dter; A.dta2 = md->dta2;
+  A.pos = md->c->d_pos; A.vel = md->d_vel; A.aa = md->d_aa;
+  A.fl = md->c->d_fl; A.rm = md->d_rm;
+  A.moff = md->d_moff; A.boff = md->d_boff; A.aoff = md->d_aoff;
+  A.bi = md->d_bi; A.bj = md->d_bj; A.ai = md->d_ai; A.aj = md->d_aj; A.ak = md->d_ak;
+  A.bk = md->d_bk; A.br0 = md->d_br0; A.afk = md->d_afk; A.ath0 = md->d_ath0;
+  return A;
+}
+
+extern "C" void nb_md_destroy(nb_md *md) {
+  if (!md) return;
+  hipSetDevice(md->c->dev);
+  if (md->have_graph) { hipGraphExecDestroy(md->gexec); hipGraphDestroy(md->graph); }
+  void *ptrs[] = {md->d_vel, md->d_rm, md->d_aa, md->d_amass, md->d_moff, md->d_boff,
+                  md->d_aoff, md->d_bi, md->d_bj, md->d_ai, md->d_aj, md->d_ak, md->d_bk,
+                  md->d_br0, md->d_afk, md->d_ath0, md->d_ep, md->d_eo};
+  for (void *p : ptrs)
+    if (p) hipFree(p);
+  delete md;
+}
+
+extern "C" int nb_md_create(nb_ctx *c, const nb_md_params *p, nb_md **out) {
+  if (!c || !p || !out || !p->amass || p->nmol <= 0 || !p->mol_start) {
+    nb_set_error("nb_md_create: bad argument");
+    return NB_ERR_ARG;
+  }
+  if (p->nres_lrf != 1) {
+    nb_set_error("nb_md_create: only nres_lrf == 1 is supported");
+    return NB_ERR_UNSUPPORTED;
+  }
+  const int n = c->natoms, nm = p->nmol;
+  if (p->mol_start[0] != 0 || p->mol_start[nm] != n) {
+    nb_set_error("nb_md_create: molecules must cover atoms 0..natoms-1 contiguously");
+    return NB_ERR_ARG;
+  }
+  std::vector<int> molof(n);
+  for (int m = 0; m < nm; m++) {
+    int na = p->mol_start[m + 1] - p->mol_start[m];
+    if (na <= 0 || na > MD_MAXA) {
+      nb_set_error("nb_md_create: molecule %d has %d atoms (1..%d supported)", m, na, MD_MAXA);
+      return NB_ERR_UNSUPPORTED;
+    }
+    for (int a = p->mol_start[m]; a < p->mol_start[m + 1]; a++) molof[a] = m;
+  }
+  // group bonded terms by molecule, keeping their order within a molecule
+  std::vector<int> boff(nm + 1, 0), aoff(nm + 1, 0), border, aorder;
+  for (int b = 0; b < p->nbond; b++) {
+    int i = p->bond_i[b], j = p->bond_j[b];
+    if (i < 0 || i >= n || j < 0 || j >= n || molof[i] != molof[j]) {
+      nb_set_error("nb_md_create: bond %d crosses molecules or is out of range", b);
+      return NB_ERR_ARG;
+    }
+    boff[molof[i] + 1]++;
+  }
+  for (int e = 0; e < p->nbend; e++) {
+    int i = p->bend_i[e], j = p->bend_j[e], k = p->bend_k[e];
+    if (i < 0 || i >= n || j < 0 || j >= n || k < 0 || k >= n || molof[i] != molof[j] ||
+        molof[k] != molof[j]) {
+      nb_set_error("nb_md_create: bend %d crosses molecules or is out of range", e);
+      return NB_ERR_ARG;
+    }
+    aoff[molof[j] + 1]++;
+  }
+  for (int m = 0; m < nm; m++) { boff[m + 1] += boff[m]; aoff[m + 1] += aoff[m]; }
+  std::vector<int> bi(p->nbond), bj(p->nbond), ai(p->nbend), aj(p->nbend), ak(p->nbend);
+  std::vector<double> bk(p->nbond), br0(p->nbond), afk(p->nbend), ath0(p->nbend);
+  {
+    std::vector<int> fill(boff.begin(), boff.end() - 1);
+    for (int b = 0; b < p->nbond; b++) {
+      int s = fill[molof[p->bond_i[b]]]++;
+      bi[s] = p->bond_i[b]; bj[s] = p->bond_j[b]; bk[s] = p->bond_k[b]; br0[s] = p->bond_r0[b];
+    }
+    std::vector<int> fa(aoff.begin(), aoff.end() - 1);
+    for (int e = 0; e < p->nbend; e++) {
+      int s = fa[molof[p->bend_j[e]]]++;
+      ai[s] = p->bend_i[e]; aj[s] = p->bend_j[e]; ak[s] = p->bend_k[e];
+      afk[s] = p->bend_fk[e]; ath0[s] = p->bend_th0[e];
+    }
+  }
+  nb_md *md = new nb_md();
+  memset(md, 0, sizeof(*md));
+  md->c = c; md->n = n; md->nmol = nm; md->nbond = p->nbond; md->nbend = p->nbend;
+  md->ninnra = p->nres_intra > 0 ? p->nres_intra : 1;
+  // md_step.c:73-78 with ninter = ninrra = 1
+  md->dt = p->dt;
+  double dtl2 = .5 * p->dt, dts2 = dtl2 / 1., dtr2 = dts2 / 1.;
+  md->dta2 = dtr2 / (double)md->ninnra;
+  md->dter = 1. * dts2;
+  std::vector<double> rm(n);
+  for (int i = 0; i < n; i++) rm[i] = 1. / p->amass[i];
+  int rc;
+  hipSetDevice(c->dev);
+#define MUP(ptr, src, cnt) if ((rc = mup(ptr, src, cnt))) { nb_md_destroy(md); return rc; }
+  MUP(&md->d_vel, (const double *)nullptr, 3 * (size_t)n);
+  MUP(&md->d_aa, (const double *)nullptr, 3 * (size_t)n);
+  MUP(&md->d_rm, rm.data(), n);
+  MUP(&md->d_amass, p->amass, n);
+  MUP(&md->d_moff, p->mol_start, nm + 1);
+  MUP(&md->d_boff, boff.data(), nm + 1);
+  MUP(&md->d_aoff, aoff.data(), nm + 1);
+  MUP(&md->d_bi, bi.data(), bi.size()); MUP(&md->d_bj, bj.data(), bj.size());
+  MUP(&md->d_bk, bk.data(), bk.size()); MUP(&md->d_br0, br0.data(), br0.size());
+  MUP(&md->d_ai, ai.data(), ai.size()); MUP(&md->d_aj, aj.data(), aj.size());
+  MUP(&md->d_ak, ak.data(), ak.size()); MUP(&md->d_afk, afk.data(), afk.size());
+  MUP(&md->d_ath0, ath0.data(), ath0.size());
+  md->ep_blocks = (nm + 255) / 256;
+  MUP(&md->d_ep, (const double *)nullptr, 3 * (size_t)md->ep_blocks);
+  MUP(&md->d_eo, (const double *)nullptr, 8);
+#undef MUP
+  (void)g_dummy;
+  *out = md;
+  return NB_OK;
+}
+
+extern "C" int nb_md_set_state(nb_md *md, const double *pos, const double *vel,
+                               const double hmat[9]) {
+  if (!md || !pos || !vel || !hmat) { nb_set_error("nb_md_set_state: null argument"); return NB_ERR_ARG; }
+  nb_ctx *c = md->c;
+  MCHK(hipSetDevice(c->dev));
+  MCHK(hipMemcpy(c->d_pos, pos, sizeof(double) * 3 * md->n, hipMemcpyHostToDevice));
+  MCHK(hipMemcpy(md->d_vel, vel, sizeof(double) * 3 * md->n, hipMemcpyHostToDevice));
+  memcpy(md->hmat, hmat, sizeof(md->hmat));
+  if (md->have_graph) {
+    hipGraphExecDestroy(md->gexec);
+    hipGraphDestroy(md->graph);
+    md->have_graph = 0;
+  }
+  return NB_OK;
+}
+
+extern "C" int nb_md_get_state(nb_md *md, double *pos, double *vel) {
+  if (!md) { nb_set_error("nb_md_get_state: null argument"); return NB_ERR_ARG; }
+  nb_ctx *c = md->c;
+  MCHK(hipSetDevice(c->dev));
+  MCHK(hipStreamSynchronize(c->stream));
+  if (pos) MCHK(hipMemcpy(pos, c->d_pos, sizeof(double) * 3 * md->n, hipMemcpyDeviceToHost));
+  if (vel) MCHK(hipMemcpy(vel, md->d_vel, sizeof(double) * 3 * md->n, hipMemcpyDeviceToHost));
+  return NB_OK;
+}
+
+int nb_launch_build_cond(nb_ctx *c, const Box &b, hipStream_t s);
+int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_fl,
+                    hipStream_t s);
+int nb_build_now(nb_ctx *c, const double hmat[9], hipStream_t s);
+
+// step_init (md_step.c:32-58): lists, force(...,1), force(...,2).
+extern "C" int nb_md_init(nb_md *md) {
+  if (!md) { nb_set_error("nb_md_init: null argument"); return NB_ERR_ARG; }
+  nb_ctx *c = md->c;
+  MCHK(hipSetDevice(c->dev));
+  int rc = nb_build_now(c, md->hmat, c->stream);
+  if (rc) return rc;
+  md->box = c->box_built;
+  rc = nb_launch_force(c, 1, md->box, nullptr, c->d_fl, c->stream);
+  if (rc) return rc;
+  InnerArgs A = inner_args(md);
+  k_md_bonded<<<(md->nmol + MD_TPB - 1) / MD_TPB, MD_TPB, 0, c->stream>>>(A);
+  MCHK(hipGetLastError());
+  MCHK(hipStreamSynchronize(c->stream));
+  md->nrebuild_host = 0;
+  return NB_OK;
+}
+
+static int md_enqueue_step(nb_md *md, hipStream_t s) {
+  nb_ctx *c = md->c;
+  InnerArgs A = inner_args(md);
+  k_md_inner<<<(md->nmol + MD_TPB - 1) / MD_TPB, MD_TPB, 0, s>>>(A);
+  int rc = nb_launch_rebuild_check(c, c->d_pos, s);
+  if (rc) return rc;
+  rc = nb_launch_build_cond(c, md->box, s);
+  if (rc) return rc;
+  rc = nb_launch_force(c, 1, md->box, nullptr, c->d_fl, s);
+  if (rc) return rc;
+  k_md_kick<<<(3 * md->n + 255) / 256, 256, 0, s>>>(md->d_vel, c->d_fl, md->d_rm, md->n, md->dter);
+  MCHK(hipGetLastError());
+  return NB_OK;
+}
+
+extern "C" int nb_md_step(nb_md *md, int nsteps, void *stream, int use_graph) {
+  if (!md || nsteps < 0) { nb_set_error("nb_md_step: bad argument"); return NB_ERR_ARG; }
+  nb_ctx *c = md->c;
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  MCHK(hipSetDevice(c->dev));
+  if (use_graph && !md->have_graph) {
+    MCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int rc = md_enqueue_step(md, s);
+    hipGraph_t g;
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc) return rc;
+    MCHK(e);
+    MCHK(hipGraphInstantiate(&md->gexec, g, nullptr, nullptr, 0));
+    md->graph = g;
+    md->have_graph = 1;
+  }
+  for (int k = 0; k < nsteps; k++) {
+    if (use_graph) {
+      MCHK(hipGraphLaunch(md->gexec, s));
+    } else {
+      int rc = md_enqueue_step(md, s);
+      if (rc) return rc;
+    }
+  }
+  return NB_OK;
+}
+
+// Energies of the current state (getvireal + geteng for NVE):
+// poter = U + vlrc/V, potra = vbond + vbend, zke, ham = poter + potra + zke.
+extern "C" int nb_md_energies(nb_md *md, double vlrc, nb_md_energy *e) {
+  if (!md || !e) { nb_set_error("nb_md_energies: null argument"); return NB_ERR_ARG; }
+  nb_ctx *c = md->c;
+  hipStream_t s = c->stream;
+  MCHK(hipSetDevice(c->dev));
+  MCHK(hipStreamSynchronize(s));
+  // nonbonded energies of the current positions: rerun force(...,1) into
+  // scratch (fs) so the integrator's fl is untouched
+  int rc = nb_launch_force(c, 1, md->box, nullptr, c->d_fs, s);
+  if (rc) return rc;
+  InnerArgs A = inner_args(md);
+  k_md_energy<<<md->ep_blocks, 256, 0, s>>>(A, md->d_amass, md->d_ep);
+  MCHK(hipGetLastError());
+  std::vector<double> ep(3 * md->ep_blocks);
+  double E[64];
+  MCHK(hipMemcpyAsync(ep.data(), md->d_ep, ep.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  MCHK(hipMemcpyAsync(E, c->d_eout, sizeof(E), hipMemcpyDeviceToHost, s));
+  MCHK(hipMemcpyAsync(c->h_flags, c->d_flags, 8 * sizeof(int), hipMemcpyDeviceToHost, s));
+  MCHK(hipStreamSynchronize(s));
+  double p2m = 0., vb = 0., va = 0.;
+  for (int b = 0; b < md->ep_blocks; b++) { p2m += ep[3 * b]; vb += ep[3 * b + 1]; va += ep[3 * b + 2]; }
+  const double *h = md->hmat;
+  double vol = h[0] * (h[4] * h[8] - h[5] * h[7]) + h[1] * (h[5] * h[6] - h[3] * h[8]) +
+               h[2] * (h[3] * h[7] - h[4] * h[6]);
+  memset(e, 0, sizeof(*e));
+  e->vinter = .5 * E[0];
+  e->velec = .5 * E[9];
+  double vew = 0.;
+  if (c->iperd == 3 && c->ncharge > 0) {
+    double bk = c->ewald_bkg / vol;
+    vew = (c->ktot > 0 ? E[32] : 0.) + bk + c->ewald_self;
+    if (c->necorr > 0) vew += .5 * E[20];
+  }
+  e->vewald = vew;
+  e->vbond = vb;
+  e->vbend = va;
+  e->zke = .5 * p2m;
+  e->poter = e->vinter + e->velec + e->vewald + vlrc / vol;
+  e->potra = vb + va;
+  e->ham = e->poter + e->potra + e->zke;
+  e->overflow = c->h_flags[0] > c->cap;
+  return NB_OK;
+}
+
+extern "C" int nb_md_rebuilds(nb_md *md, int *count) {
+  if (!md || !count) { nb_set_error("nb_md_rebuilds: null argument"); return NB_ERR_ARG; }
+  nb_ctx *c = md->c;
+  MCHK(hipSetDevice(c->dev));
+  MCHK(hipStreamSynchronize(c->stream));
+  MCHK(hipDeviceSynchronize());
+  int f[8];
+  MCHK(hipMemcpy(f, c->d_flags, sizeof(f), hipMemcpyDeviceToHost));
+  *count = f[6];
+  if (f[0] > c->cap) {
+    nb_set_error("neighbour list overflow in a device-side rebuild (%d > %d slots)", f[0], c->cap);
+    return NB_ERR_CAPACITY;
+  }
+  return NB_OK;
+}

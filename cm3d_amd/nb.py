@@ -73,10 +73,30 @@ class NBEnergy(C.Structure):
         return out
 
 
+class NBMDParams(C.Structure):
+    _fields_ = [("amass", _dp), ("nmol", C.c_int), ("mol_start", _ip),
+                ("nbond", C.c_int), ("bond_i", _ip), ("bond_j", _ip),
+                ("bond_k", _dp), ("bond_r0", _dp),
+                ("nbend", C.c_int), ("bend_i", _ip), ("bend_j", _ip), ("bend_k", _ip),
+                ("bend_fk", _dp), ("bend_th0", _dp),
+                ("dt", C.c_double), ("nres_intra", C.c_int), ("nres_lrf", C.c_int)]
+
+
+class NBMDEnergy(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("vinter", "velec", "vewald", "vbond", "vbend",
+                                          "zke", "poter", "potra", "ham")] + \
+               [("overflow", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 # every symbol include/cm3d_nb.h declares
 EXPORTS = ["nb_create", "nb_destroy", "nb_last_error", "nb_build_lists",
            "nb_need_rebuild", "nb_force", "nb_force_device",
-           "nb_build_lists_device", "nb_list_counts", "nb_get_pairs",
+           "nb_build_lists_device", "nb_list_counts", "nb_get_pairs", "nb_timing",
+           "nb_md_create", "nb_md_destroy", "nb_md_set_state", "nb_md_get_state",
+           "nb_md_init", "nb_md_step", "nb_md_energies", "nb_md_rebuilds",
            "nb_table_pair", "nb_table_coul", "nb_kvectors", "nb_ewald_consts"]
 
 _LIB = None
@@ -108,6 +128,16 @@ def lib():
     L.nb_build_lists_device.argtypes = [C.c_void_p, C.c_void_p, _dp, C.c_void_p]
     L.nb_list_counts.argtypes = [C.c_void_p, _ip]
     L.nb_get_pairs.argtypes = [C.c_void_p, C.c_int, _ip, _ip, C.c_int, _ip]
+    L.nb_timing.argtypes = [C.c_void_p, C.c_int, _dp]
+    L.nb_md_create.argtypes = [C.c_void_p, C.POINTER(NBMDParams), C.POINTER(C.c_void_p)]
+    L.nb_md_destroy.argtypes = [C.c_void_p]
+    L.nb_md_destroy.restype = None
+    L.nb_md_set_state.argtypes = [C.c_void_p, _dp, _dp, _dp]
+    L.nb_md_get_state.argtypes = [C.c_void_p, _dp, _dp]
+    L.nb_md_init.argtypes = [C.c_void_p]
+    L.nb_md_step.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int]
+    L.nb_md_energies.argtypes = [C.c_void_p, C.c_double, C.POINTER(NBMDEnergy)]
+    L.nb_md_rebuilds.argtypes = [C.c_void_p, _ip]
     L.nb_table_pair.argtypes = [C.c_int, _dp, C.c_double, C.c_double,
                                 C.c_double, C.c_double, C.c_double, C.c_double,
                                 C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
@@ -227,6 +257,17 @@ class NBContext:
                                           C.c_void_p(stream) if stream else None),
               "nb_build_lists_device")
 
+    def timing_start(self):
+        check(lib().nb_timing(self._h, 1, None), "nb_timing")
+
+    def timing_read(self):
+        """Average ms per force call: pack, pair, ecorr, ewald, total."""
+        out = np.zeros(6)
+        check(lib().nb_timing(self._h, 2, out.ctypes.data_as(_dp)), "nb_timing")
+        check(lib().nb_timing(self._h, 0, None), "nb_timing")
+        return dict(calls=int(out[0]), pack=out[1], pair=out[2], ecorr=out[3],
+                    ewald=out[4], total=out[5])
+
     def list_counts(self):
         """Half-pair counts as cm3d reports them: nprs, nprt, nprse, nprte."""
         c = (C.c_int * 4)()
@@ -287,3 +328,87 @@ def ewald_consts(qch, alpha):
     check(lib().nb_ewald_consts(len(q), qp, alpha, s.ctypes.data_as(_dp),
                                 b.ctypes.data_as(_dp)), "nb_ewald_consts")
     return s[0], b[0]
+
+
+class NBMD:
+    """Device-resident RESPA MD of one system around an NBContext (cm3d's
+    step_init + step(), integrate/md_step.c:32-159, nres_lrf = 1).
+
+    sysd needs ``amass`` and, for molecules, ``mol_start`` plus the bonded
+    arrays ``bond_i/j/k/r0`` and ``bend_i/j/k/fk/th0`` (water from
+    cm3d_amd.systems has them); without ``mol_start`` every atom is its own
+    molecule."""
+
+    def __init__(self, ctx, sysd, dt_ps, nres_intra=1):
+        self.ctx = ctx
+        n = ctx.natoms
+        self.n = n
+        self._keep = []
+        p = NBMDParams()
+
+        def put(name, arr, conv):
+            a, ptr = conv(arr)
+            self._keep.append(a)
+            setattr(p, name, ptr)
+            return len(a)
+
+        put("amass", sysd["amass"], _d)
+        ms = sysd.get("mol_start")
+        if ms is None:
+            ms = np.arange(n + 1, dtype=np.int32)
+        p.nmol = put("mol_start", ms, _i) - 1
+        p.nbond = len(sysd.get("bond_i", []))
+        p.nbend = len(sysd.get("bend_i", []))
+        for name in ("bond_i", "bond_j", "bend_i", "bend_j", "bend_k"):
+            put(name, sysd.get(name, np.zeros(0, np.int32)), _i)
+        for name, key in (("bond_k", "bond_k"), ("bond_r0", "bond_r0"),
+                          ("bend_fk", "bend_fk"), ("bend_th0", "bend_th0")):
+            put(name, sysd.get(key, np.zeros(0)), _d)
+        p.dt = float(dt_ps)
+        p.nres_intra = int(nres_intra)
+        p.nres_lrf = 1
+        h = C.c_void_p()
+        check(lib().nb_md_create(ctx.handle, C.byref(p), C.byref(h)), "nb_md_create")
+        self._h = h
+        self.vlrc = float(np.asarray(sysd.get("vlrc", 0.0)).reshape(-1)[0])
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().nb_md_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, pos, vel, hmat):
+        p, pp = _d(pos)
+        v, vp = _d(vel)
+        h, hp = _d(hmat)
+        check(lib().nb_md_set_state(self._h, pp, vp, hp), "nb_md_set_state")
+
+    def get_state(self):
+        pos = np.zeros(3 * self.n)
+        vel = np.zeros(3 * self.n)
+        check(lib().nb_md_get_state(self._h, pos.ctypes.data_as(_dp),
+                                    vel.ctypes.data_as(_dp)), "nb_md_get_state")
+        return pos, vel
+
+    def init(self):
+        check(lib().nb_md_init(self._h), "nb_md_init")
+
+    def step(self, nsteps=1, stream=None, graph=True):
+        check(lib().nb_md_step(self._h, int(nsteps), C.c_void_p(stream) if stream else None,
+                               int(bool(graph))), "nb_md_step")
+
+    def energies(self):
+        e = NBMDEnergy()
+        check(lib().nb_md_energies(self._h, self.vlrc, C.byref(e)), "nb_md_energies")
+        return e.as_dict()
+
+    def rebuilds(self):
+        c = C.c_int(0)
+        check(lib().nb_md_rebuilds(self._h, C.byref(c)), "nb_md_rebuilds")
+        return c.value

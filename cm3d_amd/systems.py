@@ -202,6 +202,7 @@ def water(nside=16, density=1.0, rc=10.0, alpha=0.35, kmax=9, skin=1.0, rheal=0.
              bend_i=(m3 + 1).astype(np.int32), bend_j=m3.astype(np.int32),
              bend_k=(m3 + 2).astype(np.int32), bend_fk=np.full(nmol, ka_kcal * KCAL),
              bend_th0=np.full(nmol, math.radians(theta)))
+    s["mol_start"] = (3 * np.arange(nmol + 1)).astype(np.int32)
     s["vel"] = maxwell_velocities(s["amass"], temp, seed + 1)
     s["name"] = f"water_{nmol}"
     return s

@@ -132,6 +132,55 @@ int nb_list_counts(nb_ctx *ctx, int counts[4]);
  * NB_ERR_CAPACITY (and sets *n) when cap is too small. */
 int nb_get_pairs(nb_ctx *ctx, int which, int *ip, int *jp, int cap, int *n);
 
+/* Per-phase timing of nb_force / nb_force_device with HIP events on the
+ * stream the kernels run on (no counterpart in cm3d; diagnostics).
+ * mode 1: start recording (up to 2048 calls), 0: stop, 2: read the averages
+ * in ms: out = {calls, position pack, pair kernel, ecorr (with the pair
+ * energy reduction), Ewald (four kernels), whole call}. */
+int nb_timing(nb_ctx *ctx, int mode, double out[6]);
+
+/* ---- device-resident MD step (cm3d's step(), integrate/md_step.c:62-159,
+ * for nres_lrf == 1) ----
+ * Velocity Verlet with the RESPA inner loop over harmonic bonds and bends
+ * (energy/intra/md_fbond.c:76-128, md_fbend.c:126-260, harmonic cases),
+ * nrefsh's rebuild test and the nonbonded force, all kept on the device.
+ * Molecules are contiguous atom ranges of at most 8 atoms; bonded terms may
+ * not cross molecules.  Units: dt in ps, masses in cm3d units (g/mol x
+ * MCONV), bond k in K/A^2, bend k in K/rad^2, angles in radians. */
+typedef struct nb_md nb_md;
+typedef struct nb_md_params {
+  const double *amass;                          /* [natoms]                */
+  int nmol;
+  const int *mol_start;                         /* [nmol+1]                */
+  int nbond;
+  const int *bond_i, *bond_j;
+  const double *bond_k, *bond_r0;
+  int nbend;
+  const int *bend_i, *bend_j, *bend_k;          /* bend_j is the apex      */
+  const double *bend_fk, *bend_th0;
+  double dt;                                    /* outer step, ps          */
+  int nres_intra;                               /* inner steps (ninnra)    */
+  int nres_lrf;                                 /* must be 1               */
+} nb_md_params;
+typedef struct nb_md_energy {
+  double vinter, velec, vewald, vbond, vbend, zke, poter, potra, ham;
+  int overflow;                                 /* a list build overflowed */
+} nb_md_energy;
+int nb_md_create(nb_ctx *ctx, const nb_md_params *p, nb_md **md);
+void nb_md_destroy(nb_md *md);
+int nb_md_set_state(nb_md *md, const double *pos, const double *vel,
+                    const double hmat[9]);
+int nb_md_get_state(nb_md *md, double *pos, double *vel);
+/* step_init (md_step.c:32-58): list build, force(...,1), bonded forces. */
+int nb_md_init(nb_md *md);
+/* nsteps outer steps on the stream (NULL = the context's); use_graph
+ * captures one step into a hipGraph once and replays it.  Asynchronous. */
+int nb_md_step(nb_md *md, int nsteps, void *stream, int use_graph);
+/* Energies of the current state; vlrc is cm3d's simparms->vlrc. */
+int nb_md_energies(nb_md *md, double vlrc, nb_md_energy *e);
+/* Lists built so far by the device-side rebuild test (synchronises). */
+int nb_md_rebuilds(nb_md *md, int *count);
+
 /* ---- set-up helpers (host only) ---- */
 #define NB_POT_LJ 0
 #define NB_POT_WILLIAMS 1

@@ -101,3 +101,34 @@ def test_need_rebuild_criterion():
     p[n + 11] += 0.51 * skin  # one atom beyond skin/2
     assert c.need_rebuild(p) == O.need_rebuild(n, skin, p, d["pos"]) == True
     c.close()
+
+
+MIXED_FORCE_TOL = 1e-4      # force RMS error vs FP64 oracle
+
+
+@pytest.mark.parametrize("name", ["ar864", "water512", "mix600", "ref_lj256"])
+def test_mixed_forces_within_tolerance(name):
+    """FP32 pair math + 64-bit fixed-point forces vs cm3d's FP64 forces."""
+    d = load_fixture(name)
+    print(f"[mixed] {name}", flush=True)
+    c = _ctx(d, "mixed")
+    c.build_lists(d["pos"], d["hmat"])
+    fs, fl, e = c.force(d["pos"], d["hmat"], iflg=1)
+    err = rel_rms(fl, d["fxl"])
+    print(f"  mixed force rel RMS error {err:.3e}")
+    assert err < MIXED_FORCE_TOL
+    for mine, ref in (("spotl", "e_vinter"), ("spote", "e_velec")):
+        assert abs(e[mine] - d[ref][0]) <= 1e-5 * max(abs(d[ref][0]), 1.0), (mine, e[mine], d[ref][0])
+    # Ewald and ecorr stay FP64
+    assert _rel(e["vk_ewald"], d["vk_ewald"][0]) < 1e-10 or abs(e["vk_ewald"]) < 1e-12
+    c.close()
+
+
+def test_mixed_bitwise_reproducible():
+    d = load_fixture("water512")
+    c = _ctx(d, "mixed")
+    c.build_lists(d["pos"], d["hmat"])
+    a = c.force(d["pos"], d["hmat"], 1)
+    b = c.force(d["pos"], d["hmat"], 1)
+    assert np.array_equal(a[1], b[1]) and np.array_equal(a[0], b[0])
+    c.close()
