@@ -1,0 +1,58 @@
+// nb_launch.h -- one launch path for single systems and batches.
+//
+// Every kernel has a __device__ body taking (args, block index) and two
+// entry points: the single-system __global__ (args by value, blockIdx.x) and
+// the batched one, which reads its system's args from an array in HBM and
+// its (system, local block) from a block map.  A launch site either launches
+// the single-system kernel or, when the context is recording, appends a
+// LaunchRec; a batch of systems records the same step for each of them and
+// then issues ONE grid per kernel covering every system's blocks, so the
+// launch count per step does not grow with the number of systems.
+#pragma once
+#include <vector>
+
+#include "nb_internal.h"
+
+enum KernelId {
+  K_ZERO, K_COPY, K_CELL, K_SCAN, K_SCAT, K_ORDER, K_PACK64, K_LIST, K_CHECK,
+  K_PAIR64_0T, K_PAIR64_1T, K_PAIR64_1F, K_RED_PAIR, K_RED_ECORR, K_RED_EW,
+  K_ECORR, K_EWCHG, K_EWSFAC, K_EWKRED, K_EWFORCE, K_PACKMIX, K_MIX_0T,
+  K_MIX_1T, K_MIX_1F, K_MDINNER, K_MDKICK, K_COUNT
+};
+
+struct LaunchRec {
+  int kid, nblk, tpb, lds;
+  std::vector<char> args;
+};
+
+typedef void (*BatchFn)(const void *args, const int2 *bmap, int nblk, int tpb, int lds,
+                        hipStream_t s);
+extern BatchFn g_batch_fn[K_COUNT];
+
+#define NB_KERNEL_PAIR(name, ArgsT, TPB)                                             \
+  __global__ __launch_bounds__(TPB) void name(ArgsT A) { name##_body(A, blockIdx.x); } \
+  __global__ __launch_bounds__(TPB) void name##_b(const ArgsT *AA, const int2 *bm) {  \
+    const int2 m = bm[blockIdx.x];                                                   \
+    name##_body(AA[m.x], m.y);                                                       \
+  }                                                                                  \
+  static void name##_bl(const void *a, const int2 *bm, int nb, int tpb, int lds,     \
+                        hipStream_t s) {                                             \
+    name##_b<<<nb, tpb, lds, s>>>((const ArgsT *)a, bm);                             \
+  }
+
+static inline void nb_record(nb_ctx *c, int kid, int nblk, int tpb, int lds, const void *a,
+                             size_t bytes) {
+  LaunchRec r;
+  r.kid = kid; r.nblk = nblk; r.tpb = tpb; r.lds = lds;
+  r.args.assign((const char *)a, (const char *)a + bytes);
+  c->rec->push_back(r);
+}
+
+// Launch (or record) kernel KERN with id KID.
+#define NB_GO(c, KID, KERN, NBLK, TPB, LDS, ARGS, S)                              \
+  do {                                                                            \
+    if ((c)->rec)                                                                 \
+      nb_record((c), (KID), (NBLK), (TPB), (LDS), &(ARGS), sizeof(ARGS));         \
+    else if ((NBLK) > 0)                                                          \
+      KERN<<<(NBLK), (TPB), (LDS), (S)>>>(ARGS);                                  \
+  } while (0)

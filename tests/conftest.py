@@ -29,8 +29,11 @@ def fixture(request):
     return request.param, load_fixture(request.param)
 
 
-def rel_rms(a, b):
+def rel_rms(a, b, floor=1.0):
+    """RMS(a - b) / max(RMS(b), floor).  The floor (1 K/A for forces) keeps
+    the measure meaningful where forces vanish by symmetry (a perfect
+    lattice), where both sides are pure rounding noise."""
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)
-    den = np.sqrt(np.mean(b * b))
-    return float(np.sqrt(np.mean((a - b) ** 2)) / den) if den > 0 else float(np.abs(a - b).max())
+    den = max(float(np.sqrt(np.mean(b * b))), floor)
+    return float(np.sqrt(np.mean((a - b) ** 2)) / den)
