@@ -15,7 +15,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libcm3d_nb.so")
-SOURCES = ["nb_api.hip", "nb_kernels.hip", "nb_mixed.hip", "nb_md.hip", "nb_setup.cpp"]
+SOURCES = ["nb_api.hip", "nb_kernels.hip", "nb_mixed.hip", "nb_md.hip", "nb_batch.hip",
+           "nb_setup.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 

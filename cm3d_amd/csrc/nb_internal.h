@@ -20,7 +20,11 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "../../include/cm3d_nb.h"
+
+struct LaunchRec;
 
 #define NB_WAVE 64
 #define NB_FLAG_VT 1u  // vdW t list  (rmaxl+skin)
@@ -127,6 +131,8 @@ struct nb_ctx {
   // optional per-phase timing: a ring of event records, one per force call
   int timing, trec;
   hipEvent_t *tev;            // NB_TIME_REC x NB_TIME_EV events
+  // when set, launch sites append to this list instead of launching
+  std::vector<LaunchRec> *rec;
 };
 #define NB_TIME_REC 2048
 #define NB_TIME_EV 6             // start | pack | pair | ecorr | ewald | end

@@ -1,20 +1,23 @@
-// nb_kernels.hip -- gfx950 kernels of the nonbonded path.
+// nb_kernels.hip -- gfx950 kernels of the nonbonded path (FP64 form) and
+// their launchers.
 //
-//   list build   k_cell_assign -> k_cell_scan -> k_cell_scatter ->
-//                k_cell_order -> k_pack -> k_build_list
+//   list build   k_cell -> k_scan -> k_scat -> k_order -> k_pack64 -> k_list
 //                (cm3d get_ngbr / get_ngbr_e, energy/control/ngbr_list.c:297,
 //                 elec_ngbr.c:220, set_link.c:203)
-//   pair forces  k_pair64 / k_pair_mixed (force_npol md_fnopol.c:46, force_e
-//                finter_e.c:42, ecorr md_ewald.c:385)
-//   reciprocal   k_ew_charges -> k_ew_sfac -> k_ew_kreduce -> k_ew_force
+//   pair forces  k_pair64_* (force_npol md_fnopol.c:46, force_e finter_e.c:42),
+//                k_ecorr (ecorr md_ewald.c:385); the mixed form is nb_mixed.hip
+//   reciprocal   k_ewchg -> k_ewsfac -> k_ewkred -> k_ewforce
 //                (fk_ewald md_ewald.c:185)
-//   reductions   k_reduce_rows (the getsvirter / getsvirter_e / getvewald sums)
+//   reductions   k_red_* (the getsvirter / getsvirter_e / getvewald sums)
 //
-// Every kernel is launched on the context's stream; none allocates.
+// Every kernel is a __device__ body plus a single-system and a batched entry
+// point (nb_launch.h); none allocates.
 #include <stdarg.h>
 #include <stdio.h>
 
-#include "nb_internal.h"
+#include "nb_launch.h"
+
+BatchFn g_batch_fn[K_COUNT];
 
 static __device__ __forceinline__ int lane_id() { return threadIdx.x & (NB_WAVE - 1); }
 
@@ -23,6 +26,50 @@ static __device__ __forceinline__ int mbcnt(unsigned long long m) {
                                    __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
 }
 
+// Sum NACC per-thread accumulators over a 256-thread block (wave shuffles,
+// then the 4 wave totals in LDS, fixed order) and store them to out[0..NACC).
+template <int NACC>
+static __device__ __forceinline__ void block_reduce_store(const double *acc, double *out) {
+  __shared__ double red[4][NACC];
+  const int w = threadIdx.x >> 6, lane = lane_id();
+#pragma unroll
+  for (int k = 0; k < NACC; k++) {
+    double v = nb_wave_sum(acc[k]);
+    if (lane == 0) red[w][k] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < NACC) {
+    double v = red[0][threadIdx.x] + red[1][threadIdx.x];
+    v += red[2][threadIdx.x] + red[3][threadIdx.x];
+    out[threadIdx.x] = v;
+  }
+}
+
+// Every list-build kernel takes `cond`: when non-null and *cond == 0 it
+// returns at once, so a device-side rebuild test can gate a captured build.
+#define NB_SKIP(cond) \
+  if ((cond) && *(cond) == 0) return
+
+// ---------------------------------------------------------------------------
+// small utility kernels
+// ---------------------------------------------------------------------------
+struct ZeroArgs { int *p; int n; const int *cond; };
+static __device__ __forceinline__ void k_zero_body(const ZeroArgs &A, int bid) {
+  NB_SKIP(A.cond);
+  int i = bid * blockDim.x + threadIdx.x;
+  if (i < A.n) A.p[i] = 0;
+}
+NB_KERNEL_PAIR(k_zero, ZeroArgs, 256)
+
+struct CopyArgs { double *dst; const double *src; int n; const int *cond; int *counter; };
+static __device__ __forceinline__ void k_copy_body(const CopyArgs &A, int bid) {
+  NB_SKIP(A.cond);
+  int i = bid * blockDim.x + threadIdx.x;
+  if (i < A.n) A.dst[i] = A.src[i];
+  if (i == 0 && A.counter) atomicAdd(A.counter, 1);
+}
+NB_KERNEL_PAIR(k_copy, CopyArgs, 256)
+
 // ---------------------------------------------------------------------------
 // Cell sort.  Cells tile the box in fractional coordinates with widths no
 // smaller than the longest list cutoff, so every pair within it under cm3d's
@@ -30,62 +77,43 @@ static __device__ __forceinline__ int mbcnt(unsigned long long m) {
 // Binning follows get_lnklist (set_link.c:218-236): wrap s -= anint(s), then
 // (int)((s+.5)*n) clamped.
 // ---------------------------------------------------------------------------
-// Every list-build kernel takes `cond`: when non-null and *cond == 0 it
-// returns at once, so a device-side rebuild test can gate a captured build.
-#define NB_SKIP(cond) if ((cond) && *(cond) == 0) return
-
-__global__ void k_zero_cond(int *__restrict__ p, int n, const int *__restrict__ cond) {
-  NB_SKIP(cond);
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0;
-}
-
-__global__ void k_copy_cond(double *__restrict__ dst, const double *__restrict__ src, int n,
-                            const int *__restrict__ cond, int *__restrict__ counter) {
-  NB_SKIP(cond);
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) dst[i] = src[i];
-  if (i == 0 && counter) atomicAdd(counter, 1);
-}
-
-__global__ void k_cell_assign(const double *__restrict__ pos, int n, Box b,
-                              int ncx, int ncy, int ncz, int *__restrict__ cell,
-                              int *__restrict__ count, const int *__restrict__ cond) {
-  NB_SKIP(cond);
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double x = pos[i], y = pos[n + i], z = pos[2 * n + i];
-  const double *hi = b.hi;
+struct CellArgs {
+  const double *pos; int n; Box b; int ncx, ncy, ncz; int *cell; int *count; const int *cond;
+};
+static __device__ __forceinline__ void k_cell_body(const CellArgs &A, int bid) {
+  NB_SKIP(A.cond);
+  int i = bid * blockDim.x + threadIdx.x;
+  if (i >= A.n) return;
+  const int n = A.n;
+  double x = A.pos[i], y = A.pos[n + i], z = A.pos[2 * n + i];
+  const double *hi = A.b.hi;
   double sx = x * hi[0] + y * hi[1] + z * hi[2];
   double sy = x * hi[3] + y * hi[4] + z * hi[5];
   double sz = x * hi[6] + y * hi[7] + z * hi[8];
   sx -= nb_anint(sx);
   sy -= nb_anint(sy);
   sz -= nb_anint(sz);
-  int a = (int)((sx + .5) * ncx), c = (int)((sy + .5) * ncy), e = (int)((sz + .5) * ncz);
-  a = a < 0 ? 0 : (a > ncx - 1 ? ncx - 1 : a);
-  c = c < 0 ? 0 : (c > ncy - 1 ? ncy - 1 : c);
-  e = e < 0 ? 0 : (e > ncz - 1 ? ncz - 1 : e);
-  int id = (a * ncy + c) * ncz + e;
-  cell[i] = id;
-  atomicAdd(&count[id], 1);
+  int a = (int)((sx + .5) * A.ncx), c = (int)((sy + .5) * A.ncy), e = (int)((sz + .5) * A.ncz);
+  a = a < 0 ? 0 : (a > A.ncx - 1 ? A.ncx - 1 : a);
+  c = c < 0 ? 0 : (c > A.ncy - 1 ? A.ncy - 1 : c);
+  e = e < 0 ? 0 : (e > A.ncz - 1 ? A.ncz - 1 : e);
+  int id = (a * A.ncy + c) * A.ncz + e;
+  A.cell[i] = id;
+  atomicAdd(&A.count[id], 1);
 }
+NB_KERNEL_PAIR(k_cell, CellArgs, 256)
 
 // Exclusive scan of the cell counts in one workgroup of 1024 threads.
-__global__ __launch_bounds__(1024) void k_cell_scan(const int *__restrict__ count,
-                                                    int ncells,
-                                                    int *__restrict__ start,
-                                                    int *__restrict__ fill,
-                                                    const int *__restrict__ cond) {
-  NB_SKIP(cond);
+struct ScanArgs { const int *count; int ncells; int *start; int *fill; const int *cond; };
+static __device__ __forceinline__ void k_scan_body(const ScanArgs &A, int) {
+  NB_SKIP(A.cond);
   __shared__ int wsum[16];
   __shared__ int carry;
   if (threadIdx.x == 0) carry = 0;
   __syncthreads();
-  for (int base = 0; base < ncells; base += 1024) {
+  for (int base = 0; base < A.ncells; base += 1024) {
     int i = base + threadIdx.x;
-    int v = i < ncells ? count[i] : 0;
-    // inclusive scan within the wave
+    int v = i < A.ncells ? A.count[i] : 0;
     int x = v;
 #pragma unroll
     for (int o = 1; o < NB_WAVE; o <<= 1) {
@@ -101,35 +129,36 @@ __global__ __launch_bounds__(1024) void k_cell_scan(const int *__restrict__ coun
     }
     __syncthreads();
     int excl = carry + wsum[w] + x - v;
-    if (i < ncells) { start[i] = excl; fill[i] = excl; }
+    if (i < A.ncells) { A.start[i] = excl; A.fill[i] = excl; }
     __syncthreads();
     if (threadIdx.x == 1023) carry = excl + v;
     __syncthreads();
   }
 }
+NB_KERNEL_PAIR(k_scan, ScanArgs, 1024)
 
-__global__ void k_cell_scatter(const int *__restrict__ cell, int n,
-                               int *__restrict__ fill, int *__restrict__ tmp,
-                               const int *__restrict__ cond) {
-  NB_SKIP(cond);
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  int slot = atomicAdd(&fill[cell[i]], 1);
-  tmp[slot] = i;
+struct ScatArgs { const int *cell; int n; int *fill; int *tmp; const int *cond; };
+static __device__ __forceinline__ void k_scat_body(const ScatArgs &A, int bid) {
+  NB_SKIP(A.cond);
+  int i = bid * blockDim.x + threadIdx.x;
+  if (i >= A.n) return;
+  int slot = atomicAdd(&A.fill[A.cell[i]], 1);
+  A.tmp[slot] = i;
 }
+NB_KERNEL_PAIR(k_scat, ScatArgs, 256)
 
 // Order each cell's atoms by atom index (the scatter's atomics leave them in
 // arbitrary order): every member's rank is the number of smaller indices in
 // the cell.  Makes the sorted order, and so every list, deterministic.
 #define NB_CELL_LDS 4096
-__global__ __launch_bounds__(256) void k_cell_order(
-    const int *__restrict__ count, const int *__restrict__ start,
-    const int *__restrict__ tmp, int *__restrict__ perm, int *__restrict__ iperm,
-    int *__restrict__ scell, const int *__restrict__ cond) {
-  NB_SKIP(cond);
+struct OrderArgs {
+  const int *count, *start, *tmp; int *perm, *iperm, *scell; const int *cond;
+};
+static __device__ __forceinline__ void k_order_body(const OrderArgs &A, int bid) {
+  NB_SKIP(A.cond);
   __shared__ int seg[NB_CELL_LDS];
-  int c = blockIdx.x, n = count[c], st = start[c];
-  const int *src = tmp + st;
+  int c = bid, n = A.count[c], st = A.start[c];
+  const int *src = A.tmp + st;
   if (n <= NB_CELL_LDS) {
     for (int t = threadIdx.x; t < n; t += blockDim.x) seg[t] = src[t];
     __syncthreads();
@@ -138,24 +167,27 @@ __global__ __launch_bounds__(256) void k_cell_order(
   for (int t = threadIdx.x; t < n; t += blockDim.x) {
     int a = src[t], r = 0;
     for (int u = 0; u < n; u++) r += src[u] < a;
-    perm[st + r] = a;
-    iperm[a] = st + r;
-    scell[st + r] = c;
+    A.perm[st + r] = a;
+    A.iperm[a] = st + r;
+    A.scell[st + r] = c;
   }
 }
+NB_KERNEL_PAIR(k_order, OrderArgs, 256)
 
 // Sorted-order records refreshed at every evaluation: {x,y,z,q} in FP64.
-__global__ void k_pack64(const double *__restrict__ pos, const double *__restrict__ q,
-                         const int *__restrict__ itype, const int *__restrict__ perm,
-                         int n, double4 *__restrict__ rec, int *__restrict__ stype,
-                         const int *__restrict__ cond) {
-  NB_SKIP(cond);
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  int a = perm[s];
-  rec[s] = make_double4(pos[a], pos[n + a], pos[2 * n + a], q[a]);
-  if (stype) stype[s] = itype[a];
+struct Pack64Args {
+  const double *pos, *q; const int *itype, *perm; int n; double4 *rec; int *stype;
+  const int *cond;
+};
+static __device__ __forceinline__ void k_pack64_body(const Pack64Args &A, int bid) {
+  NB_SKIP(A.cond);
+  int s = bid * blockDim.x + threadIdx.x;
+  if (s >= A.n) return;
+  const int n = A.n, a = A.perm[s];
+  A.rec[s] = make_double4(A.pos[a], A.pos[n + a], A.pos[2 * n + a], A.q[a]);
+  if (A.stype) A.stype[s] = A.itype[a];
 }
+NB_KERNEL_PAIR(k_pack64, Pack64Args, 256)
 
 // ---------------------------------------------------------------------------
 // Verlet list build, one wavefront per (sorted) atom.  Each wave walks the
@@ -182,10 +214,10 @@ static __device__ __forceinline__ int wrapc(int v, int n) {
   return v < 0 ? v + n : (v >= n ? v - n : v);
 }
 
-__global__ __launch_bounds__(256) void k_build_list(ListArgs A) {
+static __device__ __forceinline__ void k_list_body(const ListArgs &A, int bid) {
   NB_SKIP(A.cond);
   const int lane = lane_id();
-  const int s = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int s = bid * 4 + (threadIdx.x >> 6);
   if (s >= A.n) return;
   const double4 ri = A.rec[s];
   const int ti = A.stype[s], ai = A.perm[s];
@@ -259,16 +291,19 @@ __global__ __launch_bounds__(256) void k_build_list(ListArgs A) {
     if (hte) atomicAdd(&A.flags[5], hte);
   }
 }
+NB_KERNEL_PAIR(k_list, ListArgs, 256)
 
 // nrefsh criterion (md_finter.c:61-73): any |r - r_build|^2 > skin^2/4.
-__global__ void k_rebuild_check(const double *__restrict__ pos,
-                                const double *__restrict__ posb, int n,
-                                double skin2q, int *__restrict__ flag) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double x = posb[i] - pos[i], y = posb[n + i] - pos[n + i], z = posb[2 * n + i] - pos[2 * n + i];
-  if (x * x + y * y + z * z > skin2q) atomicOr(flag, 1);
+struct CheckArgs { const double *pos, *posb; int n; double skin2q; int *flag; };
+static __device__ __forceinline__ void k_check_body(const CheckArgs &A, int bid) {
+  int i = bid * blockDim.x + threadIdx.x;
+  if (i >= A.n) return;
+  const int n = A.n;
+  double x = A.posb[i] - A.pos[i], y = A.posb[n + i] - A.pos[n + i];
+  double z = A.posb[2 * n + i] - A.pos[2 * n + i];
+  if (x * x + y * y + z * z > A.skin2q) atomicOr(A.flag, 1);
 }
+NB_KERNEL_PAIR(k_check, CheckArgs, 256)
 
 // ---------------------------------------------------------------------------
 // Pair forces, FP64.  One wavefront per sorted atom i walks i's full list,
@@ -277,51 +312,19 @@ __global__ void k_rebuild_check(const double *__restrict__ pos,
 // and accumulates F_i; the wave then reduces F_i with shuffles.  Energies and
 // virials count every pair twice and are halved at the end.  The tables
 // (interleaved {dV/r, switch, V} per r^2 node) are staged in LDS when they
-// fit.  Excluded charged pairs get the ecorr term here too.
+// fit.
+// IFLG 1: fl = full force, fs = switched force (when WANT_FS), energies and
+// virials; IFLG 0: fs = short-range force over the s lists, no energies
+// (cm3d computes them only at the long level).
 // ---------------------------------------------------------------------------
-struct PairArgs {
-  int n, ntypes, ninter, ntable, cap, tab_lds, ctab_off;
-  const uint32_t *nbr;
-  const int *nnb, *stype, *perm, *map;
-  const double4 *rec;
-  const InterPar *ip;
-  const double *vtab3, *ctab3;
-  double rmins_coul2, dx2coul, rmaxl_coul2, clast;
-  Box b;
-  double *fl, *fs;
-  double *epart;
-};
-
-// accumulator slots of the pair kernel (NB_NE_PAIR = 19 with the clamp count)
+// accumulator slots (NB_NE_PAIR = 19 with the clamp count)
 enum {
   E_SPOTL = 0, E_SPOTHL, E_SPRSL, E_SL0, E_SL1, E_SL2, E_SL4, E_SL5, E_SL8,
   E_SPOTE, E_SPOTHE, E_SPRSE, E_SE0, E_SE1, E_SE2, E_SE4, E_SE5, E_SE8, E_CLAMP
 };
 
-// Sum NACC per-thread accumulators over a 256-thread block (wave shuffles,
-// then the 4 wave totals in LDS, fixed order) and store them to out[0..NACC).
-template <int NACC>
-static __device__ __forceinline__ void block_reduce_store(const double *acc, double *out) {
-  __shared__ double red[4][NACC];
-  const int w = threadIdx.x >> 6, lane = lane_id();
-#pragma unroll
-  for (int k = 0; k < NACC; k++) {
-    double v = nb_wave_sum(acc[k]);
-    if (lane == 0) red[w][k] = v;
-  }
-  __syncthreads();
-  if (threadIdx.x < NACC) {
-    double v = red[0][threadIdx.x] + red[1][threadIdx.x];
-    v += red[2][threadIdx.x] + red[3][threadIdx.x];
-    out[threadIdx.x] = v;
-  }
-}
-
-// IFLG 1: fl = full force, fs = switched force (when WANT_FS), energies and
-// virials; IFLG 0: fs = short-range force over the s lists, no energies
-// (cm3d computes them only at the long level).
 template <int IFLG, bool WANT_FS>
-__global__ __launch_bounds__(256, 2) void k_pair64(PairArgs A) {
+static __device__ __forceinline__ void pair64_body(const PairArgs &A, int bid) {
   extern __shared__ double tabs[];
   const double *vt = A.vtab3, *ct = A.ctab3;
   if (A.tab_lds) {
@@ -336,11 +339,11 @@ __global__ __launch_bounds__(256, 2) void k_pair64(PairArgs A) {
 #pragma unroll
   for (int k = 0; k < NB_NE_PAIR; k++) acc[k] = 0.;
   const int lane = lane_id();
-  const int nw = gridDim.x * 4;
+  const int nw = A.nblocks * 4;
   constexpr unsigned fv = IFLG ? NB_FLAG_VT : NB_FLAG_VS;
   constexpr unsigned fe = IFLG ? NB_FLAG_ET : NB_FLAG_ES;
   const int ntm4 = A.ntable - 4;
-  for (int s = blockIdx.x * 4 + (threadIdx.x >> 6); s < A.n; s += nw) {
+  for (int s = bid * 4 + (threadIdx.x >> 6); s < A.n; s += nw) {
     const double4 ri = A.rec[s];
     const int ti = A.stype[s];
     const int nb = A.nnb[s];
@@ -421,27 +424,34 @@ __global__ __launch_bounds__(256, 2) void k_pair64(PairArgs A) {
       if (WANT_FS || !IFLG) { A.fs[ai] = sx; A.fs[n + ai] = sy; A.fs[2 * n + ai] = sz; }
     }
   }
-  block_reduce_store<NB_NE_PAIR>(acc, A.epart + (size_t)blockIdx.x * NB_NE_PAIR);
+  block_reduce_store<NB_NE_PAIR>(acc, A.epart + (size_t)bid * NB_NE_PAIR);
 }
+#define k_pair64_0t_body pair64_body<0, true>
+#define k_pair64_1t_body pair64_body<1, true>
+#define k_pair64_1f_body pair64_body<1, false>
+NB_KERNEL_PAIR(k_pair64_0t, PairArgs, 256)
+NB_KERNEL_PAIR(k_pair64_1t, PairArgs, 256)
+NB_KERNEL_PAIR(k_pair64_1f, PairArgs, 256)
 
 // ecorr (md_ewald.c:385-453): one thread per atom over its excluded charged
 // partners (both directions, so energies and virials are halved later), on
 // the caller's unwrapped positions with no minimum image.  Adds to fl.
-__global__ __launch_bounds__(256) void k_ecorr(const int *__restrict__ eoff,
-                                               const int *__restrict__ elist,
-                                               const double *__restrict__ pos,
-                                               const double *__restrict__ q, int n,
-                                               double kappa, double *__restrict__ fl,
-                                               double *__restrict__ epart) {
+struct EcorrArgs {
+  const int *eoff, *elist; const double *pos, *q; int n; double kappa; double *fl;
+  double *epart;
+};
+static __device__ __forceinline__ void k_ecorr_body(const EcorrArgs &A, int bid) {
   double acc[NB_NE_ECORR];
 #pragma unroll
   for (int k = 0; k < NB_NE_ECORR; k++) acc[k] = 0.;
-  int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a < n) {
-    const double spi = 1. / sqrt(M_PI), ka2 = kappa * kappa;
+  int a = bid * blockDim.x + threadIdx.x;
+  if (a < A.n) {
+    const int n = A.n;
+    const double spi = 1. / sqrt(M_PI), kappa = A.kappa, ka2 = kappa * kappa;
+    const double *pos = A.pos, *q = A.q;
     double fx = 0., fy = 0., fz = 0.;
-    for (int e = eoff[a]; e < eoff[a + 1]; e++) {
-      int b = elist[e];
+    for (int e = A.eoff[a]; e < A.eoff[a + 1]; e++) {
+      int b = A.elist[e];
       double dx = pos[a] - pos[b], dy = pos[n + a] - pos[n + b], dz = pos[2 * n + a] - pos[2 * n + b];
       double d2 = dx * dx + dy * dy + dz * dz, d = sqrt(d2);
       double qr = q[a] * q[b] / d, er = erf(d * kappa);
@@ -454,34 +464,36 @@ __global__ __launch_bounds__(256) void k_ecorr(const int *__restrict__ eoff,
       acc[8] += dv * dz * dx; acc[9] += dv * dz * dy; acc[10] += dv * dz * dz;
       fx += dx * dv; fy += dy * dv; fz += dz * dv;
     }
-    if (eoff[a + 1] > eoff[a]) { fl[a] += fx; fl[n + a] += fy; fl[2 * n + a] += fz; }
+    if (A.eoff[a + 1] > A.eoff[a]) { A.fl[a] += fx; A.fl[n + a] += fy; A.fl[2 * n + a] += fz; }
   }
-  block_reduce_store<NB_NE_ECORR>(acc, epart + (size_t)blockIdx.x * NB_NE_ECORR);
+  block_reduce_store<NB_NE_ECORR>(acc, A.epart + (size_t)bid * NB_NE_ECORR);
 }
+NB_KERNEL_PAIR(k_ecorr, EcorrArgs, 256)
 
 // ---------------------------------------------------------------------------
 // Reciprocal-space Ewald, FP64 throughout.
-// k_ew_charges: per charge, fractional coordinates s = h^-1 r wrapped to
+// k_ewchg: per charge, fractional coordinates s = h^-1 r wrapped to
 //   [-1/2,1/2) (exact for e^{2 pi i k.s} with integer k), stored as 2 pi s,
 //   the charge times sqrt(4 pi) (md_ewald.c:220-224), and e^{2 pi i s_c}, the
 //   c-axis step of cm3d's recursion (md_ewald.c:227-232, 273-278).
-// k_ew_sfac: one workgroup per (segment, charge chunk); each thread starts a
+// k_ewsfac: one workgroup per (segment, charge chunk); each thread starts a
 //   segment with one sincos and walks its <= 16 k vectors by the recursion,
 //   accumulating q e^{i k.r} over its charges; a block reduction writes the
 //   chunk's partial S(k).
-// k_ew_kreduce: per k, sum the chunk partials in fixed order, form preg,
-//   prep, |S|^2 and the energy / virial terms (md_ewald.c:247-302).
-// k_ew_force: per charge, walk all segments (staged in LDS) and add
+// k_ewkred: per k, sum the chunk partials in fixed order, form preg, prep,
+//   |S|^2 and the energy / virial terms (md_ewald.c:247-302).
+// k_ewforce: per charge, walk all segments (staged in LDS) and add
 //   q k (Re S' sin - Im S' cos) to fl (md_ewald.c:303-323).
 // ---------------------------------------------------------------------------
-__global__ void k_ew_charges(const double *__restrict__ pos, const double *__restrict__ q,
-                             const int *__restrict__ qidx, int nq, int n, Box b,
-                             double4 *__restrict__ chg, double2 *__restrict__ stp) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nq) return;
-  int a = qidx[c];
-  double x = pos[a], y = pos[n + a], z = pos[2 * n + a];
-  const double *hi = b.hi;
+struct EwChgArgs {
+  const double *pos, *q; const int *qidx; int nq, n; Box b; double4 *chg; double2 *stp;
+};
+static __device__ __forceinline__ void k_ewchg_body(const EwChgArgs &A, int bid) {
+  int c = bid * blockDim.x + threadIdx.x;
+  if (c >= A.nq) return;
+  const int n = A.n, a = A.qidx[c];
+  double x = A.pos[a], y = A.pos[n + a], z = A.pos[2 * n + a];
+  const double *hi = A.b.hi;
   double sa = x * hi[0] + y * hi[1] + z * hi[2];
   double sb = x * hi[3] + y * hi[4] + z * hi[5];
   double sc = x * hi[6] + y * hi[7] + z * hi[8];
@@ -489,26 +501,28 @@ __global__ void k_ew_charges(const double *__restrict__ pos, const double *__res
   sb -= rint(sb);
   sc -= rint(sc);
   const double tpi = 2. * M_PI;
-  chg[c] = make_double4(tpi * sa, tpi * sb, tpi * sc, q[a] * sqrt(4.0 * M_PI));
+  A.chg[c] = make_double4(tpi * sa, tpi * sb, tpi * sc, A.q[a] * sqrt(4.0 * M_PI));
   double sn, cs;
   sincos(tpi * sc, &sn, &cs);
-  stp[c] = make_double2(cs, sn);
+  A.stp[c] = make_double2(cs, sn);
 }
+NB_KERNEL_PAIR(k_ewchg, EwChgArgs, 256)
 
-__global__ __launch_bounds__(256) void k_ew_sfac(const EwSeg *__restrict__ seg,
-                                                 const double4 *__restrict__ chg,
-                                                 const double2 *__restrict__ stp,
-                                                 int nq, int chunk, int ktot,
-                                                 double2 *__restrict__ part) {
-  const EwSeg sg = seg[blockIdx.x];
-  const int c0 = blockIdx.y * chunk;
-  const int c1 = min(nq, c0 + chunk);
+struct EwSfacArgs {
+  const EwSeg *seg; const double4 *chg; const double2 *stp; int nq, chunk, ktot, nseg;
+  double2 *part;
+};
+static __device__ __forceinline__ void k_ewsfac_body(const EwSfacArgs &A, int bid) {
+  const int gseg = bid % A.nseg, gchunk = bid / A.nseg;
+  const EwSeg sg = A.seg[gseg];
+  const int c0 = gchunk * A.chunk;
+  const int c1 = min(A.nq, c0 + A.chunk);
   double re[NB_EW_SEG], im[NB_EW_SEG];
 #pragma unroll
   for (int m = 0; m < NB_EW_SEG; m++) { re[m] = 0.; im[m] = 0.; }
   for (int c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
-    double4 r = chg[c];
-    double2 st = stp[c];
+    double4 r = A.chg[c];
+    double2 st = A.stp[c];
     double th = sg.ka * r.x + sg.kb * r.y + sg.kc0 * r.z;
     double sn, cs;
     sincos(th, &sn, &cs);
@@ -533,38 +547,40 @@ __global__ __launch_bounds__(256) void k_ew_sfac(const EwSeg *__restrict__ seg,
   __syncthreads();
   if (threadIdx.x < sg.cnt) {
     int m = threadIdx.x;
-    double a = 0., bb = 0.;
-    for (int k = 0; k < (int)(blockDim.x >> 6); k++) { a += red[k][2 * m]; bb += red[k][2 * m + 1]; }
-    part[(size_t)blockIdx.y * ktot + sg.k0 + m] = make_double2(a, bb);
+    double a = red[0][2 * m] + red[1][2 * m], bb = red[0][2 * m + 1] + red[1][2 * m + 1];
+    a += red[2][2 * m] + red[3][2 * m];
+    bb += red[2][2 * m + 1] + red[3][2 * m + 1];
+    A.part[(size_t)gchunk * A.ktot + sg.k0 + m] = make_double2(a, bb);
   }
 }
+NB_KERNEL_PAIR(k_ewsfac, EwSfacArgs, 256)
 
 struct KData { double kx, ky, kz, cr, ci; };
 
-__global__ __launch_bounds__(256) void k_ew_kreduce(const int *__restrict__ kabc, int ktot,
-                                                    const double2 *__restrict__ part,
-                                                    int nchunk, Box b, double rvol,
-                                                    double falp2, KData *__restrict__ kd,
-                                                    double *__restrict__ epart) {
+struct EwKredArgs {
+  const int *kabc; int ktot; const double2 *part; int nchunk; Box b; double rvol, falp2;
+  KData *kd; double *epart;
+};
+static __device__ __forceinline__ void k_ewkred_body(const EwKredArgs &A, int bid) {
   double acc[NB_NE_EW];
 #pragma unroll
   for (int k = 0; k < NB_NE_EW; k++) acc[k] = 0.;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < ktot) {
+  int i = bid * blockDim.x + threadIdx.x;
+  if (i < A.ktot) {
     double sr = 0., si = 0.;
-    for (int c = 0; c < nchunk; c++) {
-      double2 p = part[(size_t)c * ktot + i];
+    for (int c = 0; c < A.nchunk; c++) {
+      double2 p = A.part[(size_t)c * A.ktot + i];
       sr += p.x;
       si += p.y;
     }
-    const double tpi = 2. * M_PI, *hi = b.hi;
-    double six = tpi * kabc[3 * i], siy = tpi * kabc[3 * i + 1], siz = tpi * kabc[3 * i + 2];
+    const double tpi = 2. * M_PI, *hi = A.b.hi;
+    double six = tpi * A.kabc[3 * i], siy = tpi * A.kabc[3 * i + 1], siz = tpi * A.kabc[3 * i + 2];
     double aka = six * hi[0] + siy * hi[3] + siz * hi[6];
     double akb = six * hi[1] + siy * hi[4] + siz * hi[7];
     double akc = six * hi[2] + siy * hi[5] + siz * hi[8];
     double g2 = aka * aka + akb * akb + akc * akc;
-    double preg = exp(-g2 * falp2) * rvol / g2;
-    double prep = -2. * preg * (g2 * falp2 + 1.) / g2;
+    double preg = exp(-g2 * A.falp2) * A.rvol / g2;
+    double prep = -2. * preg * (g2 * A.falp2 + 1.) / g2;
     double sm = sr * sr + si * si;
     acc[0] = preg * sm;
     acc[1] = prep * sm * g2;
@@ -575,32 +591,32 @@ __global__ __launch_bounds__(256) void k_ew_kreduce(const int *__restrict__ kabc
     d.kx = aka; d.ky = akb; d.kz = akc;
     d.cr = sr * 2. * preg;
     d.ci = si * 2. * preg;
-    kd[i] = d;
+    A.kd[i] = d;
   }
-  block_reduce_store<NB_NE_EW>(acc, epart + (size_t)blockIdx.x * NB_NE_EW);
+  block_reduce_store<NB_NE_EW>(acc, A.epart + (size_t)bid * NB_NE_EW);
 }
+NB_KERNEL_PAIR(k_ewkred, EwKredArgs, 256)
 
-#define NB_EW_TILE 64   // segments per LDS tile in k_ew_force
-__global__ __launch_bounds__(256) void k_ew_force(const EwSeg *__restrict__ seg, int nseg,
-                                                  const KData *__restrict__ kd,
-                                                  const double4 *__restrict__ chg,
-                                                  const double2 *__restrict__ stp,
-                                                  const int *__restrict__ qidx, int nq,
-                                                  int n, double *__restrict__ fl) {
+#define NB_EW_TILE 64   // segments per LDS tile in k_ewforce
+struct EwForceArgs {
+  const EwSeg *seg; int nseg; const KData *kd; const double4 *chg; const double2 *stp;
+  const int *qidx; int nq, n; double *fl;
+};
+static __device__ __forceinline__ void k_ewforce_body(const EwForceArgs &A, int bid) {
   __shared__ EwSeg ss[NB_EW_TILE];
   __shared__ KData sk[NB_EW_TILE * NB_EW_SEG];
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  int c = bid * blockDim.x + threadIdx.x;
   double4 r = make_double4(0., 0., 0., 0.);
   double2 st = make_double2(1., 0.);
-  if (c < nq) { r = chg[c]; st = stp[c]; }
+  if (c < A.nq) { r = A.chg[c]; st = A.stp[c]; }
   double fx = 0., fy = 0., fz = 0.;
-  for (int t0 = 0; t0 < nseg; t0 += NB_EW_TILE) {
-    int nt = min(NB_EW_TILE, nseg - t0);
+  for (int t0 = 0; t0 < A.nseg; t0 += NB_EW_TILE) {
+    int nt = min(NB_EW_TILE, A.nseg - t0);
     __syncthreads();
-    for (int t = threadIdx.x; t < nt; t += blockDim.x) ss[t] = seg[t0 + t];
+    for (int t = threadIdx.x; t < nt; t += blockDim.x) ss[t] = A.seg[t0 + t];
     __syncthreads();
     int kbase = ss[0].k0, kend = ss[nt - 1].k0 + ss[nt - 1].cnt;
-    for (int t = threadIdx.x; t < kend - kbase; t += blockDim.x) sk[t] = kd[kbase + t];
+    for (int t = threadIdx.x; t < kend - kbase; t += blockDim.x) sk[t] = A.kd[kbase + t];
     __syncthreads();
     for (int g = 0; g < nt; g++) {
       const EwSeg sg = ss[g];
@@ -619,28 +635,34 @@ __global__ __launch_bounds__(256) void k_ew_force(const EwSeg *__restrict__ seg,
       }
     }
   }
-  if (c < nq) {
-    int a = qidx[c];
-    fl[a] += r.w * fx;
-    fl[n + a] += r.w * fy;
-    fl[2 * n + a] += r.w * fz;
+  if (c < A.nq) {
+    const int a = A.qidx[c], n = A.n;
+    A.fl[a] += r.w * fx;
+    A.fl[n + a] += r.w * fy;
+    A.fl[2 * n + a] += r.w * fz;
   }
 }
+NB_KERNEL_PAIR(k_ewforce, EwForceArgs, 256)
 
 // Column sums of a [rows][COLS] partial table in one 256-thread block, with
 // a fixed assignment of rows to threads and a fixed tree (deterministic).
 template <int COLS>
-__global__ __launch_bounds__(256) void k_reduce_rows(const double *__restrict__ part,
-                                                     int rows, double *__restrict__ out) {
+static __device__ __forceinline__ void reduce_body(const RedArgs &A, int) {
   double acc[COLS];
 #pragma unroll
   for (int k = 0; k < COLS; k++) acc[k] = 0.;
-  for (int r = threadIdx.x; r < rows; r += blockDim.x) {
+  for (int r = threadIdx.x; r < A.rows; r += blockDim.x) {
 #pragma unroll
-    for (int k = 0; k < COLS; k++) acc[k] += part[(size_t)r * COLS + k];
+    for (int k = 0; k < COLS; k++) acc[k] += A.part[(size_t)r * COLS + k];
   }
-  block_reduce_store<COLS>(acc, out);
+  block_reduce_store<COLS>(acc, A.out);
 }
+#define k_red_pair_body reduce_body<NB_NE_PAIR>
+#define k_red_ecorr_body reduce_body<NB_NE_ECORR>
+#define k_red_ew_body reduce_body<NB_NE_EW>
+NB_KERNEL_PAIR(k_red_pair, RedArgs, 256)
+NB_KERNEL_PAIR(k_red_ecorr, RedArgs, 256)
+NB_KERNEL_PAIR(k_red_ew, RedArgs, 256)
 
 // ---------------------------------------------------------------------------
 // launchers
@@ -661,18 +683,22 @@ static int blocks(int n, int t) { return (n + t - 1) / t; }
 // No memset nodes: zeroing is done by kernels so the sequence can be captured.
 static int launch_build_impl(nb_ctx *c, const Box &b, hipStream_t s, const int *cond) {
   const int n = c->natoms;
-  k_zero_cond<<<blocks(c->ncells, 256), 256, 0, s>>>(c->d_ccount, c->ncells, cond);
-  k_zero_cond<<<1, 64, 0, s>>>(c->d_flags + 2, 4, cond);
-  k_zero_cond<<<1, 64, 0, s>>>(c->d_flags, 1, cond);
-  k_cell_assign<<<blocks(n, 256), 256, 0, s>>>(c->d_pos, n, b, c->ncx, c->ncy, c->ncz,
-                                                c->d_cell, c->d_ccount, cond);
-  k_cell_scan<<<1, 1024, 0, s>>>(c->d_ccount, c->ncells, c->d_cstart, c->d_cfill, cond);
-  k_cell_scatter<<<blocks(n, 256), 256, 0, s>>>(c->d_cell, n, c->d_cfill, c->d_tmp, cond);
-  // scell reuses d_cell: the sorted cell id of each sorted slot
-  k_cell_order<<<c->ncells, 256, 0, s>>>(c->d_ccount, c->d_cstart, c->d_tmp, c->d_perm,
-                                         c->d_iperm, c->d_cell, cond);
-  k_pack64<<<blocks(n, 256), 256, 0, s>>>(c->d_pos, c->d_qch, c->d_itype, c->d_perm, n,
-                                          (double4 *)c->d_rec64, c->d_stype, cond);
+  ZeroArgs z1 = {c->d_ccount, c->ncells, cond};
+  NB_GO(c, K_ZERO, k_zero, blocks(c->ncells, 256), 256, 0, z1, s);
+  ZeroArgs z2 = {c->d_flags, 6, cond};  // overflow max and half-pair counts
+  NB_GO(c, K_ZERO, k_zero, 1, 256, 0, z2, s);
+  CellArgs ca = {c->d_pos, n, b, c->ncx, c->ncy, c->ncz, c->d_cell, c->d_ccount, cond};
+  NB_GO(c, K_CELL, k_cell, blocks(n, 256), 256, 0, ca, s);
+  ScanArgs sa = {c->d_ccount, c->ncells, c->d_cstart, c->d_cfill, cond};
+  NB_GO(c, K_SCAN, k_scan, 1, 1024, 0, sa, s);
+  ScatArgs sc = {c->d_cell, n, c->d_cfill, c->d_tmp, cond};
+  NB_GO(c, K_SCAT, k_scat, blocks(n, 256), 256, 0, sc, s);
+  // the sorted cell id of each sorted slot overwrites d_cell
+  OrderArgs oa = {c->d_ccount, c->d_cstart, c->d_tmp, c->d_perm, c->d_iperm, c->d_cell, cond};
+  NB_GO(c, K_ORDER, k_order, c->ncells, 256, 0, oa, s);
+  Pack64Args pa = {c->d_pos, c->d_qch, c->d_itype, c->d_perm, n, (double4 *)c->d_rec64,
+                   c->d_stype, cond};
+  NB_GO(c, K_PACK64, k_pack64, blocks(n, 256), 256, 0, pa, s);
   ListArgs A;
   A.n = n; A.ntypes = c->ntypes; A.cap = c->cap;
   A.ncx = c->ncx; A.ncy = c->ncy; A.ncz = c->ncz;
@@ -681,10 +707,10 @@ static int launch_build_impl(nb_ctx *c, const Box &b, hipStream_t s, const int *
   A.xoff = c->d_xoff; A.xlist = c->d_xlist; A.map = c->d_map; A.ip = c->d_ip;
   A.rmaxs_skin2_e = c->rmaxs_skin2_e; A.rmaxl_skin2_e = c->rmaxl_skin2_e;
   A.b = b; A.nbr = c->d_nbr; A.nnb = c->d_nnb; A.flags = c->d_flags; A.cond = cond;
-  k_build_list<<<blocks(n, 4), 256, 0, s>>>(A);
-  k_copy_cond<<<blocks(3 * n, 256), 256, 0, s>>>(c->d_posb, c->d_pos, 3 * n, cond,
-                                                 c->d_flags + 6);
-  LCHK(hipGetLastError());
+  NB_GO(c, K_LIST, k_list, blocks(n, 4), 256, 0, A, s);
+  CopyArgs cp = {c->d_posb, c->d_pos, 3 * n, cond, c->d_flags + 6};
+  NB_GO(c, K_COPY, k_copy, blocks(3 * n, 256), 256, 0, cp, s);
+  if (!c->rec) LCHK(hipGetLastError());
   return NB_OK;
 }
 
@@ -697,10 +723,11 @@ int nb_launch_build_cond(nb_ctx *c, const Box &b, hipStream_t s) {
 }
 
 int nb_launch_rebuild_check(nb_ctx *c, const double *d_pos, hipStream_t s) {
-  k_zero_cond<<<1, 64, 0, s>>>(c->d_flags + 1, 1, nullptr);
-  k_rebuild_check<<<blocks(c->natoms, 256), 256, 0, s>>>(d_pos, c->d_posb, c->natoms,
-                                                         .25 * c->skin * c->skin, c->d_flags + 1);
-  LCHK(hipGetLastError());
+  ZeroArgs z = {c->d_flags + 1, 1, nullptr};
+  NB_GO(c, K_ZERO, k_zero, 1, 256, 0, z, s);
+  CheckArgs ch = {d_pos, c->d_posb, c->natoms, .25 * c->skin * c->skin, c->d_flags + 1};
+  NB_GO(c, K_CHECK, k_check, blocks(c->natoms, 256), 256, 0, ch, s);
+  if (!c->rec) LCHK(hipGetLastError());
   return NB_OK;
 }
 
@@ -713,12 +740,14 @@ int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_f
     int rc = nb_launch_pair_mixed(c, iflg, b, d_fs, d_fl, s);
     if (rc) return rc;
   } else {
-    k_pack64<<<blocks(n, 256), 256, 0, s>>>(c->d_pos, c->d_qch, c->d_itype, c->d_perm, n,
-                                            (double4 *)c->d_rec64, nullptr, nullptr);
+    Pack64Args pa = {c->d_pos, c->d_qch, c->d_itype, c->d_perm, n, (double4 *)c->d_rec64,
+                     nullptr, nullptr};
+    NB_GO(c, K_PACK64, k_pack64, blocks(n, 256), 256, 0, pa, s);
     nb_time_mark(c, 1, s);
     PairArgs P;
     P.n = n; P.ntypes = c->ntypes; P.ninter = c->ninter; P.ntable = c->ntable;
     P.cap = c->cap; P.tab_lds = c->tab_lds_bytes > 0; P.ctab_off = c->ctab_off;
+    P.nblocks = c->npair_blocks;
     P.nbr = c->d_nbr; P.nnb = c->d_nnb; P.stype = c->d_stype; P.perm = c->d_perm;
     P.map = c->d_map; P.rec = (const double4 *)c->d_rec64; P.ip = c->d_ip;
     P.vtab3 = c->d_vtab3; P.ctab3 = c->d_ctab3;
@@ -727,46 +756,72 @@ int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_f
     P.b = b; P.fl = d_fl; P.fs = d_fs; P.epart = c->d_epair;
     const int g = c->npair_blocks, lds = c->tab_lds_bytes;
     if (iflg == 0)
-      k_pair64<0, true><<<g, 256, lds, s>>>(P);
+      NB_GO(c, K_PAIR64_0T, k_pair64_0t, g, 256, lds, P, s);
     else if (d_fs)
-      k_pair64<1, true><<<g, 256, lds, s>>>(P);
+      NB_GO(c, K_PAIR64_1T, k_pair64_1t, g, 256, lds, P, s);
     else
-      k_pair64<1, false><<<g, 256, lds, s>>>(P);
-    LCHK(hipGetLastError());
+      NB_GO(c, K_PAIR64_1F, k_pair64_1f, g, 256, lds, P, s);
+    if (!c->rec) LCHK(hipGetLastError());
   }
   nb_time_mark(c, 2, s);
-  k_reduce_rows<NB_NE_PAIR><<<1, 256, 0, s>>>(c->d_epair, c->npair_blocks, c->d_eout);
-  if (iflg == 1 && c->iperd == 3 && c->necorr > 0) {
-    k_ecorr<<<c->necorr_blocks, 256, 0, s>>>(c->d_eoff, c->d_elist, c->d_pos, c->d_qch, n,
-                                             c->kappa, d_fl, c->d_eec);
-    k_reduce_rows<NB_NE_ECORR><<<1, 256, 0, s>>>(c->d_eec, c->necorr_blocks, c->d_eout + 20);
-  }
+  RedArgs rp = {c->d_epair, c->npair_blocks, c->d_eout};
+  NB_GO(c, K_RED_PAIR, k_red_pair, 1, 256, 0, rp, s);
+  const bool ec = iflg == 1 && c->iperd == 3 && c->necorr > 0;
+  EcorrArgs ea = {c->d_eoff, c->d_elist, c->d_pos, c->d_qch, n, c->kappa, d_fl, c->d_eec};
+  NB_GO(c, K_ECORR, k_ecorr, ec ? c->necorr_blocks : 0, 256, 0, ea, s);
+  RedArgs re = {c->d_eec, c->necorr_blocks, c->d_eout + 20};
+  NB_GO(c, K_RED_ECORR, k_red_ecorr, ec ? 1 : 0, 256, 0, re, s);
   nb_time_mark(c, 3, s);
-  if (iflg == 1 && c->iperd == 3 && c->ktot > 0 && c->ncharge > 0) {
-    const int nq = c->ncharge;
-    k_ew_charges<<<blocks(nq, 256), 256, 0, s>>>(c->d_pos, c->d_qch, c->d_qidx, nq, n, b,
-                                                 (double4 *)c->d_chg, (double2 *)c->d_stp);
-    int chunk = (nq + c->nchunk - 1) / c->nchunk;
-    dim3 gs(c->nseg, c->nchunk);
-    k_ew_sfac<<<gs, 256, 0, s>>>(c->d_seg, (const double4 *)c->d_chg, (const double2 *)c->d_stp,
-                                 nq, chunk, c->ktot, (double2 *)c->d_spart);
-    double det = b.h[0] * (b.h[4] * b.h[8] - b.h[5] * b.h[7]) +
-                 b.h[1] * (b.h[5] * b.h[6] - b.h[3] * b.h[8]) +
-                 b.h[2] * (b.h[3] * b.h[7] - b.h[4] * b.h[6]);
-    double falp2 = 1. / (4. * c->kappa * c->kappa);
-    k_ew_kreduce<<<c->new_blocks, 256, 0, s>>>(c->d_kabc, c->ktot, (const double2 *)c->d_spart,
-                                               c->nchunk, b, 1. / det, falp2,
-                                               (KData *)c->d_kd, c->d_eew);
-    k_reduce_rows<NB_NE_EW><<<1, 256, 0, s>>>(c->d_eew, c->new_blocks, c->d_eout + 32);
-    k_ew_force<<<blocks(nq, 256), 256, 0, s>>>(c->d_seg, c->nseg, (const KData *)c->d_kd,
-                                               (const double4 *)c->d_chg, (const double2 *)c->d_stp,
-                                               c->d_qidx, nq, n, d_fl);
-    LCHK(hipGetLastError());
-  }
+  const bool ew = iflg == 1 && c->iperd == 3 && c->ktot > 0 && c->ncharge > 0;
+  const int nq = c->ncharge;
+  EwChgArgs ch = {c->d_pos, c->d_qch, c->d_qidx, nq, n, b, (double4 *)c->d_chg,
+                  (double2 *)c->d_stp};
+  NB_GO(c, K_EWCHG, k_ewchg, ew ? blocks(nq, 256) : 0, 256, 0, ch, s);
+  EwSfacArgs sf = {c->d_seg, (const double4 *)c->d_chg, (const double2 *)c->d_stp, nq,
+                   (nq + c->nchunk - 1) / c->nchunk, c->ktot, c->nseg, (double2 *)c->d_spart};
+  NB_GO(c, K_EWSFAC, k_ewsfac, ew ? c->nseg * c->nchunk : 0, 256, 0, sf, s);
+  double det = b.h[0] * (b.h[4] * b.h[8] - b.h[5] * b.h[7]) +
+               b.h[1] * (b.h[5] * b.h[6] - b.h[3] * b.h[8]) +
+               b.h[2] * (b.h[3] * b.h[7] - b.h[4] * b.h[6]);
+  double falp2 = c->kappa > 0 ? 1. / (4. * c->kappa * c->kappa) : 0.;
+  EwKredArgs kr = {c->d_kabc, c->ktot, (const double2 *)c->d_spart, c->nchunk, b, 1. / det,
+                   falp2, (KData *)c->d_kd, c->d_eew};
+  NB_GO(c, K_EWKRED, k_ewkred, ew ? c->new_blocks : 0, 256, 0, kr, s);
+  RedArgs rw = {c->d_eew, c->new_blocks, c->d_eout + 32};
+  NB_GO(c, K_RED_EW, k_red_ew, ew ? 1 : 0, 256, 0, rw, s);
+  EwForceArgs ef = {c->d_seg, c->nseg, (const KData *)c->d_kd, (const double4 *)c->d_chg,
+                    (const double2 *)c->d_stp, c->d_qidx, nq, n, d_fl};
+  NB_GO(c, K_EWFORCE, k_ewforce, ew ? blocks(nq, 256) : 0, 256, 0, ef, s);
+  if (!c->rec) LCHK(hipGetLastError());
   nb_time_mark(c, 4, s);
   nb_time_mark(c, 5, s);
-  if (c->timing) c->trec++;
+  if (c->timing && !c->rec) c->trec++;
   return NB_OK;
 }
 
 size_t nb_kdata_bytes() { return sizeof(KData); }
+
+static struct RegisterKernels {
+  RegisterKernels() {
+    g_batch_fn[K_ZERO] = k_zero_bl;
+    g_batch_fn[K_COPY] = k_copy_bl;
+    g_batch_fn[K_CELL] = k_cell_bl;
+    g_batch_fn[K_SCAN] = k_scan_bl;
+    g_batch_fn[K_SCAT] = k_scat_bl;
+    g_batch_fn[K_ORDER] = k_order_bl;
+    g_batch_fn[K_PACK64] = k_pack64_bl;
+    g_batch_fn[K_LIST] = k_list_bl;
+    g_batch_fn[K_CHECK] = k_check_bl;
+    g_batch_fn[K_PAIR64_0T] = k_pair64_0t_bl;
+    g_batch_fn[K_PAIR64_1T] = k_pair64_1t_bl;
+    g_batch_fn[K_PAIR64_1F] = k_pair64_1f_bl;
+    g_batch_fn[K_RED_PAIR] = k_red_pair_bl;
+    g_batch_fn[K_RED_ECORR] = k_red_ecorr_bl;
+    g_batch_fn[K_RED_EW] = k_red_ew_bl;
+    g_batch_fn[K_ECORR] = k_ecorr_bl;
+    g_batch_fn[K_EWCHG] = k_ewchg_bl;
+    g_batch_fn[K_EWSFAC] = k_ewsfac_bl;
+    g_batch_fn[K_EWKRED] = k_ewkred_bl;
+    g_batch_fn[K_EWFORCE] = k_ewforce_bl;
+  }
+} g_register_kernels;

@@ -20,6 +20,21 @@ enum KernelId {
   K_MIX_1T, K_MIX_1F, K_MDINNER, K_MDKICK, K_COUNT
 };
 
+struct PairArgs {
+  int n, ntypes, ninter, ntable, cap, tab_lds, ctab_off, nblocks;
+  const uint32_t *nbr;
+  const int *nnb, *stype, *perm, *map;
+  const double4 *rec;
+  const InterPar *ip;
+  const double *vtab3, *ctab3;
+  double rmins_coul2, dx2coul, rmaxl_coul2, clast;
+  Box b;
+  double *fl, *fs;
+  double *epart;
+};
+
+struct RedArgs { const double *part; int rows; double *out; };
+
 struct LaunchRec {
   int kid, nblk, tpb, lds;
   std::vector<char> args;
@@ -31,9 +46,11 @@ extern BatchFn g_batch_fn[K_COUNT];
 
 #define NB_KERNEL_PAIR(name, ArgsT, TPB)                                             \
   __global__ __launch_bounds__(TPB) void name(ArgsT A) { name##_body(A, blockIdx.x); } \
-  __global__ __launch_bounds__(TPB) void name##_b(const ArgsT *AA, const int2 *bm) {  \
+  __global__ __launch_bounds__(TPB) void name##_b(const ArgsT *__restrict__ AA, const int2 *__restrict__ bm) {\
     const int2 m = bm[blockIdx.x];                                                   \
-    name##_body(AA[m.x], m.y);                                                       \
+    const int sys = __builtin_amdgcn_readfirstlane(m.x);                             \
+    const int lb = __builtin_amdgcn_readfirstlane(m.y);                              \
+    name##_body(AA[sys], lb);                                                        \
   }                                                                                  \
   static void name##_bl(const void *a, const int2 *bm, int nb, int tpb, int lds,     \
                         hipStream_t s) {                                             \

@@ -20,7 +20,7 @@
 
 #include <vector>
 
-#include "nb_internal.h"
+#include "nb_launch.h"
 
 #define MD_MAXA 8          // atoms per molecule held in LDS by k_md_inner
 #define MD_TPB 64          // molecules per workgroup
@@ -92,9 +92,9 @@ struct InnerArgs {
   const double *bk, *br0, *afk, *ath0;
 };
 
-__global__ __launch_bounds__(MD_TPB) void k_md_inner(InnerArgs A) {
+static __device__ __forceinline__ void k_mdinner_body(const InnerArgs &A, int bid) {
   __shared__ double sp[MD_TPB][MD_MAXA][3], sv[MD_TPB][MD_MAXA][3], sa[MD_TPB][MD_MAXA][3];
-  int m = blockIdx.x * MD_TPB + threadIdx.x;
+  int m = bid * MD_TPB + threadIdx.x;
   if (m >= A.nmol) return;
   const int n = A.n, a0 = A.moff[m], na = A.moff[m + 1] - a0;
   double (*p)[3] = sp[threadIdx.x], (*v)[3] = sv[threadIdx.x], (*aa)[3] = sa[threadIdx.x];
@@ -138,6 +138,7 @@ __global__ __launch_bounds__(MD_TPB) void k_md_inner(InnerArgs A) {
     }
   }
 }
+NB_KERNEL_PAIR(k_mdinner, InnerArgs, MD_TPB)
 
 // cm3d's force(...,2) at step_init: bonded accelerations at the current
 // positions (md_step.c:48, md_fcntr.c:79-105 with rmass).
@@ -157,12 +158,13 @@ __global__ __launch_bounds__(MD_TPB) void k_md_bonded(InnerArgs A) {
   }
 }
 
-__global__ void k_md_kick(double *__restrict__ vel, const double *__restrict__ fl,
-                          const double *__restrict__ rm, int n, double dter) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= 3 * n) return;
-  vel[i] += dter * (fl[i] * rm[i % n]);
+struct KickArgs { double *vel; const double *fl, *rm; int n; double dter; };
+static __device__ __forceinline__ void k_mdkick_body(const KickArgs &A, int bid) {
+  int i = bid * blockDim.x + threadIdx.x;
+  if (i >= 3 * A.n) return;
+  A.vel[i] += A.dter * (A.fl[i] * A.rm[i % A.n]);
 }
+NB_KERNEL_PAIR(k_mdkick, KickArgs, 256)
 
 // Kinetic, bond and bend energies (getvbond / getvbend harmonic cases,
 // md_fbond.c:165-173, md_fbend.c:318-338; p2m trace, md_fcntr.c:338-356).
@@ -390,20 +392,30 @@ extern "C" int nb_md_init(nb_md *md) {
   return NB_OK;
 }
 
-static int md_enqueue_step(nb_md *md, hipStream_t s) {
+int md_enqueue_step(nb_md *md, hipStream_t s) {
   nb_ctx *c = md->c;
   InnerArgs A = inner_args(md);
-  k_md_inner<<<(md->nmol + MD_TPB - 1) / MD_TPB, MD_TPB, 0, s>>>(A);
+  NB_GO(c, K_MDINNER, k_mdinner, (md->nmol + MD_TPB - 1) / MD_TPB, MD_TPB, 0, A, s);
   int rc = nb_launch_rebuild_check(c, c->d_pos, s);
   if (rc) return rc;
   rc = nb_launch_build_cond(c, md->box, s);
   if (rc) return rc;
   rc = nb_launch_force(c, 1, md->box, nullptr, c->d_fl, s);
   if (rc) return rc;
-  k_md_kick<<<(3 * md->n + 255) / 256, 256, 0, s>>>(md->d_vel, c->d_fl, md->d_rm, md->n, md->dter);
-  MCHK(hipGetLastError());
+  KickArgs K = {md->d_vel, c->d_fl, md->d_rm, md->n, md->dter};
+  NB_GO(c, K_MDKICK, k_mdkick, (3 * md->n + 255) / 256, 256, 0, K, s);
+  if (!c->rec) MCHK(hipGetLastError());
   return NB_OK;
 }
+
+nb_ctx *nb_md_ctx(nb_md *md) { return md->c; }
+
+static struct RegisterMD {
+  RegisterMD() {
+    g_batch_fn[K_MDINNER] = k_mdinner_bl;
+    g_batch_fn[K_MDKICK] = k_mdkick_bl;
+  }
+} g_register_md;
 
 extern "C" int nb_md_step(nb_md *md, int nsteps, void *stream, int use_graph) {
   if (!md || nsteps < 0) { nb_set_error("nb_md_step: bad argument"); return NB_ERR_ARG; }

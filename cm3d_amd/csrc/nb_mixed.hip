@@ -15,7 +15,7 @@
 //     the per-atom force does not depend on how pairs are spread over lanes
 //     or in which order partial sums meet;
 //   - energies and virials accumulate in FP64.
-#include "nb_internal.h"
+#include "nb_launch.h"
 
 #define NB_FX_SCALE 16777216.0  // 2^24 fixed-point units per K/A
 
@@ -28,14 +28,14 @@ static __device__ __forceinline__ long long wave_sum_ll(long long v) {
 }
 
 // Sorted-order records for the mixed kernel: {u_x, u_y, u_z, q as f32 bits}.
-__global__ void k_pack_mixed(const double *__restrict__ pos, const double *__restrict__ q,
-                             const int *__restrict__ perm, int n, Box b,
-                             uint4 *__restrict__ rec) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  int a = perm[s];
+struct PackMixArgs { const double *pos, *q; const int *perm; int n; Box b; uint4 *rec; };
+static __device__ __forceinline__ void k_packmix_body(const PackMixArgs &A, int bid) {
+  int s = bid * blockDim.x + threadIdx.x;
+  if (s >= A.n) return;
+  const int n = A.n, a = A.perm[s];
+  const double *pos = A.pos;
   double x = pos[a], y = pos[n + a], z = pos[2 * n + a];
-  const double *hi = b.hi;
+  const double *hi = A.b.hi;
   double sx = x * hi[0] + y * hi[1] + z * hi[2];
   double sy = x * hi[3] + y * hi[4] + z * hi[5];
   double sz = x * hi[6] + y * hi[7] + z * hi[8];
@@ -47,12 +47,13 @@ __global__ void k_pack_mixed(const double *__restrict__ pos, const double *__res
   r.x = (unsigned)(unsigned long long)(sx * two32);
   r.y = (unsigned)(unsigned long long)(sy * two32);
   r.z = (unsigned)(unsigned long long)(sz * two32);
-  r.w = __float_as_uint((float)q[a]);
-  rec[s] = r;
+  r.w = __float_as_uint((float)A.q[a]);
+  A.rec[s] = r;
 }
+NB_KERNEL_PAIR(k_packmix, PackMixArgs, 256)
 
 struct MixArgs {
-  int n, ntypes, ninter, ntable, cap, ctab_off, tab_lds;
+  int n, ntypes, ninter, ntable, cap, ctab_off, tab_lds, nblocks;
   const uint32_t *nbr;
   const int *nnb, *stype, *perm, *map;
   const uint4 *rec;
@@ -97,7 +98,7 @@ static __device__ __forceinline__ long long to_fix(double v) {
 }
 
 template <int IFLG, bool WANT_FS>
-__global__ __launch_bounds__(256) void k_pair_mixed(MixArgs A) {
+static __device__ __forceinline__ void pair_mixed_body(const MixArgs &A, int bid) {
   // FP32 pair arithmetic may contract to FMA (the FP64 path may not)
 #pragma clang fp contract(fast)
   extern __shared__ float tabsf[];
@@ -114,12 +115,12 @@ __global__ __launch_bounds__(256) void k_pair_mixed(MixArgs A) {
 #pragma unroll
   for (int k = 0; k < NB_NE_PAIR; k++) acc[k] = 0.;
   const int lane = lane_idm();
-  const int nw = gridDim.x * 4;
+  const int nw = A.nblocks * 4;
   constexpr unsigned fv = IFLG ? NB_FLAG_VT : NB_FLAG_VS;
   constexpr unsigned fe = IFLG ? NB_FLAG_ET : NB_FLAG_ES;
   const int ntm4 = A.ntable - 4;
   const float s32 = 2.3283064365386963e-10f;  // 2^-32
-  for (int s = blockIdx.x * 4 + (threadIdx.x >> 6); s < A.n; s += nw) {
+  for (int s = bid * 4 + (threadIdx.x >> 6); s < A.n; s += nw) {
     const uint4 ri = A.rec[s];
     const float qi = __uint_as_float(ri.w);
     const int ti = A.stype[s];
@@ -217,17 +218,24 @@ __global__ __launch_bounds__(256) void k_pair_mixed(MixArgs A) {
       }
     }
   }
-  block_reduce_store_m<NB_NE_PAIR>(acc, A.epart + (size_t)blockIdx.x * NB_NE_PAIR);
+  block_reduce_store_m<NB_NE_PAIR>(acc, A.epart + (size_t)bid * NB_NE_PAIR);
 }
+#define k_mix_0t_body pair_mixed_body<0, true>
+#define k_mix_1t_body pair_mixed_body<1, true>
+#define k_mix_1f_body pair_mixed_body<1, false>
+NB_KERNEL_PAIR(k_mix_0t, MixArgs, 256)
+NB_KERNEL_PAIR(k_mix_1t, MixArgs, 256)
+NB_KERNEL_PAIR(k_mix_1f, MixArgs, 256)
 
 int nb_launch_pair_mixed(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_fl,
                          hipStream_t s) {
   const int n = c->natoms;
-  k_pack_mixed<<<(n + 255) / 256, 256, 0, s>>>(c->d_pos, c->d_qch, c->d_perm, n, b,
-                                               (uint4 *)c->d_recf);
+  PackMixArgs pk = {c->d_pos, c->d_qch, c->d_perm, n, b, (uint4 *)c->d_recf};
+  NB_GO(c, K_PACKMIX, k_packmix, (n + 255) / 256, 256, 0, pk, s);
   MixArgs M;
   M.n = n; M.ntypes = c->ntypes; M.ninter = c->ninter; M.ntable = c->ntable;
   M.cap = c->cap; M.ctab_off = c->ctab_off; M.tab_lds = c->tab_lds_bytes_f > 0;
+  M.nblocks = c->npair_blocks;
   M.nbr = c->d_nbr; M.nnb = c->d_nnb; M.stype = c->d_stype; M.perm = c->d_perm;
   M.map = c->d_map; M.rec = (const uint4 *)c->d_recf; M.ip = c->d_ip;
   M.vtab3 = c->d_vtab3f; M.ctab3 = c->d_ctab3f;
@@ -237,15 +245,26 @@ int nb_launch_pair_mixed(nb_ctx *c, int iflg, const Box &b, double *d_fs, double
   M.fl = d_fl; M.fs = d_fs; M.epart = c->d_epair;
   const int g = c->npair_blocks, lds = c->tab_lds_bytes_f;
   if (iflg == 0)
-    k_pair_mixed<0, true><<<g, 256, lds, s>>>(M);
+    NB_GO(c, K_MIX_0T, k_mix_0t, g, 256, lds, M, s);
   else if (d_fs)
-    k_pair_mixed<1, true><<<g, 256, lds, s>>>(M);
+    NB_GO(c, K_MIX_1T, k_mix_1t, g, 256, lds, M, s);
   else
-    k_pair_mixed<1, false><<<g, 256, lds, s>>>(M);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) {
-    nb_set_error("k_pair_mixed: %s", hipGetErrorString(e));
-    return NB_ERR_HIP;
+    NB_GO(c, K_MIX_1F, k_mix_1f, g, 256, lds, M, s);
+  if (!c->rec) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+      nb_set_error("k_pair_mixed: %s", hipGetErrorString(e));
+      return NB_ERR_HIP;
+    }
   }
   return NB_OK;
 }
+
+static struct RegisterMixed {
+  RegisterMixed() {
+    g_batch_fn[K_PACKMIX] = k_packmix_bl;
+    g_batch_fn[K_MIX_0T] = k_mix_0t_bl;
+    g_batch_fn[K_MIX_1T] = k_mix_1t_bl;
+    g_batch_fn[K_MIX_1F] = k_mix_1f_bl;
+  }
+} g_register_mixed;

@@ -97,6 +97,7 @@ EXPORTS = ["nb_create", "nb_destroy", "nb_last_error", "nb_build_lists",
            "nb_build_lists_device", "nb_list_counts", "nb_get_pairs", "nb_timing",
            "nb_md_create", "nb_md_destroy", "nb_md_set_state", "nb_md_get_state",
            "nb_md_init", "nb_md_step", "nb_md_energies", "nb_md_rebuilds",
+           "nb_batch_create", "nb_batch_destroy", "nb_batch_step", "nb_batch_launches",
            "nb_table_pair", "nb_table_coul", "nb_kvectors", "nb_ewald_consts"]
 
 _LIB = None
@@ -138,6 +139,11 @@ def lib():
     L.nb_md_step.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int]
     L.nb_md_energies.argtypes = [C.c_void_p, C.c_double, C.POINTER(NBMDEnergy)]
     L.nb_md_rebuilds.argtypes = [C.c_void_p, _ip]
+    L.nb_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]
+    L.nb_batch_destroy.argtypes = [C.c_void_p]
+    L.nb_batch_destroy.restype = None
+    L.nb_batch_step.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int]
+    L.nb_batch_launches.argtypes = [C.c_void_p, _ip]
     L.nb_table_pair.argtypes = [C.c_int, _dp, C.c_double, C.c_double,
                                 C.c_double, C.c_double, C.c_double, C.c_double,
                                 C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
@@ -411,4 +417,37 @@ class NBMD:
     def rebuilds(self):
         c = C.c_int(0)
         check(lib().nb_md_rebuilds(self._h, C.byref(c)), "nb_md_rebuilds")
+        return c.value
+
+
+class NBBatch:
+    """Several NBMD systems advanced together: every kernel of the step is
+    launched once for all of them (nb_batch_*).  The systems must share the
+    precision; their sizes, boxes, force fields and cutoffs may differ."""
+
+    def __init__(self, mds):
+        self.mds = list(mds)
+        arr = (C.c_void_p * len(self.mds))(*[m._h for m in self.mds])
+        h = C.c_void_p()
+        check(lib().nb_batch_create(arr, len(self.mds), C.byref(h)), "nb_batch_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().nb_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, nsteps=1, stream=None, graph=True):
+        check(lib().nb_batch_step(self._h, int(nsteps), C.c_void_p(stream) if stream else None,
+                                  int(bool(graph))), "nb_batch_step")
+
+    def launches(self):
+        c = C.c_int(0)
+        check(lib().nb_batch_launches(self._h, C.byref(c)), "nb_batch_launches")
         return c.value

@@ -181,6 +181,19 @@ int nb_md_energies(nb_md *md, double vlrc, nb_md_energy *e);
 /* Lists built so far by the device-side rebuild test (synchronises). */
 int nb_md_rebuilds(nb_md *md, int *count);
 
+/* ---- batches: many independent systems stepped together ----
+ * No counterpart in cm3d (one system per process).  Each system is an
+ * nb_ctx + nb_md created as above (same precision); the batch records one
+ * step of each and launches every kernel once for all of them, so a step
+ * costs the same number of launches for 1 or 1000 systems.  Systems keep
+ * their own state and lists; nb_md_get_state / nb_md_energies still apply. */
+typedef struct nb_batch nb_batch;
+int nb_batch_create(nb_md **mds, int nsys, nb_batch **b);
+void nb_batch_destroy(nb_batch *b);
+int nb_batch_step(nb_batch *b, int nsteps, void *stream, int use_graph);
+/* Number of kernel launches one batched step issues. */
+int nb_batch_launches(nb_batch *b, int *nkernels);
+
 /* ---- set-up helpers (host only) ---- */
 #define NB_POT_LJ 0
 #define NB_POT_WILLIAMS 1
