@@ -1,23 +1,28 @@
-"""Benchmark of the MI355X nonbonded force path (one GPU).
+"""Benchmark of the MI355X nonbonded engine (one GPU).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
-Workload (the configuration the headline metric is quoted on): 4096 flexible
+Headline workload (the configuration the metric is quoted on): 4096 flexible
 3-site waters (12,288 atoms, cubic box 49.67 A, 1.0 g/cc), O-O Lennard-Jones
 and erfc real-space Coulomb tabulated with 500 nodes, rc 10 A, skin 1 A,
-Ewald alpha 0.35/A with kmax 9 (1535 half-space k vectors), FP64.
+Ewald alpha 0.35/A, kmax 9 (1535 half-space k vectors), harmonic O-H bonds
+and H-O-H bends, RESPA with 4 x 0.5 fs inner and 2 fs outer steps, 300 K,
+plain FP64.  A step is one outer MD step, entirely on the device and
+replayed from a hipGraph: the inner bonded RESPA loop, cm3d's nrefsh
+rebuild test (and the list build when it fires), force(...,1) -- LJ +
+real-space Coulomb pairs, ecorr, Ewald, energies and virials -- and the
+velocity kicks.  value = atom-steps/s of that single FP64 system.
 
-A step is one pass of the hot path over the batch: cm3d's nrefsh rebuild test
-on the current positions, a Verlet-list rebuild when it fires, and
-force(...,1) -- LJ + real-space Coulomb pair kernel, ecorr and the Ewald
-reciprocal sum with energies and virials.  Positions advance between steps
-by a seeded thermal displacement so the rebuild trigger fires as in MD.
-Inputs are resident in HBM when the timed region starts.
+Beside it, the batched form: 64 independent LJ + charge binary mixtures
+(N log-uniform in [500, 20000] from the seed, rc 9 A, kmax 7, mixed
+precision: FP32 pairs, 64-bit fixed-point forces, FP64 Ewald/energies)
+stepped by one set of batched graph-replayed launches; its aggregate
+atom-steps/s and its ratio to the FP64 single-system value are reported.
 
-One JSON line is printed: value = whole-job atom-steps/s, plus ns/day at the
-2 fs outer step, p50/p99 step latency, the roofline of the dominant kernel
-(measured with HIP events on the kernels' stream) and the CPU baseline: the
-clean-room C oracle timed on this host on a bounded sample.
+Also printed: ns/day, p50/p99 step latency, energy drift per ns for both
+forms, the roofline of the dominant kernel from HIP events on the kernels'
+stream, and the CPU baseline -- the clean-room C oracle of cm3d's CPU path
+timed on this host on a bounded sample of the same workload.
 """
 import argparse
 import json
@@ -33,26 +38,33 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+DT_PS = 0.002
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--nside", type=int, default=16, help="waters per box edge")
+    ap.add_argument("--nsys", type=int, default=64, help="systems in the batched form")
+    ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     return ap.parse_args()
 
 
 def cpu_baseline(sysd, budget_s):
     """The oracle (clean-room restatement of cm3d's CPU path, gcc -O2, one
-    core) on the same workload: list build + force(...,1) per step, timed
-    until about budget_s seconds have run."""
+    core) on the same system: list build + force(...,1) + bonded forces,
+    timed until about budget_s seconds have run."""
     from oracle import oracle as O
     pos = sysd["pos"]
     h = sysd["hmat"]
+    bonded = dict(ib=sysd["bond_i"], jb=sysd["bond_j"], bk=sysd["bond_k"], beq=sysd["bond_r0"],
+                  ia=sysd["bend_i"], ja=sysd["bend_j"], ka=sysd["bend_k"], ak=sysd["bend_fk"],
+                  aeq=sysd["bend_th0"])
     t0 = time.perf_counter()
     L = O.build_lists(sysd, pos, h)
     t_list = time.perf_counter() - t0
@@ -60,11 +72,87 @@ def cpu_baseline(sysd, budget_s):
     t0 = time.perf_counter()
     while True:
         O.force(sysd, L, pos, h, 1)
+        for _ in range(4):  # the four inner RESPA bonded evaluations
+            O.bonded_force(int(sysd["natoms"]), bonded, pos)
         nf += 1
         if time.perf_counter() - t0 > budget_s or nf >= 200:
             break
-    t_force = (time.perf_counter() - t0) / nf
-    return t_list, t_force, nf
+    return t_list, (time.perf_counter() - t0) / nf, nf
+
+
+def run_single(nb, S, a, torch):
+    s = S.water(nside=a.nside)
+    n = int(s["natoms"])
+    ctx = nb.NBContext(s, precision="fp64")
+    md = nb.NBMD(ctx, s, DT_PS, nres_intra=4)
+    md.set_state(s["pos"], s["vel"], s["hmat"])
+    md.init()
+    sp = torch.cuda.current_stream().cuda_stream
+    md.step(a.warmup, stream=sp, graph=True)
+    torch.cuda.synchronize()
+    e0 = md.energies()
+    r0 = md.rebuilds()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(a.steps + 1)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev[0].record()
+    for k in range(a.steps):
+        md.step(1, stream=sp, graph=True)
+        ev[k + 1].record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    lat = np.array([ev[k].elapsed_time(ev[k + 1]) for k in range(a.steps)])
+    e1 = md.energies()
+    nreb = md.rebuilds() - r0
+    # kernel durations: force(...,1) on the current state with HIP events on
+    # the stream the kernels run on
+    pos, _ = md.get_state()
+    ctx.timing_start()
+    for _ in range(20):
+        ctx.force(pos, s["hmat"], iflg=1)
+    tm = ctx.timing_read()
+    counts = ctx.list_counts()
+    md.close()
+    ctx.close()
+    return s, dict(n=n, wall=wall, lat=lat, e0=e0, e1=e1, rebuilds=nreb, tm=tm, counts=counts)
+
+
+def run_batch(nb, S, a, torch):
+    rng = np.random.default_rng(2026)
+    sizes = np.exp(rng.uniform(math.log(500), math.log(20000), a.nsys)).astype(int)
+    ctxs, mds, syss = [], [], []
+    for k, nat in enumerate(sizes):
+        s = S.mixture(int(nat), seed=100 + k, rc=9.0, kmax=7)
+        c = nb.NBContext(s, precision="mixed")
+        m = nb.NBMD(c, s, DT_PS, nres_intra=1)
+        m.set_state(s["pos"], s["vel"], s["hmat"])
+        m.init()
+        ctxs.append(c); mds.append(m); syss.append(s)
+    b = nb.NBBatch(mds)
+    sp = torch.cuda.current_stream().cuda_stream
+    b.step(10, stream=sp, graph=True)
+    torch.cuda.synchronize()
+    e0 = [m.energies()["ham"] for m in mds]
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(a.batch_steps + 1)]
+    t0 = time.perf_counter()
+    ev[0].record()
+    for k in range(a.batch_steps):
+        b.step(1, stream=sp, graph=True)
+        ev[k + 1].record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    lat = np.array([ev[k].elapsed_time(ev[k + 1]) for k in range(a.batch_steps)])
+    e1 = [m.energies()["ham"] for m in mds]
+    ntot = int(sum(int(s["natoms"]) for s in syss))
+    drift = float(np.mean([(y - x) / int(s["natoms"]) for x, y, s in zip(e0, e1, syss)]))
+    out = dict(natoms_total=ntot, nsys=a.nsys, wall=wall, lat=lat, launches=b.launches(),
+               drift_K_per_atom=drift, sizes=[int(x) for x in sizes])
+    b.close()
+    for m in mds:
+        m.close()
+    for c in ctxs:
+        c.close()
+    return out
 
 
 def main():
@@ -78,108 +166,85 @@ def main():
 
     if not torch.cuda.is_available():
         raise SystemExit("bench.py needs a GPU (libcm3d_nb.so has no CPU path)")
-    sysd = S.water(nside=a.nside)
-    n = int(sysd["natoms"])
-    ctx = nb.NBContext(sysd, precision="fp64")
-    dev = torch.device("cuda:0")
-    stream = torch.cuda.current_stream()
-    sp = stream.cuda_stream
-    hmat = sysd["hmat"]
-    pos0 = torch.tensor(sysd["pos"], dtype=torch.float64, device=dev)
-    pos = pos0.clone()
-    fl = torch.zeros(3 * n, dtype=torch.float64, device=dev)
-    # thermal-like displacement per step: rms 0.02 A per coordinate, seeded
-    gen = torch.Generator(device=dev)
-    gen.manual_seed(1234)
-    ctx.build_lists_device(pos.data_ptr(), hmat, sp)
-    half_skin2 = (0.5 * float(sysd["skin"])) ** 2
-    posb = pos.clone()
-
-    def step():
-        nonlocal posb
-        pos.add_(torch.randn(3 * n, dtype=torch.float64, device=dev, generator=gen), alpha=0.002)
-        d = (pos - posb).view(3, n)
-        if bool(((d * d).sum(0) > half_skin2).any()):
-            ctx.build_lists_device(pos.data_ptr(), hmat, sp)
-            posb = pos.clone()
-            step.rebuilds += 1
-        ctx.force_device(1, pos.data_ptr(), hmat, 0, fl.data_ptr(), stream=sp)
-    step.rebuilds = 0
-
-    for _ in range(a.warmup):
-        step()
-    torch.cuda.synchronize()
-    step.rebuilds = 0
-    ctx.timing_start()
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(a.steps + 1)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ev[0].record()
-    for k in range(a.steps):
-        step()
-        ev[k + 1].record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    lat = np.array([ev[k].elapsed_time(ev[k + 1]) for k in range(a.steps)])
-    tm = ctx.timing_read()
-    counts = ctx.list_counts()
-
-    ms_per_step = 1e3 * wall / a.steps
+    s, r = run_single(nb, S, a, torch)
+    n = r["n"]
+    wall = r["wall"]
     atom_steps = n * a.steps / wall
-    ns_day = (a.steps / wall) * 2e-6 * 86400.0
-    # pair kernel roofline: 4 B per listed half pair + 44 B per atom (the
-    # algorithmic minimum: one neighbour index per pair, per-atom records)
-    half_pairs = counts["nprte"] + counts["nprt"] - 0  # lists merged on device
-    pair_bytes = 4.0 * max(counts["nprte"], counts["nprt"]) + 44.0 * n
-    pair_ms = tm["pair"]
-    ktot = int(sysd["ktot"])
-    nq = int((sysd["qch"] != 0).sum())
+    ns_day = (a.steps / wall) * DT_PS * 1e-3 * 86400.0
+    tm = r["tm"]
+    counts = r["counts"]
+    # dominant kernel roofline.  Pair kernel: 4 B per listed half pair (the
+    # merged vdW/coulomb list holds the coulomb te pairs, which include the
+    # O-O vdW pairs) + 44 B per atom.  Ewald: 26 FLOP per (k, charge).
+    half_pairs = max(counts["nprte"], counts["nprt"])
+    pair_bytes = 4.0 * half_pairs + 44.0 * n
+    ktot = int(s["ktot"])
+    nq = int((s["qch"] != 0).sum())
     ew_flop = 26.0 * ktot * nq
-    ew_ms = tm["ewald"]
-    pair_gbs = pair_bytes / (pair_ms * 1e-3) / 1e9 if pair_ms > 0 else 0.0
-    ew_tfs = ew_flop / (ew_ms * 1e-3) / 1e12 if ew_ms > 0 else 0.0
-    if pair_ms >= ew_ms:
-        roof = {"kernel": "k_pair64", "bound": "hbm", "achieved": round(pair_gbs, 2),
+    pair_gbs = pair_bytes / (tm["pair"] * 1e-3) / 1e9
+    ew_tfs = ew_flop / (tm["ewald"] * 1e-3) / 1e12
+    if tm["pair"] >= tm["ewald"]:
+        roof = {"kernel": "k_pair64_1f", "bound": "hbm", "achieved": round(pair_gbs, 2),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(pair_gbs / HBM_PEAK_GBS, 5),
-                "traffic": None, "ms_per_launch": round(pair_ms, 5)}
+                "traffic": None, "ms_per_launch": round(tm["pair"], 5),
+                "bytes_per_launch": pair_bytes}
     else:
-        roof = {"kernel": "ewald (k_ew_*)", "bound": "valu-fp64", "achieved": round(ew_tfs, 3),
-                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(ew_tfs / FP64_PEAK_TFLOPS, 5), "traffic": None,
-                "ms_per_launch": round(ew_ms, 5)}
+        roof = {"kernel": "Ewald (k_ewchg+k_ewsfac+k_ewkred+k_red_ew+k_ewforce)",
+                "bound": "valu-fp64", "achieved": round(ew_tfs, 3), "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(ew_tfs / FP64_PEAK_TFLOPS, 5),
+                "traffic": None, "ms_per_launch": round(tm["ewald"], 5),
+                "flop_per_launch": ew_flop}
+    t_ns = a.steps * DT_PS * 1e-3
+    drift = (r["e1"]["ham"] - r["e0"]["ham"]) / n / t_ns
     out = {
         "metric": "atom-steps/s", "value": round(atom_steps, 1), "unit": "atom-steps/s",
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
-        "ms_per_step": round(ms_per_step, 5), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"water{n // 3} flexible 3-site, {n} atoms, rc 10 A skin 1 A, "
-                               f"Ewald alpha 0.35 kmax 9 ({ktot} k), FP64, nonbonded force path",
-                   "natoms": n, "ktot": ktot},
-        "ns_per_day": round(ns_day, 3),
-        "p50_step_ms": round(float(np.percentile(lat, 50)), 5),
-        "p99_step_ms": round(float(np.percentile(lat, 99)), 5),
-        "rebuilds": step.rebuilds,
+        "ms_per_step": round(1e3 * wall / a.steps, 5), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"water{n // 3}: {n}-atom flexible 3-site water, rc 10 A, skin 1 A, "
+                               f"Ewald alpha 0.35 kmax 9 ({ktot} k), RESPA 4x0.5/2 fs, FP64, "
+                               f"device-resident MD step (hipGraph)",
+                   "natoms": n, "ktot": ktot, "dt_fs": 2.0, "nres_intra": 4},
+        "ns_per_day": round(ns_day, 2),
+        "p50_step_ms": round(float(np.percentile(r["lat"], 50)), 5),
+        "p99_step_ms": round(float(np.percentile(r["lat"], 99)), 5),
+        "energy_drift_K_per_atom_per_ns": round(drift, 6),
+        "list_rebuilds": r["rebuilds"],
         "pairs_half": counts,
-        "phase_ms": {k: round(v, 5) for k, v in tm.items() if k != "calls"},
+        "force_phase_ms": {k: round(v, 5) for k, v in tm.items() if k != "calls"},
         "pair_interactions_per_s": round(counts["nprte"] * a.steps / wall, 1),
-        "ewald_tflops": round(ew_tfs, 4),
-        "pair_gbs": round(pair_gbs, 2),
         "roofline": roof,
+        "pair_kernel_gbs": round(pair_gbs, 2),
+        "ewald_tflops": round(ew_tfs, 4),
     }
+    if not a.no_batch:
+        b = run_batch(nb, S, a, torch)
+        bas = b["natoms_total"] * a.batch_steps / b["wall"]
+        bt_ns = a.batch_steps * DT_PS * 1e-3
+        out["batched"] = {
+            "workload": f"{b['nsys']} LJ+charge mixtures, N log-uniform 500-20000 "
+                        f"({b['natoms_total']} atoms), rc 9 A kmax 7, mixed precision, "
+                        f"one batched graph per step",
+            "atom_steps_per_s": round(bas, 1),
+            "vs_fp64_single": round(bas / atom_steps, 3),
+            "ms_per_step": round(1e3 * b["wall"] / a.batch_steps, 5),
+            "p50_step_ms": round(float(np.percentile(b["lat"], 50)), 5),
+            "p99_step_ms": round(float(np.percentile(b["lat"], 99)), 5),
+            "launches_per_step": b["launches"],
+            "energy_drift_K_per_atom_per_ns": round(b["drift_K_per_atom"] / bt_ns, 6),
+            "dtype": "fp32 pairs / int64 fixed-point forces / f64 Ewald+energies",
+        }
     if not a.no_cpu:
-        t_list, t_force, nf = cpu_baseline(sysd, a.cpu_seconds)
-        # per step: one force evaluation; the list build amortised over the
-        # measured GPU rebuild rate
-        rate = step.rebuilds / a.steps
-        t_step = t_force + rate * t_list
-        out["cpu_baseline"] = {"value": round(n / t_step, 1), "unit": "atom-steps/s",
-                               "cores": 1, "kind": "port",
-                               "sample": f"{nf} force(...,1) evaluations + 1 list build of the same "
-                                         f"{n}-atom system; step = force + list build x "
-                                         f"{rate:.3f} (measured rebuild rate)",
+        t_list, t_step, nf = cpu_baseline(s, a.cpu_seconds)
+        rate = r["rebuilds"] / a.steps
+        t = t_step + rate * t_list
+        out["cpu_baseline"] = {"value": round(n / t, 1), "unit": "atom-steps/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"{nf} steps of the same {n}-atom water: force(...,1) "
+                                         f"+ 4 bonded evaluations each, plus list builds at the "
+                                         f"GPU run's measured rate ({rate:.3f}/step)",
                                "host_cores_total": os.cpu_count()}
     print(json.dumps(out), flush=True)
-    ctx.close()
 
 
 if __name__ == "__main__":

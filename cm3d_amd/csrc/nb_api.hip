@@ -71,6 +71,7 @@ static Box make_box(const double h[9]) {
   b.hi[6] = (h[3] * h[7] - h[4] * h[6]) * rvol;
   b.hi[7] = (h[1] * h[6] - h[0] * h[7]) * rvol;
   b.hi[8] = (h[0] * h[4] - h[1] * h[3]) * rvol;
+  b.ortho = h[1] == 0. && h[2] == 0. && h[3] == 0. && h[5] == 0. && h[6] == 0. && h[7] == 0.;
   return b;
 }
 
@@ -265,7 +266,7 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
     }
   }
   c->nseg = (int)seg.size();
-  c->nchunk = std::max(1, (ncharge + 4095) / 4096);
+  c->nchunk = std::max(1, (ncharge + 2047) / 2048);
 
   // device uploads
   if ((rc = upload(&c->d_itype, p->itype, n))) return fail(rc);
@@ -299,7 +300,7 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
   if ((rc = dalloc((char **)&c->d_kd, nb_kdata_bytes() * std::max(c->ktot, 1)))) return fail(rc);
   c->npair_blocks = std::min((n + 3) / 4, 2048);
   c->new_blocks = std::max(1, (c->ktot + 255) / 256);
-  c->necorr_blocks = (n + 255) / 256;
+  c->necorr_blocks = (n + 63) / 64;
   if ((rc = dalloc(&c->d_epair, (size_t)c->npair_blocks * NB_NE_PAIR))) return fail(rc);
   if ((rc = dalloc(&c->d_eec, (size_t)c->necorr_blocks * NB_NE_ECORR))) return fail(rc);
   if ((rc = dalloc(&c->d_eew, (size_t)c->new_blocks * NB_NE_EW))) return fail(rc);

@@ -26,22 +26,31 @@ static __device__ __forceinline__ int mbcnt(unsigned long long m) {
                                    __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
 }
 
-// Sum NACC per-thread accumulators over a 256-thread block (wave shuffles,
-// then the 4 wave totals in LDS, fixed order) and store them to out[0..NACC).
-template <int NACC>
-static __device__ __forceinline__ void block_reduce_store(const double *acc, double *out) {
-  __shared__ double red[4][NACC];
+// Sum NACC per-thread accumulators over a block of NW wavefronts (wave
+// shuffles, then the wave totals in LDS in a fixed order) and store them to
+// out[k * stride], k < NACC -- column-major partial tables, one column per
+// accumulator, so the final reduction reads each column contiguously.
+template <int NACC, int NW = 4>
+static __device__ __forceinline__ void block_reduce_store(const double *acc, double *out,
+                                                          int stride) {
+  __shared__ double red[NW][NACC];
   const int w = threadIdx.x >> 6, lane = lane_id();
 #pragma unroll
   for (int k = 0; k < NACC; k++) {
     double v = nb_wave_sum(acc[k]);
-    if (lane == 0) red[w][k] = v;
+    if (NW == 1) {
+      if (lane == 0) out[(size_t)k * stride] = v;
+    } else if (lane == 0) {
+      red[w][k] = v;
+    }
   }
+  if (NW == 1) return;
   __syncthreads();
   if (threadIdx.x < NACC) {
-    double v = red[0][threadIdx.x] + red[1][threadIdx.x];
-    v += red[2][threadIdx.x] + red[3][threadIdx.x];
-    out[threadIdx.x] = v;
+    double v = red[0][threadIdx.x];
+#pragma unroll
+    for (int q = 1; q < NW; q++) v += red[q][threadIdx.x];
+    out[(size_t)threadIdx.x * stride] = v;
   }
 }
 
@@ -242,7 +251,7 @@ static __device__ __forceinline__ void k_list_body(const ListArgs &A, int bid) {
             int tj = A.stype[s2];
             aj = A.perm[s2];
             double dx = ri.x - rj.x, dy = ri.y - rj.y, dz = ri.z - rj.z;
-            nb_minimg(dx, dy, dz, A.b.h, A.b.hi);
+            nb_minimg_box(dx, dy, dz, A.b);
             double d2 = dx * dx + dy * dy + dz * dz;
             unsigned ex = 0;
             for (int e = x0; e < x1; e++) {
@@ -356,7 +365,7 @@ static __device__ __forceinline__ void pair64_body(const PairArgs &A, int bid) {
       int j = (int)(e & NB_IDX_MASK);
       double4 rj = A.rec[j];
       double dx = ri.x - rj.x, dy = ri.y - rj.y, dz = ri.z - rj.z;
-      nb_minimg(dx, dy, dz, A.b.h, A.b.hi);
+      nb_minimg_box(dx, dy, dz, A.b);
       double d2 = dx * dx + dy * dy + dz * dz;
       if (fl & fv) {
         int in = A.map[ti * A.ntypes + A.stype[j]];
@@ -424,7 +433,7 @@ static __device__ __forceinline__ void pair64_body(const PairArgs &A, int bid) {
       if (WANT_FS || !IFLG) { A.fs[ai] = sx; A.fs[n + ai] = sy; A.fs[2 * n + ai] = sz; }
     }
   }
-  block_reduce_store<NB_NE_PAIR>(acc, A.epart + (size_t)bid * NB_NE_PAIR);
+  block_reduce_store<NB_NE_PAIR>(acc, A.epart + bid, A.nblocks);
 }
 #define k_pair64_0t_body pair64_body<0, true>
 #define k_pair64_1t_body pair64_body<1, true>
@@ -437,7 +446,7 @@ NB_KERNEL_PAIR(k_pair64_1f, PairArgs, 256)
 // partners (both directions, so energies and virials are halved later), on
 // the caller's unwrapped positions with no minimum image.  Adds to fl.
 struct EcorrArgs {
-  const int *eoff, *elist; const double *pos, *q; int n; double kappa; double *fl;
+  const int *eoff, *elist; const double *pos, *q; int n, nblocks; double kappa; double *fl;
   double *epart;
 };
 static __device__ __forceinline__ void k_ecorr_body(const EcorrArgs &A, int bid) {
@@ -466,9 +475,9 @@ static __device__ __forceinline__ void k_ecorr_body(const EcorrArgs &A, int bid)
     }
     if (A.eoff[a + 1] > A.eoff[a]) { A.fl[a] += fx; A.fl[n + a] += fy; A.fl[2 * n + a] += fz; }
   }
-  block_reduce_store<NB_NE_ECORR>(acc, A.epart + (size_t)bid * NB_NE_ECORR);
+  block_reduce_store<NB_NE_ECORR, 1>(acc, A.epart + bid, A.nblocks);
 }
-NB_KERNEL_PAIR(k_ecorr, EcorrArgs, 256)
+NB_KERNEL_PAIR(k_ecorr, EcorrArgs, 64)
 
 // ---------------------------------------------------------------------------
 // Reciprocal-space Ewald, FP64 throughout.
@@ -558,7 +567,7 @@ NB_KERNEL_PAIR(k_ewsfac, EwSfacArgs, 256)
 struct KData { double kx, ky, kz, cr, ci; };
 
 struct EwKredArgs {
-  const int *kabc; int ktot; const double2 *part; int nchunk; Box b; double rvol, falp2;
+  const int *kabc; int ktot; const double2 *part; int nchunk, nblocks; Box b; double rvol, falp2;
   KData *kd; double *epart;
 };
 static __device__ __forceinline__ void k_ewkred_body(const EwKredArgs &A, int bid) {
@@ -593,49 +602,45 @@ static __device__ __forceinline__ void k_ewkred_body(const EwKredArgs &A, int bi
     d.ci = si * 2. * preg;
     A.kd[i] = d;
   }
-  block_reduce_store<NB_NE_EW>(acc, A.epart + (size_t)bid * NB_NE_EW);
+  block_reduce_store<NB_NE_EW>(acc, A.epart + bid, A.nblocks);
 }
 NB_KERNEL_PAIR(k_ewkred, EwKredArgs, 256)
 
-#define NB_EW_TILE 64   // segments per LDS tile in k_ewforce
+// One wavefront per charge; its lanes take the k segments round-robin, each
+// starting its segment with one sincos and walking it by the recursion, and
+// the wave sums the three force components at the end.  The k data (40 B
+// per k) are read by every wave and stay cache-resident.
 struct EwForceArgs {
   const EwSeg *seg; int nseg; const KData *kd; const double4 *chg; const double2 *stp;
   const int *qidx; int nq, n; double *fl;
 };
 static __device__ __forceinline__ void k_ewforce_body(const EwForceArgs &A, int bid) {
-  __shared__ EwSeg ss[NB_EW_TILE];
-  __shared__ KData sk[NB_EW_TILE * NB_EW_SEG];
-  int c = bid * blockDim.x + threadIdx.x;
-  double4 r = make_double4(0., 0., 0., 0.);
-  double2 st = make_double2(1., 0.);
-  if (c < A.nq) { r = A.chg[c]; st = A.stp[c]; }
+  const int c = bid * 4 + (threadIdx.x >> 6), lane = lane_id();
+  if (c >= A.nq) return;
+  const double4 r = A.chg[c];
+  const double2 st = A.stp[c];
   double fx = 0., fy = 0., fz = 0.;
-  for (int t0 = 0; t0 < A.nseg; t0 += NB_EW_TILE) {
-    int nt = min(NB_EW_TILE, A.nseg - t0);
-    __syncthreads();
-    for (int t = threadIdx.x; t < nt; t += blockDim.x) ss[t] = A.seg[t0 + t];
-    __syncthreads();
-    int kbase = ss[0].k0, kend = ss[nt - 1].k0 + ss[nt - 1].cnt;
-    for (int t = threadIdx.x; t < kend - kbase; t += blockDim.x) sk[t] = A.kd[kbase + t];
-    __syncthreads();
-    for (int g = 0; g < nt; g++) {
-      const EwSeg sg = ss[g];
-      double th = sg.ka * r.x + sg.kb * r.y + sg.kc0 * r.z;
-      double sn, cs;
-      sincos(th, &sn, &cs);
-      const KData *K = sk + (sg.k0 - kbase);
-      for (int m = 0; m < sg.cnt; m++) {
-        double tt = K[m].cr * sn - K[m].ci * cs;
-        fx += K[m].kx * tt;
-        fy += K[m].ky * tt;
-        fz += K[m].kz * tt;
-        double c2 = cs * st.x - sn * st.y;
-        sn = sn * st.x + cs * st.y;
-        cs = c2;
-      }
+  for (int g = lane; g < A.nseg; g += NB_WAVE) {
+    const EwSeg sg = A.seg[g];
+    double th = sg.ka * r.x + sg.kb * r.y + sg.kc0 * r.z;
+    double sn, cs;
+    sincos(th, &sn, &cs);
+    const KData *K = A.kd + sg.k0;
+    for (int m = 0; m < sg.cnt; m++) {
+      const KData kk = K[m];
+      double tt = kk.cr * sn - kk.ci * cs;
+      fx += kk.kx * tt;
+      fy += kk.ky * tt;
+      fz += kk.kz * tt;
+      double c2 = cs * st.x - sn * st.y;
+      sn = sn * st.x + cs * st.y;
+      cs = c2;
     }
   }
-  if (c < A.nq) {
+  fx = nb_wave_sum(fx);
+  fy = nb_wave_sum(fy);
+  fz = nb_wave_sum(fz);
+  if (lane == 0) {
     const int a = A.qidx[c], n = A.n;
     A.fl[a] += r.w * fx;
     A.fl[n + a] += r.w * fy;
@@ -644,22 +649,18 @@ static __device__ __forceinline__ void k_ewforce_body(const EwForceArgs &A, int 
 }
 NB_KERNEL_PAIR(k_ewforce, EwForceArgs, 256)
 
-// Column sums of a [rows][COLS] partial table in one 256-thread block, with
-// a fixed assignment of rows to threads and a fixed tree (deterministic).
-template <int COLS>
-static __device__ __forceinline__ void reduce_body(const RedArgs &A, int) {
-  double acc[COLS];
-#pragma unroll
-  for (int k = 0; k < COLS; k++) acc[k] = 0.;
-  for (int r = threadIdx.x; r < A.rows; r += blockDim.x) {
-#pragma unroll
-    for (int k = 0; k < COLS; k++) acc[k] += A.part[(size_t)r * COLS + k];
-  }
-  block_reduce_store<COLS>(acc, A.out);
+// Column sums of a column-major [COLS][rows] partial table: one block per
+// column, a fixed assignment of rows to threads and a fixed tree
+// (deterministic).
+static __device__ __forceinline__ void reduce_body(const RedArgs &A, int bid) {
+  const double *col = A.part + (size_t)bid * A.rows;
+  double acc = 0.;
+  for (int r = threadIdx.x; r < A.rows; r += blockDim.x) acc += col[r];
+  block_reduce_store<1>(&acc, A.out + bid, 1);
 }
-#define k_red_pair_body reduce_body<NB_NE_PAIR>
-#define k_red_ecorr_body reduce_body<NB_NE_ECORR>
-#define k_red_ew_body reduce_body<NB_NE_EW>
+#define k_red_pair_body reduce_body
+#define k_red_ecorr_body reduce_body
+#define k_red_ew_body reduce_body
 NB_KERNEL_PAIR(k_red_pair, RedArgs, 256)
 NB_KERNEL_PAIR(k_red_ecorr, RedArgs, 256)
 NB_KERNEL_PAIR(k_red_ew, RedArgs, 256)
@@ -765,12 +766,13 @@ int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_f
   }
   nb_time_mark(c, 2, s);
   RedArgs rp = {c->d_epair, c->npair_blocks, c->d_eout};
-  NB_GO(c, K_RED_PAIR, k_red_pair, 1, 256, 0, rp, s);
+  NB_GO(c, K_RED_PAIR, k_red_pair, NB_NE_PAIR, 256, 0, rp, s);
   const bool ec = iflg == 1 && c->iperd == 3 && c->necorr > 0;
-  EcorrArgs ea = {c->d_eoff, c->d_elist, c->d_pos, c->d_qch, n, c->kappa, d_fl, c->d_eec};
-  NB_GO(c, K_ECORR, k_ecorr, ec ? c->necorr_blocks : 0, 256, 0, ea, s);
+  EcorrArgs ea = {c->d_eoff, c->d_elist, c->d_pos, c->d_qch, n, c->necorr_blocks, c->kappa,
+                  d_fl, c->d_eec};
+  NB_GO(c, K_ECORR, k_ecorr, ec ? c->necorr_blocks : 0, 64, 0, ea, s);
   RedArgs re = {c->d_eec, c->necorr_blocks, c->d_eout + 20};
-  NB_GO(c, K_RED_ECORR, k_red_ecorr, ec ? 1 : 0, 256, 0, re, s);
+  NB_GO(c, K_RED_ECORR, k_red_ecorr, ec ? NB_NE_ECORR : 0, 256, 0, re, s);
   nb_time_mark(c, 3, s);
   const bool ew = iflg == 1 && c->iperd == 3 && c->ktot > 0 && c->ncharge > 0;
   const int nq = c->ncharge;
@@ -784,14 +786,14 @@ int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_f
                b.h[1] * (b.h[5] * b.h[6] - b.h[3] * b.h[8]) +
                b.h[2] * (b.h[3] * b.h[7] - b.h[4] * b.h[6]);
   double falp2 = c->kappa > 0 ? 1. / (4. * c->kappa * c->kappa) : 0.;
-  EwKredArgs kr = {c->d_kabc, c->ktot, (const double2 *)c->d_spart, c->nchunk, b, 1. / det,
-                   falp2, (KData *)c->d_kd, c->d_eew};
+  EwKredArgs kr = {c->d_kabc, c->ktot, (const double2 *)c->d_spart, c->nchunk, c->new_blocks,
+                   b, 1. / det, falp2, (KData *)c->d_kd, c->d_eew};
   NB_GO(c, K_EWKRED, k_ewkred, ew ? c->new_blocks : 0, 256, 0, kr, s);
   RedArgs rw = {c->d_eew, c->new_blocks, c->d_eout + 32};
-  NB_GO(c, K_RED_EW, k_red_ew, ew ? 1 : 0, 256, 0, rw, s);
+  NB_GO(c, K_RED_EW, k_red_ew, ew ? NB_NE_EW : 0, 256, 0, rw, s);
   EwForceArgs ef = {c->d_seg, c->nseg, (const KData *)c->d_kd, (const double4 *)c->d_chg,
                     (const double2 *)c->d_stp, c->d_qidx, nq, n, d_fl};
-  NB_GO(c, K_EWFORCE, k_ewforce, ew ? blocks(nq, 256) : 0, 256, 0, ef, s);
+  NB_GO(c, K_EWFORCE, k_ewforce, ew ? blocks(nq, 4) : 0, 256, 0, ef, s);
   if (!c->rec) LCHK(hipGetLastError());
   nb_time_mark(c, 4, s);
   nb_time_mark(c, 5, s);

@@ -76,8 +76,10 @@ static __device__ __forceinline__ float interp3f(const float *t, int k, int c, f
   return f0 + .5f * p * (f1 - fm1 + p * (fm1 - 2.f * f0 + f1));
 }
 
+// block totals of NACC accumulators stored column-major (out[k * stride])
 template <int NACC>
-static __device__ __forceinline__ void block_reduce_store_m(const double *acc, double *out) {
+static __device__ __forceinline__ void block_reduce_store_m(const double *acc, double *out,
+                                                            int stride) {
   __shared__ double red[4][NACC];
   const int w = threadIdx.x >> 6, lane = lane_idm();
 #pragma unroll
@@ -89,7 +91,7 @@ static __device__ __forceinline__ void block_reduce_store_m(const double *acc, d
   if (threadIdx.x < NACC) {
     double v = red[0][threadIdx.x] + red[1][threadIdx.x];
     v += red[2][threadIdx.x] + red[3][threadIdx.x];
-    out[threadIdx.x] = v;
+    out[(size_t)threadIdx.x * stride] = v;
   }
 }
 
@@ -218,7 +220,7 @@ static __device__ __forceinline__ void pair_mixed_body(const MixArgs &A, int bid
       }
     }
   }
-  block_reduce_store_m<NB_NE_PAIR>(acc, A.epart + (size_t)bid * NB_NE_PAIR);
+  block_reduce_store_m<NB_NE_PAIR>(acc, A.epart + bid, A.nblocks);
 }
 #define k_mix_0t_body pair_mixed_body<0, true>
 #define k_mix_1t_body pair_mixed_body<1, true>
