@@ -686,8 +686,12 @@ static int launch_build_impl(nb_ctx *c, const Box &b, hipStream_t s, const int *
   const int n = c->natoms;
   ZeroArgs z1 = {c->d_ccount, c->ncells, cond};
   NB_GO(c, K_ZERO, k_zero, blocks(c->ncells, 256), 256, 0, z1, s);
-  ZeroArgs z2 = {c->d_flags, 6, cond};  // overflow max and half-pair counts
+  // overflow max (flags[0]) and half-pair counts (flags[2..5]); flags[1] is
+  // the rebuild flag `cond` itself and must survive the whole build
+  ZeroArgs z2 = {c->d_flags + 2, 4, cond};
   NB_GO(c, K_ZERO, k_zero, 1, 256, 0, z2, s);
+  ZeroArgs z3 = {c->d_flags, 1, cond};
+  NB_GO(c, K_ZERO, k_zero, 1, 256, 0, z3, s);
   CellArgs ca = {c->d_pos, n, b, c->ncx, c->ncy, c->ncz, c->d_cell, c->d_ccount, cond};
   NB_GO(c, K_CELL, k_cell, blocks(n, 256), 256, 0, ca, s);
   ScanArgs sa = {c->d_ccount, c->ncells, c->d_cstart, c->d_cfill, cond};

@@ -55,3 +55,34 @@ def test_batch_equals_alone(prec):
         m.close()
     for c in ctxs:
         c.close()
+
+
+def test_batch_equals_alone_across_rebuilds():
+    """Long enough that every system rebuilds its lists on the device, each
+    at its own steps: the batched step still reproduces each system alone
+    bit for bit."""
+    from cm3d_amd import nb
+    nsteps = 160
+    syss = _systems()
+    print(f"[batch-long] fp64 {nsteps} steps", flush=True)
+    alone = [_run_alone(s, "fp64", nsteps, graph=True) for s in syss]
+    ctxs, mds = [], []
+    for s in syss:
+        c = nb.NBContext(s, precision="fp64")
+        m = nb.NBMD(c, s, 0.002, nres_intra=4 if "mol_start" in s else 1)
+        m.set_state(s["pos"], s["vel"], s["hmat"])
+        m.init()
+        ctxs.append(c)
+        mds.append(m)
+    b = nb.NBBatch(mds)
+    b.step(nsteps, graph=True)
+    for k, (m, ref) in enumerate(zip(mds, alone)):
+        pos, vel = m.get_state()
+        assert m.rebuilds() > 0, k
+        assert np.array_equal(pos, ref[0]), k
+        assert np.array_equal(vel, ref[1]), k
+    b.close()
+    for m in mds:
+        m.close()
+    for c in ctxs:
+        c.close()
