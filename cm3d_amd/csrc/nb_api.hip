@@ -85,8 +85,9 @@ extern "C" void nb_destroy(nb_ctx *c) {
   hipSetDevice(c->dev);
   void *ptrs[] = {c->d_itype, c->d_map, c->d_ip, c->d_vtab3, c->d_ctab3, c->d_vtab3f,
                   c->d_ctab3f, c->d_qch, c->d_xoff, c->d_xlist, c->d_eoff, c->d_elist,
-                  c->d_qidx, c->d_seg, c->d_kabc, c->d_pos, c->d_posb, c->d_fs, c->d_fl,
-                  c->d_perm, c->d_iperm, c->d_cell, c->d_ccount, c->d_cstart, c->d_cfill,
+                  c->d_qidx, c->d_seg, c->d_sseg, c->d_tseg, c->d_rtile, c->d_fpart, c->d_kabc, c->d_pos, c->d_posb, c->d_fs, c->d_fl,
+                  c->d_perm, c->d_iperm, c->d_cell, c->d_ccount, c->d_cstart, c->d_cfill, c->d_cwork,
+                  c->d_hcnt, c->d_pcut,
                   c->d_tmp, c->d_rec64, c->d_stype, c->d_nbr, c->d_nnb, c->d_flags,
                   c->d_chg, c->d_stp, c->d_spart, c->d_kd, c->d_epair, c->d_eew, c->d_eec, c->d_eout,
                   c->d_fxp, c->d_recf};
@@ -191,6 +192,16 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
     }
     c->clast = p->coultab[nt - 4];
   }
+  // list cutoffs per type pair: null pairs get -1 (no r^2 is below it)
+  std::vector<double> pcut(2 * (size_t)p->ntypes * p->ntypes, -1.);
+  for (int a = 0; a < p->ntypes; a++)
+    for (int b = 0; b < p->ntypes; b++) {
+      int k = p->map[a * p->ntypes + b];
+      if (k < 0) continue;
+      if (k >= p->ninter) { nb_set_error("nb_create: map entry %d out of range", k); return fail(NB_ERR_ARG); }
+      pcut[2 * ((size_t)a * p->ntypes + b)] = p->rmaxl_skin2[k];
+      pcut[2 * ((size_t)a * p->ntypes + b) + 1] = p->rmaxs_skin2[k];
+    }
   // list cutoff: the longest (r+skin) of any interaction / the coulomb list
   double r2 = 0.;
   for (int k = 0; k < p->ninter; k++) r2 = std::max(r2, p->rmaxl_skin2[k]);
@@ -256,22 +267,64 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
   std::vector<int> kabc(3 * (size_t)std::max(c->ktot, 1), 0);
   for (int k = 0; k < c->ktot; k++) {
     kabc[3 * k] = p->ka[k]; kabc[3 * k + 1] = p->kb[k]; kabc[3 * k + 2] = p->kc[k];
-    bool cont = !seg.empty() && seg.back().ka == p->ka[k] && seg.back().kb == p->kb[k] &&
-                seg.back().kc0 + seg.back().cnt == p->kc[k] && seg.back().cnt < NB_EW_SEG;
-    if (cont) {
+    bool row = !seg.empty() && seg.back().ka == p->ka[k] && seg.back().kb == p->kb[k] &&
+               seg.back().kc0 + seg.back().cnt == p->kc[k];
+    if (row && seg.back().cnt < NB_EW_SEG) {
       seg.back().cnt++;
     } else {
-      EwSeg s = {k, 1, p->ka[k], p->kb[k], p->kc[k], 0};
+      EwSeg s = {k, 1, p->ka[k], p->kb[k], p->kc[k], row ? 1 : 0};
       seg.push_back(s);
     }
   }
   c->nseg = (int)seg.size();
-  c->nchunk = std::max(1, (ncharge + 2047) / 2048);
+  // structure factor: one lane per segment, longest first; a workgroup of 4
+  // waves takes 64 segments and one chunk of charges; chunks sized for
+  // >= ~512 workgroups
+  std::vector<EwSeg> sseg(seg);
+  std::stable_sort(sseg.begin(), sseg.end(),
+                   [](const EwSeg &a, const EwSeg &b) { return a.cnt > b.cnt; });
+  c->nsw = std::max(1, (c->nseg + NB_WAVE - 1) / NB_WAVE);
+  {
+    int maxch = std::max(1, (ncharge + 31) / 32);  // >= 32 charges a workgroup
+    int nch = std::min(std::max(1, (512 + c->nsw - 1) / c->nsw), maxch);
+    c->qc = std::max(1, (ncharge + nch - 1) / nch);
+    c->nchunk = std::max(1, (ncharge + c->qc - 1) / c->qc);
+  }
+  // force: one lane per charge, a workgroup per (256 charges, tile range);
+  // tiles are runs of whole segments small enough for LDS and for enough
+  // ranges to give >= ~1024 workgroups; ranges balanced by k count
+  c->nqb = std::max(1, (ncharge + 255) / 256);
+  std::vector<int> tseg(1, 0), rtile(1, 0);
+  {
+    int nrt = std::min(std::max(1, (1024 + c->nqb - 1) / c->nqb), std::max(1, c->nseg));
+    int lim = std::min(NB_EW_KT, std::max(NB_EW_SEG, (c->ktot + nrt - 1) / std::max(1, nrt)));
+    int nk = 0;
+    std::vector<int> tk;  // k per tile
+    for (int g = 0; g < c->nseg; g++) {
+      if (nk > 0 && nk + seg[g].cnt > lim) { tseg.push_back(g); tk.push_back(nk); nk = 0; }
+      nk += seg[g].cnt;
+    }
+    tseg.push_back(c->nseg);
+    tk.push_back(nk);
+    c->ntile = (int)tseg.size() - 1;
+    c->nrange = std::min(nrt, c->ntile);
+    long long cum = 0;
+    for (int t = 0; t < c->ntile; t++) {
+      cum += tk[t];
+      int r = (int)rtile.size();
+      if (r < c->nrange && cum * c->nrange >= (long long)r * c->ktot && t + 1 < c->ntile &&
+          c->ntile - (t + 1) >= c->nrange - r)
+        rtile.push_back(t + 1);
+    }
+    rtile.push_back(c->ntile);
+    c->nrange = (int)rtile.size() - 1;
+  }
 
   // device uploads
   if ((rc = upload(&c->d_itype, p->itype, n))) return fail(rc);
   if ((rc = upload(&c->d_map, p->map, (size_t)p->ntypes * p->ntypes))) return fail(rc);
   if ((rc = upload(&c->d_ip, ip.data(), ni))) return fail(rc);
+  if ((rc = upload(&c->d_pcut, pcut.data(), pcut.size()))) return fail(rc);
   if ((rc = upload(&c->d_vtab3, vt3.data(), vt3.size()))) return fail(rc);
   if ((rc = upload(&c->d_ctab3, ct3.data(), ct3.size()))) return fail(rc);
   if ((rc = upload(&c->d_qch, p->qch, n))) return fail(rc);
@@ -281,6 +334,9 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
   if ((rc = upload(&c->d_elist, elist.data(), elist.size()))) return fail(rc);
   if ((rc = upload(&c->d_qidx, qidx.data(), qidx.size()))) return fail(rc);
   if ((rc = upload(&c->d_seg, seg.data(), seg.size()))) return fail(rc);
+  if ((rc = upload(&c->d_sseg, sseg.data(), sseg.size()))) return fail(rc);
+  if ((rc = upload(&c->d_tseg, tseg.data(), tseg.size()))) return fail(rc);
+  if ((rc = upload(&c->d_rtile, rtile.data(), rtile.size()))) return fail(rc);
   if ((rc = upload(&c->d_kabc, kabc.data(), kabc.size()))) return fail(rc);
   if ((rc = dalloc(&c->d_pos, 3 * (size_t)n))) return fail(rc);
   if ((rc = dalloc(&c->d_posb, 3 * (size_t)n))) return fail(rc);
@@ -293,13 +349,15 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
   if ((rc = dalloc(&c->d_rec64, 4 * (size_t)n))) return fail(rc);
   if ((rc = dalloc(&c->d_stype, n))) return fail(rc);
   if ((rc = dalloc(&c->d_nnb, n))) return fail(rc);
+  if ((rc = dalloc(&c->d_hcnt, 4 * (size_t)n))) return fail(rc);
   if ((rc = dalloc(&c->d_flags, 8))) return fail(rc);
   if ((rc = dalloc(&c->d_chg, 4 * (size_t)std::max(ncharge, 1)))) return fail(rc);
   if ((rc = dalloc(&c->d_stp, 2 * (size_t)std::max(ncharge, 1)))) return fail(rc);
   if ((rc = dalloc(&c->d_spart, 2 * (size_t)c->nchunk * std::max(c->ktot, 1)))) return fail(rc);
   if ((rc = dalloc((char **)&c->d_kd, nb_kdata_bytes() * std::max(c->ktot, 1)))) return fail(rc);
+  if ((rc = dalloc(&c->d_fpart, 3 * (size_t)c->nrange * std::max(ncharge, 1)))) return fail(rc);
   c->npair_blocks = std::min((n + 3) / 4, 2048);
-  c->new_blocks = std::max(1, (c->ktot + 255) / 256);
+  c->new_blocks = std::max(1, (c->ktot + NB_WAVE - 1) / NB_WAVE);
   c->necorr_blocks = (n + 63) / 64;
   if ((rc = dalloc(&c->d_epair, (size_t)c->npair_blocks * NB_NE_PAIR))) return fail(rc);
   if ((rc = dalloc(&c->d_eec, (size_t)c->necorr_blocks * NB_NE_ECORR))) return fail(rc);
@@ -332,8 +390,8 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
   return NB_OK;
 }
 
-// Grid of cells no narrower than rlist in any direction: the perpendicular
-// width of the box along a is V/|b x c| (box vectors are hmat's columns).
+// Grid of cells for the list stencil, per box axis: the perpendicular width
+// of the box along a is V/|b x c| (box vectors are hmat's columns).
 static void cell_grid(nb_ctx *c, const Box &b, int *nx, int *ny, int *nz) {
   const double *h = b.h;
   double a[3] = {h[0], h[3], h[6]}, bb[3] = {h[1], h[4], h[7]}, cc[3] = {h[2], h[5], h[8]};
@@ -347,14 +405,23 @@ static void cell_grid(nb_ctx *c, const Box &b, int *nx, int *ny, int *nz) {
   cross(bb, cc, w); widths[0] = vol / sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
   cross(cc, a, w);  widths[1] = vol / sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
   cross(a, bb, w);  widths[2] = vol / sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
-  int ncs[3];
+  int ncs[3], reach[3];
   for (int d = 0; d < 3; d++) {
     double r = c->rlist > 0 ? c->rlist * (1. + 1e-9) : widths[d];
-    int m = (int)floor(widths[d] / r);
-    if (m > 64) m = 64;
-    ncs[d] = m >= 3 ? m : 1;
+    // half-cutoff cells with a reach of 2 when at least 5 fit (so the 5
+    // cells around are distinct), else cutoff cells with a reach of 1 when
+    // at least 3 fit, else the whole axis as one cell
+    int m2 = (int)floor(2. * widths[d] / r), m1 = (int)floor(widths[d] / r);
+    if (m2 >= 5) {
+      ncs[d] = std::min(m2, 128); reach[d] = 2;
+    } else if (m1 >= 3) {
+      ncs[d] = m1; reach[d] = 1;
+    } else {
+      ncs[d] = 1; reach[d] = 0;
+    }
   }
   *nx = ncs[0]; *ny = ncs[1]; *nz = ncs[2];
+  c->crx = reach[0]; c->cry = reach[1]; c->crz = reach[2];
 }
 
 static int ensure_capacity(nb_ctx *c, const Box &b, int want_cap) {
@@ -374,10 +441,13 @@ int nb_build_now(nb_ctx *c, const double hmat[9], hipStream_t s) {
     if (c->d_ccount) hipFree(c->d_ccount);
     if (c->d_cstart) hipFree(c->d_cstart);
     if (c->d_cfill) hipFree(c->d_cfill);
-    c->d_ccount = c->d_cstart = c->d_cfill = nullptr;
+    if (c->d_cwork) hipFree(c->d_cwork);
+    c->d_ccount = c->d_cstart = c->d_cfill = c->d_cwork = nullptr;
     RC(dalloc(&c->d_ccount, c->ncells));
     RC(dalloc(&c->d_cstart, c->ncells));
     RC(dalloc(&c->d_cfill, c->ncells));
+    RC(dalloc(&c->d_cwork, c->ncells));
+    HCHK(hipMemsetAsync(c->d_cwork, 0, c->ncells * sizeof(int), s));
     c->cells_alloc = c->ncells;
   }
   if (c->cap == 0) {
@@ -529,8 +599,14 @@ extern "C" int nb_force_device(nb_ctx *c, int iflg, const double *d_pxyz, const 
 extern "C" int nb_list_counts(nb_ctx *c, int counts[4]) {
   if (!c || !counts) { nb_set_error("nb_list_counts: null argument"); return NB_ERR_ARG; }
   if (!c->have_lists) { nb_set_error("nb_list_counts: lists not built"); return NB_ERR_ARG; }
-  counts[0] = c->h_flags[2]; counts[1] = c->h_flags[3];
-  counts[2] = c->h_flags[4]; counts[3] = c->h_flags[5];
+  HCHK(hipSetDevice(c->dev));
+  HCHK(hipStreamSynchronize(c->stream));
+  std::vector<int> h(4 * (size_t)c->natoms);
+  HCHK(hipMemcpy(h.data(), c->d_hcnt, h.size() * sizeof(int), hipMemcpyDeviceToHost));
+  long long t[4] = {0, 0, 0, 0};
+  for (int i = 0; i < c->natoms; i++)
+    for (int k = 0; k < 4; k++) t[k] += h[4 * (size_t)i + k];
+  for (int k = 0; k < 4; k++) counts[k] = (int)t[k];
   return NB_OK;
 }
 

@@ -34,6 +34,7 @@ struct LaunchRec;
 #define NB_IDX_MASK 0x0FFFFFFFu
 #define NB_MAX_ATOMS (1 << 28)
 #define NB_EW_SEG 16     // k vectors per Ewald segment (one c-axis run)
+#define NB_EW_KT 512     // k vectors per LDS tile of the Ewald force pass
 #define NB_NE_PAIR 19    // pair energy/virial accumulators + clamp count
 #define NB_NE_ECORR 11   // ecorr energy/virial accumulators
 #define NB_NE_EW 11      // Ewald accumulators
@@ -43,7 +44,8 @@ struct InterPar {   // one vdW interaction (cm3d INTER arrays at index in)
 };
 
 struct EwSeg {      // run of <= NB_EW_SEG consecutive kc in one (ka,kb) row
-  int k0, cnt, ka, kb, kc0, pad;
+  int k0, cnt, ka, kb, kc0;
+  int cont;         // continues the previous segment's row (kc0 = its end)
 };
 
 // cm3d's -DLINUX anint: (int)(x +- 0.5), truncating (md_macros.h:31-52).
@@ -122,22 +124,27 @@ struct nb_ctx {
   int *d_xoff, *d_xlist;       // exclusions, both directions, flag bits
   int *d_eoff, *d_elist;       // ecorr partners, both directions
   int *d_qidx;                 // charge -> atom
-  EwSeg *d_seg;
+  EwSeg *d_seg;                // k order
+  EwSeg *d_sseg;               // sorted longest first (structure-factor lanes)
+  int *d_tseg, *d_rtile;       // force pass: tiles (segment bounds), ranges (tile bounds)
   int *d_kabc;                 // ka,kb,kc per k (3K)
   // state
   double *d_pos, *d_posb, *d_fs, *d_fl;
-  int *d_perm, *d_iperm, *d_cell, *d_ccount, *d_cstart, *d_cfill, *d_tmp;
+  int *d_perm, *d_iperm, *d_cell, *d_ccount, *d_cstart, *d_cfill, *d_cwork, *d_tmp;
+  int *d_hcnt;                 // per sorted atom: half-pair counts s,t,se,te
+  double *d_pcut;              // per type pair: (rmaxl+skin)^2, (rmaxs+skin)^2; -1 null
   double *d_rec64;
   int *d_stype;
   uint32_t *d_nbr;
   int *d_nnb;
   int cap;
   int *d_flags;                // [0] overflow max count, [1] rebuild flag,
-                               // [2..5] half-pair counts s,t,se,te,
-                               // [6] builds done by the conditional path
+                               // [6] builds done, [7] k_cell's ticket
   int ncx, ncy, ncz, ncells, cells_alloc;
-  double *d_chg, *d_stp, *d_spart, *d_kd;
-  int nchunk;
+  int crx, cry, crz;           // cell reach of the list stencil per axis (0..2)
+  double *d_chg, *d_stp, *d_spart, *d_kd, *d_fpart;
+  int nchunk, qc, nsw;         // structure factor: charge chunks, segment waves
+  int nrange, nqb, ntile;      // force: tile ranges, charge workgroups, tiles
   double *d_epair, *d_eew, *d_eec, *d_eout;
   int npair_blocks, new_blocks, necorr_blocks;
   long long *d_fxp;            // fixed-point force accumulators (mixed)

@@ -1,7 +1,7 @@
 // nb_kernels.hip -- gfx950 kernels of the nonbonded path (FP64 form) and
 // their launchers.
 //
-//   list build   k_cell -> k_scan -> k_scat -> k_order -> k_pack64 -> k_list
+//   list build   k_cell -> k_scat -> k_order -> k_list
 //                (cm3d get_ngbr / get_ngbr_e, energy/control/ngbr_list.c:297,
 //                 elec_ngbr.c:220, set_link.c:203)
 //   pair forces  k_pair64_* (force_npol md_fnopol.c:46, force_e finter_e.c:42),
@@ -16,6 +16,7 @@
 #include <stdio.h>
 
 #include "nb_launch.h"
+#include "nb_sincos.h"
 
 BatchFn g_batch_fn[K_COUNT];
 
@@ -70,81 +71,84 @@ static __device__ __forceinline__ void k_zero_body(const ZeroArgs &A, int bid) {
 }
 NB_KERNEL_PAIR(k_zero, ZeroArgs, 256)
 
-struct CopyArgs { double *dst; const double *src; int n; const int *cond; int *counter; };
-static __device__ __forceinline__ void k_copy_body(const CopyArgs &A, int bid) {
-  NB_SKIP(A.cond);
-  int i = bid * blockDim.x + threadIdx.x;
-  if (i < A.n) A.dst[i] = A.src[i];
-  if (i == 0 && A.counter) atomicAdd(A.counter, 1);
-}
-NB_KERNEL_PAIR(k_copy, CopyArgs, 256)
 
 // ---------------------------------------------------------------------------
-// Cell sort.  Cells tile the box in fractional coordinates with widths no
-// smaller than the longest list cutoff, so every pair within it under cm3d's
-// minimum image lies in the 27 (or fewer, for thin boxes) neighbouring cells.
-// Binning follows get_lnklist (set_link.c:218-236): wrap s -= anint(s), then
-// (int)((s+.5)*n) clamped.
+// Cell sort.  Cells tile the box in fractional coordinates; along each box
+// vector they are either no narrower than half the longest list cutoff
+// (reach 2: the 5 cells around a cell's own), no narrower than the cutoff
+// (reach 1) or one cell for the whole box (reach 0), so every pair within
+// the cutoff under cm3d's minimum image lies in the (2rx+1)(2ry+1)(2rz+1)
+// cells around.  Binning follows get_lnklist (set_link.c:218-236): wrap
+// s -= anint(s), then (int)((s+.5)*n) clamped.
+//   k_cell   cell of each atom, counts by atomics; the last workgroup to
+//            finish scans the counts into cell starts (and clears the
+//            atomics' array and its own ticket for the next build)
+//   k_scat   scatter atom indices into their cell's slots (arbitrary order)
+//   k_order  per cell: order its atoms by index (deterministic sorted
+//            order), then write the sorted records {x,y,z,q}, types, the
+//            permutation both ways, and the build positions (r_build)
+//   k_list   the Verlet lists
 // ---------------------------------------------------------------------------
 struct CellArgs {
-  const double *pos; int n; Box b; int ncx, ncy, ncz; int *cell; int *count; const int *cond;
+  const double *pos; int n, nblocks, ncells; Box b; int ncx, ncy, ncz;
+  int *cell, *work, *count, *start, *fill, *flags; const int *cond;
 };
 static __device__ __forceinline__ void k_cell_body(const CellArgs &A, int bid) {
   NB_SKIP(A.cond);
-  int i = bid * blockDim.x + threadIdx.x;
-  if (i >= A.n) return;
-  const int n = A.n;
-  double x = A.pos[i], y = A.pos[n + i], z = A.pos[2 * n + i];
-  const double *hi = A.b.hi;
-  double sx = x * hi[0] + y * hi[1] + z * hi[2];
-  double sy = x * hi[3] + y * hi[4] + z * hi[5];
-  double sz = x * hi[6] + y * hi[7] + z * hi[8];
-  sx -= nb_anint(sx);
-  sy -= nb_anint(sy);
-  sz -= nb_anint(sz);
-  int a = (int)((sx + .5) * A.ncx), c = (int)((sy + .5) * A.ncy), e = (int)((sz + .5) * A.ncz);
-  a = a < 0 ? 0 : (a > A.ncx - 1 ? A.ncx - 1 : a);
-  c = c < 0 ? 0 : (c > A.ncy - 1 ? A.ncy - 1 : c);
-  e = e < 0 ? 0 : (e > A.ncz - 1 ? A.ncz - 1 : e);
-  int id = (a * A.ncy + c) * A.ncz + e;
-  A.cell[i] = id;
-  atomicAdd(&A.count[id], 1);
-}
-NB_KERNEL_PAIR(k_cell, CellArgs, 256)
-
-// Exclusive scan of the cell counts in one workgroup of 1024 threads.
-struct ScanArgs { const int *count; int ncells; int *start; int *fill; const int *cond; };
-static __device__ __forceinline__ void k_scan_body(const ScanArgs &A, int) {
-  NB_SKIP(A.cond);
-  __shared__ int wsum[16];
-  __shared__ int carry;
+  __shared__ int wsum[4];
+  __shared__ int last, carry;
+  const int i = bid * blockDim.x + threadIdx.x;
+  if (bid == 0 && threadIdx.x == 0) A.flags[0] = 0;  // overflow max of this build
+  if (i < A.n) {
+    const int n = A.n;
+    double x = A.pos[i], y = A.pos[n + i], z = A.pos[2 * n + i];
+    const double *hi = A.b.hi;
+    double sx = x * hi[0] + y * hi[1] + z * hi[2];
+    double sy = x * hi[3] + y * hi[4] + z * hi[5];
+    double sz = x * hi[6] + y * hi[7] + z * hi[8];
+    sx -= nb_anint(sx);
+    sy -= nb_anint(sy);
+    sz -= nb_anint(sz);
+    int a = (int)((sx + .5) * A.ncx), c = (int)((sy + .5) * A.ncy), e = (int)((sz + .5) * A.ncz);
+    a = a < 0 ? 0 : (a > A.ncx - 1 ? A.ncx - 1 : a);
+    c = c < 0 ? 0 : (c > A.ncy - 1 ? A.ncy - 1 : c);
+    e = e < 0 ? 0 : (e > A.ncz - 1 ? A.ncz - 1 : e);
+    const int id = (a * A.ncy + c) * A.ncz + e;
+    A.cell[i] = id;
+    atomicAdd(&A.work[id], 1);
+  }
+  // last workgroup: exclusive scan of the counts (flags[7] is the ticket)
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) last = atomicAdd(&A.flags[7], 1) == A.nblocks - 1;
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
   if (threadIdx.x == 0) carry = 0;
   __syncthreads();
-  for (int base = 0; base < A.ncells; base += 1024) {
-    int i = base + threadIdx.x;
-    int v = i < A.ncells ? A.count[i] : 0;
+  const int lane = lane_id(), w = threadIdx.x >> 6;
+  for (int base = 0; base < A.ncells; base += blockDim.x) {
+    const int k = base + threadIdx.x;
+    const int v = k < A.ncells ? __hip_atomic_load(&A.work[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
     int x = v;
 #pragma unroll
     for (int o = 1; o < NB_WAVE; o <<= 1) {
       int y = __shfl_up(x, o, NB_WAVE);
-      if (lane_id() >= o) x += y;
+      if (lane >= o) x += y;
     }
-    int w = threadIdx.x >> 6;
-    if (lane_id() == NB_WAVE - 1) wsum[w] = x;
+    if (lane == NB_WAVE - 1) wsum[w] = x;
     __syncthreads();
-    if (threadIdx.x == 0) {
-      int acc = 0;
-      for (int k = 0; k < 16; k++) { int t = wsum[k]; wsum[k] = acc; acc += t; }
-    }
+    int pre = carry;
+    for (int q = 0; q < w; q++) pre += wsum[q];
+    const int excl = pre + x - v;
+    if (k < A.ncells) { A.start[k] = excl; A.fill[k] = excl; A.count[k] = v; A.work[k] = 0; }
     __syncthreads();
-    int excl = carry + wsum[w] + x - v;
-    if (i < A.ncells) { A.start[i] = excl; A.fill[i] = excl; }
-    __syncthreads();
-    if (threadIdx.x == 1023) carry = excl + v;
+    if (threadIdx.x == blockDim.x - 1) carry = excl + v;
     __syncthreads();
   }
+  if (threadIdx.x == 0) A.flags[7] = 0;
 }
-NB_KERNEL_PAIR(k_scan, ScanArgs, 1024)
+NB_KERNEL_PAIR(k_cell, CellArgs, 256)
 
 struct ScatArgs { const int *cell; int n; int *fill; int *tmp; const int *cond; };
 static __device__ __forceinline__ void k_scat_body(const ScatArgs &A, int bid) {
@@ -161,61 +165,71 @@ NB_KERNEL_PAIR(k_scat, ScatArgs, 256)
 // the cell.  Makes the sorted order, and so every list, deterministic.
 #define NB_CELL_LDS 4096
 struct OrderArgs {
-  const int *count, *start, *tmp; int *perm, *iperm, *scell; const int *cond;
+  const int *count, *start, *tmp; int *perm, *iperm, *scell;
+  const double *pos, *q; const int *itype; double4 *rec; int *stype; double *posb; int n;
+  int *builds; const int *cond;
 };
 static __device__ __forceinline__ void k_order_body(const OrderArgs &A, int bid) {
   NB_SKIP(A.cond);
   __shared__ int seg[NB_CELL_LDS];
-  int c = bid, n = A.count[c], st = A.start[c];
+  if (bid == 0 && threadIdx.x == 0) atomicAdd(A.builds, 1);
+  const int c = bid, cnt = A.count[c], st = A.start[c], n = A.n;
   const int *src = A.tmp + st;
-  if (n <= NB_CELL_LDS) {
-    for (int t = threadIdx.x; t < n; t += blockDim.x) seg[t] = src[t];
+  if (cnt <= NB_CELL_LDS) {
+    for (int t = threadIdx.x; t < cnt; t += blockDim.x) seg[t] = src[t];
     __syncthreads();
     src = seg;
   }
-  for (int t = threadIdx.x; t < n; t += blockDim.x) {
-    int a = src[t], r = 0;
-    for (int u = 0; u < n; u++) r += src[u] < a;
-    A.perm[st + r] = a;
-    A.iperm[a] = st + r;
-    A.scell[st + r] = c;
+  for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
+    const int a = src[t];
+    int r = 0;
+    for (int u = 0; u < cnt; u++) r += src[u] < a;
+    const int sl = st + r;
+    A.perm[sl] = a;
+    A.iperm[a] = sl;
+    A.scell[sl] = c;
+    const double x = A.pos[a], y = A.pos[n + a], z = A.pos[2 * n + a];
+    A.rec[sl] = make_double4(x, y, z, A.q[a]);
+    A.stype[sl] = A.itype[a];
+    A.posb[a] = x; A.posb[n + a] = y; A.posb[2 * n + a] = z;
   }
 }
 NB_KERNEL_PAIR(k_order, OrderArgs, 256)
 
 // Sorted-order records refreshed at every evaluation: {x,y,z,q} in FP64.
 struct Pack64Args {
-  const double *pos, *q; const int *itype, *perm; int n; double4 *rec; int *stype;
-  const int *cond;
+  const double *pos, *q; const int *perm; int n; double4 *rec;
 };
 static __device__ __forceinline__ void k_pack64_body(const Pack64Args &A, int bid) {
-  NB_SKIP(A.cond);
   int s = bid * blockDim.x + threadIdx.x;
   if (s >= A.n) return;
   const int n = A.n, a = A.perm[s];
   A.rec[s] = make_double4(A.pos[a], A.pos[n + a], A.pos[2 * n + a], A.q[a]);
-  if (A.stype) A.stype[s] = A.itype[a];
 }
 NB_KERNEL_PAIR(k_pack64, Pack64Args, 256)
 
 // ---------------------------------------------------------------------------
-// Verlet list build, one wavefront per (sorted) atom.  Each wave walks the
-// neighbouring cells 64 candidates at a time, applies cm3d's filters in FP64
-// exactly as ngbr_list / ngbr_liste do (null map, exclusion, minimum image,
-// strict r^2 < (rc+skin)^2, s nested in t), and compacts the survivors with a
-// ballot + mbcnt prefix into the atom's slots, so the order is the candidate
-// order.  Both directions of every pair are stored.
+// Verlet list build, one wavefront per (sorted) atom.  The candidate cells
+// around the atom's own are walked as runs of consecutive cells along c (at
+// most 2 runs per (a,b) column, consecutive in sorted order), 64 candidates
+// at a time; each lane applies cm3d's filters in FP64 exactly as ngbr_list /
+// ngbr_liste do (null map, exclusion, minimum image, strict
+// r^2 < (rc+skin)^2, s nested in t), and the survivors are compacted with a
+// ballot + mbcnt prefix into the atom's slots.  Both directions of every pair
+// are stored; the per-atom half-pair counts (i < j) go to hcnt.
 // ---------------------------------------------------------------------------
+#define NB_PCUT_LDS 1024  // type pairs whose list cutoffs are staged in LDS
 struct ListArgs {
-  int n, ntypes, cap, ncx, ncy, ncz;
+  int n, ntypes, cap, ncx, ncy, ncz, rx, ry, rz, pcut_lds;
   const double4 *rec;
   const int *stype, *perm, *scell, *cstart, *ccount;
-  const int *xoff, *xlist, *map;
-  const InterPar *ip;
+  const int *xoff, *xlist;
+  const double2 *pcut;  // {(rmaxl+skin)^2, (rmaxs+skin)^2} per (ti,tj); -1 for null
   double rmaxs_skin2_e, rmaxl_skin2_e;
   Box b;
   uint32_t *nbr;
   int *nnb, *flags;
+  int4 *hcnt;
   const int *cond;
 };
 
@@ -225,6 +239,11 @@ static __device__ __forceinline__ int wrapc(int v, int n) {
 
 static __device__ __forceinline__ void k_list_body(const ListArgs &A, int bid) {
   NB_SKIP(A.cond);
+  __shared__ double2 pc[NB_PCUT_LDS];
+  if (A.pcut_lds) {
+    for (int t = threadIdx.x; t < A.ntypes * A.ntypes; t += blockDim.x) pc[t] = A.pcut[t];
+    __syncthreads();
+  }
   const int lane = lane_id();
   const int s = bid * 4 + (threadIdx.x >> 6);
   if (s >= A.n) return;
@@ -232,61 +251,86 @@ static __device__ __forceinline__ void k_list_body(const ListArgs &A, int bid) {
   const int ti = A.stype[s], ai = A.perm[s];
   const int c = A.scell[s];
   const int cz = c % A.ncz, cy = (c / A.ncz) % A.ncy, cx = c / (A.ncy * A.ncz);
-  const int x0 = A.xoff[ai], x1 = A.xoff[ai + 1];
+  // this atom's exclusions: the first four as wave-uniform values
+  const int x0 = A.xoff[ai], x1 = A.xoff[ai + 1], nx = x1 - x0;
+  const unsigned none = 0xFFFFFFFFu;  // index bits 0x3FFFFFFF match no atom
+  const unsigned ex0 = nx > 0 ? (unsigned)A.xlist[x0] : none;
+  const unsigned ex1 = nx > 1 ? (unsigned)A.xlist[x0 + 1] : none;
+  const unsigned ex2 = nx > 2 ? (unsigned)A.xlist[x0 + 2] : none;
+  const unsigned ex3 = nx > 3 ? (unsigned)A.xlist[x0 + 3] : none;
   const bool qi = ri.w != 0.0;
-  const int rx = A.ncx >= 3 ? 1 : 0, ry = A.ncy >= 3 ? 1 : 0, rz = A.ncz >= 3 ? 1 : 0;
+  // candidate runs: lane l < ncol takes column l (offsets ox, oy) and
+  // finds its <= 2 slot runs
+  const int wy = 2 * A.ry + 1, ncol = (2 * A.rx + 1) * wy;
+  int rs0 = 0, rl0 = 0, rs1 = 0, rl1 = 0;
+  if (lane < ncol) {
+    const int xx = wrapc(cx + lane / wy - A.rx, A.ncx), yy = wrapc(cy + lane % wy - A.ry, A.ncy);
+    const int base = (xx * A.ncy + yy) * A.ncz;
+    const int zlo = cz - A.rz, zhi = cz + A.rz;
+    const int a0 = zlo < 0 ? 0 : zlo, a1 = zhi > A.ncz - 1 ? A.ncz - 1 : zhi;
+    rs0 = A.cstart[base + a0];
+    rl0 = A.cstart[base + a1] + A.ccount[base + a1] - rs0;
+    int b0 = -1, b1 = -1;
+    if (zlo < 0) { b0 = A.ncz + zlo; b1 = A.ncz - 1; }
+    else if (zhi > A.ncz - 1) { b0 = 0; b1 = zhi - A.ncz; }
+    if (b0 >= 0) {
+      rs1 = A.cstart[base + b0];
+      rl1 = A.cstart[base + b1] + A.ccount[base + b1] - rs1;
+    }
+  }
+  const double2 *pcut = A.pcut + ti * A.ntypes;
   int count = 0, hs = 0, ht = 0, hse = 0, hte = 0;
   uint32_t *out = A.nbr + (size_t)s * A.cap;
-  for (int ox = -rx; ox <= rx; ox++)
-    for (int oy = -ry; oy <= ry; oy++)
-      for (int oz = -rz; oz <= rz; oz++) {
-        int cc = (wrapc(cx + ox, A.ncx) * A.ncy + wrapc(cy + oy, A.ncy)) * A.ncz +
-                 wrapc(cz + oz, A.ncz);
-        int st = A.cstart[cc], nn = A.ccount[cc];
-        for (int b0 = 0; b0 < nn; b0 += NB_WAVE) {
-          int t = b0 + lane, s2 = st + t, aj = -1;
-          unsigned fl = 0;
-          if (t < nn && s2 != s) {
-            double4 rj = A.rec[s2];
-            int tj = A.stype[s2];
-            aj = A.perm[s2];
-            double dx = ri.x - rj.x, dy = ri.y - rj.y, dz = ri.z - rj.z;
-            nb_minimg_box(dx, dy, dz, A.b);
-            double d2 = dx * dx + dy * dy + dz * dz;
-            unsigned ex = 0;
-            for (int e = x0; e < x1; e++) {
-              unsigned v = (unsigned)A.xlist[e];
-              if ((int)(v & 0x3FFFFFFFu) == aj) ex |= v >> 30;
-            }
-            int in = A.map[ti * A.ntypes + tj];
-            if (in >= 0 && !(ex & 1u)) {
-              const InterPar &P = A.ip[in];
-              if (d2 < P.rmaxl_skin2) {
-                fl |= NB_FLAG_VT;
-                if (d2 < P.rmaxs_skin2) fl |= NB_FLAG_VS;
-              }
-            }
-            if (qi && rj.w != 0.0 && !(ex & 2u)) {
-              if (d2 < A.rmaxl_skin2_e) {
-                fl |= NB_FLAG_ET;
-                if (d2 < A.rmaxs_skin2_e) fl |= NB_FLAG_ES;
-              }
-            }
+  for (int r = 0; r < 2 * ncol; r++) {
+    const int st = __builtin_amdgcn_readlane((r & 1) ? rs1 : rs0, r >> 1);
+    const int len = __builtin_amdgcn_readlane((r & 1) ? rl1 : rl0, r >> 1);
+    for (int b0 = 0; b0 < len; b0 += NB_WAVE) {
+      const int t = b0 + lane, s2 = st + t;
+      int aj = -1;
+      unsigned fl = 0;
+      if (t < len && s2 != s) {
+        const double4 rj = A.rec[s2];
+        const int tj = A.stype[s2];
+        aj = A.perm[s2];
+        double dx = ri.x - rj.x, dy = ri.y - rj.y, dz = ri.z - rj.z;
+        nb_minimg_box(dx, dy, dz, A.b);
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        const unsigned uj = (unsigned)aj;
+        unsigned ex = 0;
+        if ((ex0 & 0x3FFFFFFFu) == uj) ex |= ex0 >> 30;
+        if ((ex1 & 0x3FFFFFFFu) == uj) ex |= ex1 >> 30;
+        if ((ex2 & 0x3FFFFFFFu) == uj) ex |= ex2 >> 30;
+        if ((ex3 & 0x3FFFFFFFu) == uj) ex |= ex3 >> 30;
+        for (int e = x0 + 4; e < x1; e++) {
+          const unsigned v = (unsigned)A.xlist[e];
+          if ((v & 0x3FFFFFFFu) == uj) ex |= v >> 30;
+        }
+        const double2 cut = A.pcut_lds ? pc[ti * A.ntypes + tj] : pcut[tj];
+        if (!(ex & 1u) && d2 < cut.x) {  // null pairs carry -1: never true
+          fl |= NB_FLAG_VT;
+          if (d2 < cut.y) fl |= NB_FLAG_VS;
+        }
+        if (qi && rj.w != 0.0 && !(ex & 2u)) {
+          if (d2 < A.rmaxl_skin2_e) {
+            fl |= NB_FLAG_ET;
+            if (d2 < A.rmaxs_skin2_e) fl |= NB_FLAG_ES;
           }
-          unsigned long long m = __ballot(fl != 0);
-          if (fl) {
-            int slot = count + mbcnt(m);
-            if (slot < A.cap) out[slot] = (uint32_t)s2 | (fl << 28);
-            if (ai < aj) {
-              hs += (fl & NB_FLAG_VS) != 0;
-              ht += (fl & NB_FLAG_VT) != 0;
-              hse += (fl & NB_FLAG_ES) != 0;
-              hte += (fl & NB_FLAG_ET) != 0;
-            }
-          }
-          count += __popcll(m);
         }
       }
+      const unsigned long long m = __ballot(fl != 0);
+      if (fl) {
+        const int slot = count + mbcnt(m);
+        if (slot < A.cap) out[slot] = (uint32_t)s2 | (fl << 28);
+        if (ai < aj) {
+          hs += (fl & NB_FLAG_VS) != 0;
+          ht += (fl & NB_FLAG_VT) != 0;
+          hse += (fl & NB_FLAG_ES) != 0;
+          hte += (fl & NB_FLAG_ET) != 0;
+        }
+      }
+      count += __popcll(m);
+    }
+  }
   hs = nb_wave_sum(hs);
   ht = nb_wave_sum(ht);
   hse = nb_wave_sum(hse);
@@ -294,10 +338,7 @@ static __device__ __forceinline__ void k_list_body(const ListArgs &A, int bid) {
   if (lane == 0) {
     A.nnb[s] = count < A.cap ? count : A.cap;
     if (count > A.cap) atomicMax(&A.flags[0], count);
-    if (hs) atomicAdd(&A.flags[2], hs);
-    if (ht) atomicAdd(&A.flags[3], ht);
-    if (hse) atomicAdd(&A.flags[4], hse);
-    if (hte) atomicAdd(&A.flags[5], hte);
+    A.hcnt[s] = make_int4(hs, ht, hse, hte);
   }
 }
 NB_KERNEL_PAIR(k_list, ListArgs, 256)
@@ -480,20 +521,38 @@ static __device__ __forceinline__ void k_ecorr_body(const EcorrArgs &A, int bid)
 NB_KERNEL_PAIR(k_ecorr, EcorrArgs, 64)
 
 // ---------------------------------------------------------------------------
-// Reciprocal-space Ewald, FP64 throughout.
+// Reciprocal-space Ewald, FP64 throughout (fk_ewald, md_ewald.c:185-361).
+// The k vectors are cut into segments: runs of <= NB_EW_SEG consecutive kc in
+// one (ka,kb) row, along which e^{i k.r} advances by the c-axis step
+// e^{2 pi i s_c} (cm3d's recursion, md_ewald.c:261-279); a segment starts
+// with one sine/cosine evaluation (nb_sincos.h).  What a whole wavefront
+// reads at once (the charges in k_ewsfac, the k data in k_ewforce) is staged
+// in LDS and read as broadcasts.  FMA contraction is allowed in these
+// kernels: the sums only have to agree with cm3d's to rounding.
 // k_ewchg: per charge, fractional coordinates s = h^-1 r wrapped to
 //   [-1/2,1/2) (exact for e^{2 pi i k.s} with integer k), stored as 2 pi s,
-//   the charge times sqrt(4 pi) (md_ewald.c:220-224), and e^{2 pi i s_c}, the
-//   c-axis step of cm3d's recursion (md_ewald.c:227-232, 273-278).
-// k_ewsfac: one workgroup per (segment, charge chunk); each thread starts a
-//   segment with one sincos and walks its <= 16 k vectors by the recursion,
-//   accumulating q e^{i k.r} over its charges; a block reduction writes the
-//   chunk's partial S(k).
-// k_ewkred: per k, sum the chunk partials in fixed order, form preg, prep,
+//   the charge times sqrt(4 pi) (md_ewald.c:220-224), and e^{2 pi i s_c}.
+// k_ewsfac: lanes = segments (sorted longest first, so a wave's trip count is
+//   its first lane's); a workgroup walks one chunk of charges and each lane
+//   sums q e^{i k.r} for its segment's k in registers -- no cross-lane
+//   reduction -- then the 4 waves combine and write the chunk's partial S(k).
+// k_ewkred: per k, the chunk partials summed in fixed order, then preg, prep,
 //   |S|^2 and the energy / virial terms (md_ewald.c:247-302).
-// k_ewforce: per charge, walk all segments (staged in LDS) and add
-//   q k (Re S' sin - Im S' cos) to fl (md_ewald.c:303-323).
+// k_ewforce: lanes = charges; a workgroup walks one range of segments in k
+//   order, tile by tile (a segment that continues its row continues the
+//   recursion), summing k (Re S' sin - Im S' cos) (md_ewald.c:303-323) into
+//   a per-range partial.
+// k_ewfadd: per charge, the range partials in fixed order, times q, added
+//   to fl.
 // ---------------------------------------------------------------------------
+template <typename T>
+using cptr4 = __attribute__((address_space(4))) const T *;
+// The constant address space: uniform loads through it become scalar loads.
+template <typename T>
+static __device__ __forceinline__ cptr4<T> as_const4(const T *p) {
+  return (cptr4<T>)(unsigned long long)p;
+}
+
 struct EwChgArgs {
   const double *pos, *q; const int *qidx; int nq, n; Box b; double4 *chg; double2 *stp;
 };
@@ -512,76 +571,148 @@ static __device__ __forceinline__ void k_ewchg_body(const EwChgArgs &A, int bid)
   const double tpi = 2. * M_PI;
   A.chg[c] = make_double4(tpi * sa, tpi * sb, tpi * sc, A.q[a] * sqrt(4.0 * M_PI));
   double sn, cs;
-  sincos(tpi * sc, &sn, &cs);
+  nb_sincos(tpi * sc, &sn, &cs);
   A.stp[c] = make_double2(cs, sn);
 }
 NB_KERNEL_PAIR(k_ewchg, EwChgArgs, 256)
 
 struct EwSfacArgs {
-  const EwSeg *seg; const double4 *chg; const double2 *stp; int nq, chunk, ktot, nseg;
+  const EwSeg *sseg; const double4 *chg; const double2 *stp; int nq, qc, nseg, nsw, ktot;
   double2 *part;
 };
+// LDS of k_ewsfac: the charge tile (6 doubles a charge) during the sweep,
+// then the cross-wave reduction [2][2 NB_EW_SEG][64]
+#define EW_SFAC_LDS (2 * 2 * NB_EW_SEG * NB_WAVE)
+#define EW_QT (EW_SFAC_LDS / 6)  // charges per tile
+// one charge's contribution to a lane's segment: start phase by sincos, then
+// NB_EW_SEG steps of the c-axis recursion (steps past the lane's own count
+// accumulate into slots that are never stored)
+#define EW_SFAC_CHARGE(T)                                                     \
+  {                                                                           \
+    const double *cq = lds + 6 * (T);                                         \
+    const double q = cq[3], stc = cq[4], sts = cq[5];                         \
+    double sn, cs;                                                            \
+    nb_sincos(ka * cq[0] + kb * cq[1] + kc0 * cq[2], &sn, &cs);                \
+    _Pragma("unroll") for (int m = 0; m < NB_EW_SEG; m++) {                   \
+      if (m >= mmax) break;                                                   \
+      re[m] += q * cs;                                                        \
+      im[m] += q * sn;                                                        \
+      const double c2 = cs * stc - sn * sts;                                  \
+      sn = sn * stc + cs * sts;                                               \
+      cs = c2;                                                                \
+    }                                                                         \
+  }
 static __device__ __forceinline__ void k_ewsfac_body(const EwSfacArgs &A, int bid) {
-  const int gseg = bid % A.nseg, gchunk = bid / A.nseg;
-  const EwSeg sg = A.seg[gseg];
-  const int c0 = gchunk * A.chunk;
-  const int c1 = min(A.nq, c0 + A.chunk);
+#pragma clang fp contract(fast)
+  // block = 4 waves on the same 64 segments; the block's chunk of charges is
+  // staged through LDS in tiles and each wave takes every 4th charge; the 4
+  // waves' sums are then combined in LDS in a fixed order
+  __shared__ double lds[EW_SFAC_LDS];
+  static_assert(6 * EW_QT <= EW_SFAC_LDS, "charge tile exceeds the LDS buffer");
+  const int lane = lane_id(), w = threadIdx.x >> 6;
+  const int sw = bid % A.nsw, chunk = bid / A.nsw;
+  const int g = sw * NB_WAVE + lane;
+  int k0 = 0, cnt = 0;
+  double ka = 0., kb = 0., kc0 = 0.;
+  if (g < A.nseg) {
+    const EwSeg sg = A.sseg[g];
+    k0 = sg.k0; cnt = sg.cnt; ka = sg.ka; kb = sg.kb; kc0 = sg.kc0;
+  }
+  // lane 0 (always a real segment) has the wave's longest
+  const int mmax = __builtin_amdgcn_readfirstlane(cnt);
   double re[NB_EW_SEG], im[NB_EW_SEG];
 #pragma unroll
   for (int m = 0; m < NB_EW_SEG; m++) { re[m] = 0.; im[m] = 0.; }
-  for (int c = c0 + threadIdx.x; c < c1; c += blockDim.x) {
-    double4 r = A.chg[c];
-    double2 st = A.stp[c];
-    double th = sg.ka * r.x + sg.kb * r.y + sg.kc0 * r.z;
-    double sn, cs;
-    sincos(th, &sn, &cs);
+  const int c0 = chunk * A.qc, c1 = min(A.nq, c0 + A.qc);
+  for (int t0 = c0; t0 < c1; t0 += EW_QT) {
+    const int nt = min(EW_QT, c1 - t0);
+    __syncthreads();
+    for (int t = threadIdx.x; t < nt; t += blockDim.x) {
+      const double4 r = A.chg[t0 + t];
+      const double2 st = A.stp[t0 + t];
+      double *d = lds + 6 * t;
+      d[0] = r.x; d[1] = r.y; d[2] = r.z; d[3] = r.w; d[4] = st.x; d[5] = st.y;
+    }
+    __syncthreads();
+    int t = w;
+    for (; t + 4 < nt; t += 8) {  // two charges per trip for ILP
+      EW_SFAC_CHARGE(t)
+      EW_SFAC_CHARGE(t + 4)
+    }
+    if (t < nt) EW_SFAC_CHARGE(t)
+  }
+  // (w0 + w2) + (w1 + w3), through the same LDS
+  double (*red)[2 * NB_EW_SEG][NB_WAVE] = (double (*)[2 * NB_EW_SEG][NB_WAVE])lds;
+  __syncthreads();
+  if (w >= 2) {
 #pragma unroll
     for (int m = 0; m < NB_EW_SEG; m++) {
-      if (m < sg.cnt) {
-        re[m] += r.w * cs;
-        im[m] += r.w * sn;
-        double c2 = cs * st.x - sn * st.y;
-        sn = sn * st.x + cs * st.y;
-        cs = c2;
-      }
+      if (m >= mmax) break;
+      red[w - 2][2 * m][lane] = re[m];
+      red[w - 2][2 * m + 1][lane] = im[m];
     }
   }
-  __shared__ double red[4][2 * NB_EW_SEG];
-  const int w = threadIdx.x >> 6, lane = lane_id();
+  __syncthreads();
+  if (w < 2) {
 #pragma unroll
-  for (int m = 0; m < NB_EW_SEG; m++) {
-    double a = nb_wave_sum(re[m]), bb = nb_wave_sum(im[m]);
-    if (lane == 0) { red[w][2 * m] = a; red[w][2 * m + 1] = bb; }
+    for (int m = 0; m < NB_EW_SEG; m++) {
+      if (m >= mmax) break;
+      re[m] += red[w][2 * m][lane];
+      im[m] += red[w][2 * m + 1][lane];
+    }
   }
   __syncthreads();
-  if (threadIdx.x < sg.cnt) {
-    int m = threadIdx.x;
-    double a = red[0][2 * m] + red[1][2 * m], bb = red[0][2 * m + 1] + red[1][2 * m + 1];
-    a += red[2][2 * m] + red[3][2 * m];
-    bb += red[2][2 * m + 1] + red[3][2 * m + 1];
-    A.part[(size_t)gchunk * A.ktot + sg.k0 + m] = make_double2(a, bb);
+  if (w == 1) {
+#pragma unroll
+    for (int m = 0; m < NB_EW_SEG; m++) {
+      if (m >= mmax) break;
+      red[0][2 * m][lane] = re[m];
+      red[0][2 * m + 1][lane] = im[m];
+    }
+  }
+  __syncthreads();
+  if (w == 0) {
+    double2 *out = A.part + (size_t)chunk * A.ktot + k0;
+#pragma unroll
+    for (int m = 0; m < NB_EW_SEG; m++)
+      if (m < cnt) out[m] = make_double2(re[m] + red[0][2 * m][lane], im[m] + red[0][2 * m + 1][lane]);
   }
 }
+#undef EW_SFAC_CHARGE
 NB_KERNEL_PAIR(k_ewsfac, EwSfacArgs, 256)
 
 struct KData { double kx, ky, kz, cr, ci; };
 
+// per k: the chunk partials in fixed order (a workgroup = 64 k x 4 chunk
+// quarters, the quarters combined in order), then the k-space terms
 struct EwKredArgs {
   const int *kabc; int ktot; const double2 *part; int nchunk, nblocks; Box b; double rvol, falp2;
   KData *kd; double *epart;
 };
 static __device__ __forceinline__ void k_ewkred_body(const EwKredArgs &A, int bid) {
+  __shared__ double2 qs[4][NB_WAVE];
   double acc[NB_NE_EW];
 #pragma unroll
   for (int k = 0; k < NB_NE_EW; k++) acc[k] = 0.;
-  int i = bid * blockDim.x + threadIdx.x;
-  if (i < A.ktot) {
+  const int lane = lane_id(), w = threadIdx.x >> 6;
+  const int i = bid * NB_WAVE + lane;
+  {
     double sr = 0., si = 0.;
-    for (int c = 0; c < A.nchunk; c++) {
-      double2 p = A.part[(size_t)c * A.ktot + i];
-      sr += p.x;
-      si += p.y;
+    if (i < A.ktot) {
+      const int q0 = (A.nchunk * w) / 4, q1 = (A.nchunk * (w + 1)) / 4;
+#pragma unroll 8
+      for (int c = q0; c < q1; c++) {
+        double2 p = A.part[(size_t)c * A.ktot + i];
+        sr += p.x;
+        si += p.y;
+      }
     }
+    qs[w][lane] = make_double2(sr, si);
+  }
+  __syncthreads();
+  if (w == 0 && i < A.ktot) {
+    double sr = ((qs[0][lane].x + qs[1][lane].x) + qs[2][lane].x) + qs[3][lane].x;
+    double si = ((qs[0][lane].y + qs[1][lane].y) + qs[2][lane].y) + qs[3][lane].y;
     const double tpi = 2. * M_PI, *hi = A.b.hi;
     double six = tpi * A.kabc[3 * i], siy = tpi * A.kabc[3 * i + 1], siz = tpi * A.kabc[3 * i + 2];
     double aka = six * hi[0] + siy * hi[3] + siz * hi[6];
@@ -606,48 +737,83 @@ static __device__ __forceinline__ void k_ewkred_body(const EwKredArgs &A, int bi
 }
 NB_KERNEL_PAIR(k_ewkred, EwKredArgs, 256)
 
-// One wavefront per charge; its lanes take the k segments round-robin, each
-// starting its segment with one sincos and walking it by the recursion, and
-// the wave sums the three force components at the end.  The k data (40 B
-// per k) are read by every wave and stay cache-resident.
+// k_ewforce: tiles of whole segments (<= NB_EW_KT k each) are staged in LDS;
+// a range is a run of tiles (rtile[r] .. rtile[r+1]); tile t covers
+// segments tseg[t] .. tseg[t+1].
 struct EwForceArgs {
-  const EwSeg *seg; int nseg; const KData *kd; const double4 *chg; const double2 *stp;
-  const int *qidx; int nq, n; double *fl;
+  const EwSeg *seg; const int *rtile, *tseg; const KData *kd; const double4 *chg;
+  const double2 *stp; int nq, nqb; double *fpart;
 };
 static __device__ __forceinline__ void k_ewforce_body(const EwForceArgs &A, int bid) {
-  const int c = bid * 4 + (threadIdx.x >> 6), lane = lane_id();
-  if (c >= A.nq) return;
-  const double4 r = A.chg[c];
-  const double2 st = A.stp[c];
-  double fx = 0., fy = 0., fz = 0.;
-  for (int g = lane; g < A.nseg; g += NB_WAVE) {
-    const EwSeg sg = A.seg[g];
-    double th = sg.ka * r.x + sg.kb * r.y + sg.kc0 * r.z;
-    double sn, cs;
-    sincos(th, &sn, &cs);
-    const KData *K = A.kd + sg.k0;
-    for (int m = 0; m < sg.cnt; m++) {
-      const KData kk = K[m];
-      double tt = kk.cr * sn - kk.ci * cs;
-      fx += kk.kx * tt;
-      fy += kk.ky * tt;
-      fz += kk.kz * tt;
-      double c2 = cs * st.x - sn * st.y;
-      sn = sn * st.x + cs * st.y;
-      cs = c2;
+#pragma clang fp contract(fast)
+  __shared__ double kl[6 * NB_EW_KT];  // {kx, ky, kz, cr, ci, -} per k
+  const int qb = bid % A.nqb, r = bid / A.nqb;
+  const int c = qb * blockDim.x + threadIdx.x;
+  const bool act = c < A.nq;
+  double4 q4 = make_double4(0., 0., 0., 0.);
+  double2 st = make_double2(1., 0.);
+  if (act) { q4 = A.chg[c]; st = A.stp[c]; }
+  cptr4<EwSeg> seg = as_const4(A.seg);
+  cptr4<int> rt = as_const4(A.rtile), ts = as_const4(A.tseg);
+  const int t0 = rt[r], t1 = rt[r + 1], gfirst = ts[t0];
+  double fx = 0., fy = 0., fz = 0., sn = 0., cs = 1.;
+  for (int t = t0; t < t1; t++) {
+    const int g0 = ts[t], g1 = ts[t + 1];
+    const int kb0 = seg[g0].k0, nk = seg[g1 - 1].k0 + seg[g1 - 1].cnt - kb0;
+    __syncthreads();
+    for (int u = threadIdx.x; u < nk; u += blockDim.x) {
+      const KData d = A.kd[kb0 + u];
+      double *o = kl + 6 * u;
+      o[0] = d.kx; o[1] = d.ky; o[2] = d.kz; o[3] = d.cr; o[4] = d.ci;
+    }
+    __syncthreads();
+    for (int g = g0; g < g1; g++) {
+      const int cnt = seg[g].cnt;
+      if (g == gfirst || !seg[g].cont)
+        nb_sincos((double)seg[g].ka * q4.x + (double)seg[g].kb * q4.y + (double)seg[g].kc0 * q4.z,
+                  &sn, &cs);
+      const double *K = kl + 6 * (seg[g].k0 - kb0);
+      for (int m = 0; m < cnt; m++, K += 6) {
+        const double tt = K[3] * sn - K[4] * cs;
+        fx += K[0] * tt;
+        fy += K[1] * tt;
+        fz += K[2] * tt;
+        const double c2 = cs * st.x - sn * st.y;
+        sn = sn * st.x + cs * st.y;
+        cs = c2;
+      }
     }
   }
-  fx = nb_wave_sum(fx);
-  fy = nb_wave_sum(fy);
-  fz = nb_wave_sum(fz);
-  if (lane == 0) {
-    const int a = A.qidx[c], n = A.n;
-    A.fl[a] += r.w * fx;
-    A.fl[n + a] += r.w * fy;
-    A.fl[2 * n + a] += r.w * fz;
+  if (act) {
+    double *o = A.fpart + (size_t)(3 * r) * A.nq + c;
+    o[0] = fx;
+    o[A.nq] = fy;
+    o[2 * (size_t)A.nq] = fz;
   }
 }
 NB_KERNEL_PAIR(k_ewforce, EwForceArgs, 256)
+
+struct EwFaddArgs {
+  const double *fpart; const double4 *chg; const int *qidx; int nq, n, nrange; double *fl;
+};
+static __device__ __forceinline__ void k_ewfadd_body(const EwFaddArgs &A, int bid) {
+  const int c = bid * blockDim.x + threadIdx.x;
+  if (c >= A.nq) return;
+  double fx = 0., fy = 0., fz = 0.;
+  const size_t nq = A.nq;
+  for (int r = 0; r < A.nrange; r++) {
+    const double *p = A.fpart + 3 * (size_t)r * nq + c;
+    fx += p[0];
+    fy += p[nq];
+    fz += p[2 * nq];
+  }
+  const double q = A.chg[c].w;
+  const int a = A.qidx[c], n = A.n;
+  A.fl[a] += q * fx;
+  A.fl[n + a] += q * fy;
+  A.fl[2 * n + a] += q * fz;
+}
+NB_KERNEL_PAIR(k_ewfadd, EwFaddArgs, 256)
 
 // Column sums of a column-major [COLS][rows] partial table: one block per
 // column, a fixed assignment of rows to threads and a fixed tree
@@ -684,37 +850,27 @@ static int blocks(int n, int t) { return (n + t - 1) / t; }
 // No memset nodes: zeroing is done by kernels so the sequence can be captured.
 static int launch_build_impl(nb_ctx *c, const Box &b, hipStream_t s, const int *cond) {
   const int n = c->natoms;
-  ZeroArgs z1 = {c->d_ccount, c->ncells, cond};
-  NB_GO(c, K_ZERO, k_zero, blocks(c->ncells, 256), 256, 0, z1, s);
-  // overflow max (flags[0]) and half-pair counts (flags[2..5]); flags[1] is
-  // the rebuild flag `cond` itself and must survive the whole build
-  ZeroArgs z2 = {c->d_flags + 2, 4, cond};
-  NB_GO(c, K_ZERO, k_zero, 1, 256, 0, z2, s);
-  ZeroArgs z3 = {c->d_flags, 1, cond};
-  NB_GO(c, K_ZERO, k_zero, 1, 256, 0, z3, s);
-  CellArgs ca = {c->d_pos, n, b, c->ncx, c->ncy, c->ncz, c->d_cell, c->d_ccount, cond};
+  CellArgs ca = {c->d_pos, n, blocks(n, 256), c->ncells, b, c->ncx, c->ncy, c->ncz, c->d_cell,
+                 c->d_cwork, c->d_ccount, c->d_cstart, c->d_cfill, c->d_flags, cond};
   NB_GO(c, K_CELL, k_cell, blocks(n, 256), 256, 0, ca, s);
-  ScanArgs sa = {c->d_ccount, c->ncells, c->d_cstart, c->d_cfill, cond};
-  NB_GO(c, K_SCAN, k_scan, 1, 1024, 0, sa, s);
   ScatArgs sc = {c->d_cell, n, c->d_cfill, c->d_tmp, cond};
   NB_GO(c, K_SCAT, k_scat, blocks(n, 256), 256, 0, sc, s);
   // the sorted cell id of each sorted slot overwrites d_cell
-  OrderArgs oa = {c->d_ccount, c->d_cstart, c->d_tmp, c->d_perm, c->d_iperm, c->d_cell, cond};
+  OrderArgs oa = {c->d_ccount, c->d_cstart, c->d_tmp, c->d_perm, c->d_iperm, c->d_cell,
+                  c->d_pos, c->d_qch, c->d_itype, (double4 *)c->d_rec64, c->d_stype,
+                  c->d_posb, n, c->d_flags + 6, cond};
   NB_GO(c, K_ORDER, k_order, c->ncells, 256, 0, oa, s);
-  Pack64Args pa = {c->d_pos, c->d_qch, c->d_itype, c->d_perm, n, (double4 *)c->d_rec64,
-                   c->d_stype, cond};
-  NB_GO(c, K_PACK64, k_pack64, blocks(n, 256), 256, 0, pa, s);
   ListArgs A;
   A.n = n; A.ntypes = c->ntypes; A.cap = c->cap;
-  A.ncx = c->ncx; A.ncy = c->ncy; A.ncz = c->ncz;
+  A.ncx = c->ncx; A.ncy = c->ncy; A.ncz = c->ncz; A.rx = c->crx; A.ry = c->cry; A.rz = c->crz;
+  A.pcut_lds = c->ntypes * c->ntypes <= NB_PCUT_LDS;
   A.rec = (const double4 *)c->d_rec64; A.stype = c->d_stype; A.perm = c->d_perm;
   A.scell = c->d_cell; A.cstart = c->d_cstart; A.ccount = c->d_ccount;
-  A.xoff = c->d_xoff; A.xlist = c->d_xlist; A.map = c->d_map; A.ip = c->d_ip;
+  A.xoff = c->d_xoff; A.xlist = c->d_xlist; A.pcut = (const double2 *)c->d_pcut;
   A.rmaxs_skin2_e = c->rmaxs_skin2_e; A.rmaxl_skin2_e = c->rmaxl_skin2_e;
-  A.b = b; A.nbr = c->d_nbr; A.nnb = c->d_nnb; A.flags = c->d_flags; A.cond = cond;
+  A.b = b; A.nbr = c->d_nbr; A.nnb = c->d_nnb; A.flags = c->d_flags;
+  A.hcnt = (int4 *)c->d_hcnt; A.cond = cond;
   NB_GO(c, K_LIST, k_list, blocks(n, 4), 256, 0, A, s);
-  CopyArgs cp = {c->d_posb, c->d_pos, 3 * n, cond, c->d_flags + 6};
-  NB_GO(c, K_COPY, k_copy, blocks(3 * n, 256), 256, 0, cp, s);
   if (!c->rec) LCHK(hipGetLastError());
   return NB_OK;
 }
@@ -745,8 +901,7 @@ int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_f
     int rc = nb_launch_pair_mixed(c, iflg, b, d_fs, d_fl, s);
     if (rc) return rc;
   } else {
-    Pack64Args pa = {c->d_pos, c->d_qch, c->d_itype, c->d_perm, n, (double4 *)c->d_rec64,
-                     nullptr, nullptr};
+    Pack64Args pa = {c->d_pos, c->d_qch, c->d_perm, n, (double4 *)c->d_rec64};
     NB_GO(c, K_PACK64, k_pack64, blocks(n, 256), 256, 0, pa, s);
     nb_time_mark(c, 1, s);
     PairArgs P;
@@ -783,21 +938,24 @@ int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_f
   EwChgArgs ch = {c->d_pos, c->d_qch, c->d_qidx, nq, n, b, (double4 *)c->d_chg,
                   (double2 *)c->d_stp};
   NB_GO(c, K_EWCHG, k_ewchg, ew ? blocks(nq, 256) : 0, 256, 0, ch, s);
-  EwSfacArgs sf = {c->d_seg, (const double4 *)c->d_chg, (const double2 *)c->d_stp, nq,
-                   (nq + c->nchunk - 1) / c->nchunk, c->ktot, c->nseg, (double2 *)c->d_spart};
-  NB_GO(c, K_EWSFAC, k_ewsfac, ew ? c->nseg * c->nchunk : 0, 256, 0, sf, s);
+  EwSfacArgs sf = {c->d_sseg, (const double4 *)c->d_chg, (const double2 *)c->d_stp, nq, c->qc,
+                   c->nseg, c->nsw, c->ktot, (double2 *)c->d_spart};
+  NB_GO(c, K_EWSFAC, k_ewsfac, ew ? c->nsw * c->nchunk : 0, 256, 0, sf, s);
   double det = b.h[0] * (b.h[4] * b.h[8] - b.h[5] * b.h[7]) +
                b.h[1] * (b.h[5] * b.h[6] - b.h[3] * b.h[8]) +
                b.h[2] * (b.h[3] * b.h[7] - b.h[4] * b.h[6]);
   double falp2 = c->kappa > 0 ? 1. / (4. * c->kappa * c->kappa) : 0.;
   EwKredArgs kr = {c->d_kabc, c->ktot, (const double2 *)c->d_spart, c->nchunk, c->new_blocks,
                    b, 1. / det, falp2, (KData *)c->d_kd, c->d_eew};
+  // one workgroup per 64 k
   NB_GO(c, K_EWKRED, k_ewkred, ew ? c->new_blocks : 0, 256, 0, kr, s);
   RedArgs rw = {c->d_eew, c->new_blocks, c->d_eout + 32};
   NB_GO(c, K_RED_EW, k_red_ew, ew ? NB_NE_EW : 0, 256, 0, rw, s);
-  EwForceArgs ef = {c->d_seg, c->nseg, (const KData *)c->d_kd, (const double4 *)c->d_chg,
-                    (const double2 *)c->d_stp, c->d_qidx, nq, n, d_fl};
-  NB_GO(c, K_EWFORCE, k_ewforce, ew ? blocks(nq, 4) : 0, 256, 0, ef, s);
+  EwForceArgs ef = {c->d_seg, c->d_rtile, c->d_tseg, (const KData *)c->d_kd,
+                    (const double4 *)c->d_chg, (const double2 *)c->d_stp, nq, c->nqb, c->d_fpart};
+  NB_GO(c, K_EWFORCE, k_ewforce, ew ? c->nqb * c->nrange : 0, 256, 0, ef, s);
+  EwFaddArgs fa = {c->d_fpart, (const double4 *)c->d_chg, c->d_qidx, nq, n, c->nrange, d_fl};
+  NB_GO(c, K_EWFADD, k_ewfadd, ew ? blocks(nq, 256) : 0, 256, 0, fa, s);
   if (!c->rec) LCHK(hipGetLastError());
   nb_time_mark(c, 4, s);
   nb_time_mark(c, 5, s);
@@ -810,9 +968,7 @@ size_t nb_kdata_bytes() { return sizeof(KData); }
 static struct RegisterKernels {
   RegisterKernels() {
     g_batch_fn[K_ZERO] = k_zero_bl;
-    g_batch_fn[K_COPY] = k_copy_bl;
     g_batch_fn[K_CELL] = k_cell_bl;
-    g_batch_fn[K_SCAN] = k_scan_bl;
     g_batch_fn[K_SCAT] = k_scat_bl;
     g_batch_fn[K_ORDER] = k_order_bl;
     g_batch_fn[K_PACK64] = k_pack64_bl;
@@ -829,5 +985,6 @@ static struct RegisterKernels {
     g_batch_fn[K_EWSFAC] = k_ewsfac_bl;
     g_batch_fn[K_EWKRED] = k_ewkred_bl;
     g_batch_fn[K_EWFORCE] = k_ewforce_bl;
+    g_batch_fn[K_EWFADD] = k_ewfadd_bl;
   }
 } g_register_kernels;
