@@ -49,6 +49,11 @@ def parse():
     ap.add_argument("--nside", type=int, default=16, help="waters per box edge")
     ap.add_argument("--nsys", type=int, default=64, help="systems in the batched form")
     ap.add_argument("--batch-steps", type=int, default=100)
+    ap.add_argument("--equil", type=int, default=600,
+                    help="untimed steps with velocity rescaling to 300 K before timing")
+    ap.add_argument("--drift-steps", type=int, default=2000)
+    ap.add_argument("--lat-steps", type=int, default=100,
+                    help="single synchronised steps timed for the latency percentiles")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
@@ -80,6 +85,17 @@ def cpu_baseline(sysd, budget_s):
     return t_list, (time.perf_counter() - t0) / nf, nf
 
 
+def _sync_steps(torch, step, nsteps):
+    """Per-step latency as a caller sees it: launch one step, wait for it."""
+    lat = []
+    for _ in range(nsteps):
+        t = time.perf_counter()
+        step()
+        torch.cuda.synchronize()
+        lat.append(1e3 * (time.perf_counter() - t))
+    return np.array(lat)
+
+
 def run_single(nb, S, a, torch):
     s = S.water(nside=a.nside)
     n = int(s["natoms"])
@@ -87,23 +103,19 @@ def run_single(nb, S, a, torch):
     md = nb.NBMD(ctx, s, DT_PS, nres_intra=4)
     md.set_state(s["pos"], s["vel"], s["hmat"])
     md.init()
+    # the lattice start relaxes and heats: bring it to 300 K first (untimed)
+    S.equilibrate([md], [s], 300.0, a.equil, every=10)
+    pos_eq, vel_eq = md.get_state()
     sp = torch.cuda.current_stream().cuda_stream
     md.step(a.warmup, stream=sp, graph=True)
     torch.cuda.synchronize()
-    e0 = md.energies()
     r0 = md.rebuilds()
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(a.steps + 1)]
-    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    ev[0].record()
-    for k in range(a.steps):
-        md.step(1, stream=sp, graph=True)
-        ev[k + 1].record()
+    md.step(a.steps, stream=sp, graph=True)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    lat = np.array([ev[k].elapsed_time(ev[k + 1]) for k in range(a.steps)])
-    e1 = md.energies()
     nreb = md.rebuilds() - r0
+    lat = _sync_steps(torch, lambda: md.step(1, stream=sp, graph=True), a.lat_steps)
     # kernel durations: force(...,1) on the current state with HIP events on
     # the stream the kernels run on
     pos, _ = md.get_state()
@@ -112,9 +124,31 @@ def run_single(nb, S, a, torch):
         ctx.force(pos, s["hmat"], iflg=1)
     tm = ctx.timing_read()
     counts = ctx.list_counts()
-    md.close()
-    ctx.close()
-    return s, dict(n=n, wall=wall, lat=lat, e0=e0, e1=e1, rebuilds=nreb, tm=tm, counts=counts)
+    # energy drift of both precisions from the same equilibrated state, and
+    # the mixed form's force error against FP64 there
+    ctxm = nb.NBContext(s, precision="mixed")
+    mdm = nb.NBMD(ctxm, s, DT_PS, nres_intra=4)
+    drift = {}
+    for name, c, m in (("fp64", ctx, md), ("mixed", ctxm, mdm)):
+        m.set_state(pos_eq, vel_eq, s["hmat"])
+        m.init()
+        e0 = m.energies()
+        m.step(a.drift_steps, graph=True)
+        e1 = m.energies()
+        drift[name] = (e1["ham"] - e0["ham"]) / n / (a.drift_steps * DT_PS * 1e-3)
+        drift[name + "_ham0_K_per_atom"] = e0["ham"] / n
+    t0 = time.perf_counter()
+    mdm.step(a.steps, graph=True)
+    torch.cuda.synchronize()
+    wall_m = time.perf_counter() - t0
+    ctx.build_lists(pos_eq, s["hmat"])
+    _, fl64, _ = ctx.force(pos_eq, s["hmat"], iflg=1)
+    ctxm.build_lists(pos_eq, s["hmat"])
+    _, flm, _ = ctxm.force(pos_eq, s["hmat"], iflg=1)
+    ferr = float(np.sqrt(((flm - fl64) ** 2).mean() / (fl64 ** 2).mean()))
+    md.close(); mdm.close(); ctx.close(); ctxm.close()
+    return s, dict(n=n, wall=wall, lat=lat, rebuilds=nreb, tm=tm, counts=counts,
+                   drift=drift, wall_mixed=wall_m, mixed_force_err=ferr)
 
 
 def run_batch(nb, S, a, torch):
@@ -130,19 +164,17 @@ def run_batch(nb, S, a, torch):
         ctxs.append(c); mds.append(m); syss.append(s)
     b = nb.NBBatch(mds)
     sp = torch.cuda.current_stream().cuda_stream
+    S.equilibrate(mds, syss, 300.0, a.equil, every=10,
+                  stepper=lambda k: b.step(k, stream=sp, graph=True))
     b.step(10, stream=sp, graph=True)
     torch.cuda.synchronize()
     e0 = [m.energies()["ham"] for m in mds]
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(a.batch_steps + 1)]
     t0 = time.perf_counter()
-    ev[0].record()
-    for k in range(a.batch_steps):
-        b.step(1, stream=sp, graph=True)
-        ev[k + 1].record()
+    b.step(a.batch_steps, stream=sp, graph=True)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    lat = np.array([ev[k].elapsed_time(ev[k + 1]) for k in range(a.batch_steps)])
     e1 = [m.energies()["ham"] for m in mds]
+    lat = _sync_steps(torch, lambda: b.step(1, stream=sp, graph=True), a.lat_steps)
     ntot = int(sum(int(s["natoms"]) for s in syss))
     drift = float(np.mean([(y - x) / int(s["natoms"]) for x, y, s in zip(e0, e1, syss)]))
     out = dict(natoms_total=ntot, nsys=a.nsys, wall=wall, lat=lat, launches=b.launches(),
@@ -194,8 +226,6 @@ def main():
                 "unit": "TFLOP/s", "frac": round(ew_tfs / FP64_PEAK_TFLOPS, 5),
                 "traffic": None, "ms_per_launch": round(tm["ewald"], 5),
                 "flop_per_launch": ew_flop}
-    t_ns = a.steps * DT_PS * 1e-3
-    drift = (r["e1"]["ham"] - r["e0"]["ham"]) / n / t_ns
     out = {
         "metric": "atom-steps/s", "value": round(atom_steps, 1), "unit": "atom-steps/s",
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -208,7 +238,11 @@ def main():
         "ns_per_day": round(ns_day, 2),
         "p50_step_ms": round(float(np.percentile(r["lat"], 50)), 5),
         "p99_step_ms": round(float(np.percentile(r["lat"], 99)), 5),
-        "energy_drift_K_per_atom_per_ns": round(drift, 6),
+        "latency_basis": f"{a.lat_steps} single steps, each launched and waited for",
+        "energy_drift_K_per_atom_per_ns": {k: round(v, 6) for k, v in r["drift"].items()},
+        "drift_run": f"{a.drift_steps} NVE steps from the same 300 K state, each precision",
+        "mixed_single": {"atom_steps_per_s": round(n * a.steps / r["wall_mixed"], 1),
+                         "force_rel_rms_vs_fp64": r["mixed_force_err"]},
         "list_rebuilds": r["rebuilds"],
         "pairs_half": counts,
         "force_phase_ms": {k: round(v, 5) for k, v in tm.items() if k != "calls"},

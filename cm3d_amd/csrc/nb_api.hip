@@ -90,7 +90,7 @@ extern "C" void nb_destroy(nb_ctx *c) {
                   c->d_hcnt, c->d_pcut,
                   c->d_tmp, c->d_rec64, c->d_stype, c->d_nbr, c->d_nnb, c->d_flags,
                   c->d_chg, c->d_stp, c->d_spart, c->d_kd, c->d_epair, c->d_eew, c->d_eec, c->d_eout,
-                  c->d_fxp, c->d_recf};
+                  c->d_fxp, c->d_recf, c->d_tpf};
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (c->h_flags) hipHostFree(c->h_flags);
@@ -377,6 +377,22 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
     if ((rc = upload(&c->d_vtab3f, vf.data(), vf.size()))) return fail(rc);
     if ((rc = upload(&c->d_ctab3f, cf.data(), cf.size()))) return fail(rc);
     if ((rc = dalloc(&c->d_recf, (size_t)n))) return fail(rc);
+    std::vector<TPairF> tpf((size_t)p->ntypes * p->ntypes);
+    for (int a = 0; a < p->ntypes; a++)
+      for (int b = 0; b < p->ntypes; b++) {
+        TPairF &t = tpf[(size_t)a * p->ntypes + b];
+        const int k = p->map[a * p->ntypes + b];
+        if (k < 0) {
+          t.rmins2 = t.rminl2 = t.rmaxs2 = t.rmaxl2 = -1.f; t.dx2 = 0.f; t.vlast = 0.f;
+          t.in = -1;
+        } else {
+          t.rmins2 = (float)ip[k].rmins2; t.rminl2 = (float)ip[k].rminl2;
+          t.rmaxs2 = (float)ip[k].rmaxs2; t.rmaxl2 = (float)ip[k].rmaxl2;
+          t.dx2 = (float)ip[k].dx2; t.vlast = (float)ip[k].vlast; t.in = k;
+        }
+        t.pad = 0;
+      }
+    if ((rc = upload(&c->d_tpf, tpf.data(), tpf.size()))) return fail(rc);
   }
   // neighbour capacity from the density, grown on overflow at build time
   {

@@ -43,6 +43,11 @@ struct InterPar {   // one vdW interaction (cm3d INTER arrays at index in)
   double rmins2, rmaxs2, rminl2, rmaxl2, dx2, rmaxs_skin2, rmaxl_skin2, vlast;
 };
 
+struct TPairF {     // one type pair for the FP32 kernel (null pair: in -1, cutoffs -1)
+  float rmins2, rminl2, rmaxs2, rmaxl2, dx2, vlast;
+  int in, pad;
+};
+
 struct EwSeg {      // run of <= NB_EW_SEG consecutive kc in one (ka,kb) row
   int k0, cnt, ka, kb, kc0;
   int cont;         // continues the previous segment's row (kc0 = its end)
@@ -149,6 +154,7 @@ struct nb_ctx {
   int npair_blocks, new_blocks, necorr_blocks;
   long long *d_fxp;            // fixed-point force accumulators (mixed)
   float4 *d_recf;              // mixed: {sx, sy, sz (u32 bits), q}
+  TPairF *d_tpf;               // mixed: per type pair parameters
   Box box_built;
   int have_lists;
   int tab_lds_bytes, tab_lds_bytes_f, ctab_off;

@@ -281,56 +281,87 @@ static __device__ __forceinline__ void k_list_body(const ListArgs &A, int bid) {
   const double2 *pcut = A.pcut + ti * A.ntypes;
   int count = 0, hs = 0, ht = 0, hse = 0, hte = 0;
   uint32_t *out = A.nbr + (size_t)s * A.cap;
-  for (int r = 0; r < 2 * ncol; r++) {
-    const int st = __builtin_amdgcn_readlane((r & 1) ? rs1 : rs0, r >> 1);
-    const int len = __builtin_amdgcn_readlane((r & 1) ? rl1 : rl0, r >> 1);
-    for (int b0 = 0; b0 < len; b0 += NB_WAVE) {
-      const int t = b0 + lane, s2 = st + t;
-      int aj = -1;
-      unsigned fl = 0;
-      if (t < len && s2 != s) {
-        const double4 rj = A.rec[s2];
-        const int tj = A.stype[s2];
-        aj = A.perm[s2];
-        double dx = ri.x - rj.x, dy = ri.y - rj.y, dz = ri.z - rj.z;
-        nb_minimg_box(dx, dy, dz, A.b);
-        const double d2 = dx * dx + dy * dy + dz * dz;
-        const unsigned uj = (unsigned)aj;
-        unsigned ex = 0;
-        if ((ex0 & 0x3FFFFFFFu) == uj) ex |= ex0 >> 30;
-        if ((ex1 & 0x3FFFFFFFu) == uj) ex |= ex1 >> 30;
-        if ((ex2 & 0x3FFFFFFFu) == uj) ex |= ex2 >> 30;
-        if ((ex3 & 0x3FFFFFFFu) == uj) ex |= ex3 >> 30;
-        for (int e = x0 + 4; e < x1; e++) {
-          const unsigned v = (unsigned)A.xlist[e];
-          if ((v & 0x3FFFFFFFu) == uj) ex |= v >> 30;
-        }
-        const double2 cut = A.pcut_lds ? pc[ti * A.ntypes + tj] : pcut[tj];
-        if (!(ex & 1u) && d2 < cut.x) {  // null pairs carry -1: never true
-          fl |= NB_FLAG_VT;
-          if (d2 < cut.y) fl |= NB_FLAG_VS;
-        }
-        if (qi && rj.w != 0.0 && !(ex & 2u)) {
-          if (d2 < A.rmaxl_skin2_e) {
-            fl |= NB_FLAG_ET;
-            if (d2 < A.rmaxs_skin2_e) fl |= NB_FLAG_ES;
-          }
-        }
-      }
-      const unsigned long long m = __ballot(fl != 0);
-      if (fl) {
-        const int slot = count + mbcnt(m);
-        if (slot < A.cap) out[slot] = (uint32_t)s2 | (fl << 28);
-        if (ai < aj) {
-          hs += (fl & NB_FLAG_VS) != 0;
-          ht += (fl & NB_FLAG_VT) != 0;
-          hse += (fl & NB_FLAG_ES) != 0;
-          hte += (fl & NB_FLAG_ET) != 0;
-        }
-      }
-      count += __popcll(m);
-    }
+  // walk the chunks (run r, offset b) with the next chunk's candidate loads
+  // issued before the current chunk is evaluated
+  const int nrun = 2 * ncol;
+#define NB_RUN_ST(r) __builtin_amdgcn_readlane(((r) & 1) ? rs1 : rs0, (r) >> 1)
+#define NB_RUN_LEN(r) __builtin_amdgcn_readlane(((r) & 1) ? rl1 : rl0, (r) >> 1)
+#define NB_NEXT_CHUNK(r, b, st, len)              \
+  {                                               \
+    b += NB_WAVE;                                 \
+    while (b >= len) {                            \
+      if (++r >= nrun) break;                     \
+      st = NB_RUN_ST(r);                          \
+      len = NB_RUN_LEN(r);                        \
+      b = 0;                                      \
+    }                                             \
   }
+  int r = -1, b = -NB_WAVE, st = 0, len = 0;
+  NB_NEXT_CHUNK(r, b, st, len)
+  double4 rjc = make_double4(0., 0., 0., 0.);
+  int tjc = 0, ajc = -1, s2c = 0;
+  bool okc = false;
+  if (r < nrun) {
+    s2c = st + b + lane;
+    okc = b + lane < len && s2c != s;
+    if (okc) { rjc = A.rec[s2c]; tjc = A.stype[s2c]; ajc = A.perm[s2c]; }
+  }
+  while (r < nrun) {
+    int rn = r, bn = b, stn = st, lenn = len;
+    NB_NEXT_CHUNK(rn, bn, stn, lenn)
+    double4 rjn = make_double4(0., 0., 0., 0.);
+    int tjn = 0, ajn = -1, s2n = 0;
+    bool okn = false;
+    if (rn < nrun) {
+      s2n = stn + bn + lane;
+      okn = bn + lane < lenn && s2n != s;
+      if (okn) { rjn = A.rec[s2n]; tjn = A.stype[s2n]; ajn = A.perm[s2n]; }
+    }
+    unsigned fl = 0;
+    if (okc) {
+      double dx = ri.x - rjc.x, dy = ri.y - rjc.y, dz = ri.z - rjc.z;
+      nb_minimg_box(dx, dy, dz, A.b);
+      const double d2 = dx * dx + dy * dy + dz * dz;
+      const unsigned uj = (unsigned)ajc;
+      unsigned ex = 0;
+      if ((ex0 & 0x3FFFFFFFu) == uj) ex |= ex0 >> 30;
+      if ((ex1 & 0x3FFFFFFFu) == uj) ex |= ex1 >> 30;
+      if ((ex2 & 0x3FFFFFFFu) == uj) ex |= ex2 >> 30;
+      if ((ex3 & 0x3FFFFFFFu) == uj) ex |= ex3 >> 30;
+      for (int e = x0 + 4; e < x1; e++) {
+        const unsigned v = (unsigned)A.xlist[e];
+        if ((v & 0x3FFFFFFFu) == uj) ex |= v >> 30;
+      }
+      const double2 cut = A.pcut_lds ? pc[ti * A.ntypes + tjc] : pcut[tjc];
+      if (!(ex & 1u) && d2 < cut.x) {  // null pairs carry -1: never true
+        fl |= NB_FLAG_VT;
+        if (d2 < cut.y) fl |= NB_FLAG_VS;
+      }
+      if (qi && rjc.w != 0.0 && !(ex & 2u)) {
+        if (d2 < A.rmaxl_skin2_e) {
+          fl |= NB_FLAG_ET;
+          if (d2 < A.rmaxs_skin2_e) fl |= NB_FLAG_ES;
+        }
+      }
+    }
+    const unsigned long long m = __ballot(fl != 0);
+    if (fl) {
+      const int slot = count + mbcnt(m);
+      if (slot < A.cap) out[slot] = (uint32_t)s2c | (fl << 28);
+      if (ai < ajc) {
+        hs += (fl & NB_FLAG_VS) != 0;
+        ht += (fl & NB_FLAG_VT) != 0;
+        hse += (fl & NB_FLAG_ES) != 0;
+        hte += (fl & NB_FLAG_ET) != 0;
+      }
+    }
+    count += __popcll(m);
+    r = rn; b = bn; st = stn; len = lenn;
+    rjc = rjn; tjc = tjn; ajc = ajn; s2c = s2n; okc = okn;
+  }
+#undef NB_NEXT_CHUNK
+#undef NB_RUN_ST
+#undef NB_RUN_LEN
   hs = nb_wave_sum(hs);
   ht = nb_wave_sum(ht);
   hse = nb_wave_sum(hse);
@@ -373,18 +404,27 @@ enum {
   E_SPOTE, E_SPOTHE, E_SPRSE, E_SE0, E_SE1, E_SE2, E_SE4, E_SE5, E_SE8, E_CLAMP
 };
 
+#define NB_IP_LDS 16    // interactions staged in LDS
+#define NB_MAP_LDS 256  // type-map entries staged in LDS
 template <int IFLG, bool WANT_FS>
 static __device__ __forceinline__ void pair64_body(const PairArgs &A, int bid) {
   extern __shared__ double tabs[];
+  __shared__ InterPar sip[NB_IP_LDS];
+  __shared__ int smap[NB_MAP_LDS];
   const double *vt = A.vtab3, *ct = A.ctab3;
   if (A.tab_lds) {
     int nv = A.ninter * A.ntable * 3, nc = A.ntable * 3;
     for (int t = threadIdx.x; t < nv; t += blockDim.x) tabs[t] = A.vtab3[t];
     for (int t = threadIdx.x; t < nc; t += blockDim.x) tabs[A.ctab_off + t] = A.ctab3[t];
-    __syncthreads();
     vt = tabs;
     ct = tabs + A.ctab_off;
   }
+  const bool ip_lds = A.ninter <= NB_IP_LDS && A.ntypes * A.ntypes <= NB_MAP_LDS;
+  if (ip_lds) {
+    for (int t = threadIdx.x; t < A.ninter; t += blockDim.x) sip[t] = A.ip[t];
+    for (int t = threadIdx.x; t < A.ntypes * A.ntypes; t += blockDim.x) smap[t] = A.map[t];
+  }
+  __syncthreads();
   double acc[NB_NE_PAIR];
 #pragma unroll
   for (int k = 0; k < NB_NE_PAIR; k++) acc[k] = 0.;
@@ -399,71 +439,92 @@ static __device__ __forceinline__ void pair64_body(const PairArgs &A, int bid) {
     const int nb = A.nnb[s];
     const uint32_t *lst = A.nbr + (size_t)s * A.cap;
     double fx = 0., fy = 0., fz = 0., sx = 0., sy = 0., sz = 0.;
-    for (int k = lane; k < nb; k += NB_WAVE) {
-      uint32_t e = lst[k];
-      unsigned fl = e >> 28;
-      if (!(fl & (fv | fe))) continue;
-      int j = (int)(e & NB_IDX_MASK);
-      double4 rj = A.rec[j];
-      double dx = ri.x - rj.x, dy = ri.y - rj.y, dz = ri.z - rj.z;
-      nb_minimg_box(dx, dy, dz, A.b);
-      double d2 = dx * dx + dy * dy + dz * dz;
-      if (fl & fv) {
-        int in = A.map[ti * A.ntypes + A.stype[j]];
-        const InterPar &P = A.ip[in];
-        double rmax2 = IFLG ? P.rmaxl2 : P.rmaxs2;
-        if (d2 < rmax2) {
-          double p = (d2 - P.rmins2) * P.dx2;
-          int kk = (int)p;
-          if (kk < 1) { acc[E_CLAMP] += 1.; kk = 1; p = 1.; }
-          p = p - (double)kk;
-          const double *T = vt + (size_t)in * A.ntable * 3;
-          double f = nb_interp3(T, kk, 0, p);
-          double swv = 1.;
-          if ((WANT_FS || !IFLG) && d2 > P.rminl2) swv = nb_interp3(T, kk, 1, p);
-          if (IFLG) {
-            acc[E_SPRSL] -= d2 * f;
-            acc[E_SL0] -= dx * dx * f;
-            acc[E_SL1] -= dy * dx * f;
-            acc[E_SL2] -= dz * dx * f;
-            acc[E_SL4] -= dy * dy * f;
-            acc[E_SL5] -= dz * dy * f;
-            acc[E_SL8] -= dz * dz * f;
-            double gx = dx * f, gy = dy * f, gz = dz * f;
-            fx -= gx; fy -= gy; fz -= gz;
-            if (WANT_FS) { sx -= gx * swv; sy -= gy * swv; sz -= gz * swv; }
-            double v = nb_interp3(T, kk, 2, p);
-            acc[E_SPOTL] += v;
-            acc[E_SPOTHL] += v - T[3 * ntm4 + 2];
-          } else {
-            f *= swv;
-            sx -= dx * f; sy -= dy * f; sz -= dz * f;
+    // chunks of 64 list entries; the next chunk's entries and partner
+    // records are loaded before the current chunk is evaluated
+    uint32_t ec = lane < nb ? lst[lane] : 0u;
+    double4 rjc = make_double4(0., 0., 0., 0.);
+    int tjc = 0;
+    if ((ec >> 28) & (fv | fe)) {
+      const int j = (int)(ec & NB_IDX_MASK);
+      rjc = A.rec[j];
+      tjc = A.stype[j];
+    }
+    for (int k0 = 0; k0 < nb; k0 += NB_WAVE) {
+      const int kn = k0 + NB_WAVE + lane;
+      const uint32_t en = kn < nb ? lst[kn] : 0u;
+      double4 rjn = make_double4(0., 0., 0., 0.);
+      int tjn = 0;
+      if ((en >> 28) & (fv | fe)) {
+        const int j = (int)(en & NB_IDX_MASK);
+        rjn = A.rec[j];
+        tjn = A.stype[j];
+      }
+      const unsigned fl = ec >> 28;
+      if (fl & (fv | fe)) {
+        const double4 rj = rjc;
+        double dx = ri.x - rj.x, dy = ri.y - rj.y, dz = ri.z - rj.z;
+        nb_minimg_box(dx, dy, dz, A.b);
+        double d2 = dx * dx + dy * dy + dz * dz;
+        if (fl & fv) {
+          const int in = ip_lds ? smap[ti * A.ntypes + tjc] : A.map[ti * A.ntypes + tjc];
+          const InterPar &P = ip_lds ? sip[in] : A.ip[in];
+          double rmax2 = IFLG ? P.rmaxl2 : P.rmaxs2;
+          if (d2 < rmax2) {
+            double p = (d2 - P.rmins2) * P.dx2;
+            int kk = (int)p;
+            if (kk < 1) { acc[E_CLAMP] += 1.; kk = 1; p = 1.; }
+            p = p - (double)kk;
+            const double *T = vt + (size_t)in * A.ntable * 3;
+            double f = nb_interp3(T, kk, 0, p);
+            double swv = 1.;
+            if ((WANT_FS || !IFLG) && d2 > P.rminl2) swv = nb_interp3(T, kk, 1, p);
+            if (IFLG) {
+              acc[E_SPRSL] -= d2 * f;
+              acc[E_SL0] -= dx * dx * f;
+              acc[E_SL1] -= dy * dx * f;
+              acc[E_SL2] -= dz * dx * f;
+              acc[E_SL4] -= dy * dy * f;
+              acc[E_SL5] -= dz * dy * f;
+              acc[E_SL8] -= dz * dz * f;
+              double gx = dx * f, gy = dy * f, gz = dz * f;
+              fx -= gx; fy -= gy; fz -= gz;
+              if (WANT_FS) { sx -= gx * swv; sy -= gy * swv; sz -= gz * swv; }
+              double v = nb_interp3(T, kk, 2, p);
+              acc[E_SPOTL] += v;
+              acc[E_SPOTHL] += v - T[3 * ntm4 + 2];
+            } else {
+              f *= swv;
+              sx -= dx * f; sy -= dy * f; sz -= dz * f;
+            }
           }
         }
-      }
-      if ((fl & fe) && d2 < A.rmaxl_coul2) {
-        double p = (d2 - A.rmins_coul2) * A.dx2coul;
-        int kk = (int)p;
-        if (kk < 1) { acc[E_CLAMP] += 1.; kk = 1; p = 1; }
-        p = p - (double)kk;
-        double qij = ri.w * rj.w;
-        double fcc = qij * nb_interp3(ct, kk, 0, p);
-        double gx = dx * fcc, gy = dy * fcc, gz = dz * fcc;
-        if (IFLG) {
-          acc[E_SPRSE] -= d2 * fcc;
-          acc[E_SE0] -= dx * dx * fcc;
-          acc[E_SE1] -= dy * dx * fcc;
-          acc[E_SE2] -= dz * dx * fcc;
-          acc[E_SE4] -= dy * dy * fcc;
-          acc[E_SE5] -= dz * dy * fcc;
-          acc[E_SE8] -= dz * dz * fcc;
-          fx -= gx; fy -= gy; fz -= gz;
-          double v = nb_interp3(ct, kk, 2, p);
-          acc[E_SPOTHE] += qij * (v - A.clast);
-          acc[E_SPOTE] += qij * v;
+        if ((fl & fe) && d2 < A.rmaxl_coul2) {
+          double p = (d2 - A.rmins_coul2) * A.dx2coul;
+          int kk = (int)p;
+          if (kk < 1) { acc[E_CLAMP] += 1.; kk = 1; p = 1; }
+          p = p - (double)kk;
+          double qij = ri.w * rj.w;
+          double fcc = qij * nb_interp3(ct, kk, 0, p);
+          double gx = dx * fcc, gy = dy * fcc, gz = dz * fcc;
+          if (IFLG) {
+            acc[E_SPRSE] -= d2 * fcc;
+            acc[E_SE0] -= dx * dx * fcc;
+            acc[E_SE1] -= dy * dx * fcc;
+            acc[E_SE2] -= dz * dx * fcc;
+            acc[E_SE4] -= dy * dy * fcc;
+            acc[E_SE5] -= dz * dy * fcc;
+            acc[E_SE8] -= dz * dz * fcc;
+            fx -= gx; fy -= gy; fz -= gz;
+            double v = nb_interp3(ct, kk, 2, p);
+            acc[E_SPOTHE] += qij * (v - A.clast);
+            acc[E_SPOTE] += qij * v;
+          }
+          if (WANT_FS || !IFLG) { sx -= gx; sy -= gy; sz -= gz; }
         }
-        if (WANT_FS || !IFLG) { sx -= gx; sy -= gy; sz -= gz; }
       }
+      ec = en;
+      rjc = rjn;
+      tjc = tjn;
     }
     const int ai = A.perm[s];
     if (IFLG) { fx = nb_wave_sum(fx); fy = nb_wave_sum(fy); fz = nb_wave_sum(fz); }
@@ -479,9 +540,9 @@ static __device__ __forceinline__ void pair64_body(const PairArgs &A, int bid) {
 #define k_pair64_0t_body pair64_body<0, true>
 #define k_pair64_1t_body pair64_body<1, true>
 #define k_pair64_1f_body pair64_body<1, false>
-NB_KERNEL_PAIR(k_pair64_0t, PairArgs, 256)
-NB_KERNEL_PAIR(k_pair64_1t, PairArgs, 256)
-NB_KERNEL_PAIR(k_pair64_1f, PairArgs, 256)
+NB_KERNEL_PAIR_W(k_pair64_0t, PairArgs, 256, 4)
+NB_KERNEL_PAIR_W(k_pair64_1t, PairArgs, 256, 3)
+NB_KERNEL_PAIR_W(k_pair64_1f, PairArgs, 256, 4)
 
 // ecorr (md_ewald.c:385-453): one thread per atom over its excluded charged
 // partners (both directions, so energies and virials are halved later), on

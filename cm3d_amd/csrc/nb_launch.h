@@ -44,9 +44,13 @@ typedef void (*BatchFn)(const void *args, const int2 *bmap, int nblk, int tpb, i
                         hipStream_t s);
 extern BatchFn g_batch_fn[K_COUNT];
 
-#define NB_KERNEL_PAIR(name, ArgsT, TPB)                                             \
-  __global__ __launch_bounds__(TPB) void name(ArgsT A) { name##_body(A, blockIdx.x); } \
-  __global__ __launch_bounds__(TPB) void name##_b(const ArgsT *__restrict__ AA, const int2 *__restrict__ bm) {\
+#define NB_KERNEL_PAIR(name, ArgsT, TPB) NB_KERNEL_PAIR_W(name, ArgsT, TPB, 1)
+// WPE: waves per SIMD the register allocation must leave room for
+#define NB_KERNEL_PAIR_W(name, ArgsT, TPB, WPE)                                      \
+  __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE)))        \
+  void name(ArgsT A) { name##_body(A, blockIdx.x); }                                 \
+  __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(WPE)))        \
+  void name##_b(const ArgsT *__restrict__ AA, const int2 *__restrict__ bm) {         \
     const int2 m = bm[blockIdx.x];                                                   \
     const int sys = __builtin_amdgcn_readfirstlane(m.x);                             \
     const int lb = __builtin_amdgcn_readfirstlane(m.y);                              \

@@ -9,12 +9,12 @@
 //     precision is 2^-24 of the pair distance whatever the box size.
 //   - table interpolation, forces and per-pair energies in FP32 from FP32
 //     copies of the tables (LDS-resident);
-//   - each lane sums its pairs' FP32 forces in FP64 (exact for a handful of
-//     terms), the lane sums are converted to 64-bit fixed point (2^-24 K/A
-//     resolution) and reduced across the wavefront in integer arithmetic, so
-//     the per-atom force does not depend on how pairs are spread over lanes
-//     or in which order partial sums meet;
-//   - energies and virials accumulate in FP64.
+//   - each lane sums its few pair forces for an atom in FP32, the lane sums
+//     are converted to 64-bit fixed point (2^-24 K/A resolution) and reduced
+//     across the wavefront in integer arithmetic, so the per-atom force does
+//     not depend on the order in which the lanes' partial sums meet;
+//   - energies and virials: per-lane FP32 sums for an atom, accumulated in
+//     FP64 across atoms and reduced in FP64 in a fixed order.
 #include "nb_launch.h"
 
 #define NB_FX_SCALE 16777216.0  // 2^24 fixed-point units per K/A
@@ -52,12 +52,14 @@ static __device__ __forceinline__ void k_packmix_body(const PackMixArgs &A, int 
 }
 NB_KERNEL_PAIR(k_packmix, PackMixArgs, 256)
 
+#define NB_TPF_LDS 256  // type pairs whose FP32 parameters are staged in LDS
+
 struct MixArgs {
   int n, ntypes, ninter, ntable, cap, ctab_off, tab_lds, nblocks;
   const uint32_t *nbr;
-  const int *nnb, *stype, *perm, *map;
+  const int *nnb, *stype, *perm;
   const uint4 *rec;
-  const InterPar *ip;
+  const TPairF *tpf;
   const float *vtab3, *ctab3;
   float h[9];
   float rmins_coul2, dx2coul, rmaxl_coul2, clast;
@@ -99,20 +101,31 @@ static __device__ __forceinline__ long long to_fix(double v) {
   return __double2ll_rn(v * NB_FX_SCALE);
 }
 
+// One wavefront per sorted atom; each lane takes every 64th list entry.  A
+// lane's pair terms for the atom are summed in FP32 (a handful of terms),
+// then the force lane sums go to fixed point and are reduced in integers,
+// and the energy / virial lane sums are added in FP64 across atoms.  The
+// next chunk's list entries and partner records are loaded before the
+// current chunk is evaluated.
 template <int IFLG, bool WANT_FS>
 static __device__ __forceinline__ void pair_mixed_body(const MixArgs &A, int bid) {
   // FP32 pair arithmetic may contract to FMA (the FP64 path may not)
 #pragma clang fp contract(fast)
   extern __shared__ float tabsf[];
+  __shared__ TPairF stp[NB_TPF_LDS];
   const float *vt = A.vtab3, *ct = A.ctab3;
   if (A.tab_lds) {
     int nv = A.ninter * A.ntable * 3, nc = A.ntable * 3;
     for (int t = threadIdx.x; t < nv; t += blockDim.x) tabsf[t] = A.vtab3[t];
     for (int t = threadIdx.x; t < nc; t += blockDim.x) tabsf[A.ctab_off + t] = A.ctab3[t];
-    __syncthreads();
     vt = tabsf;
     ct = tabsf + A.ctab_off;
   }
+  const int nt2 = A.ntypes * A.ntypes;
+  const bool tp_lds = nt2 <= NB_TPF_LDS;
+  if (tp_lds)
+    for (int t = threadIdx.x; t < nt2; t += blockDim.x) stp[t] = A.tpf[t];
+  __syncthreads();
   double acc[NB_NE_PAIR];
 #pragma unroll
   for (int k = 0; k < NB_NE_PAIR; k++) acc[k] = 0.;
@@ -120,7 +133,6 @@ static __device__ __forceinline__ void pair_mixed_body(const MixArgs &A, int bid
   const int nw = A.nblocks * 4;
   constexpr unsigned fv = IFLG ? NB_FLAG_VT : NB_FLAG_VS;
   constexpr unsigned fe = IFLG ? NB_FLAG_ET : NB_FLAG_ES;
-  const int ntm4 = A.ntable - 4;
   const float s32 = 2.3283064365386963e-10f;  // 2^-32
   for (int s = bid * 4 + (threadIdx.x >> 6); s < A.n; s += nw) {
     const uint4 ri = A.rec[s];
@@ -128,79 +140,103 @@ static __device__ __forceinline__ void pair_mixed_body(const MixArgs &A, int bid
     const int ti = A.stype[s];
     const int nb = A.nnb[s];
     const uint32_t *lst = A.nbr + (size_t)s * A.cap;
-    double fx = 0., fy = 0., fz = 0., sx = 0., sy = 0., sz = 0.;
-    for (int k = lane; k < nb; k += NB_WAVE) {
-      uint32_t e = lst[k];
-      unsigned fl = e >> 28;
-      if (!(fl & (fv | fe))) continue;
-      int j = (int)(e & NB_IDX_MASK);
-      uint4 rj = A.rec[j];
-      float ux = (float)(int)(ri.x - rj.x) * s32;
-      float uy = (float)(int)(ri.y - rj.y) * s32;
-      float uz = (float)(int)(ri.z - rj.z) * s32;
-      float dx = ux * A.h[0] + uy * A.h[1] + uz * A.h[2];
-      float dy = ux * A.h[3] + uy * A.h[4] + uz * A.h[5];
-      float dz = ux * A.h[6] + uy * A.h[7] + uz * A.h[8];
-      float d2 = dx * dx + dy * dy + dz * dz;
-      if (fl & fv) {
-        int in = A.map[ti * A.ntypes + A.stype[j]];
-        const InterPar &P = A.ip[in];
-        float rmax2 = (float)(IFLG ? P.rmaxl2 : P.rmaxs2);
-        if (d2 < rmax2) {
-          float p = (d2 - (float)P.rmins2) * (float)P.dx2;
-          int kk = (int)p;
-          if (kk < 1) { acc[M_CLAMP] += 1.; kk = 1; p = 1.f; }
-          if (kk > A.ntable - 3) kk = A.ntable - 3;
-          p = p - (float)kk;
-          const float *T = vt + (size_t)in * A.ntable * 3;
-          float f = interp3f(T, kk, 0, p);
-          float swv = 1.f;
-          if ((WANT_FS || !IFLG) && d2 > (float)P.rminl2) swv = interp3f(T, kk, 1, p);
-          if (IFLG) {
-            acc[M_SPRSL] -= (double)(d2 * f);
-            acc[M_SL0] -= (double)(dx * dx * f);
-            acc[M_SL1] -= (double)(dy * dx * f);
-            acc[M_SL2] -= (double)(dz * dx * f);
-            acc[M_SL4] -= (double)(dy * dy * f);
-            acc[M_SL5] -= (double)(dz * dy * f);
-            acc[M_SL8] -= (double)(dz * dz * f);
-            float gx = dx * f, gy = dy * f, gz = dz * f;
-            fx -= (double)gx; fy -= (double)gy; fz -= (double)gz;
-            if (WANT_FS) { sx -= (double)(gx * swv); sy -= (double)(gy * swv); sz -= (double)(gz * swv); }
-            float v = interp3f(T, kk, 2, p);
-            acc[M_SPOTL] += (double)v;
-            acc[M_SPOTHL] += (double)(v - T[3 * ntm4 + 2]);
-          } else {
-            f *= swv;
-            sx -= (double)(dx * f); sy -= (double)(dy * f); sz -= (double)(dz * f);
+    const TPairF *tprow = (tp_lds ? stp : A.tpf) + ti * A.ntypes;
+    float fx = 0.f, fy = 0.f, fz = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
+    float e[NB_NE_PAIR];
+#pragma unroll
+    for (int k = 0; k < NB_NE_PAIR; k++) e[k] = 0.f;
+    uint32_t ec = lane < nb ? lst[lane] : 0u;
+    uint4 rjc = make_uint4(0u, 0u, 0u, 0u);
+    int tjc = 0;
+    if ((ec >> 28) & (fv | fe)) {
+      const int j = (int)(ec & NB_IDX_MASK);
+      rjc = A.rec[j];
+      tjc = A.stype[j];
+    }
+    for (int k0 = 0; k0 < nb; k0 += NB_WAVE) {
+      const int kn = k0 + NB_WAVE + lane;
+      const uint32_t en = kn < nb ? lst[kn] : 0u;
+      uint4 rjn = make_uint4(0u, 0u, 0u, 0u);
+      int tjn = 0;
+      if ((en >> 28) & (fv | fe)) {
+        const int j = (int)(en & NB_IDX_MASK);
+        rjn = A.rec[j];
+        tjn = A.stype[j];
+      }
+      const unsigned fl = ec >> 28;
+      if (fl & (fv | fe)) {
+        const uint4 rj = rjc;
+        float ux = (float)(int)(ri.x - rj.x) * s32;
+        float uy = (float)(int)(ri.y - rj.y) * s32;
+        float uz = (float)(int)(ri.z - rj.z) * s32;
+        float dx = ux * A.h[0] + uy * A.h[1] + uz * A.h[2];
+        float dy = ux * A.h[3] + uy * A.h[4] + uz * A.h[5];
+        float dz = ux * A.h[6] + uy * A.h[7] + uz * A.h[8];
+        float d2 = dx * dx + dy * dy + dz * dz;
+        if (fl & fv) {
+          const TPairF P = tprow[tjc];
+          const float rmax2 = IFLG ? P.rmaxl2 : P.rmaxs2;
+          if (d2 < rmax2) {
+            float p = (d2 - P.rmins2) * P.dx2;
+            int kk = (int)p;
+            if (kk < 1) { e[M_CLAMP] += 1.f; kk = 1; p = 1.f; }
+            if (kk > A.ntable - 3) kk = A.ntable - 3;
+            p = p - (float)kk;
+            const float *T = vt + (size_t)P.in * A.ntable * 3;
+            float f = interp3f(T, kk, 0, p);
+            float swv = 1.f;
+            if ((WANT_FS || !IFLG) && d2 > P.rminl2) swv = interp3f(T, kk, 1, p);
+            if (IFLG) {
+              e[M_SPRSL] -= d2 * f;
+              e[M_SL0] -= dx * dx * f;
+              e[M_SL1] -= dy * dx * f;
+              e[M_SL2] -= dz * dx * f;
+              e[M_SL4] -= dy * dy * f;
+              e[M_SL5] -= dz * dy * f;
+              e[M_SL8] -= dz * dz * f;
+              float gx = dx * f, gy = dy * f, gz = dz * f;
+              fx -= gx; fy -= gy; fz -= gz;
+              if (WANT_FS) { sx -= gx * swv; sy -= gy * swv; sz -= gz * swv; }
+              float v = interp3f(T, kk, 2, p);
+              e[M_SPOTL] += v;
+              e[M_SPOTHL] += v - P.vlast;
+            } else {
+              f *= swv;
+              sx -= dx * f; sy -= dy * f; sz -= dz * f;
+            }
           }
         }
-      }
-      if ((fl & fe) && d2 < A.rmaxl_coul2) {
-        float p = (d2 - A.rmins_coul2) * A.dx2coul;
-        int kk = (int)p;
-        if (kk < 1) { acc[M_CLAMP] += 1.; kk = 1; p = 1.f; }
-        if (kk > A.ntable - 3) kk = A.ntable - 3;
-        p = p - (float)kk;
-        float qij = qi * __uint_as_float(rj.w);
-        float fcc = qij * interp3f(ct, kk, 0, p);
-        float gx = dx * fcc, gy = dy * fcc, gz = dz * fcc;
-        if (IFLG) {
-          acc[M_SPRSE] -= (double)(d2 * fcc);
-          acc[M_SE0] -= (double)(dx * dx * fcc);
-          acc[M_SE1] -= (double)(dy * dx * fcc);
-          acc[M_SE2] -= (double)(dz * dx * fcc);
-          acc[M_SE4] -= (double)(dy * dy * fcc);
-          acc[M_SE5] -= (double)(dz * dy * fcc);
-          acc[M_SE8] -= (double)(dz * dz * fcc);
-          fx -= (double)gx; fy -= (double)gy; fz -= (double)gz;
-          float v = interp3f(ct, kk, 2, p);
-          acc[M_SPOTHE] += (double)(qij * (v - A.clast));
-          acc[M_SPOTE] += (double)(qij * v);
+        if ((fl & fe) && d2 < A.rmaxl_coul2) {
+          float p = (d2 - A.rmins_coul2) * A.dx2coul;
+          int kk = (int)p;
+          if (kk < 1) { e[M_CLAMP] += 1.f; kk = 1; p = 1.f; }
+          if (kk > A.ntable - 3) kk = A.ntable - 3;
+          p = p - (float)kk;
+          float qij = qi * __uint_as_float(rj.w);
+          float fcc = qij * interp3f(ct, kk, 0, p);
+          float gx = dx * fcc, gy = dy * fcc, gz = dz * fcc;
+          if (IFLG) {
+            e[M_SPRSE] -= d2 * fcc;
+            e[M_SE0] -= dx * dx * fcc;
+            e[M_SE1] -= dy * dx * fcc;
+            e[M_SE2] -= dz * dx * fcc;
+            e[M_SE4] -= dy * dy * fcc;
+            e[M_SE5] -= dz * dy * fcc;
+            e[M_SE8] -= dz * dz * fcc;
+            fx -= gx; fy -= gy; fz -= gz;
+            float v = interp3f(ct, kk, 2, p);
+            e[M_SPOTHE] += qij * (v - A.clast);
+            e[M_SPOTE] += qij * v;
+          }
+          if (WANT_FS || !IFLG) { sx -= gx; sy -= gy; sz -= gz; }
         }
-        if (WANT_FS || !IFLG) { sx -= (double)gx; sy -= (double)gy; sz -= (double)gz; }
       }
+      ec = en;
+      rjc = rjn;
+      tjc = tjn;
     }
+#pragma unroll
+    for (int k = 0; k < NB_NE_PAIR; k++) acc[k] += (double)e[k];
     const int ai = A.perm[s];
     const int n = A.n;
     if (IFLG) {
@@ -239,7 +275,7 @@ int nb_launch_pair_mixed(nb_ctx *c, int iflg, const Box &b, double *d_fs, double
   M.cap = c->cap; M.ctab_off = c->ctab_off; M.tab_lds = c->tab_lds_bytes_f > 0;
   M.nblocks = c->npair_blocks;
   M.nbr = c->d_nbr; M.nnb = c->d_nnb; M.stype = c->d_stype; M.perm = c->d_perm;
-  M.map = c->d_map; M.rec = (const uint4 *)c->d_recf; M.ip = c->d_ip;
+  M.rec = (const uint4 *)c->d_recf; M.tpf = (const TPairF *)c->d_tpf;
   M.vtab3 = c->d_vtab3f; M.ctab3 = c->d_ctab3f;
   for (int k = 0; k < 9; k++) M.h[k] = (float)b.h[k];
   M.rmins_coul2 = (float)c->rmins_coul2; M.dx2coul = (float)c->dx2coul;

@@ -244,3 +244,33 @@ def mixture(natoms=2000, seed=7, rc=9.0, alpha=0.35, kmax=7, skin=1.0, rheal=0.5
     s["vel"] = maxwell_velocities(s["amass"], temp, seed + 1)
     s["name"] = f"mixture_{natoms}"
     return s
+
+
+def kinetic_temperature(vel, amass):
+    """Instantaneous temperature (K) over 3N-3 degrees of freedom (kB = 1
+    in cm3d's units: sum m v^2 is in K)."""
+    n = len(amass)
+    v = np.asarray(vel).reshape(3, n)
+    return float((amass[None, :] * v * v).sum() / (3 * n - 3))
+
+
+def equilibrate(mds, systems, temp, nsteps, every=10, stepper=None):
+    """Bring lattice starts to temperature: advance nsteps outer steps,
+    rescaling every system's velocities to temp every `every` steps (a set-up
+    device, not one of cm3d's thermostats).  mds: NBMD objects (already
+    init()ed); stepper(k) advances all of them by k steps (default: each
+    md.step(k))."""
+    if stepper is None:
+        def stepper(k):
+            for m in mds:
+                m.step(k, graph=True)
+    done = 0
+    while done < nsteps:
+        k = min(every, nsteps - done)
+        stepper(k)
+        done += k
+        for m, s in zip(mds, systems):
+            pos, vel = m.get_state()
+            t = kinetic_temperature(vel, s["amass"])
+            if t > 0:
+                m.set_state(pos, vel * math.sqrt(temp / t), s["hmat"])
