@@ -57,6 +57,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--only-batch", action="store_true",
+                    help="run the batched form alone (profiling); prints its own JSON")
     return ap.parse_args()
 
 
@@ -198,6 +200,11 @@ def main():
 
     if not torch.cuda.is_available():
         raise SystemExit("bench.py needs a GPU (libcm3d_nb.so has no CPU path)")
+    if a.only_batch:
+        b = run_batch(nb, S, a, torch)
+        print(json.dumps({"batched_atom_steps_per_s": b["natoms_total"] * a.batch_steps / b["wall"],
+                          "ms_per_step": 1e3 * b["wall"] / a.batch_steps}), flush=True)
+        return
     s, r = run_single(nb, S, a, torch)
     n = r["n"]
     wall = r["wall"]

@@ -356,7 +356,10 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
   if ((rc = dalloc(&c->d_spart, 2 * (size_t)c->nchunk * std::max(c->ktot, 1)))) return fail(rc);
   if ((rc = dalloc((char **)&c->d_kd, nb_kdata_bytes() * std::max(c->ktot, 1)))) return fail(rc);
   if ((rc = dalloc(&c->d_fpart, 3 * (size_t)c->nrange * std::max(ncharge, 1)))) return fail(rc);
-  c->npair_blocks = std::min((n + 3) / 4, 2048);
+  // pair kernels: a wave per atom in turn; the FP32 kernel gives each wave
+  // about 4 atoms so its once-per-wave energy reduction is amortised
+  c->npair_blocks = p->precision == NB_PREC_MIXED ? std::max(1, std::min((n + 15) / 16, 2048))
+                                                  : std::min((n + 3) / 4, 2048);
   c->new_blocks = std::max(1, (c->ktot + NB_WAVE - 1) / NB_WAVE);
   c->necorr_blocks = (n + 63) / 64;
   if ((rc = dalloc(&c->d_epair, (size_t)c->npair_blocks * NB_NE_PAIR))) return fail(rc);

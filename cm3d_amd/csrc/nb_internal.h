@@ -82,11 +82,48 @@ __device__ __forceinline__ double nb_interp3(const double *t, int k, int c,
   return f0 + .5 * p * (f1 - fm1 + p * (fm1 - 2. * f0 + f1));
 }
 
+// Wavefront sums with DPP lane moves (VALU, no LDS round trips): swaps
+// within quads, then rotations by 4 and 8 lanes within each row of 16,
+// leave a row total in every lane of a row; the totals of rows 0..3 are then
+// read from lanes 0, 16, 32, 48 and added in a fixed order.  Deterministic,
+// and every lane returns the same value.
+template <int CTRL>
+__device__ __forceinline__ int nb_dpp_i(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ long long nb_dpp(long long v) {
+  const int lo = nb_dpp_i<CTRL>((int)v), hi = nb_dpp_i<CTRL>((int)(v >> 32));
+  return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+template <int CTRL>
+__device__ __forceinline__ int nb_dpp(int v) { return nb_dpp_i<CTRL>(v); }
+template <int CTRL>
+__device__ __forceinline__ double nb_dpp(double v) {
+  return __longlong_as_double(nb_dpp<CTRL>((long long)__double_as_longlong(v)));
+}
+template <int CTRL>
+__device__ __forceinline__ float nb_dpp(float v) {
+  return __int_as_float(nb_dpp_i<CTRL>(__float_as_int(v)));
+}
+__device__ __forceinline__ int nb_readlane(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ long long nb_readlane(long long v, int l) {
+  const int lo = __builtin_amdgcn_readlane((int)v, l), hi = __builtin_amdgcn_readlane((int)(v >> 32), l);
+  return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double nb_readlane(double v, int l) {
+  return __longlong_as_double(nb_readlane((long long)__double_as_longlong(v), l));
+}
+__device__ __forceinline__ float nb_readlane(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
 template <typename T>
 __device__ __forceinline__ T nb_wave_sum(T v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, NB_WAVE);
-  return v;
+  v += nb_dpp<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += nb_dpp<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += nb_dpp<0x124>(v);  // row_ror:4
+  v += nb_dpp<0x128>(v);  // row_ror:8
+  return (nb_readlane(v, 0) + nb_readlane(v, 16)) + (nb_readlane(v, 32) + nb_readlane(v, 48));
 }
 
 struct Box {

@@ -13,19 +13,15 @@
 //     are converted to 64-bit fixed point (2^-24 K/A resolution) and reduced
 //     across the wavefront in integer arithmetic, so the per-atom force does
 //     not depend on the order in which the lanes' partial sums meet;
-//   - energies and virials: per-lane FP32 sums for an atom, accumulated in
-//     FP64 across atoms and reduced in FP64 in a fixed order.
+//   - energies and virials: per-lane FP32 sums (a few tens of terms), then
+//     reduced in FP64 in a fixed order.
 #include "nb_launch.h"
 
 #define NB_FX_SCALE 16777216.0  // 2^24 fixed-point units per K/A
 
 static __device__ __forceinline__ int lane_idm() { return threadIdx.x & (NB_WAVE - 1); }
 
-static __device__ __forceinline__ long long wave_sum_ll(long long v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, NB_WAVE);
-  return v;
-}
+static __device__ __forceinline__ long long wave_sum_ll(long long v) { return nb_wave_sum(v); }
 
 // Sorted-order records for the mixed kernel: {u_x, u_y, u_z, q as f32 bits}.
 struct PackMixArgs { const double *pos, *q; const int *perm; int n; Box b; uint4 *rec; };
@@ -101,12 +97,13 @@ static __device__ __forceinline__ long long to_fix(double v) {
   return __double2ll_rn(v * NB_FX_SCALE);
 }
 
-// One wavefront per sorted atom; each lane takes every 64th list entry.  A
-// lane's pair terms for the atom are summed in FP32 (a handful of terms),
-// then the force lane sums go to fixed point and are reduced in integers,
-// and the energy / virial lane sums are added in FP64 across atoms.  The
-// next chunk's list entries and partner records are loaded before the
-// current chunk is evaluated.
+// One wavefront per sorted atom at a time (a wave takes several atoms in
+// turn); each lane takes every 64th list entry.  A lane's pair forces for the
+// atom are summed in FP32 (a handful of terms), then go to fixed point and
+// are reduced across the wave in integers.  Energies and virials: FP32 sums
+// per lane over all its pairs (a few tens), reduced in FP64 once per
+// workgroup.  The next chunk's list entries and partner records are loaded
+// before the current chunk is evaluated.
 template <int IFLG, bool WANT_FS>
 static __device__ __forceinline__ void pair_mixed_body(const MixArgs &A, int bid) {
   // FP32 pair arithmetic may contract to FMA (the FP64 path may not)
@@ -126,9 +123,9 @@ static __device__ __forceinline__ void pair_mixed_body(const MixArgs &A, int bid
   if (tp_lds)
     for (int t = threadIdx.x; t < nt2; t += blockDim.x) stp[t] = A.tpf[t];
   __syncthreads();
-  double acc[NB_NE_PAIR];
+  float e[NB_NE_PAIR];  // this lane's energy / virial sums over its pairs
 #pragma unroll
-  for (int k = 0; k < NB_NE_PAIR; k++) acc[k] = 0.;
+  for (int k = 0; k < NB_NE_PAIR; k++) e[k] = 0.f;
   const int lane = lane_idm();
   const int nw = A.nblocks * 4;
   constexpr unsigned fv = IFLG ? NB_FLAG_VT : NB_FLAG_VS;
@@ -142,9 +139,6 @@ static __device__ __forceinline__ void pair_mixed_body(const MixArgs &A, int bid
     const uint32_t *lst = A.nbr + (size_t)s * A.cap;
     const TPairF *tprow = (tp_lds ? stp : A.tpf) + ti * A.ntypes;
     float fx = 0.f, fy = 0.f, fz = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
-    float e[NB_NE_PAIR];
-#pragma unroll
-    for (int k = 0; k < NB_NE_PAIR; k++) e[k] = 0.f;
     uint32_t ec = lane < nb ? lst[lane] : 0u;
     uint4 rjc = make_uint4(0u, 0u, 0u, 0u);
     int tjc = 0;
@@ -235,8 +229,6 @@ static __device__ __forceinline__ void pair_mixed_body(const MixArgs &A, int bid
       rjc = rjn;
       tjc = tjn;
     }
-#pragma unroll
-    for (int k = 0; k < NB_NE_PAIR; k++) acc[k] += (double)e[k];
     const int ai = A.perm[s];
     const int n = A.n;
     if (IFLG) {
@@ -256,6 +248,9 @@ static __device__ __forceinline__ void pair_mixed_body(const MixArgs &A, int bid
       }
     }
   }
+  double acc[NB_NE_PAIR];
+#pragma unroll
+  for (int k = 0; k < NB_NE_PAIR; k++) acc[k] = (double)e[k];
   block_reduce_store_m<NB_NE_PAIR>(acc, A.epart + bid, A.nblocks);
 }
 #define k_mix_0t_body pair_mixed_body<0, true>
