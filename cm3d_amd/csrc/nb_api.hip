@@ -85,7 +85,7 @@ extern "C" void nb_destroy(nb_ctx *c) {
   hipSetDevice(c->dev);
   void *ptrs[] = {c->d_itype, c->d_map, c->d_ip, c->d_vtab3, c->d_ctab3, c->d_vtab3f,
                   c->d_ctab3f, c->d_qch, c->d_xoff, c->d_xlist, c->d_eoff, c->d_elist,
-                  c->d_qidx, c->d_seg, c->d_sseg, c->d_tseg, c->d_rtile, c->d_fpart, c->d_kabc, c->d_pos, c->d_posb, c->d_fs, c->d_fl,
+                  c->d_qidx, c->d_qpos, c->d_seg, c->d_sseg, c->d_tseg, c->d_rtile, c->d_fpart, c->d_kabc, c->d_pos, c->d_posb, c->d_fs, c->d_fl,
                   c->d_perm, c->d_iperm, c->d_cell, c->d_ccount, c->d_cstart, c->d_cfill, c->d_cwork,
                   c->d_hcnt, c->d_pcut,
                   c->d_tmp, c->d_rec64, c->d_stype, c->d_nbr, c->d_nnb, c->d_flags,
@@ -176,9 +176,9 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
     ip[k].rmaxl_skin2 = p->rmaxl_skin2[k]; ip[k].vlast = p->vtab[(size_t)k * nt + nt - 4];
   }
   int ncharge = 0;
-  std::vector<int> qidx;
+  std::vector<int> qidx, qpos(n, -1);
   for (int i = 0; i < n; i++)
-    if (p->qch[i] != 0.) { qidx.push_back(i); ncharge++; }
+    if (p->qch[i] != 0.) { qpos[i] = ncharge; qidx.push_back(i); ncharge++; }
   c->ncharge = ncharge;
   if (ncharge > 0) {
     if (!p->coultab || !p->dcoultab || !p->switchse) {
@@ -333,6 +333,7 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
   if ((rc = upload(&c->d_eoff, eoff.data(), n + 1))) return fail(rc);
   if ((rc = upload(&c->d_elist, elist.data(), elist.size()))) return fail(rc);
   if ((rc = upload(&c->d_qidx, qidx.data(), qidx.size()))) return fail(rc);
+  if ((rc = upload(&c->d_qpos, qpos.data(), qpos.size()))) return fail(rc);
   if ((rc = upload(&c->d_seg, seg.data(), seg.size()))) return fail(rc);
   if ((rc = upload(&c->d_sseg, sseg.data(), sseg.size()))) return fail(rc);
   if ((rc = upload(&c->d_tseg, tseg.data(), tseg.size()))) return fail(rc);
@@ -361,9 +362,9 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
   c->npair_blocks = p->precision == NB_PREC_MIXED ? std::max(1, std::min((n + 15) / 16, 2048))
                                                   : std::min((n + 3) / 4, 2048);
   c->new_blocks = std::max(1, (c->ktot + NB_WAVE - 1) / NB_WAVE);
-  c->necorr_blocks = (n + 63) / 64;
+  c->nfin_blocks = std::max(1, (ncharge + 255) / 256);
   if ((rc = dalloc(&c->d_epair, (size_t)c->npair_blocks * NB_NE_PAIR))) return fail(rc);
-  if ((rc = dalloc(&c->d_eec, (size_t)c->necorr_blocks * NB_NE_ECORR))) return fail(rc);
+  if ((rc = dalloc(&c->d_eec, (size_t)c->nfin_blocks * NB_NE_ECORR))) return fail(rc);
   if ((rc = dalloc(&c->d_eew, (size_t)c->new_blocks * NB_NE_EW))) return fail(rc);
   if ((rc = dalloc(&c->d_eout, 64))) return fail(rc);
   if (hipHostMalloc((void **)&c->h_flags, 16 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
@@ -568,7 +569,7 @@ static int force_on(nb_ctx *c, int iflg, const double hmat[9], double *d_fs, dou
   if (iflg != 0 && iflg != 1) { nb_set_error("nb_force: iflg must be 0 or 1"); return NB_ERR_ARG; }
   if (!c->have_lists) { nb_set_error("nb_force: lists not built (call nb_build_lists)"); return NB_ERR_ARG; }
   Box b = make_box(hmat);
-  RC(nb_launch_force(c, iflg, b, d_fs, d_fl, s));
+  RC(nb_launch_force(c, iflg, b, d_fs, d_fl, s, 0));
   if (e) {
     double E[64];
     HCHK(hipMemcpyAsync(E, c->d_eout, sizeof(E), hipMemcpyDeviceToHost, s));

@@ -166,6 +166,7 @@ struct nb_ctx {
   int *d_xoff, *d_xlist;       // exclusions, both directions, flag bits
   int *d_eoff, *d_elist;       // ecorr partners, both directions
   int *d_qidx;                 // charge -> atom
+  int *d_qpos;                 // atom -> charge (-1: uncharged)
   EwSeg *d_seg;                // k order
   EwSeg *d_sseg;               // sorted longest first (structure-factor lanes)
   int *d_tseg, *d_rtile;       // force pass: tiles (segment bounds), ranges (tile bounds)
@@ -188,7 +189,7 @@ struct nb_ctx {
   int nchunk, qc, nsw;         // structure factor: charge chunks, segment waves
   int nrange, nqb, ntile;      // force: tile ranges, charge workgroups, tiles
   double *d_epair, *d_eew, *d_eec, *d_eout;
-  int npair_blocks, new_blocks, necorr_blocks;
+  int npair_blocks, new_blocks, nfin_blocks;
   long long *d_fxp;            // fixed-point force accumulators (mixed)
   float4 *d_recf;              // mixed: {sx, sy, sz (u32 bits), q}
   TPairF *d_tpf;               // mixed: per type pair parameters
@@ -210,10 +211,12 @@ struct nb_ctx {
 int nb_launch_build(nb_ctx *c, const Box &b, hipStream_t s);
 int nb_launch_build_cond(nb_ctx *c, const Box &b, hipStream_t s);
 int nb_build_now(nb_ctx *c, const double hmat[9], hipStream_t s);
+#define NB_FORCE_PACKED 1  // sorted records / Ewald charge data already current
+#define NB_FORCE_NORED 2   // leave the energy columns unreduced
 int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs,
-                    double *d_fl, hipStream_t s);
+                    double *d_fl, hipStream_t s, int flags);
 int nb_launch_rebuild_check(nb_ctx *c, const double *d_pos, hipStream_t s);
 void nb_time_mark(nb_ctx *c, int ev, hipStream_t s);
 int nb_launch_pair_mixed(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_fl,
-                         hipStream_t s);
+                         hipStream_t s, bool packed);
 void nb_set_error(const char *fmt, ...);

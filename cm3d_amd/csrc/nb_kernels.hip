@@ -4,55 +4,26 @@
 //   list build   k_cell -> k_scat -> k_order -> k_list
 //                (cm3d get_ngbr / get_ngbr_e, energy/control/ngbr_list.c:297,
 //                 elec_ngbr.c:220, set_link.c:203)
-//   pair forces  k_pair64_* (force_npol md_fnopol.c:46, force_e finter_e.c:42),
-//                k_ecorr (ecorr md_ewald.c:385); the mixed form is nb_mixed.hip
-//   reciprocal   k_ewchg -> k_ewsfac -> k_ewkred -> k_ewforce
-//                (fk_ewald md_ewald.c:185)
-//   reductions   k_red_* (the getsvirter / getsvirter_e / getvewald sums)
+//   pair forces  k_pair64_* (force_npol md_fnopol.c:46, force_e finter_e.c:42);
+//                the mixed form is nb_mixed.hip
+//   reciprocal   k_ewchg -> k_ewsfac -> k_ewkred -> k_ewforce -> k_ewfin
+//                (fk_ewald md_ewald.c:185; k_ewfin also does ecorr,
+//                md_ewald.c:385)
+//   reductions   k_red (the getsvirter / getsvirter_e / getvewald sums)
 //
 // Every kernel is a __device__ body plus a single-system and a batched entry
 // point (nb_launch.h); none allocates.
 #include <stdarg.h>
+#include <string.h>
 #include <stdio.h>
 
-#include "nb_launch.h"
-#include "nb_sincos.h"
+#include "nb_device.h"
 
 BatchFn g_batch_fn[K_COUNT];
-
-static __device__ __forceinline__ int lane_id() { return threadIdx.x & (NB_WAVE - 1); }
 
 static __device__ __forceinline__ int mbcnt(unsigned long long m) {
   return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
                                    __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-}
-
-// Sum NACC per-thread accumulators over a block of NW wavefronts (wave
-// shuffles, then the wave totals in LDS in a fixed order) and store them to
-// out[k * stride], k < NACC -- column-major partial tables, one column per
-// accumulator, so the final reduction reads each column contiguously.
-template <int NACC, int NW = 4>
-static __device__ __forceinline__ void block_reduce_store(const double *acc, double *out,
-                                                          int stride) {
-  __shared__ double red[NW][NACC];
-  const int w = threadIdx.x >> 6, lane = lane_id();
-#pragma unroll
-  for (int k = 0; k < NACC; k++) {
-    double v = nb_wave_sum(acc[k]);
-    if (NW == 1) {
-      if (lane == 0) out[(size_t)k * stride] = v;
-    } else if (lane == 0) {
-      red[w][k] = v;
-    }
-  }
-  if (NW == 1) return;
-  __syncthreads();
-  if (threadIdx.x < NACC) {
-    double v = red[0][threadIdx.x];
-#pragma unroll
-    for (int q = 1; q < NW; q++) v += red[q][threadIdx.x];
-    out[(size_t)threadIdx.x * stride] = v;
-  }
 }
 
 // Every list-build kernel takes `cond`: when non-null and *cond == 0 it
@@ -85,8 +56,9 @@ NB_KERNEL_PAIR(k_zero, ZeroArgs, 256)
 //            atomics' array and its own ticket for the next build)
 //   k_scat   scatter atom indices into their cell's slots (arbitrary order)
 //   k_order  per cell: order its atoms by index (deterministic sorted
-//            order), then write the sorted records {x,y,z,q}, types, the
-//            permutation both ways, and the build positions (r_build)
+//            order), then write the sorted records {x,y,z,q} (and the mixed
+//            kernel's), types, the permutation both ways, and the build
+//            positions (r_build)
 //   k_list   the Verlet lists
 // ---------------------------------------------------------------------------
 struct CellArgs {
@@ -167,7 +139,7 @@ NB_KERNEL_PAIR(k_scat, ScatArgs, 256)
 struct OrderArgs {
   const int *count, *start, *tmp; int *perm, *iperm, *scell;
   const double *pos, *q; const int *itype; double4 *rec; int *stype; double *posb; int n;
-  int *builds; const int *cond;
+  int *builds; uint4 *recf; Box b; const int *cond;
 };
 static __device__ __forceinline__ void k_order_body(const OrderArgs &A, int bid) {
   NB_SKIP(A.cond);
@@ -190,6 +162,7 @@ static __device__ __forceinline__ void k_order_body(const OrderArgs &A, int bid)
     A.scell[sl] = c;
     const double x = A.pos[a], y = A.pos[n + a], z = A.pos[2 * n + a];
     A.rec[sl] = make_double4(x, y, z, A.q[a]);
+    if (A.recf) A.recf[sl] = nb_mixed_record(x, y, z, A.q[a], A.b.hi);
     A.stype[sl] = A.itype[a];
     A.posb[a] = x; A.posb[n + a] = y; A.posb[2 * n + a] = z;
   }
@@ -544,43 +517,6 @@ NB_KERNEL_PAIR_W(k_pair64_0t, PairArgs, 256, 4)
 NB_KERNEL_PAIR_W(k_pair64_1t, PairArgs, 256, 3)
 NB_KERNEL_PAIR_W(k_pair64_1f, PairArgs, 256, 4)
 
-// ecorr (md_ewald.c:385-453): one thread per atom over its excluded charged
-// partners (both directions, so energies and virials are halved later), on
-// the caller's unwrapped positions with no minimum image.  Adds to fl.
-struct EcorrArgs {
-  const int *eoff, *elist; const double *pos, *q; int n, nblocks; double kappa; double *fl;
-  double *epart;
-};
-static __device__ __forceinline__ void k_ecorr_body(const EcorrArgs &A, int bid) {
-  double acc[NB_NE_ECORR];
-#pragma unroll
-  for (int k = 0; k < NB_NE_ECORR; k++) acc[k] = 0.;
-  int a = bid * blockDim.x + threadIdx.x;
-  if (a < A.n) {
-    const int n = A.n;
-    const double spi = 1. / sqrt(M_PI), kappa = A.kappa, ka2 = kappa * kappa;
-    const double *pos = A.pos, *q = A.q;
-    double fx = 0., fy = 0., fz = 0.;
-    for (int e = A.eoff[a]; e < A.eoff[a + 1]; e++) {
-      int b = A.elist[e];
-      double dx = pos[a] - pos[b], dy = pos[n + a] - pos[n + b], dz = pos[2 * n + a] - pos[2 * n + b];
-      double d2 = dx * dx + dy * dy + dz * dz, d = sqrt(d2);
-      double qr = q[a] * q[b] / d, er = erf(d * kappa);
-      acc[0] += -qr * er;
-      double dv = -qr * (-2. * kappa * exp(-ka2 * d2) * spi + er / d);
-      acc[1] += d * dv;
-      dv /= d;
-      acc[2] += dv * dx * dx; acc[3] += dv * dx * dy; acc[4] += dv * dx * dz;
-      acc[5] += dv * dy * dx; acc[6] += dv * dy * dy; acc[7] += dv * dy * dz;
-      acc[8] += dv * dz * dx; acc[9] += dv * dz * dy; acc[10] += dv * dz * dz;
-      fx += dx * dv; fy += dy * dv; fz += dz * dv;
-    }
-    if (A.eoff[a + 1] > A.eoff[a]) { A.fl[a] += fx; A.fl[n + a] += fy; A.fl[2 * n + a] += fz; }
-  }
-  block_reduce_store<NB_NE_ECORR, 1>(acc, A.epart + bid, A.nblocks);
-}
-NB_KERNEL_PAIR(k_ecorr, EcorrArgs, 64)
-
 // ---------------------------------------------------------------------------
 // Reciprocal-space Ewald, FP64 throughout (fk_ewald, md_ewald.c:185-361).
 // The k vectors are cut into segments: runs of <= NB_EW_SEG consecutive kc in
@@ -621,19 +557,7 @@ static __device__ __forceinline__ void k_ewchg_body(const EwChgArgs &A, int bid)
   int c = bid * blockDim.x + threadIdx.x;
   if (c >= A.nq) return;
   const int n = A.n, a = A.qidx[c];
-  double x = A.pos[a], y = A.pos[n + a], z = A.pos[2 * n + a];
-  const double *hi = A.b.hi;
-  double sa = x * hi[0] + y * hi[1] + z * hi[2];
-  double sb = x * hi[3] + y * hi[4] + z * hi[5];
-  double sc = x * hi[6] + y * hi[7] + z * hi[8];
-  sa -= rint(sa);
-  sb -= rint(sb);
-  sc -= rint(sc);
-  const double tpi = 2. * M_PI;
-  A.chg[c] = make_double4(tpi * sa, tpi * sb, tpi * sc, A.q[a] * sqrt(4.0 * M_PI));
-  double sn, cs;
-  nb_sincos(tpi * sc, &sn, &cs);
-  A.stp[c] = make_double2(cs, sn);
+  nb_charge_record(A.pos[a], A.pos[n + a], A.pos[2 * n + a], A.q[a], A.b.hi, A.chg + c, A.stp + c);
 }
 NB_KERNEL_PAIR(k_ewchg, EwChgArgs, 256)
 
@@ -854,43 +778,65 @@ static __device__ __forceinline__ void k_ewforce_body(const EwForceArgs &A, int 
 }
 NB_KERNEL_PAIR(k_ewforce, EwForceArgs, 256)
 
-struct EwFaddArgs {
-  const double *fpart; const double4 *chg; const int *qidx; int nq, n, nrange; double *fl;
+// k_ewfin, one thread per charge: ecorr (md_ewald.c:385-453) over the
+// atom's excluded charged partners (both directions, so energies and
+// virials are halved later), on the caller's unwrapped positions with no
+// minimum image, added to fl; then the reciprocal force, the range partials
+// summed in fixed order and times q, added to fl.  The ecorr energy / virial
+// partials go to a column-major table, one row per workgroup.
+struct EwFinArgs {
+  const double *fpart; const double4 *chg; const int *qidx; int nq, n, nrange, ew, ec;
+  const int *eoff, *elist; const double *pos, *q; double kappa; double *fl;
+  int nblocks; double *epart;
 };
-static __device__ __forceinline__ void k_ewfadd_body(const EwFaddArgs &A, int bid) {
+static __device__ __forceinline__ void k_ewfin_body(const EwFinArgs &A, int bid) {
+  double acc[NB_NE_ECORR];
+#pragma unroll
+  for (int k = 0; k < NB_NE_ECORR; k++) acc[k] = 0.;
   const int c = bid * blockDim.x + threadIdx.x;
-  if (c >= A.nq) return;
-  double fx = 0., fy = 0., fz = 0.;
-  const size_t nq = A.nq;
-  for (int r = 0; r < A.nrange; r++) {
-    const double *p = A.fpart + 3 * (size_t)r * nq + c;
-    fx += p[0];
-    fy += p[nq];
-    fz += p[2 * nq];
+  if (c < A.nq) {
+    const int a = A.qidx[c], n = A.n;
+    double gx = A.fl[a], gy = A.fl[n + a], gz = A.fl[2 * n + a];
+    if (A.ec && A.eoff[a + 1] > A.eoff[a]) {
+      const double spi = 1. / sqrt(M_PI), kappa = A.kappa, ka2 = kappa * kappa;
+      const double *pos = A.pos, *q = A.q;
+      double fx = 0., fy = 0., fz = 0.;
+      for (int e = A.eoff[a]; e < A.eoff[a + 1]; e++) {
+        int b = A.elist[e];
+        double dx = pos[a] - pos[b], dy = pos[n + a] - pos[n + b], dz = pos[2 * n + a] - pos[2 * n + b];
+        double d2 = dx * dx + dy * dy + dz * dz, d = sqrt(d2);
+        double qr = q[a] * q[b] / d, er = erf(d * kappa);
+        acc[0] += -qr * er;
+        double dv = -qr * (-2. * kappa * exp(-ka2 * d2) * spi + er / d);
+        acc[1] += d * dv;
+        dv /= d;
+        acc[2] += dv * dx * dx; acc[3] += dv * dx * dy; acc[4] += dv * dx * dz;
+        acc[5] += dv * dy * dx; acc[6] += dv * dy * dy; acc[7] += dv * dy * dz;
+        acc[8] += dv * dz * dx; acc[9] += dv * dz * dy; acc[10] += dv * dz * dz;
+        fx += dx * dv; fy += dy * dv; fz += dz * dv;
+      }
+      gx += fx; gy += fy; gz += fz;
+    }
+    if (A.ew) {
+      double fx = 0., fy = 0., fz = 0.;
+      const size_t nq = A.nq;
+      for (int r = 0; r < A.nrange; r++) {
+        const double *p = A.fpart + 3 * (size_t)r * nq + c;
+        fx += p[0];
+        fy += p[nq];
+        fz += p[2 * nq];
+      }
+      const double qc = A.chg[c].w;
+      gx += qc * fx; gy += qc * fy; gz += qc * fz;
+    }
+    A.fl[a] = gx; A.fl[n + a] = gy; A.fl[2 * n + a] = gz;
   }
-  const double q = A.chg[c].w;
-  const int a = A.qidx[c], n = A.n;
-  A.fl[a] += q * fx;
-  A.fl[n + a] += q * fy;
-  A.fl[2 * n + a] += q * fz;
+  if (A.ec) block_reduce_store<NB_NE_ECORR>(acc, A.epart + bid, A.nblocks);
 }
-NB_KERNEL_PAIR(k_ewfadd, EwFaddArgs, 256)
+NB_KERNEL_PAIR(k_ewfin, EwFinArgs, 256)
 
-// Column sums of a column-major [COLS][rows] partial table: one block per
-// column, a fixed assignment of rows to threads and a fixed tree
-// (deterministic).
-static __device__ __forceinline__ void reduce_body(const RedArgs &A, int bid) {
-  const double *col = A.part + (size_t)bid * A.rows;
-  double acc = 0.;
-  for (int r = threadIdx.x; r < A.rows; r += blockDim.x) acc += col[r];
-  block_reduce_store<1>(&acc, A.out + bid, 1);
-}
-#define k_red_pair_body reduce_body
-#define k_red_ecorr_body reduce_body
-#define k_red_ew_body reduce_body
-NB_KERNEL_PAIR(k_red_pair, RedArgs, 256)
-NB_KERNEL_PAIR(k_red_ecorr, RedArgs, 256)
-NB_KERNEL_PAIR(k_red_ew, RedArgs, 256)
+static __device__ __forceinline__ void k_red_body(const RedSet &R, int bid) { reduce_col(R, bid); }
+NB_KERNEL_PAIR(k_red, RedSet, 256)
 
 // ---------------------------------------------------------------------------
 // launchers
@@ -919,7 +865,8 @@ static int launch_build_impl(nb_ctx *c, const Box &b, hipStream_t s, const int *
   // the sorted cell id of each sorted slot overwrites d_cell
   OrderArgs oa = {c->d_ccount, c->d_cstart, c->d_tmp, c->d_perm, c->d_iperm, c->d_cell,
                   c->d_pos, c->d_qch, c->d_itype, (double4 *)c->d_rec64, c->d_stype,
-                  c->d_posb, n, c->d_flags + 6, cond};
+                  c->d_posb, n, c->d_flags + 6,
+                  c->prec == NB_PREC_MIXED ? (uint4 *)c->d_recf : nullptr, b, cond};
   NB_GO(c, K_ORDER, k_order, c->ncells, 256, 0, oa, s);
   ListArgs A;
   A.n = n; A.ntypes = c->ntypes; A.cap = c->cap;
@@ -953,17 +900,38 @@ int nb_launch_rebuild_check(nb_ctx *c, const double *d_pos, hipStream_t s) {
   return NB_OK;
 }
 
+// The energy-column reductions of force(...,1): which tables, how many rows.
+RedSet nb_red_set(const nb_ctx *c, int iflg) {
+  RedSet R;
+  memset(&R, 0, sizeof(R));
+  const bool ec = iflg == 1 && c->iperd == 3 && c->necorr > 0;
+  const bool ew = iflg == 1 && c->iperd == 3 && c->ktot > 0 && c->ncharge > 0;
+  R.part[0] = c->d_epair; R.rows[0] = c->npair_blocks; R.cols[0] = NB_NE_PAIR;
+  R.out[0] = c->d_eout;
+  R.part[1] = c->d_eec; R.rows[1] = c->nfin_blocks; R.cols[1] = ec ? NB_NE_ECORR : 0;
+  R.out[1] = c->d_eout + 20;
+  R.part[2] = c->d_eew; R.rows[2] = c->new_blocks; R.cols[2] = ew ? NB_NE_EW : 0;
+  R.out[2] = c->d_eout + 32;
+  R.ncols = R.cols[0] + R.cols[1] + R.cols[2];
+  return R;
+}
+
+// force(...,1) / force(...,0).  flags: NB_FORCE_PACKED -- the sorted records
+// and the per-charge Ewald data are already current (the MD step writes
+// them); NB_FORCE_NORED -- leave the energy columns unreduced (the MD kick
+// reduces them).
 int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_fl,
-                    hipStream_t s) {
+                    hipStream_t s, int flags) {
   const int n = c->natoms;
+  const bool packed = flags & NB_FORCE_PACKED;
   nb_time_mark(c, 0, s);
   if (c->prec == NB_PREC_MIXED) {
     nb_time_mark(c, 1, s);
-    int rc = nb_launch_pair_mixed(c, iflg, b, d_fs, d_fl, s);
+    int rc = nb_launch_pair_mixed(c, iflg, b, d_fs, d_fl, s, packed);
     if (rc) return rc;
   } else {
     Pack64Args pa = {c->d_pos, c->d_qch, c->d_perm, n, (double4 *)c->d_rec64};
-    NB_GO(c, K_PACK64, k_pack64, blocks(n, 256), 256, 0, pa, s);
+    NB_GO(c, K_PACK64, k_pack64, packed ? 0 : blocks(n, 256), 256, 0, pa, s);
     nb_time_mark(c, 1, s);
     PairArgs P;
     P.n = n; P.ntypes = c->ntypes; P.ninter = c->ninter; P.ntable = c->ntable;
@@ -985,20 +953,13 @@ int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_f
     if (!c->rec) LCHK(hipGetLastError());
   }
   nb_time_mark(c, 2, s);
-  RedArgs rp = {c->d_epair, c->npair_blocks, c->d_eout};
-  NB_GO(c, K_RED_PAIR, k_red_pair, NB_NE_PAIR, 256, 0, rp, s);
-  const bool ec = iflg == 1 && c->iperd == 3 && c->necorr > 0;
-  EcorrArgs ea = {c->d_eoff, c->d_elist, c->d_pos, c->d_qch, n, c->necorr_blocks, c->kappa,
-                  d_fl, c->d_eec};
-  NB_GO(c, K_ECORR, k_ecorr, ec ? c->necorr_blocks : 0, 64, 0, ea, s);
-  RedArgs re = {c->d_eec, c->necorr_blocks, c->d_eout + 20};
-  NB_GO(c, K_RED_ECORR, k_red_ecorr, ec ? NB_NE_ECORR : 0, 256, 0, re, s);
   nb_time_mark(c, 3, s);
+  const bool ec = iflg == 1 && c->iperd == 3 && c->necorr > 0;
   const bool ew = iflg == 1 && c->iperd == 3 && c->ktot > 0 && c->ncharge > 0;
   const int nq = c->ncharge;
   EwChgArgs ch = {c->d_pos, c->d_qch, c->d_qidx, nq, n, b, (double4 *)c->d_chg,
                   (double2 *)c->d_stp};
-  NB_GO(c, K_EWCHG, k_ewchg, ew ? blocks(nq, 256) : 0, 256, 0, ch, s);
+  NB_GO(c, K_EWCHG, k_ewchg, ew && !packed ? blocks(nq, 256) : 0, 256, 0, ch, s);
   EwSfacArgs sf = {c->d_sseg, (const double4 *)c->d_chg, (const double2 *)c->d_stp, nq, c->qc,
                    c->nseg, c->nsw, c->ktot, (double2 *)c->d_spart};
   NB_GO(c, K_EWSFAC, k_ewsfac, ew ? c->nsw * c->nchunk : 0, 256, 0, sf, s);
@@ -1006,17 +967,21 @@ int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_f
                b.h[1] * (b.h[5] * b.h[6] - b.h[3] * b.h[8]) +
                b.h[2] * (b.h[3] * b.h[7] - b.h[4] * b.h[6]);
   double falp2 = c->kappa > 0 ? 1. / (4. * c->kappa * c->kappa) : 0.;
+  // one workgroup per 64 k
   EwKredArgs kr = {c->d_kabc, c->ktot, (const double2 *)c->d_spart, c->nchunk, c->new_blocks,
                    b, 1. / det, falp2, (KData *)c->d_kd, c->d_eew};
-  // one workgroup per 64 k
   NB_GO(c, K_EWKRED, k_ewkred, ew ? c->new_blocks : 0, 256, 0, kr, s);
-  RedArgs rw = {c->d_eew, c->new_blocks, c->d_eout + 32};
-  NB_GO(c, K_RED_EW, k_red_ew, ew ? NB_NE_EW : 0, 256, 0, rw, s);
   EwForceArgs ef = {c->d_seg, c->d_rtile, c->d_tseg, (const KData *)c->d_kd,
                     (const double4 *)c->d_chg, (const double2 *)c->d_stp, nq, c->nqb, c->d_fpart};
   NB_GO(c, K_EWFORCE, k_ewforce, ew ? c->nqb * c->nrange : 0, 256, 0, ef, s);
-  EwFaddArgs fa = {c->d_fpart, (const double4 *)c->d_chg, c->d_qidx, nq, n, c->nrange, d_fl};
-  NB_GO(c, K_EWFADD, k_ewfadd, ew ? blocks(nq, 256) : 0, 256, 0, fa, s);
+  EwFinArgs fa = {c->d_fpart, (const double4 *)c->d_chg, c->d_qidx, nq, n, c->nrange, ew, ec,
+                  c->d_eoff, c->d_elist, c->d_pos, c->d_qch, c->kappa, d_fl, c->nfin_blocks,
+                  c->d_eec};
+  NB_GO(c, K_EWFIN, k_ewfin, (ew || ec) ? c->nfin_blocks : 0, 256, 0, fa, s);
+  if (!(flags & NB_FORCE_NORED)) {
+    RedSet R = nb_red_set(c, iflg);
+    NB_GO(c, K_RED, k_red, R.ncols, 256, 0, R, s);
+  }
   if (!c->rec) LCHK(hipGetLastError());
   nb_time_mark(c, 4, s);
   nb_time_mark(c, 5, s);
@@ -1038,14 +1003,11 @@ static struct RegisterKernels {
     g_batch_fn[K_PAIR64_0T] = k_pair64_0t_bl;
     g_batch_fn[K_PAIR64_1T] = k_pair64_1t_bl;
     g_batch_fn[K_PAIR64_1F] = k_pair64_1f_bl;
-    g_batch_fn[K_RED_PAIR] = k_red_pair_bl;
-    g_batch_fn[K_RED_ECORR] = k_red_ecorr_bl;
-    g_batch_fn[K_RED_EW] = k_red_ew_bl;
-    g_batch_fn[K_ECORR] = k_ecorr_bl;
+    g_batch_fn[K_RED] = k_red_bl;
     g_batch_fn[K_EWCHG] = k_ewchg_bl;
     g_batch_fn[K_EWSFAC] = k_ewsfac_bl;
     g_batch_fn[K_EWKRED] = k_ewkred_bl;
     g_batch_fn[K_EWFORCE] = k_ewforce_bl;
-    g_batch_fn[K_EWFADD] = k_ewfadd_bl;
+    g_batch_fn[K_EWFIN] = k_ewfin_bl;
   }
 } g_register_kernels;

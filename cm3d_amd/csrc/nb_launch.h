@@ -15,9 +15,9 @@
 
 enum KernelId {
   K_ZERO, K_CELL, K_SCAT, K_ORDER, K_PACK64, K_LIST, K_CHECK,
-  K_PAIR64_0T, K_PAIR64_1T, K_PAIR64_1F, K_RED_PAIR, K_RED_ECORR, K_RED_EW,
-  K_ECORR, K_EWCHG, K_EWSFAC, K_EWKRED, K_EWFORCE, K_PACKMIX, K_MIX_0T,
-  K_MIX_1T, K_MIX_1F, K_MDINNER, K_MDKICK, K_EWFADD, K_COUNT
+  K_PAIR64_0T, K_PAIR64_1T, K_PAIR64_1F, K_RED, K_EWCHG, K_EWSFAC, K_EWKRED,
+  K_EWFORCE, K_EWFIN, K_PACKMIX, K_MIX_0T, K_MIX_1T, K_MIX_1F, K_MDINNER, K_MDKICK,
+  K_COUNT
 };
 
 struct PairArgs {
@@ -33,7 +33,6 @@ struct PairArgs {
   double *epart;
 };
 
-struct RedArgs { const double *part; int rows; double *out; };
 
 struct LaunchRec {
   int kid, nblk, tpb, lds;

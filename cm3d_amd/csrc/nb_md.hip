@@ -8,19 +8,21 @@
 //                 { v += dta2*aa; r += dta2*v; r += dta2*v; aa = bonded(r)/m;
 //                   v += dta2*aa } -- molecules never share bonded terms, so
 //                 the whole inner RESPA loop runs in one launch with the
-//                 molecule's atoms in LDS
-//   rebuild test  nrefsh's |r - r_build| > skin/2 on device (flag in HBM)
+//                 molecule's atoms in LDS; its epilogue applies nrefsh's
+//                 test |r - r_build| > skin/2 (flag in HBM) and writes the
+//                 sorted pair records and the per-charge Ewald data
 //   list build    the cell sort + list kernels, each returning at once when
 //                 the flag is clear (so the step is graph-capturable)
-//   force(...,1)  the pair, ecorr and Ewald kernels
-//   k_md_kick     v += dter * fl / m
+//   force(...,1)  the pair and Ewald (+ ecorr) kernels
+//   k_md_kick     v += dter * fl / m, plus the energy-column reductions and
+//                 the flag reset for the next step
 // Positions and velocities stay FP64 (cm3d's integrator arithmetic, no FMA).
 // A step can be captured once into a hipGraph and replayed.
 #include <string.h>
 
 #include <vector>
 
-#include "nb_launch.h"
+#include "nb_device.h"
 
 #define MD_MAXA 8          // atoms per molecule held in LDS by k_md_inner
 #define MD_TPB 64          // molecules per workgroup
@@ -90,6 +92,14 @@ struct InnerArgs {
   const double *fl, *rm;
   const int *moff, *boff, *aoff, *bi, *bj, *ai, *aj, *ak;
   const double *bk, *br0, *afk, *ath0;
+  // epilogue, for the force evaluation that follows: the nrefsh test, the
+  // sorted record of each atom, the Ewald data of each charge
+  int epilogue;
+  const double *posb; double skin2q; int *flag;
+  const int *iperm, *qpos; const double *q;
+  double4 *rec64; uint4 *recf;
+  double4 *chg; double2 *stp;
+  Box b;
 };
 
 static __device__ __forceinline__ void k_mdinner_body(const InnerArgs &A, int bid) {
@@ -129,6 +139,7 @@ static __device__ __forceinline__ void k_mdinner_body(const InnerArgs &A, int bi
     for (int a = 0; a < na; a++)
       for (int d = 0; d < 3; d++) v[a][d] += dta2 * aa[a][d];
   }
+  bool moved = false;
   for (int a = 0; a < na; a++) {
     int g = a0 + a;
     for (int d = 0; d < 3; d++) {
@@ -136,7 +147,22 @@ static __device__ __forceinline__ void k_mdinner_body(const InnerArgs &A, int bi
       A.vel[d * n + g] = v[a][d];
       A.aa[d * n + g] = aa[a][d];
     }
+    if (A.epilogue) {
+      const double x = p[a][0], y = p[a][1], z = p[a][2];
+      // nrefsh (md_finter.c:61-73): |r - r_build|^2 > skin^2 / 4
+      const double ex = A.posb[g] - x, ey = A.posb[n + g] - y, ez = A.posb[2 * n + g] - z;
+      moved |= ex * ex + ey * ey + ez * ez > A.skin2q;
+      const int sl = A.iperm[g];
+      const double qg = A.q[g];
+      if (A.recf) A.recf[sl] = nb_mixed_record(x, y, z, qg, A.b.hi);
+      else A.rec64[sl] = make_double4(x, y, z, qg);
+      if (A.chg) {
+        const int c = A.qpos[g];
+        if (c >= 0) nb_charge_record(x, y, z, qg, A.b.hi, A.chg + c, A.stp + c);
+      }
+    }
   }
+  if (__ballot(moved) != 0 && lane_id() == 0) atomicOr(A.flag, 1);
 }
 NB_KERNEL_PAIR(k_mdinner, InnerArgs, MD_TPB)
 
@@ -158,8 +184,16 @@ __global__ __launch_bounds__(MD_TPB) void k_md_bonded(InnerArgs A) {
   }
 }
 
-struct KickArgs { double *vel; const double *fl, *rm; int n; double dter; };
+// v += dter * fl / m; the workgroups past the kick reduce the energy
+// columns of this step's force(...,1) and the first clears the rebuild flag
+// for the next step.
+struct KickArgs { double *vel; const double *fl, *rm; int n, nkick; double dter; int *flag; RedSet R; };
 static __device__ __forceinline__ void k_mdkick_body(const KickArgs &A, int bid) {
+  if (bid >= A.nkick) {
+    reduce_col(A.R, bid - A.nkick);
+    return;
+  }
+  if (bid == 0 && threadIdx.x == 0) *A.flag = 0;
   int i = bid * blockDim.x + threadIdx.x;
   if (i >= 3 * A.n) return;
   A.vel[i] += A.dter * (A.fl[i] * A.rm[i % A.n]);
@@ -235,6 +269,16 @@ static InnerArgs inner_args(nb_md *md) {
   A.moff = md->d_moff; A.boff = md->d_boff; A.aoff = md->d_aoff;
   A.bi = md->d_bi; A.bj = md->d_bj; A.ai = md->d_ai; A.aj = md->d_aj; A.ak = md->d_ak;
   A.bk = md->d_bk; A.br0 = md->d_br0; A.afk = md->d_afk; A.ath0 = md->d_ath0;
+  nb_ctx *c = md->c;
+  A.epilogue = 1;
+  A.posb = c->d_posb; A.skin2q = .25 * c->skin * c->skin; A.flag = c->d_flags + 1;
+  A.iperm = c->d_iperm; A.qpos = c->d_qpos; A.q = c->d_qch;
+  A.rec64 = (double4 *)c->d_rec64;
+  A.recf = c->prec == NB_PREC_MIXED ? (uint4 *)c->d_recf : nullptr;
+  const bool ew = c->iperd == 3 && c->ktot > 0 && c->ncharge > 0;
+  A.chg = ew ? (double4 *)c->d_chg : nullptr;
+  A.stp = ew ? (double2 *)c->d_stp : nullptr;
+  A.b = md->box;
   return A;
 }
 
@@ -369,10 +413,7 @@ extern "C" int nb_md_get_state(nb_md *md, double *pos, double *vel) {
   return NB_OK;
 }
 
-int nb_launch_build_cond(nb_ctx *c, const Box &b, hipStream_t s);
-int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_fl,
-                    hipStream_t s);
-int nb_build_now(nb_ctx *c, const double hmat[9], hipStream_t s);
+RedSet nb_red_set(const nb_ctx *c, int iflg);
 
 // step_init (md_step.c:32-58): lists, force(...,1), force(...,2).
 extern "C" int nb_md_init(nb_md *md) {
@@ -382,11 +423,13 @@ extern "C" int nb_md_init(nb_md *md) {
   int rc = nb_build_now(c, md->hmat, c->stream);
   if (rc) return rc;
   md->box = c->box_built;
-  rc = nb_launch_force(c, 1, md->box, nullptr, c->d_fl, c->stream);
+  rc = nb_launch_force(c, 1, md->box, nullptr, c->d_fl, c->stream, 0);
   if (rc) return rc;
   InnerArgs A = inner_args(md);
   k_md_bonded<<<(md->nmol + MD_TPB - 1) / MD_TPB, MD_TPB, 0, c->stream>>>(A);
   MCHK(hipGetLastError());
+  // the step's rebuild flag starts clear (its inner kernel sets it)
+  MCHK(hipMemsetAsync(c->d_flags + 1, 0, sizeof(int), c->stream));
   MCHK(hipStreamSynchronize(c->stream));
   md->nrebuild_host = 0;
   return NB_OK;
@@ -394,16 +437,18 @@ extern "C" int nb_md_init(nb_md *md) {
 
 int md_enqueue_step(nb_md *md, hipStream_t s) {
   nb_ctx *c = md->c;
+  // inner RESPA loop + nrefsh test + records for the force evaluation
   InnerArgs A = inner_args(md);
   NB_GO(c, K_MDINNER, k_mdinner, (md->nmol + MD_TPB - 1) / MD_TPB, MD_TPB, 0, A, s);
-  int rc = nb_launch_rebuild_check(c, c->d_pos, s);
+  int rc = nb_launch_build_cond(c, md->box, s);
   if (rc) return rc;
-  rc = nb_launch_build_cond(c, md->box, s);
+  rc = nb_launch_force(c, 1, md->box, nullptr, c->d_fl, s, NB_FORCE_PACKED | NB_FORCE_NORED);
   if (rc) return rc;
-  rc = nb_launch_force(c, 1, md->box, nullptr, c->d_fl, s);
-  if (rc) return rc;
-  KickArgs K = {md->d_vel, c->d_fl, md->d_rm, md->n, md->dter};
-  NB_GO(c, K_MDKICK, k_mdkick, (3 * md->n + 255) / 256, 256, 0, K, s);
+  // kick + energy-column reductions + flag reset
+  const int nkick = (3 * md->n + 255) / 256;
+  KickArgs K = {md->d_vel, c->d_fl, md->d_rm, md->n, nkick, md->dter, c->d_flags + 1,
+                nb_red_set(c, 1)};
+  NB_GO(c, K_MDKICK, k_mdkick, nkick + K.R.ncols, 256, 0, K, s);
   if (!c->rec) MCHK(hipGetLastError());
   return NB_OK;
 }
@@ -454,7 +499,7 @@ extern "C" int nb_md_energies(nb_md *md, double vlrc, nb_md_energy *e) {
   MCHK(hipStreamSynchronize(s));
   // nonbonded energies of the current positions: rerun force(...,1) into
   // scratch (fs) so the integrator's fl is untouched
-  int rc = nb_launch_force(c, 1, md->box, nullptr, c->d_fs, s);
+  int rc = nb_launch_force(c, 1, md->box, nullptr, c->d_fs, s, 0);
   if (rc) return rc;
   InnerArgs A = inner_args(md);
   k_md_energy<<<md->ep_blocks, 256, 0, s>>>(A, md->d_amass, md->d_ep);

@@ -15,36 +15,20 @@
 //     not depend on the order in which the lanes' partial sums meet;
 //   - energies and virials: per-lane FP32 sums (a few tens of terms), then
 //     reduced in FP64 in a fixed order.
-#include "nb_launch.h"
+#include "nb_device.h"
 
 #define NB_FX_SCALE 16777216.0  // 2^24 fixed-point units per K/A
 
-static __device__ __forceinline__ int lane_idm() { return threadIdx.x & (NB_WAVE - 1); }
 
 static __device__ __forceinline__ long long wave_sum_ll(long long v) { return nb_wave_sum(v); }
 
-// Sorted-order records for the mixed kernel: {u_x, u_y, u_z, q as f32 bits}.
+// Sorted-order records for the mixed kernel (nb_mixed_record).
 struct PackMixArgs { const double *pos, *q; const int *perm; int n; Box b; uint4 *rec; };
 static __device__ __forceinline__ void k_packmix_body(const PackMixArgs &A, int bid) {
   int s = bid * blockDim.x + threadIdx.x;
   if (s >= A.n) return;
   const int n = A.n, a = A.perm[s];
-  const double *pos = A.pos;
-  double x = pos[a], y = pos[n + a], z = pos[2 * n + a];
-  const double *hi = A.b.hi;
-  double sx = x * hi[0] + y * hi[1] + z * hi[2];
-  double sy = x * hi[3] + y * hi[4] + z * hi[5];
-  double sz = x * hi[6] + y * hi[7] + z * hi[8];
-  sx -= floor(sx);
-  sy -= floor(sy);
-  sz -= floor(sz);
-  const double two32 = 4294967296.0;
-  uint4 r;
-  r.x = (unsigned)(unsigned long long)(sx * two32);
-  r.y = (unsigned)(unsigned long long)(sy * two32);
-  r.z = (unsigned)(unsigned long long)(sz * two32);
-  r.w = __float_as_uint((float)A.q[a]);
-  A.rec[s] = r;
+  A.rec[s] = nb_mixed_record(A.pos[a], A.pos[n + a], A.pos[2 * n + a], A.q[a], A.b.hi);
 }
 NB_KERNEL_PAIR(k_packmix, PackMixArgs, 256)
 
@@ -79,7 +63,7 @@ template <int NACC>
 static __device__ __forceinline__ void block_reduce_store_m(const double *acc, double *out,
                                                             int stride) {
   __shared__ double red[4][NACC];
-  const int w = threadIdx.x >> 6, lane = lane_idm();
+  const int w = threadIdx.x >> 6, lane = lane_id();
 #pragma unroll
   for (int k = 0; k < NACC; k++) {
     double v = nb_wave_sum(acc[k]);
@@ -126,7 +110,7 @@ static __device__ __forceinline__ void pair_mixed_body(const MixArgs &A, int bid
   float e[NB_NE_PAIR];  // this lane's energy / virial sums over its pairs
 #pragma unroll
   for (int k = 0; k < NB_NE_PAIR; k++) e[k] = 0.f;
-  const int lane = lane_idm();
+  const int lane = lane_id();
   const int nw = A.nblocks * 4;
   constexpr unsigned fv = IFLG ? NB_FLAG_VT : NB_FLAG_VS;
   constexpr unsigned fe = IFLG ? NB_FLAG_ET : NB_FLAG_ES;
@@ -261,10 +245,10 @@ NB_KERNEL_PAIR(k_mix_1t, MixArgs, 256)
 NB_KERNEL_PAIR(k_mix_1f, MixArgs, 256)
 
 int nb_launch_pair_mixed(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_fl,
-                         hipStream_t s) {
+                         hipStream_t s, bool packed) {
   const int n = c->natoms;
   PackMixArgs pk = {c->d_pos, c->d_qch, c->d_perm, n, b, (uint4 *)c->d_recf};
-  NB_GO(c, K_PACKMIX, k_packmix, (n + 255) / 256, 256, 0, pk, s);
+  NB_GO(c, K_PACKMIX, k_packmix, packed ? 0 : (n + 255) / 256, 256, 0, pk, s);
   MixArgs M;
   M.n = n; M.ntypes = c->ntypes; M.ninter = c->ninter; M.ntable = c->ntable;
   M.cap = c->cap; M.ctab_off = c->ctab_off; M.tab_lds = c->tab_lds_bytes_f > 0;
