@@ -31,7 +31,7 @@ extern "C" const char *nb_last_error(void) { return g_err; }
   } while (0)
 
 template <typename T>
-static int dalloc(T **p, size_t count) {
+static int dalloc(nb_ctx *c, T **p, size_t count) {
   *p = nullptr;
   if (count == 0) count = 1;
   hipError_t e = hipMalloc((void **)p, count * sizeof(T));
@@ -39,12 +39,13 @@ static int dalloc(T **p, size_t count) {
     nb_set_error("hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
     return NB_ERR_HIP;
   }
+  c->dev_bytes += count * sizeof(T);
   return NB_OK;
 }
 
 template <typename T>
-static int upload(T **p, const T *src, size_t count) {
-  int rc = dalloc(p, count);
+static int upload(nb_ctx *c, T **p, const T *src, size_t count) {
+  int rc = dalloc(c, p, count);
   if (rc) return rc;
   if (count && src) HCHK(hipMemcpy(*p, src, count * sizeof(T), hipMemcpyHostToDevice));
   return NB_OK;
@@ -321,52 +322,52 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
   }
 
   // device uploads
-  if ((rc = upload(&c->d_itype, p->itype, n))) return fail(rc);
-  if ((rc = upload(&c->d_map, p->map, (size_t)p->ntypes * p->ntypes))) return fail(rc);
-  if ((rc = upload(&c->d_ip, ip.data(), ni))) return fail(rc);
-  if ((rc = upload(&c->d_pcut, pcut.data(), pcut.size()))) return fail(rc);
-  if ((rc = upload(&c->d_vtab3, vt3.data(), vt3.size()))) return fail(rc);
-  if ((rc = upload(&c->d_ctab3, ct3.data(), ct3.size()))) return fail(rc);
-  if ((rc = upload(&c->d_qch, p->qch, n))) return fail(rc);
-  if ((rc = upload(&c->d_xoff, xoff.data(), n + 1))) return fail(rc);
-  if ((rc = upload(&c->d_xlist, xlist.data(), xlist.size()))) return fail(rc);
-  if ((rc = upload(&c->d_eoff, eoff.data(), n + 1))) return fail(rc);
-  if ((rc = upload(&c->d_elist, elist.data(), elist.size()))) return fail(rc);
-  if ((rc = upload(&c->d_qidx, qidx.data(), qidx.size()))) return fail(rc);
-  if ((rc = upload(&c->d_qpos, qpos.data(), qpos.size()))) return fail(rc);
-  if ((rc = upload(&c->d_seg, seg.data(), seg.size()))) return fail(rc);
-  if ((rc = upload(&c->d_sseg, sseg.data(), sseg.size()))) return fail(rc);
-  if ((rc = upload(&c->d_tseg, tseg.data(), tseg.size()))) return fail(rc);
-  if ((rc = upload(&c->d_rtile, rtile.data(), rtile.size()))) return fail(rc);
-  if ((rc = upload(&c->d_kabc, kabc.data(), kabc.size()))) return fail(rc);
-  if ((rc = dalloc(&c->d_pos, 3 * (size_t)n))) return fail(rc);
-  if ((rc = dalloc(&c->d_posb, 3 * (size_t)n))) return fail(rc);
-  if ((rc = dalloc(&c->d_fs, 3 * (size_t)n))) return fail(rc);
-  if ((rc = dalloc(&c->d_fl, 3 * (size_t)n))) return fail(rc);
-  if ((rc = dalloc(&c->d_perm, n))) return fail(rc);
-  if ((rc = dalloc(&c->d_iperm, n))) return fail(rc);
-  if ((rc = dalloc(&c->d_cell, n))) return fail(rc);
-  if ((rc = dalloc(&c->d_tmp, n))) return fail(rc);
-  if ((rc = dalloc(&c->d_rec64, 4 * (size_t)n))) return fail(rc);
-  if ((rc = dalloc(&c->d_stype, n))) return fail(rc);
-  if ((rc = dalloc(&c->d_nnb, n))) return fail(rc);
-  if ((rc = dalloc(&c->d_hcnt, 4 * (size_t)n))) return fail(rc);
-  if ((rc = dalloc(&c->d_flags, 8))) return fail(rc);
-  if ((rc = dalloc(&c->d_chg, 4 * (size_t)std::max(ncharge, 1)))) return fail(rc);
-  if ((rc = dalloc(&c->d_stp, 2 * (size_t)std::max(ncharge, 1)))) return fail(rc);
-  if ((rc = dalloc(&c->d_spart, 2 * (size_t)c->nchunk * std::max(c->ktot, 1)))) return fail(rc);
-  if ((rc = dalloc((char **)&c->d_kd, nb_kdata_bytes() * std::max(c->ktot, 1)))) return fail(rc);
-  if ((rc = dalloc(&c->d_fpart, 3 * (size_t)c->nrange * std::max(ncharge, 1)))) return fail(rc);
+  if ((rc = upload(c, &c->d_itype, p->itype, n))) return fail(rc);
+  if ((rc = upload(c, &c->d_map, p->map, (size_t)p->ntypes * p->ntypes))) return fail(rc);
+  if ((rc = upload(c, &c->d_ip, ip.data(), ni))) return fail(rc);
+  if ((rc = upload(c, &c->d_pcut, pcut.data(), pcut.size()))) return fail(rc);
+  if ((rc = upload(c, &c->d_vtab3, vt3.data(), vt3.size()))) return fail(rc);
+  if ((rc = upload(c, &c->d_ctab3, ct3.data(), ct3.size()))) return fail(rc);
+  if ((rc = upload(c, &c->d_qch, p->qch, n))) return fail(rc);
+  if ((rc = upload(c, &c->d_xoff, xoff.data(), n + 1))) return fail(rc);
+  if ((rc = upload(c, &c->d_xlist, xlist.data(), xlist.size()))) return fail(rc);
+  if ((rc = upload(c, &c->d_eoff, eoff.data(), n + 1))) return fail(rc);
+  if ((rc = upload(c, &c->d_elist, elist.data(), elist.size()))) return fail(rc);
+  if ((rc = upload(c, &c->d_qidx, qidx.data(), qidx.size()))) return fail(rc);
+  if ((rc = upload(c, &c->d_qpos, qpos.data(), qpos.size()))) return fail(rc);
+  if ((rc = upload(c, &c->d_seg, seg.data(), seg.size()))) return fail(rc);
+  if ((rc = upload(c, &c->d_sseg, sseg.data(), sseg.size()))) return fail(rc);
+  if ((rc = upload(c, &c->d_tseg, tseg.data(), tseg.size()))) return fail(rc);
+  if ((rc = upload(c, &c->d_rtile, rtile.data(), rtile.size()))) return fail(rc);
+  if ((rc = upload(c, &c->d_kabc, kabc.data(), kabc.size()))) return fail(rc);
+  if ((rc = dalloc(c, &c->d_pos, 3 * (size_t)n))) return fail(rc);
+  if ((rc = dalloc(c, &c->d_posb, 3 * (size_t)n))) return fail(rc);
+  if ((rc = dalloc(c, &c->d_fs, 3 * (size_t)n))) return fail(rc);
+  if ((rc = dalloc(c, &c->d_fl, 3 * (size_t)n))) return fail(rc);
+  if ((rc = dalloc(c, &c->d_perm, n))) return fail(rc);
+  if ((rc = dalloc(c, &c->d_iperm, n))) return fail(rc);
+  if ((rc = dalloc(c, &c->d_cell, n))) return fail(rc);
+  if ((rc = dalloc(c, &c->d_tmp, n))) return fail(rc);
+  if ((rc = dalloc(c, &c->d_rec64, 4 * (size_t)n))) return fail(rc);
+  if ((rc = dalloc(c, &c->d_stype, n))) return fail(rc);
+  if ((rc = dalloc(c, &c->d_nnb, n))) return fail(rc);
+  if ((rc = dalloc(c, &c->d_hcnt, 4 * (size_t)n))) return fail(rc);
+  if ((rc = dalloc(c, &c->d_flags, 8))) return fail(rc);
+  if ((rc = dalloc(c, &c->d_chg, 4 * (size_t)std::max(ncharge, 1)))) return fail(rc);
+  if ((rc = dalloc(c, &c->d_stp, 2 * (size_t)std::max(ncharge, 1)))) return fail(rc);
+  if ((rc = dalloc(c, &c->d_spart, 2 * (size_t)c->nchunk * std::max(c->ktot, 1)))) return fail(rc);
+  if ((rc = dalloc(c, (char **)&c->d_kd, nb_kdata_bytes() * std::max(c->ktot, 1)))) return fail(rc);
+  if ((rc = dalloc(c, &c->d_fpart, 3 * (size_t)c->nrange * std::max(ncharge, 1)))) return fail(rc);
   // pair kernels: a wave per atom in turn; the FP32 kernel gives each wave
   // about 4 atoms so its once-per-wave energy reduction is amortised
   c->npair_blocks = p->precision == NB_PREC_MIXED ? std::max(1, std::min((n + 15) / 16, 2048))
                                                   : std::min((n + 3) / 4, 2048);
   c->new_blocks = std::max(1, (c->ktot + NB_WAVE - 1) / NB_WAVE);
   c->nfin_blocks = std::max(1, (ncharge + 255) / 256);
-  if ((rc = dalloc(&c->d_epair, (size_t)c->npair_blocks * NB_NE_PAIR))) return fail(rc);
-  if ((rc = dalloc(&c->d_eec, (size_t)c->nfin_blocks * NB_NE_ECORR))) return fail(rc);
-  if ((rc = dalloc(&c->d_eew, (size_t)c->new_blocks * NB_NE_EW))) return fail(rc);
-  if ((rc = dalloc(&c->d_eout, 64))) return fail(rc);
+  if ((rc = dalloc(c, &c->d_epair, (size_t)c->npair_blocks * NB_NE_PAIR))) return fail(rc);
+  if ((rc = dalloc(c, &c->d_eec, (size_t)c->nfin_blocks * NB_NE_ECORR))) return fail(rc);
+  if ((rc = dalloc(c, &c->d_eew, (size_t)c->new_blocks * NB_NE_EW))) return fail(rc);
+  if ((rc = dalloc(c, &c->d_eout, 64))) return fail(rc);
   if (hipHostMalloc((void **)&c->h_flags, 16 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
     nb_set_error("hipHostMalloc failed");
     return fail(NB_ERR_HIP);
@@ -378,9 +379,9 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
   c->tab_lds_bytes_f = tab_bytes / 2 <= 60 * 1024 ? (int)(tab_bytes / 2) : 0;
   if (c->prec == NB_PREC_MIXED) {
     std::vector<float> vf(vt3.begin(), vt3.end()), cf(ct3.begin(), ct3.end());
-    if ((rc = upload(&c->d_vtab3f, vf.data(), vf.size()))) return fail(rc);
-    if ((rc = upload(&c->d_ctab3f, cf.data(), cf.size()))) return fail(rc);
-    if ((rc = dalloc(&c->d_recf, (size_t)n))) return fail(rc);
+    if ((rc = upload(c, &c->d_vtab3f, vf.data(), vf.size()))) return fail(rc);
+    if ((rc = upload(c, &c->d_ctab3f, cf.data(), cf.size()))) return fail(rc);
+    if ((rc = dalloc(c, &c->d_recf, (size_t)n))) return fail(rc);
     std::vector<TPairF> tpf((size_t)p->ntypes * p->ntypes);
     for (int a = 0; a < p->ntypes; a++)
       for (int b = 0; b < p->ntypes; b++) {
@@ -396,7 +397,7 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
         }
         t.pad = 0;
       }
-    if ((rc = upload(&c->d_tpf, tpf.data(), tpf.size()))) return fail(rc);
+    if ((rc = upload(c, &c->d_tpf, tpf.data(), tpf.size()))) return fail(rc);
   }
   // neighbour capacity from the density, grown on overflow at build time
   {
@@ -446,9 +447,13 @@ static void cell_grid(nb_ctx *c, const Box &b, int *nx, int *ny, int *nz) {
 
 static int ensure_capacity(nb_ctx *c, const Box &b, int want_cap) {
   if (want_cap <= c->cap && c->d_nbr) return NB_OK;
-  if (c->d_nbr) { hipFree(c->d_nbr); c->d_nbr = nullptr; }
+  if (c->d_nbr) {
+    hipFree(c->d_nbr);
+    c->d_nbr = nullptr;
+    c->dev_bytes -= (size_t)c->cap * c->natoms * sizeof(uint32_t);
+  }
   c->cap = (want_cap + 63) & ~63;
-  return dalloc(&c->d_nbr, (size_t)c->cap * c->natoms);
+  return dalloc(c, &c->d_nbr, (size_t)c->cap * c->natoms);
 }
 
 int nb_build_now(nb_ctx *c, const double hmat[9], hipStream_t s) {
@@ -458,15 +463,16 @@ int nb_build_now(nb_ctx *c, const double hmat[9], hipStream_t s) {
   c->ncx = nx; c->ncy = ny; c->ncz = nz;
   c->ncells = nx * ny * nz;
   if (c->ncells > c->cells_alloc) {
+    if (c->d_ccount) c->dev_bytes -= 4 * (size_t)c->cells_alloc * sizeof(int);
     if (c->d_ccount) hipFree(c->d_ccount);
     if (c->d_cstart) hipFree(c->d_cstart);
     if (c->d_cfill) hipFree(c->d_cfill);
     if (c->d_cwork) hipFree(c->d_cwork);
     c->d_ccount = c->d_cstart = c->d_cfill = c->d_cwork = nullptr;
-    RC(dalloc(&c->d_ccount, c->ncells));
-    RC(dalloc(&c->d_cstart, c->ncells));
-    RC(dalloc(&c->d_cfill, c->ncells));
-    RC(dalloc(&c->d_cwork, c->ncells));
+    RC(dalloc(c, &c->d_ccount, c->ncells));
+    RC(dalloc(c, &c->d_cstart, c->ncells));
+    RC(dalloc(c, &c->d_cfill, c->ncells));
+    RC(dalloc(c, &c->d_cwork, c->ncells));
     HCHK(hipMemsetAsync(c->d_cwork, 0, c->ncells * sizeof(int), s));
     c->cells_alloc = c->ncells;
   }
@@ -614,6 +620,12 @@ extern "C" int nb_force_device(nb_ctx *c, int iflg, const double *d_pxyz, const 
     HCHK(hipMemcpyAsync(c->d_pos, d_pxyz, sizeof(double) * 3 * c->natoms,
                         hipMemcpyDeviceToDevice, s));
   return force_on(c, iflg, hmat, d_fs, iflg ? d_fl : nullptr, e, s);
+}
+
+extern "C" int nb_footprint(nb_ctx *c, size_t *bytes) {
+  if (!c || !bytes) { nb_set_error("nb_footprint: null argument"); return NB_ERR_ARG; }
+  *bytes = c->dev_bytes;
+  return NB_OK;
 }
 
 extern "C" int nb_list_counts(nb_ctx *c, int counts[4]) {

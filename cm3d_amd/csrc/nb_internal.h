@@ -195,6 +195,7 @@ struct nb_ctx {
   TPairF *d_tpf;               // mixed: per type pair parameters
   Box box_built;
   int have_lists;
+  size_t dev_bytes;            // device memory held by this context (and its MD state)
   int tab_lds_bytes, tab_lds_bytes_f, ctab_off;
   // host mirrors
   int *h_flags;

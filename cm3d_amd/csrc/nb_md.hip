@@ -43,6 +43,7 @@ struct nb_md {
   hipGraphExec_t gexec;
   int have_graph;
   int nrebuild_host;
+  size_t bytes;  // device memory of this MD state (counted in its context's footprint)
 };
 
 // Harmonic bonds then bends of one molecule, in cm3d's term order, on the
@@ -253,9 +254,10 @@ static thread_local char g_dummy;  // keeps this TU's statics distinct
   } while (0)
 
 template <typename T>
-static int mup(T **p, const T *src, size_t cnt) {
+static int mup(nb_ctx *c, T **p, const T *src, size_t cnt) {
   if (cnt == 0) cnt = 1;
   MCHK(hipMalloc((void **)p, cnt * sizeof(T)));
+  c->dev_bytes += cnt * sizeof(T);
   if (src) MCHK(hipMemcpy(*p, src, cnt * sizeof(T), hipMemcpyHostToDevice));
   return NB_OK;
 }
@@ -286,6 +288,7 @@ extern "C" void nb_md_destroy(nb_md *md) {
   if (!md) return;
   hipSetDevice(md->c->dev);
   if (md->have_graph) { hipGraphExecDestroy(md->gexec); hipGraphDestroy(md->graph); }
+  md->c->dev_bytes -= md->bytes;
   void *ptrs[] = {md->d_vel, md->d_rm, md->d_aa, md->d_amass, md->d_moff, md->d_boff,
                   md->d_aoff, md->d_bi, md->d_bj, md->d_ai, md->d_aj, md->d_ak, md->d_bk,
                   md->d_br0, md->d_afk, md->d_ath0, md->d_ep, md->d_eo};
@@ -365,7 +368,9 @@ extern "C" int nb_md_create(nb_ctx *c, const nb_md_params *p, nb_md **out) {
   for (int i = 0; i < n; i++) rm[i] = 1. / p->amass[i];
   int rc;
   hipSetDevice(c->dev);
-#define MUP(ptr, src, cnt) if ((rc = mup(ptr, src, cnt))) { nb_md_destroy(md); return rc; }
+  const size_t bytes0 = c->dev_bytes;
+#define MUP(ptr, src, cnt) \
+  if ((rc = mup(c, ptr, src, cnt))) { md->bytes = c->dev_bytes - bytes0; nb_md_destroy(md); return rc; }
   MUP(&md->d_vel, (const double *)nullptr, 3 * (size_t)n);
   MUP(&md->d_aa, (const double *)nullptr, 3 * (size_t)n);
   MUP(&md->d_rm, rm.data(), n);
@@ -382,6 +387,7 @@ extern "C" int nb_md_create(nb_ctx *c, const nb_md_params *p, nb_md **out) {
   MUP(&md->d_ep, (const double *)nullptr, 3 * (size_t)md->ep_blocks);
   MUP(&md->d_eo, (const double *)nullptr, 8);
 #undef MUP
+  md->bytes = c->dev_bytes - bytes0;
   (void)g_dummy;
   *out = md;
   return NB_OK;

@@ -95,6 +95,7 @@ class NBMDEnergy(C.Structure):
 EXPORTS = ["nb_create", "nb_destroy", "nb_last_error", "nb_build_lists",
            "nb_need_rebuild", "nb_force", "nb_force_device",
            "nb_build_lists_device", "nb_list_counts", "nb_get_pairs", "nb_timing",
+           "nb_footprint",
            "nb_md_create", "nb_md_destroy", "nb_md_set_state", "nb_md_get_state",
            "nb_md_init", "nb_md_step", "nb_md_energies", "nb_md_rebuilds",
            "nb_batch_create", "nb_batch_destroy", "nb_batch_step", "nb_batch_launches",
@@ -128,6 +129,7 @@ def lib():
                                   C.c_void_p]
     L.nb_build_lists_device.argtypes = [C.c_void_p, C.c_void_p, _dp, C.c_void_p]
     L.nb_list_counts.argtypes = [C.c_void_p, _ip]
+    L.nb_footprint.argtypes = [C.c_void_p, C.POINTER(C.c_size_t)]
     L.nb_get_pairs.argtypes = [C.c_void_p, C.c_int, _ip, _ip, C.c_int, _ip]
     L.nb_timing.argtypes = [C.c_void_p, C.c_int, _dp]
     L.nb_md_create.argtypes = [C.c_void_p, C.POINTER(NBMDParams), C.POINTER(C.c_void_p)]
@@ -279,6 +281,12 @@ class NBContext:
         c = (C.c_int * 4)()
         check(lib().nb_list_counts(self._h, c), "nb_list_counts")
         return dict(nprs=c[0], nprt=c[1], nprse=c[2], nprte=c[3])
+
+    def footprint(self):
+        """Device bytes held by this context and its MD state."""
+        b = C.c_size_t(0)
+        check(lib().nb_footprint(self._h, C.byref(b)), "nb_footprint")
+        return int(b.value)
 
     def get_pairs(self, which):
         """Sorted (i<j) pair arrays of list 's', 't', 'se' or 'te'."""

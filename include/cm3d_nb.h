@@ -126,6 +126,11 @@ int nb_build_lists_device(nb_ctx *ctx, const double *d_pxyz,
 /* Pair counts of the current lists as cm3d counts them (half pairs):
  * counts[0..3] = nprs, nprt, nprse, nprte. */
 int nb_list_counts(nb_ctx *ctx, int counts[4]);
+/* Device memory held by the context and any MD state attached to it, in
+ * bytes (grows when a list build needs more neighbour capacity).  No
+ * counterpart in cm3d: it is what a server admitting jobs by HBM footprint
+ * accounts with. */
+int nb_footprint(nb_ctx *ctx, size_t *bytes);
 
 /* Copy one list as half pairs (i<j) in ascending (i,j) order.
  * which: 0 = s (vdW short), 1 = t (vdW), 2 = se, 3 = te.  Fails with
