@@ -62,6 +62,9 @@ def parse():
     ap.add_argument("--serve-rate", type=float, default=20.0, help="mean arrivals per second")
     ap.add_argument("--serve-slots", type=int, default=64)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="run only a 256 MiB device copy (counter calibration) and 5 FP64 "
+                         "force(...,1) calls on the water system, for rocprofv3 --pmc")
     ap.add_argument("--only-batch", action="store_true",
                     help="run the batched form alone (profiling); prints its own JSON")
     return ap.parse_args()
@@ -126,9 +129,14 @@ def run_single(nb, S, a, torch):
     # kernel durations: force(...,1) on the current state with HIP events on
     # the stream the kernels run on
     pos, _ = md.get_state()
+    ctx.build_lists(pos, s["hmat"])
+    d_pos = torch.from_numpy(pos).cuda()
+    d_fl = torch.zeros_like(d_pos)
+    torch.cuda.synchronize()
     ctx.timing_start()
-    for _ in range(20):
-        ctx.force(pos, s["hmat"], iflg=1)
+    for _ in range(20):  # force(...,1) as the MD step calls it: fl only (k_pair64_1f)
+        ctx.force_device(1, d_pos.data_ptr(), s["hmat"], 0, d_fl.data_ptr(), stream=sp)
+    torch.cuda.synchronize()
     tm = ctx.timing_read()
     counts = ctx.list_counts()
     # energy drift of both precisions from the same equilibrated state, and
@@ -225,6 +233,23 @@ def main():
 
     if not torch.cuda.is_available():
         raise SystemExit("bench.py needs a GPU (libcm3d_nb.so has no CPU path)")
+    if a.traffic_probe:
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")  # 256 MiB
+        dst = torch.empty_like(src)
+        for _ in range(3):  # reads 256 MiB, writes 256 MiB per launch
+            torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+        s = S.water(nside=a.nside)
+        ctx = nb.NBContext(s, precision="fp64")
+        ctx.build_lists(s["pos"], s["hmat"])
+        d_pos = torch.from_numpy(s["pos"]).cuda()
+        d_fl = torch.zeros_like(d_pos)
+        for _ in range(5):
+            ctx.force_device(1, d_pos.data_ptr(), s["hmat"], 0, d_fl.data_ptr())
+        torch.cuda.synchronize()
+        ctx.close()
+        print(json.dumps({"traffic_probe": "done", "neg_bytes_each_way": 256 << 20}), flush=True)
+        return
     if a.only_batch:
         b = run_batch(nb, S, a, torch)
         print(json.dumps({"batched_atom_steps_per_s": b["natoms_total"] * a.batch_steps / b["wall"],
