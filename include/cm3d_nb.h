@@ -41,6 +41,7 @@
  */
 #ifndef CM3D_NB_H
 #define CM3D_NB_H
+#include <stddef.h>
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -140,8 +141,8 @@ int nb_get_pairs(nb_ctx *ctx, int which, int *ip, int *jp, int cap, int *n);
 /* Per-phase timing of nb_force / nb_force_device with HIP events on the
  * stream the kernels run on (no counterpart in cm3d; diagnostics).
  * mode 1: start recording (up to 2048 calls), 0: stop, 2: read the averages
- * in ms: out = {calls, position pack, pair kernel, ecorr (with the pair
- * energy reduction), Ewald (four kernels), whole call}. */
+ * in ms: out = {calls, position pack, pair kernel, (gap between pair and
+ * Ewald phases), Ewald + ecorr + energy reductions, whole call}. */
 int nb_timing(nb_ctx *ctx, int mode, double out[6]);
 
 /* ---- device-resident MD step (cm3d's step(), integrate/md_step.c:62-159,
