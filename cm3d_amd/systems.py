@@ -274,3 +274,31 @@ def equilibrate(mds, systems, temp, nsteps, every=10, stepper=None):
             t = kinetic_temperature(vel, s["amass"])
             if t > 0:
                 m.set_state(pos, vel * math.sqrt(temp / t), s["hmat"])
+
+
+def rocksalt(ncell=4, a=5.64, alpha=0.35, kmax=10, rc=10.0, skin=1.0, rheal=0.5,
+             ntable=5000, mass=(22.99, 35.45), seed=5):
+    """NaCl rock-salt crystal: ncell^3 conventional cells (8 ions each,
+    charges +-1 e) with lattice constant a, centred on the origin.  LJ with a
+    negligible well keeps the vdW path on; the electrostatic energy of the
+    perfect lattice is -M N/2 e^2/(a/2), M = 1.747565 (Madelung)."""
+    m = 2 * ncell
+    g = np.stack(np.meshgrid(np.arange(m), np.arange(m), np.arange(m), indexing="ij"),
+                 -1).reshape(-1, 3)
+    n = len(g)
+    L = ncell * a
+    xyz = g * (a / 2.) - L / 2.
+    itype = ((g.sum(axis=1)) % 2).astype(np.int32)   # 0: Na+, 1: Cl-
+    q = np.where(itype == 0, 1.0, -1.0) * SQRT_CCONV
+    s = _base(n, 2, ntable, skin, rheal)
+    s.update(itype=itype, map=np.array([0, 0, 0, 0], np.int32), qch=q,
+             hmat=np.diag([L, L, L]).reshape(-1).astype(np.float64),
+             pos=xyz.T.reshape(-1).copy(),
+             amass=np.where(itype == 0, mass[0], mass[1]) * MCONV)
+    _pair_tables(s, [(nb.POT_LJ, [1e-12, 2.5], 1.0, rc, rc)], ntable, skin, rheal)
+    _lrc(s)
+    _exclusions(s, [])
+    _coulomb(s, ntable, alpha, 0.5, rc, rc, rheal, skin, kmax)
+    s["vel"] = maxwell_velocities(s["amass"], 10.0, seed)
+    s["name"] = f"nacl_{n}"
+    return s

@@ -848,7 +848,8 @@ double orc_ewald_recip_direct(const orc_sys *s, const double *pos,
     double akb = (s->ka[i] * hi[1] + s->kb[i] * hi[4] + s->kc[i] * hi[7]) * tpi;
     double akc = (s->ka[i] * hi[2] + s->kb[i] * hi[5] + s->kc[i] * hi[8]) * tpi;
     double g2 = aka * aka + akb * akb + akc * akc;
-    double preg = 2. * exp(-g2 * falp2) / (g2 * vol), sr = 0., si = 0.;
+    /* half-space k list: E = (4 pi / V) sum_k e^{-k^2/4a^2} / k^2 |S(k)|^2 */
+    double preg = 4. * M_PI * exp(-g2 * falp2) / (g2 * vol), sr = 0., si = 0.;
     for (j = 0; j < n; j++) {
       double q = s->qch[j], rk;
       if (q == 0.) continue;

@@ -1,0 +1,116 @@
+"""Pin the oracle by what the mathematics fixes (CPU only): the Ewald total of
+a rock-salt crystal equals the Madelung closed form and does not depend on
+the real/reciprocal split; cm3d's k-space recursion equals the direct
+cos/sin sum; the tabulated real-space Coulomb and LJ agree with the analytic
+erfc and LJ sums to the table's interpolation accuracy; bonded forces are the
+gradient of the bonded energy; pair and reciprocal forces obey Newton's third
+law.  A dropped term, a wrong sign, a transposed box or a wrong k ordering
+fails at least one of these."""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import load_fixture
+
+MADELUNG_NACL = 1.747564594633182
+
+
+def _coulomb_total(s, pos=None):
+    from oracle import oracle as O
+    pos = s["pos"] if pos is None else pos
+    L = O.build_lists(s, pos, s["hmat"])
+    fs, fl, e = O.force(s, L, pos, s["hmat"], 1)
+    return e["spote"] + e["vk_ewald"] + e["pot_ecorr"], fl, e
+
+
+def test_rocksalt_madelung():
+    from cm3d_amd import systems as S
+    s = S.rocksalt(ncell=4, a=5.64, alpha=0.35, kmax=10)
+    n = int(s["natoms"])
+    u, fl, e = _coulomb_total(s)
+    exact = -MADELUNG_NACL * (n / 2) * S.CCONV / (5.64 / 2)
+    assert abs(u - exact) < 2e-6 * abs(exact), (u, exact)
+    # every ion sits at an inversion centre: no force
+    assert np.abs(fl).max() < 1e-6 * S.CCONV / 2.82 ** 2
+
+
+def test_ewald_split_independent_of_alpha():
+    from cm3d_amd import systems as S
+    # rc 10 A < L/2; erfc(alpha rc) and exp(-(pi kmax / alpha L)^2) both
+    # below ~1e-7 for each split
+    s1 = S.rocksalt(ncell=4, alpha=0.35, kmax=10)
+    s2 = S.rocksalt(ncell=4, alpha=0.45, kmax=14)
+    rng = np.random.default_rng(3)
+    pos = s1["pos"] + rng.normal(0, 0.15, s1["pos"].shape)   # off-lattice
+    u1, f1, _ = _coulomb_total(s1, pos)
+    u2, f2, _ = _coulomb_total(s2, pos)
+    assert abs(u1 - u2) < 3e-6 * abs(u1), (u1, u2)
+    assert np.sqrt(((f1 - f2) ** 2).mean() / (f1 ** 2).mean()) < 1e-4
+
+
+@pytest.mark.parametrize("name", ["water512", "mix600", "ref_abutane"])
+def test_kspace_recursion_equals_direct_sum(name):
+    from oracle import oracle as O
+    d = load_fixture(name)
+    L = O.build_lists(d, d["pos"], d["hmat"])
+    _, _, e = O.force(d, L, d["pos"], d["hmat"], 1)
+    direct = O.ewald_recip_direct(d, d["pos"], d["hmat"])
+    assert abs(e["vk_ewald"] - direct) <= 1e-10 * abs(direct), (e["vk_ewald"], direct)
+
+
+def test_real_space_coulomb_table_vs_erfc():
+    from cm3d_amd import systems as S
+    from oracle import oracle as O
+    s = S.mixture(600, seed=11, rc=9.0, kmax=5, ntable=5000)
+    L = O.build_lists(s, s["pos"], s["hmat"])
+    _, _, e = O.force(s, L, s["pos"], s["hmat"], 1)
+    direct = O.coul_real_direct(s, s["pos"], s["hmat"], 9.0)
+    assert abs(e["spote"] - direct) < 1e-6 * abs(direct), (e["spote"], direct)
+
+
+def test_lj_table_vs_analytic():
+    from cm3d_amd import systems as S
+    from oracle import oracle as O
+    s = S.lj_fcc(ncells=4, ntable=5000)
+    rng = np.random.default_rng(1)
+    pos = s["pos"] + rng.normal(0, 0.1, s["pos"].shape)
+    L = O.build_lists(s, pos, s["hmat"])
+    _, fl, e = O.force(s, L, pos, s["hmat"], 1)
+    eps = np.array([0.238 * S.KCAL])
+    sig = np.array([3.405])
+    v, f = O.lj_direct(s, pos, s["hmat"], eps, sig)
+    assert abs(e["spotl"] - v) < 1e-6 * abs(v), (e["spotl"], v)
+    assert np.sqrt(((fl - f) ** 2).mean() / (f ** 2).mean()) < 1e-5
+
+
+def test_bonded_forces_are_energy_gradient():
+    from cm3d_amd import systems as S
+    from oracle import oracle as O
+    s = S.water(nside=2, seed=4)
+    n = int(s["natoms"])
+    b = dict(ib=s["bond_i"], jb=s["bond_j"], bk=s["bond_k"], beq=s["bond_r0"],
+             ia=s["bend_i"], ja=s["bend_j"], ka=s["bend_k"], ak=s["bend_fk"],
+             aeq=s["bend_th0"])
+    rng = np.random.default_rng(2)
+    pos = s["pos"] + rng.normal(0, 0.03, s["pos"].shape)     # strain bonds and bends
+    _, f = O.bonded_force(n, b, pos)
+    h = 1e-5
+    for k in rng.choice(3 * n, 12, replace=False):
+        p1, p2 = pos.copy(), pos.copy()
+        p1[k] += h
+        p2[k] -= h
+        fd = -(O.bonded_force(n, b, p1)[0] - O.bonded_force(n, b, p2)[0]) / (2 * h)
+        assert abs(fd - f[k]) < 1e-6 * max(1.0, abs(f[k])), (k, fd, f[k])
+
+
+@pytest.mark.parametrize("name", ["water512", "ar864", "mix600"])
+def test_newton_third_law(name):
+    from oracle import oracle as O
+    d = load_fixture(name)
+    L = O.build_lists(d, d["pos"], d["hmat"])
+    fs, fl, _ = O.force(d, L, d["pos"], d["hmat"], 1)
+    n = int(np.asarray(d["natoms"]).reshape(-1)[0])
+    tot = fl.reshape(3, n).sum(axis=1)
+    scale = max(np.abs(fl).sum() / n, 1.0)  # lattice starts have ~zero forces
+    assert np.abs(tot).max() < 1e-9 * scale * n, (tot, scale)
