@@ -273,10 +273,17 @@ def main():
     pair_gbs = pair_bytes / (tm["pair"] * 1e-3) / 1e9
     ew_tfs = ew_flop / (tm["ewald"] * 1e-3) / 1e12
     if tm["pair"] >= tm["ewald"]:
+        traffic, tsrc = None, None
+        tpath = os.path.join(ROOT, "profiles", "pair_kernel_water4096_traffic.json")
+        if os.path.exists(tpath) and a.nside == 16:
+            with open(tpath) as f:
+                tj = json.load(f)
+            traffic = round(float(tj["traffic_bytes_per_launch"]), 0)
+            tsrc = "profiles/pair_kernel_water4096_traffic.md (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated)"
         roof = {"kernel": "k_pair64_1f", "bound": "hbm", "achieved": round(pair_gbs, 2),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(pair_gbs / HBM_PEAK_GBS, 5),
-                "traffic": None, "ms_per_launch": round(tm["pair"], 5),
-                "bytes_per_launch": pair_bytes}
+                "traffic": traffic, "traffic_source": tsrc,
+                "ms_per_launch": round(tm["pair"], 5), "bytes_per_launch": pair_bytes}
     else:
         roof = {"kernel": "Ewald (k_ewchg+k_ewsfac+k_ewkred+k_red_ew+k_ewforce)",
                 "bound": "valu-fp64", "achieved": round(ew_tfs, 3), "peak": FP64_PEAK_TFLOPS,
