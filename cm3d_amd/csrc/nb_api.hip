@@ -291,10 +291,10 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
     c->qc = std::max(1, (ncharge + nch - 1) / nch);
     c->nchunk = std::max(1, (ncharge + c->qc - 1) / c->qc);
   }
-  // force: one lane per charge, a workgroup per (256 charges, tile range);
+  // force: two charges per lane, a workgroup per (512 charges, tile range);
   // tiles are runs of whole segments small enough for LDS and for enough
   // ranges to give >= ~1024 workgroups; ranges balanced by k count
-  c->nqb = std::max(1, (ncharge + 255) / 256);
+  c->nqb = std::max(1, (ncharge + 511) / 512);  // 256 lanes x 2 charges
   std::vector<int> tseg(1, 0), rtile(1, 0);
   {
     int nrt = std::min(std::max(1, (1024 + c->nqb - 1) / c->nqb), std::max(1, c->nseg));

@@ -666,7 +666,8 @@ static __device__ __forceinline__ void k_ewsfac_body(const EwSfacArgs &A, int bi
 #undef EW_SFAC_CHARGE
 NB_KERNEL_PAIR(k_ewsfac, EwSfacArgs, 256)
 
-struct KData { double kx, ky, kz, cr, ci; };
+// per k: 2 preg S(k), the amplitude the force pass needs
+struct KData { double cr, ci; };
 
 // per k: the chunk partials in fixed order (a workgroup = 64 k x 4 chunk
 // quarters, the quarters combined in order), then the k-space terms
@@ -713,7 +714,6 @@ static __device__ __forceinline__ void k_ewkred_body(const EwKredArgs &A, int bi
     acc[5] = prep * sm * aka * akb; acc[6] = prep * sm * akb * akb; acc[7] = prep * sm * akc * akb;
     acc[8] = prep * sm * aka * akc; acc[9] = prep * sm * akb * akc; acc[10] = prep * sm * akc * akc;
     KData d;
-    d.kx = aka; d.ky = akb; d.kz = akc;
     d.cr = sr * 2. * preg;
     d.ci = si * 2. * preg;
     A.kd[i] = d;
@@ -724,56 +724,95 @@ NB_KERNEL_PAIR(k_ewkred, EwKredArgs, 256)
 
 // k_ewforce: tiles of whole segments (<= NB_EW_KT k each) are staged in LDS;
 // a range is a run of tiles (rtile[r] .. rtile[r+1]); tile t covers
-// segments tseg[t] .. tseg[t+1].
+// segments tseg[t] .. tseg[t+1].  Each lane takes two charges.  Within a
+// segment (ka, kb fixed) the k vector is A + kc Gc with A = 2 pi (ka h^-T_a +
+// kb h^-T_b) and Gc = 2 pi h^-T_c, so sum_k k tt_k = A sum tt + Gc sum kc tt:
+// per k and charge, tt = Re/Im part of 2 preg S' e^{ik.r} (2 ops), two sums
+// (2) and the recursion (4).
 struct EwForceArgs {
   const EwSeg *seg; const int *rtile, *tseg; const KData *kd; const double4 *chg;
-  const double2 *stp; int nq, nqb; double *fpart;
+  const double2 *stp; int nq, nqb; double *fpart; Box b;
 };
+#define EWF_Q 2  // charges per lane
 static __device__ __forceinline__ void k_ewforce_body(const EwForceArgs &A, int bid) {
 #pragma clang fp contract(fast)
-  __shared__ double kl[6 * NB_EW_KT];  // {kx, ky, kz, cr, ci, -} per k
+  __shared__ double2 kl[NB_EW_KT];  // {2 preg Re S, 2 preg Im S} per k of the tile
   const int qb = bid % A.nqb, r = bid / A.nqb;
-  const int c = qb * blockDim.x + threadIdx.x;
-  const bool act = c < A.nq;
-  double4 q4 = make_double4(0., 0., 0., 0.);
-  double2 st = make_double2(1., 0.);
-  if (act) { q4 = A.chg[c]; st = A.stp[c]; }
+  const int cbase = qb * blockDim.x * EWF_Q + threadIdx.x;
+  double4 q4[EWF_Q];
+  double2 st[EWF_Q];
+  double fx[EWF_Q], fy[EWF_Q], fz[EWF_Q], sn[EWF_Q], cs[EWF_Q];
+#pragma unroll
+  for (int u = 0; u < EWF_Q; u++) {
+    const int c = cbase + u * blockDim.x;
+    q4[u] = make_double4(0., 0., 0., 0.);
+    st[u] = make_double2(1., 0.);
+    if (c < A.nq) { q4[u] = A.chg[c]; st[u] = A.stp[c]; }
+    fx[u] = fy[u] = fz[u] = 0.;
+    sn[u] = 0.;
+    cs[u] = 1.;
+  }
+  const double tpi = 2. * M_PI, *hi = A.b.hi;
+  const double gcx = tpi * hi[6], gcy = tpi * hi[7], gcz = tpi * hi[8];
   cptr4<EwSeg> seg = as_const4(A.seg);
   cptr4<int> rt = as_const4(A.rtile), ts = as_const4(A.tseg);
   const int t0 = rt[r], t1 = rt[r + 1], gfirst = ts[t0];
-  double fx = 0., fy = 0., fz = 0., sn = 0., cs = 1.;
   for (int t = t0; t < t1; t++) {
     const int g0 = ts[t], g1 = ts[t + 1];
     const int kb0 = seg[g0].k0, nk = seg[g1 - 1].k0 + seg[g1 - 1].cnt - kb0;
     __syncthreads();
     for (int u = threadIdx.x; u < nk; u += blockDim.x) {
       const KData d = A.kd[kb0 + u];
-      double *o = kl + 6 * u;
-      o[0] = d.kx; o[1] = d.ky; o[2] = d.kz; o[3] = d.cr; o[4] = d.ci;
+      kl[u] = make_double2(d.cr, d.ci);
     }
     __syncthreads();
     for (int g = g0; g < g1; g++) {
       const int cnt = seg[g].cnt;
-      if (g == gfirst || !seg[g].cont)
-        nb_sincos((double)seg[g].ka * q4.x + (double)seg[g].kb * q4.y + (double)seg[g].kc0 * q4.z,
-                  &sn, &cs);
-      const double *K = kl + 6 * (seg[g].k0 - kb0);
-      for (int m = 0; m < cnt; m++, K += 6) {
-        const double tt = K[3] * sn - K[4] * cs;
-        fx += K[0] * tt;
-        fy += K[1] * tt;
-        fz += K[2] * tt;
-        const double c2 = cs * st.x - sn * st.y;
-        sn = sn * st.x + cs * st.y;
-        cs = c2;
+      const double ka = seg[g].ka, kb = seg[g].kb, kc0 = seg[g].kc0;
+      if (g == gfirst || !seg[g].cont) {
+#pragma unroll
+        for (int u = 0; u < EWF_Q; u++)
+          nb_sincos(ka * q4[u].x + kb * q4[u].y + kc0 * q4[u].z, &sn[u], &cs[u]);
+      }
+      double s0[EWF_Q], s1[EWF_Q];
+#pragma unroll
+      for (int u = 0; u < EWF_Q; u++) { s0[u] = 0.; s1[u] = 0.; }
+      const double2 *K = kl + (seg[g].k0 - kb0);
+      for (int m = 0; m < cnt; m++) {
+        const double2 kk = K[m];
+        const double fm = m;
+#pragma unroll
+        for (int u = 0; u < EWF_Q; u++) {
+          const double tt = kk.x * sn[u] - kk.y * cs[u];
+          s0[u] += tt;
+          s1[u] += fm * tt;
+          const double c2 = cs[u] * st[u].x - sn[u] * st[u].y;
+          sn[u] = sn[u] * st[u].x + cs[u] * st[u].y;
+          cs[u] = c2;
+        }
+      }
+      // k = A + kc Gc with kc = kc0 + m: sum k tt = A s0 + Gc (kc0 s0 + s1)
+      const double ax = tpi * (ka * hi[0] + kb * hi[3]);
+      const double ay = tpi * (ka * hi[1] + kb * hi[4]);
+      const double az = tpi * (ka * hi[2] + kb * hi[5]);
+#pragma unroll
+      for (int u = 0; u < EWF_Q; u++) {
+        const double w = kc0 * s0[u] + s1[u];
+        fx[u] += ax * s0[u] + gcx * w;
+        fy[u] += ay * s0[u] + gcy * w;
+        fz[u] += az * s0[u] + gcz * w;
       }
     }
   }
-  if (act) {
-    double *o = A.fpart + (size_t)(3 * r) * A.nq + c;
-    o[0] = fx;
-    o[A.nq] = fy;
-    o[2 * (size_t)A.nq] = fz;
+#pragma unroll
+  for (int u = 0; u < EWF_Q; u++) {
+    const int c = cbase + u * blockDim.x;
+    if (c < A.nq) {
+      double *o = A.fpart + (size_t)(3 * r) * A.nq + c;
+      o[0] = fx[u];
+      o[A.nq] = fy[u];
+      o[2 * (size_t)A.nq] = fz[u];
+    }
   }
 }
 NB_KERNEL_PAIR(k_ewforce, EwForceArgs, 256)
@@ -972,7 +1011,8 @@ int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_f
                    b, 1. / det, falp2, (KData *)c->d_kd, c->d_eew};
   NB_GO(c, K_EWKRED, k_ewkred, ew ? c->new_blocks : 0, 256, 0, kr, s);
   EwForceArgs ef = {c->d_seg, c->d_rtile, c->d_tseg, (const KData *)c->d_kd,
-                    (const double4 *)c->d_chg, (const double2 *)c->d_stp, nq, c->nqb, c->d_fpart};
+                    (const double4 *)c->d_chg, (const double2 *)c->d_stp, nq, c->nqb, c->d_fpart,
+                    b};
   NB_GO(c, K_EWFORCE, k_ewforce, ew ? c->nqb * c->nrange : 0, 256, 0, ef, s);
   EwFinArgs fa = {c->d_fpart, (const double4 *)c->d_chg, c->d_qidx, nq, n, c->nrange, ew, ec,
                   c->d_eoff, c->d_elist, c->d_pos, c->d_qch, c->kappa, d_fl, c->nfin_blocks,

@@ -43,6 +43,7 @@ struct nb_md {
   hipGraphExec_t gexec;
   int have_graph;
   int nrebuild_host;
+  int maxa;      // atoms in the largest molecule
   size_t bytes;  // device memory of this MD state (counted in its context's footprint)
 };
 
@@ -86,6 +87,81 @@ static __device__ void mol_bonded(const double (*p)[3], double (*f)[3], int na, 
   }
 }
 
+// The same harmonic bond / bend arithmetic on a molecule held in registers
+// (at most NA atoms): atom slots are picked by unrolled compares, so every
+// index is static; operations and their order are those of mol_bonded.
+template <int NA>
+static __device__ __forceinline__ void sel3(const double (&p)[NA][3], int i, double &x,
+                                            double &y, double &z) {
+  x = p[0][0]; y = p[0][1]; z = p[0][2];
+#pragma unroll
+  for (int a = 1; a < NA; a++) {
+    const bool h = i == a;
+    x = h ? p[a][0] : x;
+    y = h ? p[a][1] : y;
+    z = h ? p[a][2] : z;
+  }
+}
+template <int NA>
+static __device__ __forceinline__ void add3(double (&f)[NA][3], int i, double x, double y,
+                                            double z) {
+#pragma unroll
+  for (int a = 0; a < NA; a++) {  // +0. leaves the other slots' values unchanged
+    const bool h = i == a;
+    f[a][0] += h ? x : 0.;
+    f[a][1] += h ? y : 0.;
+    f[a][2] += h ? z : 0.;
+  }
+}
+template <int NA>
+static __device__ __forceinline__ void mol_bonded_r(const double (&p)[NA][3], double (&f)[NA][3],
+                                                    int a0, int b0, int b1, int e0, int e1,
+                                                    const int *bi, const int *bj,
+                                                    const double *bk, const double *br0,
+                                                    const int *ai, const int *aj,
+                                                    const int *ak, const double *afk,
+                                                    const double *ath0) {
+#pragma unroll
+  for (int a = 0; a < NA; a++) f[a][0] = f[a][1] = f[a][2] = 0.;
+  for (int b = b0; b < b1; b++) {
+    const int i = bi[b] - a0, j = bj[b] - a0;
+    double xi, yi, zi, xj, yj, zj;
+    sel3(p, i, xi, yi, zi);
+    sel3(p, j, xj, yj, zj);
+    double dx = xi - xj, dy = yi - yj, dz = zi - zj;
+    double r = sqrt(dx * dx + dy * dy + dz * dz), dr = r - br0[b];
+    double fr = bk[b] * dr / r;
+    dx *= fr; dy *= fr; dz *= fr;
+    add3(f, i, -dx, -dy, -dz);
+    add3(f, j, dx, dy, dz);
+  }
+  for (int e = e0; e < e1; e++) {
+    const int i = ai[e] - a0, j = aj[e] - a0, k = ak[e] - a0;
+    double xi, yi, zi, xj, yj, zj, xk, yk, zk;
+    sel3(p, i, xi, yi, zi);
+    sel3(p, j, xj, yj, zj);
+    sel3(p, k, xk, yk, zk);
+    double ax = xi - xj, ay = yi - yj, az = zi - zj;
+    double bx = xk - xj, by = yk - yj, bz = zk - zj;
+    double al = ax * ax + ay * ay + az * az, bl = bx * bx + by * by + bz * bz;
+    double ab = ax * bx + ay * by + az * bz, r11 = 1. / sqrt(al * bl);
+    double c = ab * r11, dudc;
+    if (fabs(c) >= 1.) {
+      dudc = 0. * -1.;
+    } else {
+      double th = acos(c), dth = th - ath0[e];
+      double dudth = afk[e] * dth, dthdcos = -1. / sqrt(1. - c * c);
+      dudc = dudth * dthdcos;
+    }
+    double d1x = c * ax / al - bx * r11, d1y = c * ay / al - by * r11, d1z = c * az / al - bz * r11;
+    double d3x = c * bx / bl - ax * r11, d3y = c * by / bl - ay * r11, d3z = c * bz / bl - az * r11;
+    double d2x = -d1x - d3x, d2y = -d1y - d3y, d2z = -d1z - d3z;
+    add3(f, i, dudc * d1x, dudc * d1y, dudc * d1z);
+    add3(f, j, dudc * d2x, dudc * d2y, dudc * d2z);
+    add3(f, k, dudc * d3x, dudc * d3y, dudc * d3z);
+  }
+}
+
 struct InnerArgs {
   int n, nmol, ninnra, kick_first;
   double dter, dta2;
@@ -103,12 +179,97 @@ struct InnerArgs {
   Box b;
 };
 
-static __device__ __forceinline__ void k_mdinner_body(const InnerArgs &A, int bid) {
-  __shared__ double sp[MD_TPB][MD_MAXA][3], sv[MD_TPB][MD_MAXA][3], sa[MD_TPB][MD_MAXA][3];
-  int m = bid * MD_TPB + threadIdx.x;
-  if (m >= A.nmol) return;
-  const int n = A.n, a0 = A.moff[m], na = A.moff[m + 1] - a0;
-  double (*p)[3] = sp[threadIdx.x], (*v)[3] = sv[threadIdx.x], (*aa)[3] = sa[threadIdx.x];
+// End of the inner loop for one atom: write it back and, for the force
+// evaluation that follows, apply nrefsh's test (md_finter.c:61-73:
+// |r - r_build|^2 > skin^2/4) and write its sorted record and Ewald data.
+static __device__ __forceinline__ bool md_atom_out(const InnerArgs &A, int g, double x, double y,
+                                                   double z, double vx, double vy, double vz,
+                                                   double ax, double ay, double az) {
+  const int n = A.n;
+  A.pos[g] = x; A.pos[n + g] = y; A.pos[2 * n + g] = z;
+  A.vel[g] = vx; A.vel[n + g] = vy; A.vel[2 * n + g] = vz;
+  A.aa[g] = ax; A.aa[n + g] = ay; A.aa[2 * n + g] = az;
+  if (!A.epilogue) return false;
+  const double ex = A.posb[g] - x, ey = A.posb[n + g] - y, ez = A.posb[2 * n + g] - z;
+  const bool moved = ex * ex + ey * ey + ez * ez > A.skin2q;
+  const int sl = A.iperm[g];
+  const double qg = A.q[g];
+  if (A.recf) A.recf[sl] = nb_mixed_record(x, y, z, qg, A.b.hi);
+  else A.rec64[sl] = make_double4(x, y, z, qg);
+  if (A.chg) {
+    const int c = A.qpos[g];
+    if (c >= 0) nb_charge_record(x, y, z, qg, A.b.hi, A.chg + c, A.stp + c);
+  }
+  return moved;
+}
+
+// Molecules of at most NA atoms, all in registers.
+template <int NA>
+static __device__ __forceinline__ bool mdinner_regs(const InnerArgs &A, int m, int a0, int na) {
+  const int n = A.n;
+  double p[NA][3], v[NA][3], aa[NA][3];
+#pragma unroll
+  for (int a = 0; a < NA; a++) {
+    if (a >= na) break;
+    const int g = a0 + a;
+    const double rm = A.rm[g];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      p[a][d] = A.pos[d * n + g];
+      v[a][d] = A.vel[d * n + g];
+      aa[a][d] = A.aa[d * n + g];
+      if (A.kick_first) v[a][d] += A.dter * (A.fl[d * n + g] * rm);
+    }
+  }
+  const int b0 = A.boff[m], b1 = A.boff[m + 1], e0 = A.aoff[m], e1 = A.aoff[m + 1];
+  const bool bonded = (b1 > b0) || (e1 > e0);
+  const double dta2 = A.dta2;
+  for (int it = 0; it < A.ninnra; it++) {
+#pragma unroll
+    for (int a = 0; a < NA; a++) {
+      if (a >= na) break;
+  #pragma unroll
+    for (int d = 0; d < 3; d++) {
+        v[a][d] += dta2 * aa[a][d];
+        p[a][d] += dta2 * v[a][d];
+        p[a][d] += dta2 * v[a][d];
+      }
+    }
+    if (bonded) {
+      mol_bonded_r<NA>(p, aa, a0, b0, b1, e0, e1, A.bi, A.bj, A.bk, A.br0, A.ai, A.aj, A.ak,
+                       A.afk, A.ath0);
+#pragma unroll
+      for (int a = 0; a < NA; a++) {
+        if (a >= na) break;
+        const double rm = A.rm[a0 + a];
+#pragma unroll
+        for (int d = 0; d < 3; d++) aa[a][d] *= rm;
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < NA; a++) {
+      if (a >= na) break;
+#pragma unroll
+      for (int d = 0; d < 3; d++) v[a][d] += dta2 * aa[a][d];
+    }
+  }
+  bool moved = false;
+#pragma unroll
+  for (int a = 0; a < NA; a++) {
+    if (a >= na) break;
+    moved |= md_atom_out(A, a0 + a, p[a][0], p[a][1], p[a][2], v[a][0], v[a][1], v[a][2],
+                         aa[a][0], aa[a][1], aa[a][2]);
+  }
+  return moved;
+}
+
+// Larger molecules (up to MD_MAXA atoms) in this thread's slice of the
+// workgroup's dynamic LDS.
+static __device__ __forceinline__ bool mdinner_lds(const InnerArgs &A, int m, int a0, int na) {
+  extern __shared__ double mdl[];
+  double (*p)[3] = (double (*)[3])(mdl + (size_t)threadIdx.x * 9 * MD_MAXA);
+  double (*v)[3] = p + MD_MAXA, (*aa)[3] = p + 2 * MD_MAXA;
+  const int n = A.n;
   for (int a = 0; a < na; a++) {
     int g = a0 + a;
     double rm = A.rm[g];
@@ -141,27 +302,20 @@ static __device__ __forceinline__ void k_mdinner_body(const InnerArgs &A, int bi
       for (int d = 0; d < 3; d++) v[a][d] += dta2 * aa[a][d];
   }
   bool moved = false;
-  for (int a = 0; a < na; a++) {
-    int g = a0 + a;
-    for (int d = 0; d < 3; d++) {
-      A.pos[d * n + g] = p[a][d];
-      A.vel[d * n + g] = v[a][d];
-      A.aa[d * n + g] = aa[a][d];
-    }
-    if (A.epilogue) {
-      const double x = p[a][0], y = p[a][1], z = p[a][2];
-      // nrefsh (md_finter.c:61-73): |r - r_build|^2 > skin^2 / 4
-      const double ex = A.posb[g] - x, ey = A.posb[n + g] - y, ez = A.posb[2 * n + g] - z;
-      moved |= ex * ex + ey * ey + ez * ez > A.skin2q;
-      const int sl = A.iperm[g];
-      const double qg = A.q[g];
-      if (A.recf) A.recf[sl] = nb_mixed_record(x, y, z, qg, A.b.hi);
-      else A.rec64[sl] = make_double4(x, y, z, qg);
-      if (A.chg) {
-        const int c = A.qpos[g];
-        if (c >= 0) nb_charge_record(x, y, z, qg, A.b.hi, A.chg + c, A.stp + c);
-      }
-    }
+  for (int a = 0; a < na; a++)
+    moved |= md_atom_out(A, a0 + a, p[a][0], p[a][1], p[a][2], v[a][0], v[a][1], v[a][2],
+                         aa[a][0], aa[a][1], aa[a][2]);
+  return moved;
+}
+
+static __device__ __forceinline__ void k_mdinner_body(const InnerArgs &A, int bid) {
+  const int m = bid * MD_TPB + threadIdx.x;
+  bool moved = false;
+  if (m < A.nmol) {
+    const int a0 = A.moff[m], na = A.moff[m + 1] - a0;
+    if (na == 1) moved = mdinner_regs<1>(A, m, a0, na);
+    else if (na <= 3) moved = mdinner_regs<3>(A, m, a0, na);
+    else moved = mdinner_lds(A, m, a0, na);
   }
   if (__ballot(moved) != 0 && lane_id() == 0) atomicOr(A.flag, 1);
 }
@@ -312,6 +466,7 @@ extern "C" int nb_md_create(nb_ctx *c, const nb_md_params *p, nb_md **out) {
     return NB_ERR_ARG;
   }
   std::vector<int> molof(n);
+  int maxa = 1;
   for (int m = 0; m < nm; m++) {
     int na = p->mol_start[m + 1] - p->mol_start[m];
     if (na <= 0 || na > MD_MAXA) {
@@ -319,6 +474,7 @@ extern "C" int nb_md_create(nb_ctx *c, const nb_md_params *p, nb_md **out) {
       return NB_ERR_UNSUPPORTED;
     }
     for (int a = p->mol_start[m]; a < p->mol_start[m + 1]; a++) molof[a] = m;
+    if (na > maxa) maxa = na;
   }
   // group bonded terms by molecule, keeping their order within a molecule
   std::vector<int> boff(nm + 1, 0), aoff(nm + 1, 0), border, aorder;
@@ -358,6 +514,7 @@ extern "C" int nb_md_create(nb_ctx *c, const nb_md_params *p, nb_md **out) {
   nb_md *md = new nb_md();
   memset(md, 0, sizeof(*md));
   md->c = c; md->n = n; md->nmol = nm; md->nbond = p->nbond; md->nbend = p->nbend;
+  md->maxa = maxa;
   md->ninnra = p->nres_intra > 0 ? p->nres_intra : 1;
   // md_step.c:73-78 with ninter = ninrra = 1
   md->dt = p->dt;
@@ -445,7 +602,8 @@ int md_enqueue_step(nb_md *md, hipStream_t s) {
   nb_ctx *c = md->c;
   // inner RESPA loop + nrefsh test + records for the force evaluation
   InnerArgs A = inner_args(md);
-  NB_GO(c, K_MDINNER, k_mdinner, (md->nmol + MD_TPB - 1) / MD_TPB, MD_TPB, 0, A, s);
+  const int lds = md->maxa > 3 ? MD_TPB * 9 * MD_MAXA * (int)sizeof(double) : 0;
+  NB_GO(c, K_MDINNER, k_mdinner, (md->nmol + MD_TPB - 1) / MD_TPB, MD_TPB, lds, A, s);
   int rc = nb_launch_build_cond(c, md->box, s);
   if (rc) return rc;
   rc = nb_launch_force(c, 1, md->box, nullptr, c->d_fl, s, NB_FORCE_PACKED | NB_FORCE_NORED);
