@@ -76,10 +76,7 @@ static Box make_box(const double h[9]) {
   return b;
 }
 
-static double det9(const double h[9]) {
-  return h[0] * (h[4] * h[8] - h[5] * h[7]) + h[1] * (h[5] * h[6] - h[3] * h[8]) +
-         h[2] * (h[3] * h[7] - h[4] * h[6]);
-}
+static double det9(const double h[9]) { return nb_det9(h); }
 
 extern "C" void nb_destroy(nb_ctx *c) {
   if (!c) return;
@@ -533,8 +530,8 @@ extern "C" int nb_need_rebuild(nb_ctx *c, const double *pxyz, int *flag) {
   return NB_OK;
 }
 
-static void fill_energy(nb_ctx *c, int iflg, const double *E, const double hmat[9],
-                        nb_energy *e) {
+void nb_fill_energy(nb_ctx *c, int iflg, const double *E, const double hmat[9],
+                    nb_energy *e) {
   memset(e, 0, sizeof(*e));
   e->nclamp = (int)(E[NB_NE_PAIR - 1] * .5 + .25);
   if (iflg != 1) return;
@@ -580,7 +577,7 @@ static int force_on(nb_ctx *c, int iflg, const double hmat[9], double *d_fs, dou
     double E[64];
     HCHK(hipMemcpyAsync(E, c->d_eout, sizeof(E), hipMemcpyDeviceToHost, s));
     HCHK(hipStreamSynchronize(s));
-    fill_energy(c, iflg, E, hmat, e);
+    nb_fill_energy(c, iflg, E, hmat, e);
   }
   return NB_OK;
 }
@@ -601,7 +598,7 @@ extern "C" int nb_force(nb_ctx *c, int iflg, const double *pxyz, const double hm
     double E[64];
     HCHK(hipMemcpyAsync(E, c->d_eout, sizeof(E), hipMemcpyDeviceToHost, c->stream));
     HCHK(hipStreamSynchronize(c->stream));
-    fill_energy(c, iflg, E, hmat, e);
+    nb_fill_energy(c, iflg, E, hmat, e);
   } else {
     HCHK(hipStreamSynchronize(c->stream));
   }

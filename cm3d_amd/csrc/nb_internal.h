@@ -217,6 +217,13 @@ int nb_build_now(nb_ctx *c, const double hmat[9], hipStream_t s);
 int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs,
                     double *d_fl, hipStream_t s, int flags);
 int nb_launch_rebuild_check(nb_ctx *c, const double *d_pos, hipStream_t s);
+// nb_energy from the reduced columns E of a force(...,iflg) evaluation
+void nb_fill_energy(nb_ctx *c, int iflg, const double *E, const double hmat[9], nb_energy *e);
+// det(h), cm3d's get_deth
+static inline double nb_det9(const double h[9]) {
+  return h[0] * (h[4] * h[8] - h[5] * h[7]) + h[1] * (h[5] * h[6] - h[3] * h[8]) +
+         h[2] * (h[3] * h[7] - h[4] * h[6]);
+}
 void nb_time_mark(nb_ctx *c, int ev, hipStream_t s);
 int nb_launch_pair_mixed(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_fl,
                          hipStream_t s, bool packed);

@@ -18,6 +18,10 @@
 //                 the flag reset for the next step
 // Positions and velocities stay FP64 (cm3d's integrator arithmetic, no FMA).
 // A step can be captured once into a hipGraph and replayed.
+//
+// NVT (a Nose-Hoover chain, cm3d's intnhc at every inner substep) splits the
+// inner loop at each chain update, which needs the kinetic sum of the whole
+// system: k_md_nhc launches 0..nres_intra replace k_md_inner (see there).
 #include <string.h>
 
 #include <vector>
@@ -26,6 +30,7 @@
 
 #define MD_MAXA 8          // atoms per molecule held in LDS by k_md_inner
 #define MD_TPB 64          // molecules per workgroup
+#define MD_MAXCHAIN 16     // Nose-Hoover chain links
 
 struct nb_md {
   nb_ctx *c;
@@ -45,6 +50,13 @@ struct nb_md {
   int nrebuild_host;
   int maxa;      // atoms in the largest molecule
   size_t bytes;  // device memory of this MD state (counted in its context's footprint)
+  // Nose-Hoover chain (nchain 0: NVE).  d_nhc: veta[nc], eta[nc], gkt[nc],
+  // mnh[nc]; d_ppart: per-workgroup kinetic sums; d_ticket: last-workgroup
+  // counter (zero between launches)
+  int nchain;
+  double *d_nhc, *d_ppart;
+  int *d_ithm, *d_ticket;
+  size_t nhc_bytes;
 };
 
 // Harmonic bonds then bends of one molecule, in cm3d's term order, on the
@@ -321,6 +333,186 @@ static __device__ __forceinline__ void k_mdinner_body(const InnerArgs &A, int bi
 }
 NB_KERNEL_PAIR(k_mdinner, InnerArgs, MD_TPB)
 
+// ---------------------------------------------------------------------------
+// NVT inner loop.  cm3d's substep (md_step.c:108-128) is
+//   v += dta2 aa; r += dta2 v; intnhc; r += dta2 v; aa = bonded(r)/m;
+//   v += dta2 aa
+// and intnhc (intnhc_new, md_therm.new.c:278-332) scales the thermostatted
+// velocities by exp(-dta2 veta0), updates the chain from their kinetic sum
+// and scales them again by exp(-dta2 veta0) with the new veta0.  The chain
+// is not touched between two intnhc calls, so the second scale of one
+// substep and the first of the next use the same factor.  Launch `phase`:
+//   0          v += dter fl/m (the outer kick), then substep 0 up to its
+//              chain update
+//   1..nri-1   finish substep phase-1, then substep `phase` up to its update
+//   nri        finish the last substep, then the epilogue of k_md_inner
+// Launches 0..nri-1 end with the chain update, done by the last workgroup to
+// finish from the per-workgroup kinetic sums, summed in workgroup order, so
+// the result is independent of scheduling (and of batching).
+struct NhcArgs {
+  InnerArgs I;
+  int phase, nchain, nblocks;
+  const int *ithm;
+  const double *amass;
+  double *nhc, *ppart;
+  int *ticket;
+};
+
+// The chain update of intnhc_new (md_therm.new.c:294-326) with forcen_new's
+// forces (md_therm.new.c:174-197); nhc = veta[nc], eta[nc], gkt[nc], mnh[nc].
+static __device__ void nhc_update(double *nhc, int nc, double p2m, double dta2) {
+  double *veta = nhc, *eta = nhc + nc;
+  const double *gkt = nhc + 2 * nc, *mnh = nhc + 3 * nc;
+  const double dta = 2. * dta2;
+  double feta[MD_MAXCHAIN];
+  for (int i = 0; i < nc - 1; i++) veta[i] *= exp(-dta2 * veta[i + 1]);
+  feta[0] = (p2m - gkt[0]) / mnh[0];
+  for (int i = 1; i < nc; i++) feta[i] = (mnh[i - 1] * veta[i - 1] * veta[i - 1] - gkt[i]) / mnh[i];
+  for (int i = 0; i < nc; i++) veta[i] += dta2 * feta[i];
+  for (int i = 0; i < nc; i++) eta[i] += dta * veta[i];
+  veta[0] += dta2 * feta[0];
+  for (int i = 1; i < nc; i++)
+    veta[i] += dta2 * (mnh[i - 1] * veta[i - 1] * veta[i - 1] - gkt[i]) / mnh[i];
+  for (int i = nc - 2; i >= 0; i--) veta[i] *= exp(-dta2 * veta[i + 1]);
+}
+
+// One molecule through launch N.phase; p, v, aa: NA x 3 storage (registers
+// for NA <= 3, this thread's LDS slice otherwise).  s = exp(-dta2 veta0).
+template <int NA>
+static __device__ __forceinline__ bool mdnhc_mol(const NhcArgs &N, int m, int a0, int na,
+                                                 double s, double &p2m, double (*p)[3],
+                                                 double (*v)[3], double (*aa)[3]) {
+  const InnerArgs &A = N.I;
+  const int n = A.n;
+  const double dta2 = A.dta2;
+  double sc[NA];
+#pragma unroll
+  for (int a = 0; a < NA; a++) {
+    if (a >= na) break;
+    const int g = a0 + a;
+    const double rm = A.rm[g];
+    sc[a] = N.ithm[g] == 0 ? s : 1.;  // exp(-dta2 * 0) for the others
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      p[a][d] = A.pos[d * n + g];
+      v[a][d] = A.vel[d * n + g];
+      aa[a][d] = A.aa[d * n + g];
+      if (N.phase == 0) v[a][d] += A.dter * (A.fl[d * n + g] * rm);
+    }
+  }
+  if (N.phase > 0) {
+#pragma unroll
+    for (int a = 0; a < NA; a++) {
+      if (a >= na) break;
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        v[a][d] *= sc[a];
+        p[a][d] += dta2 * v[a][d];
+      }
+    }
+    const int b0 = A.boff[m], b1 = A.boff[m + 1], e0 = A.aoff[m], e1 = A.aoff[m + 1];
+    if ((b1 > b0) || (e1 > e0)) {
+      if constexpr (NA <= 3)
+        mol_bonded_r<NA>(*reinterpret_cast<double (*)[NA][3]>(p),
+                         *reinterpret_cast<double (*)[NA][3]>(aa), a0, b0, b1, e0, e1, A.bi,
+                         A.bj, A.bk, A.br0, A.ai, A.aj, A.ak, A.afk, A.ath0);
+      else
+        mol_bonded(p, aa, na, a0, b0, b1, e0, e1, A.bi, A.bj, A.bk, A.br0, A.ai, A.aj, A.ak,
+                   A.afk, A.ath0);
+#pragma unroll
+      for (int a = 0; a < NA; a++) {
+        if (a >= na) break;
+        const double rm = A.rm[a0 + a];
+#pragma unroll
+        for (int d = 0; d < 3; d++) aa[a][d] *= rm;
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < NA; a++) {
+      if (a >= na) break;
+#pragma unroll
+      for (int d = 0; d < 3; d++) v[a][d] += dta2 * aa[a][d];
+    }
+  }
+  if (N.phase == A.ninnra) {
+    bool moved = false;
+#pragma unroll
+    for (int a = 0; a < NA; a++) {
+      if (a >= na) break;
+      moved |= md_atom_out(A, a0 + a, p[a][0], p[a][1], p[a][2], v[a][0], v[a][1], v[a][2],
+                           aa[a][0], aa[a][1], aa[a][2]);
+    }
+    return moved;
+  }
+#pragma unroll
+  for (int a = 0; a < NA; a++) {
+    if (a >= na) break;
+    const int g = a0 + a;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      v[a][d] += dta2 * aa[a][d];
+      p[a][d] += dta2 * v[a][d];
+      v[a][d] *= sc[a];
+    }
+    if (N.ithm[g] == 0)  // forcen_new's p2m (md_therm.new.c:186-188)
+      p2m += N.amass[g] * (v[a][0] * v[a][0] + v[a][1] * v[a][1] + v[a][2] * v[a][2]);
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      A.pos[d * n + g] = p[a][d];
+      A.vel[d * n + g] = v[a][d];
+      A.aa[d * n + g] = aa[a][d];
+    }
+  }
+  return false;
+}
+
+static __device__ __forceinline__ void k_mdnhc_body(const NhcArgs &N, int bid) {
+  const InnerArgs &A = N.I;
+  const int m = bid * MD_TPB + threadIdx.x;
+  const double s = exp(-A.dta2 * N.nhc[0]);
+  bool moved = false;
+  double p2m = 0.;
+  if (m < A.nmol) {
+    const int a0 = A.moff[m], na = A.moff[m + 1] - a0;
+    if (na == 1) {
+      double p[1][3], v[1][3], aa[1][3];
+      moved = mdnhc_mol<1>(N, m, a0, na, s, p2m, p, v, aa);
+    } else if (na <= 3) {
+      double p[3][3], v[3][3], aa[3][3];
+      moved = mdnhc_mol<3>(N, m, a0, na, s, p2m, p, v, aa);
+    } else {
+      extern __shared__ double mdl[];
+      double (*p)[3] = (double (*)[3])(mdl + (size_t)threadIdx.x * 9 * MD_MAXA);
+      moved = mdnhc_mol<MD_MAXA>(N, m, a0, na, s, p2m, p, p + MD_MAXA, p + 2 * MD_MAXA);
+    }
+  }
+  if (N.phase == A.ninnra) {
+    if (__ballot(moved) != 0 && lane_id() == 0) atomicOr(A.flag, 1);
+    return;
+  }
+  // workgroup = one wave: its kinetic sum, then the chain update in the last
+  // workgroup to arrive
+  static_assert(MD_TPB == NB_WAVE, "one wave per workgroup");
+  __shared__ int last;
+  const double tot = nb_wave_sum(p2m);
+  if (threadIdx.x == 0) N.ppart[bid] = tot;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) last = atomicAdd(N.ticket, 1) == N.nblocks - 1;
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  double q = 0.;
+  for (int b = threadIdx.x; b < N.nblocks; b += MD_TPB)
+    q += __hip_atomic_load(&N.ppart[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  q = nb_wave_sum(q);
+  if (threadIdx.x == 0) {
+    nhc_update(N.nhc, N.nchain, q, A.dta2);
+    *N.ticket = 0;
+  }
+}
+NB_KERNEL_PAIR(k_mdnhc, NhcArgs, MD_TPB)
+
 // cm3d's force(...,2) at step_init: bonded accelerations at the current
 // positions (md_step.c:48, md_fcntr.c:79-105 with rmass).
 __global__ __launch_bounds__(MD_TPB) void k_md_bonded(InnerArgs A) {
@@ -355,11 +547,14 @@ static __device__ __forceinline__ void k_mdkick_body(const KickArgs &A, int bid)
 }
 NB_KERNEL_PAIR(k_mdkick, KickArgs, 256)
 
-// Kinetic, bond and bend energies (getvbond / getvbend harmonic cases,
-// md_fbond.c:165-173, md_fbend.c:318-338; p2m trace, md_fcntr.c:338-356).
+// Kinetic, bond and bend energies and the bond virial (getvbond / getvbend
+// harmonic cases, md_fbond.c:165-190, md_fbend.c:318-338 -- the bend virial
+// is compiled only with -DPRESSURE, which cm3d's Linux builds leave off;
+// p2m trace, md_fcntr.c:338-356).
+#define MD_NE 4
 __global__ __launch_bounds__(256) void k_md_energy(InnerArgs A, const double *__restrict__ amass,
                                                    double *__restrict__ epart) {
-  double acc[3] = {0., 0., 0.};
+  double acc[MD_NE] = {0., 0., 0., 0.};
   int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m < A.nmol) {
     const int n = A.n, a0 = A.moff[m], a1 = A.moff[m + 1];
@@ -369,8 +564,11 @@ __global__ __launch_bounds__(256) void k_md_energy(InnerArgs A, const double *__
       int i = A.bi[b], j = A.bj[b];
       double dx = A.pos[i] - A.pos[j], dy = A.pos[n + i] - A.pos[n + j];
       double dz = A.pos[2 * n + i] - A.pos[2 * n + j];
-      double dr = sqrt(dx * dx + dy * dy + dz * dz) - A.br0[b];
+      const double r = sqrt(dx * dx + dy * dy + dz * dz);
+      double dr = r - A.br0[b];
       acc[1] += .5 * A.bk[b] * dr * dr;
+      const double fk = A.bk[b] * dr / r;
+      acc[3] -= fk * r * r;
     }
     for (int e = A.aoff[m]; e < A.aoff[m + 1]; e++) {
       int i = A.ai[e], j = A.aj[e], k = A.ak[e];
@@ -383,15 +581,15 @@ __global__ __launch_bounds__(256) void k_md_energy(InnerArgs A, const double *__
       acc[2] += .5 * A.afk[e] * dth * dth;
     }
   }
-  __shared__ double red[4][3];
+  __shared__ double red[4][MD_NE];
   int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int k = 0; k < 3; k++) {
+  for (int k = 0; k < MD_NE; k++) {
     double v = nb_wave_sum(acc[k]);
     if (lane == 0) red[w][k] = v;
   }
   __syncthreads();
-  if (threadIdx.x < 3)
-    epart[blockIdx.x * 3 + threadIdx.x] =
+  if (threadIdx.x < MD_NE)
+    epart[blockIdx.x * MD_NE + threadIdx.x] =
         red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
 }
 
@@ -438,14 +636,35 @@ static InnerArgs inner_args(nb_md *md) {
   return A;
 }
 
+static void md_drop_graph(nb_md *md) {
+  if (md->have_graph) {
+    hipGraphExecDestroy(md->gexec);
+    hipGraphDestroy(md->graph);
+    md->have_graph = 0;
+  }
+}
+
+static void md_free_nhc(nb_md *md) {
+  void *ptrs[] = {md->d_nhc, md->d_ppart, md->d_ithm, md->d_ticket};
+  for (void *q : ptrs)
+    if (q) hipFree(q);
+  md->d_nhc = md->d_ppart = nullptr;
+  md->d_ithm = md->d_ticket = nullptr;
+  md->c->dev_bytes -= md->nhc_bytes;
+  md->nhc_bytes = 0;
+  md->nchain = 0;
+}
+
 extern "C" void nb_md_destroy(nb_md *md) {
   if (!md) return;
   hipSetDevice(md->c->dev);
   if (md->have_graph) { hipGraphExecDestroy(md->gexec); hipGraphDestroy(md->graph); }
   md->c->dev_bytes -= md->bytes;
+  md->c->dev_bytes -= md->nhc_bytes;
   void *ptrs[] = {md->d_vel, md->d_rm, md->d_aa, md->d_amass, md->d_moff, md->d_boff,
                   md->d_aoff, md->d_bi, md->d_bj, md->d_ai, md->d_aj, md->d_ak, md->d_bk,
-                  md->d_br0, md->d_afk, md->d_ath0, md->d_ep, md->d_eo};
+                  md->d_br0, md->d_afk, md->d_ath0, md->d_ep, md->d_eo, md->d_nhc,
+                  md->d_ppart, md->d_ithm, md->d_ticket};
   for (void *p : ptrs)
     if (p) hipFree(p);
   delete md;
@@ -541,7 +760,7 @@ extern "C" int nb_md_create(nb_ctx *c, const nb_md_params *p, nb_md **out) {
   MUP(&md->d_ak, ak.data(), ak.size()); MUP(&md->d_afk, afk.data(), afk.size());
   MUP(&md->d_ath0, ath0.data(), ath0.size());
   md->ep_blocks = (nm + 255) / 256;
-  MUP(&md->d_ep, (const double *)nullptr, 3 * (size_t)md->ep_blocks);
+  MUP(&md->d_ep, (const double *)nullptr, MD_NE * (size_t)md->ep_blocks);
   MUP(&md->d_eo, (const double *)nullptr, 8);
 #undef MUP
   md->bytes = c->dev_bytes - bytes0;
@@ -558,11 +777,7 @@ extern "C" int nb_md_set_state(nb_md *md, const double *pos, const double *vel,
   MCHK(hipMemcpy(c->d_pos, pos, sizeof(double) * 3 * md->n, hipMemcpyHostToDevice));
   MCHK(hipMemcpy(md->d_vel, vel, sizeof(double) * 3 * md->n, hipMemcpyHostToDevice));
   memcpy(md->hmat, hmat, sizeof(md->hmat));
-  if (md->have_graph) {
-    hipGraphExecDestroy(md->gexec);
-    hipGraphDestroy(md->graph);
-    md->have_graph = 0;
-  }
+  md_drop_graph(md);
   return NB_OK;
 }
 
@@ -573,6 +788,57 @@ extern "C" int nb_md_get_state(nb_md *md, double *pos, double *vel) {
   MCHK(hipStreamSynchronize(c->stream));
   if (pos) MCHK(hipMemcpy(pos, c->d_pos, sizeof(double) * 3 * md->n, hipMemcpyDeviceToHost));
   if (vel) MCHK(hipMemcpy(vel, md->d_vel, sizeof(double) * 3 * md->n, hipMemcpyDeviceToHost));
+  return NB_OK;
+}
+
+// set_therm + samveta's result (md_therm.new.c:158-169, 248-275) for one
+// "global" thermostat: eta starts at 0 and veta at veta0.
+extern "C" int nb_md_set_thermostat(nb_md *md, int nchain, const double *gkt, const double *mnh,
+                                    const double *veta0, const int *ithm) {
+  if (!md || nchain < 0 || nchain > MD_MAXCHAIN) {
+    nb_set_error("nb_md_set_thermostat: bad argument (nchain 0..%d)", MD_MAXCHAIN);
+    return NB_ERR_ARG;
+  }
+  nb_ctx *c = md->c;
+  MCHK(hipSetDevice(c->dev));
+  MCHK(hipStreamSynchronize(c->stream));
+  md_drop_graph(md);
+  md_free_nhc(md);
+  if (nchain == 0) return NB_OK;
+  if (!gkt || !mnh || !veta0 || !ithm) {
+    nb_set_error("nb_md_set_thermostat: null argument");
+    return NB_ERR_ARG;
+  }
+  for (int i = 0; i < nchain; i++)
+    if (!(mnh[i] > 0.)) {
+      nb_set_error("nb_md_set_thermostat: mnh[%d] = %g must be positive", i, mnh[i]);
+      return NB_ERR_ARG;
+    }
+  for (int i = 0; i < md->n; i++)
+    if (ithm[i] != 0 && ithm[i] != -1) {
+      nb_set_error("nb_md_set_thermostat: ithm[%d] = %d (0: global thermostat, -1: none)", i,
+                   ithm[i]);
+      return NB_ERR_UNSUPPORTED;
+    }
+  std::vector<double> h(4 * nchain, 0.);
+  for (int i = 0; i < nchain; i++) {
+    h[i] = veta0[i];
+    h[2 * nchain + i] = gkt[i];
+    h[3 * nchain + i] = mnh[i];
+  }
+  const int nblk = (md->nmol + MD_TPB - 1) / MD_TPB;
+  const size_t b0 = c->dev_bytes;
+  int rc = 0;
+  const int zero = 0;
+  if ((rc = mup(c, &md->d_nhc, h.data(), h.size())) || (rc = mup(c, &md->d_ithm, ithm, md->n)) ||
+      (rc = mup(c, &md->d_ppart, (const double *)nullptr, nblk)) ||
+      (rc = mup(c, &md->d_ticket, &zero, 1))) {
+    md->nhc_bytes = c->dev_bytes - b0;
+    md_free_nhc(md);
+    return rc;
+  }
+  md->nhc_bytes = c->dev_bytes - b0;
+  md->nchain = nchain;
   return NB_OK;
 }
 
@@ -603,7 +869,20 @@ int md_enqueue_step(nb_md *md, hipStream_t s) {
   // inner RESPA loop + nrefsh test + records for the force evaluation
   InnerArgs A = inner_args(md);
   const int lds = md->maxa > 3 ? MD_TPB * 9 * MD_MAXA * (int)sizeof(double) : 0;
-  NB_GO(c, K_MDINNER, k_mdinner, (md->nmol + MD_TPB - 1) / MD_TPB, MD_TPB, lds, A, s);
+  const int nblk = (md->nmol + MD_TPB - 1) / MD_TPB;
+  if (md->nchain == 0) {
+    NB_GO(c, K_MDINNER, k_mdinner, nblk, MD_TPB, lds, A, s);
+  } else {
+    NhcArgs N;
+    N.I = A;
+    N.nchain = md->nchain; N.nblocks = nblk;
+    N.ithm = md->d_ithm; N.amass = md->d_amass;
+    N.nhc = md->d_nhc; N.ppart = md->d_ppart; N.ticket = md->d_ticket;
+    for (int ph = 0; ph <= md->ninnra; ph++) {
+      N.phase = ph;
+      NB_GO(c, K_MDNHC, k_mdnhc, nblk, MD_TPB, lds, N, s);
+    }
+  }
   int rc = nb_launch_build_cond(c, md->box, s);
   if (rc) return rc;
   rc = nb_launch_force(c, 1, md->box, nullptr, c->d_fl, s, NB_FORCE_PACKED | NB_FORCE_NORED);
@@ -623,6 +902,7 @@ static struct RegisterMD {
   RegisterMD() {
     g_batch_fn[K_MDINNER] = k_mdinner_bl;
     g_batch_fn[K_MDKICK] = k_mdkick_bl;
+    g_batch_fn[K_MDNHC] = k_mdnhc_bl;
   }
 } g_register_md;
 
@@ -653,9 +933,12 @@ extern "C" int nb_md_step(nb_md *md, int nsteps, void *stream, int use_graph) {
   return NB_OK;
 }
 
-// Energies of the current state (getvireal + geteng for NVE):
-// poter = U + vlrc/V, potra = vbond + vbend, zke, ham = poter + potra + zke.
-extern "C" int nb_md_energies(nb_md *md, double vlrc, nb_md_energy *e) {
+// Energies of the current state (getvireal + geteng, md_fcntr.c:136-231 and
+// 259-306): poter = U (+ vlrc/V), poth = US (+ vlrc/V, US with vhinter),
+// potra = vbond + vbend, zke, the chain's potn / zken, cm3d's conserved
+// ham = poth + potra + zke + potn + zken (md_averages.new.c:100), and
+// prsi = (p2m + W + WI) / (3V) with W less 3 wlrc/V.
+extern "C" int nb_md_energies(nb_md *md, double vlrc, double wlrc, int ndof, nb_md_energy *e) {
   if (!md || !e) { nb_set_error("nb_md_energies: null argument"); return NB_ERR_ARG; }
   nb_ctx *c = md->c;
   hipStream_t s = c->stream;
@@ -668,33 +951,53 @@ extern "C" int nb_md_energies(nb_md *md, double vlrc, nb_md_energy *e) {
   InnerArgs A = inner_args(md);
   k_md_energy<<<md->ep_blocks, 256, 0, s>>>(A, md->d_amass, md->d_ep);
   MCHK(hipGetLastError());
-  std::vector<double> ep(3 * md->ep_blocks);
+  std::vector<double> ep(MD_NE * md->ep_blocks);
   double E[64];
   MCHK(hipMemcpyAsync(ep.data(), md->d_ep, ep.size() * sizeof(double), hipMemcpyDeviceToHost, s));
   MCHK(hipMemcpyAsync(E, c->d_eout, sizeof(E), hipMemcpyDeviceToHost, s));
   MCHK(hipMemcpyAsync(c->h_flags, c->d_flags, 8 * sizeof(int), hipMemcpyDeviceToHost, s));
   MCHK(hipStreamSynchronize(s));
-  double p2m = 0., vb = 0., va = 0.;
-  for (int b = 0; b < md->ep_blocks; b++) { p2m += ep[3 * b]; vb += ep[3 * b + 1]; va += ep[3 * b + 2]; }
-  const double *h = md->hmat;
-  double vol = h[0] * (h[4] * h[8] - h[5] * h[7]) + h[1] * (h[5] * h[6] - h[3] * h[8]) +
-               h[2] * (h[3] * h[7] - h[4] * h[6]);
-  memset(e, 0, sizeof(*e));
-  e->vinter = .5 * E[0];
-  e->velec = .5 * E[9];
-  double vew = 0.;
-  if (c->iperd == 3 && c->ncharge > 0) {
-    double bk = c->ewald_bkg / vol;
-    vew = (c->ktot > 0 ? E[32] : 0.) + bk + c->ewald_self;
-    if (c->necorr > 0) vew += .5 * E[20];
+  double p2m = 0., vb = 0., va = 0., wb = 0.;
+  for (int b = 0; b < md->ep_blocks; b++) {
+    p2m += ep[MD_NE * b]; vb += ep[MD_NE * b + 1]; va += ep[MD_NE * b + 2]; wb += ep[MD_NE * b + 3];
   }
-  e->vewald = vew;
+  const int nc = md->nchain;
+  std::vector<double> nhc(4 * nc);
+  if (nc) MCHK(hipMemcpy(nhc.data(), md->d_nhc, nhc.size() * sizeof(double), hipMemcpyDeviceToHost));
+  nb_energy ne;
+  nb_fill_energy(c, 1, E, md->hmat, &ne);
+  const double vol = nb_det9(md->hmat);
+  memset(e, 0, sizeof(*e));
+  e->vinter = ne.spotl;
+  e->vhinter = ne.spothl;
+  e->velec = ne.spote;
+  e->winter = ne.sprsl;
+  e->welec = ne.sprse;
+  if (c->iperd == 3) {  // getvewald (md_ewald.c:364-373)
+    e->vewald = ne.vk_ewald + ne.pot_ecorr;
+    e->wewald = ne.vp_ewald + ne.prs_ecorr;
+  }
   e->vbond = vb;
   e->vbend = va;
+  e->wbond = wb;
   e->zke = .5 * p2m;
-  e->poter = e->vinter + e->velec + e->vewald + vlrc / vol;
+  e->poter = e->vinter + e->velec + e->vewald;
+  e->poth = e->vhinter + e->velec + e->vewald;
+  double W = e->winter + e->welec + e->wewald;
+  if (c->iperd == 3) {
+    e->poter += vlrc / vol;
+    e->poth += vlrc / vol;
+    W -= 3. * wlrc * (1. / vol);
+  }
   e->potra = vb + va;
-  e->ham = e->poter + e->potra + e->zke;
+  // getengn_new (md_therm.new.c:200-222)
+  for (int i = 0; i < nc; i++) e->potn += nhc[2 * nc + i] * nhc[nc + i];
+  double sv = 0.;
+  for (int i = 0; i < nc; i++) sv += nhc[3 * nc + i] * nhc[i] * nhc[i];
+  e->zken = .5 * sv;
+  e->ham = e->poth + e->potra + e->zke + e->potn + e->zken;
+  e->prsi = (p2m + W + wb) / (3. * vol);
+  e->tiout = ndof > 0 ? p2m / (double)ndof : 0.;
   e->overflow = c->h_flags[0] > c->cap;
   return NB_OK;
 }

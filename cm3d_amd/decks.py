@@ -1,0 +1,402 @@
+"""cm3d's on-disk inputs: the control deck, the set / topology / data-base
+meta-key files and the initial-configuration file, read into a system dict
+for the engine; plus cm3d's random numbers and velocity sampling, so a cold
+start (istart = 1) draws the same velocities cm3d draws.
+
+Formats (reference tree):
+  control deck      \\keyword{value} pairs, everything else ignored
+                    (input/md_readsim.c, keywords input/md_setsimkey.c,
+                    defaults input/md_setsimdef.c)
+  set file          ~mol_def[\\mol_parm_file \\mol_therm_opt \\nmol \\mol_index]
+                    (input/md_readset.c)
+  topology          ~mol_name_def[...], ~atom_def[\\atom_typ \\atom_ind \\mass
+                    \\charge], ~bond_def[\\atom1 \\atom2], ~bend_def[\\atom1
+                    \\atom2 \\atom3] (input/md_getmols.c)
+  data base         ~inter_parm[\\atom1 \\atom2 \\pot_type \\eps \\sig \\min_dist
+                    \\max_dist \\res_dist], ~bond_parm[... \\pot_type{harm} \\fk
+                    \\eq], ~bend_parm[... \\fk \\eq (degrees)]
+                    (input/md_getparm.c, input/md_searchbase.c)
+  initial file      "natoms istart", natoms lines of x y z, then the 9 numbers
+                    of hmat (input/md_readpos.c:52-115)
+  .ham / .eng       instantaneous energies (output/md_saveinst.c:33-39)
+
+Supported: lennard-jones and null pair types, harmonic bonds and bends,
+charges with Ewald (alp_ewald, kmax, rcute_*), thermostat options none and
+global, NVE and NVT.  Anything else raises.
+"""
+import ctypes
+import ctypes.util
+import math
+import os
+import re
+
+import numpy as np
+
+from . import nb
+from . import systems as S
+
+# input/md_setsimdef.c
+DEFAULTS = dict(calculation="md", istart=1, ntime=1, ensemble="nve", dt=1.0, t_ext=300.0,
+                p_ext=0.0, iperd=3, nres_lrf=1, nres_intra=1, nres_tors=1, len_nhc=0,
+                nbar=0, skin=0.5, rheal_res=0.5, ntable=500, tau_nhc=1000.0,
+                alp_ewald=0.15, kmax=7, ishift=0, rcute_max=10.0, rcute_min=1.0,
+                rcute_resp=7.0, iwr_screen=0, iwr_inst=0, iseed=None, scaleeps=1.0,
+                scale_temp=1.0, inter_file="data_base", bond_file="data_base",
+                bend_file="data_base", mol_set_file=None, dpi_file=None, ham_file=None,
+                eng_file=None, long_rc="on")
+
+_INT_KEYS = {"istart", "ntime", "iperd", "nres_lrf", "nres_intra", "nres_tors", "len_nhc",
+             "nbar", "ntable", "kmax", "ishift", "iwr_screen", "iwr_inst", "iseed"}
+_FLOAT_KEYS = {"dt", "t_ext", "p_ext", "skin", "rheal_res", "tau_nhc", "alp_ewald",
+               "rcute_max", "rcute_min", "rcute_resp", "scaleeps"}
+
+_KEY = re.compile(r"\\([A-Za-z_0-9]+)\{([^}]*)\}")
+_META = re.compile(r"~([A-Za-z_0-9]+)\s*\[(.*?)\]", re.S)
+
+
+def parse_keys(text):
+    """All \\key{value} pairs in order (cm3d ignores any other text)."""
+    return [(k.lower(), v.strip()) for k, v in _KEY.findall(text)]
+
+
+def parse_meta(text):
+    """All ~metakey[ \\key{value} ... ] blocks, in order."""
+    return [(m.lower(), dict(parse_keys(body))) for m, body in _META.findall(text)]
+
+
+def read_control(path):
+    with open(path) as f:
+        text = f.read()
+    d = dict(DEFAULTS)
+    for k, v in parse_keys(text):
+        if k in _INT_KEYS:
+            d[k] = int(v)
+        elif k in _FLOAT_KEYS:
+            d[k] = float(v)
+        else:
+            d[k] = v
+    # input/md_readsim.c:474-475: fs -> ps as a product with TCONV = 1./1000.
+    # (not a division: the two round differently, e.g. for 50 fs)
+    d["dt_ps"] = d["dt"] * (1. / 1000.)
+    d["tau_nhc_ps"] = d["tau_nhc"] * (1. / 1000.)
+    ens = d["ensemble"].lower()
+    if ens not in ("nve", "nvt"):
+        raise ValueError(f"ensemble {d['ensemble']!r}: only nve and nvt are supported")
+    if ens == "nve" and d["len_nhc"]:
+        raise ValueError("a chain length with the NVE ensemble")  # md_readsim.c consistency
+    if ens == "nvt" and d["len_nhc"] <= 0:
+        raise ValueError("the NVT ensemble needs len_nhc > 0")
+    if d["nres_lrf"] != 1 or d["nres_tors"] != 1:
+        raise ValueError("only nres_lrf = nres_tors = 1 is supported")
+    return d
+
+
+def read_init(path):
+    """Cold-start initial file: header 'natoms istart', positions, hmat."""
+    with open(path) as f:
+        tok = f.read().split()
+    n, ist = int(tok[0]), int(tok[1])
+    xyz = np.array(tok[2:2 + 3 * n], float).reshape(n, 3)
+    hmat = np.array(tok[2 + 3 * n:2 + 3 * n + 9], float)
+    if len(hmat) != 9:
+        raise ValueError(f"{path}: hmat missing")
+    return n, ist, xyz.T.reshape(-1).copy(), hmat
+
+
+def _load_sincos():
+    try:
+        libm = ctypes.CDLL(ctypes.util.find_library("m") or "libm.so.6")
+        f = libm.sincos
+    except (OSError, AttributeError):
+        return None
+    f.restype = None
+    f.argtypes = [ctypes.c_double, ctypes.POINTER(ctypes.c_double),
+                  ctypes.POINTER(ctypes.c_double)]
+    return f
+
+
+_SINCOS = _load_sincos()
+
+
+def _sincos(x):
+    """(sin x, cos x) as cm3d's compiled ggauss gets them: its loop computes
+    both of one argument, which the compiler turns into one libm sincos call,
+    and glibc's sincos differs from separate sin / cos in the last bits for
+    a few arguments."""
+    if _SINCOS is None:
+        return math.sin(x), math.cos(x)
+    sn, cs = ctypes.c_double(), ctypes.c_double()
+    _SINCOS(x, ctypes.byref(sn), ctypes.byref(cs))
+    return sn.value, cs.value
+
+
+class Ran1:
+    """cm3d's randme(): Numerical Recipes ran1 (Park-Miller minimal standard
+    with a 32-entry Bays-Durham shuffle), seeded as srandme does
+    (utils/randme.c:60-110)."""
+    IA, IM, IQ, IR, NTAB = 16807, 2147483647, 127773, 2836, 32
+    NDIV = 1 + (2147483647 - 1) // 32
+    AM = 1.0 / 2147483647
+    RNMX = 1.0 - 1.2e-12
+
+    def __init__(self, iseed):
+        self.seed = iseed if iseed < 0 else -iseed
+        self.iy = 0
+        self.iv = [0] * self.NTAB
+        self.next()
+
+    def _step(self, idum):
+        k = idum // self.IQ if idum >= 0 else -((-idum) // self.IQ)  # C truncation
+        idum = self.IA * (idum - k * self.IQ) - self.IR * k
+        if idum < 0:
+            idum += self.IM
+        return idum
+
+    def next(self):
+        if self.seed <= 0 or not self.iy:
+            self.seed = 1 if -self.seed < 1 else -self.seed
+            for j in range(self.NTAB + 7, -1, -1):
+                self.seed = self._step(self.seed)
+                if j < self.NTAB:
+                    self.iv[j] = self.seed
+            self.iy = self.iv[0]
+        self.seed = self._step(self.seed)
+        j = self.iy // self.NDIV
+        self.iy = self.iv[j]
+        self.iv[j] = self.seed
+        t = self.AM * self.iy
+        return self.RNMX if t > self.RNMX else t
+
+    def ggauss(self, nran):
+        """utils/utils.c:53-85: uniforms first, then Box-Muller pairs that use
+        the same uniform for the angle and the radius; an odd count takes its
+        last value from gauss1 (two fresh uniforms) before the pairs."""
+        unif = [self.next() for _ in range(nran)]
+        g = [0.0] * nran
+        if nran % 2 == 1:
+            phi = 2. * math.pi * self.next()
+            al = math.sqrt(-math.log(self.next())) * math.sqrt(2.0)
+            g[nran - 1] = al * math.cos(phi)
+        for i in range(0, nran - 1, 2):
+            phi = 2. * math.pi * unif[i]
+            al = math.sqrt(-math.log(unif[i])) * math.sqrt(2.0)
+            sn, cs = _sincos(phi)
+            g[i] = al * cs
+            g[i + 1] = al * sn
+        return np.array(g)
+
+
+def samvel(rng, amass, temp, ndof):
+    """integrate/md_zeroam.c:33-46: gaussian velocities (vx|vy|vz contiguous),
+    zero total momentum, rescale to temp over ndof degrees of freedom."""
+    n = len(amass)
+    v = rng.ggauss(3 * n)
+    vx, vy, vz = v[:n], v[n:2 * n], v[2 * n:]
+    tm = xv = yv = zv = 0.0
+    if n > 1:
+        for i in range(n):
+            m = amass[i]
+            tm += m
+            xv += vx[i] * m
+            yv += vy[i] * m
+            zv += vz[i] * m
+        xv /= tm; yv /= tm; zv /= tm
+        vx -= xv; vy -= yv; vz -= zv
+    s = 0.0
+    for i in range(n):
+        s += amass[i] * (vx[i] * vx[i] + vy[i] * vy[i] + vz[i] * vz[i])
+    if s > 0:
+        f = math.sqrt(float(ndof) * temp / s)
+        v = np.concatenate([vx, vy, vz]) * f
+    return v
+
+
+def samveta(rng, nchain, mnh, temp):
+    """integrate/md_therm.new.c:248-275 for one thermostat: chain velocities
+    from ggauss, zero mean momentum, scaled to temp per link."""
+    veta = rng.ggauss(nchain) if nchain else np.zeros(0)
+    if nchain > 1:
+        fact = tm = 0.
+        for i in range(nchain):
+            fact += mnh[i] * veta[i]
+            tm += mnh[i]
+        fact /= tm
+        veta = veta - fact
+    fact = 0.
+    for i in range(nchain):
+        fact += mnh[i] * veta[i] * veta[i]
+    if nchain > 0:
+        veta = veta * math.sqrt(nchain * temp / fact)
+    return veta
+
+
+def _read_meta_file(path):
+    with open(path) as f:
+        return parse_meta(f.read())
+
+
+def load(deck_path):
+    """Read a control deck and everything it names.  Returns (sysd, run):
+    sysd is a system dict for NBContext / NBMD (cm3d_amd.systems layout),
+    run holds the MD controls (dt_ps, nsteps, ensemble, thermostat, output
+    intervals and files) and the initial velocities of a cold start."""
+    base = os.path.dirname(os.path.abspath(deck_path))
+    ctl = read_control(deck_path)
+
+    def fp(name):
+        return name if os.path.isabs(name) else os.path.join(base, name)
+
+    if not ctl["mol_set_file"] or not ctl["dpi_file"]:
+        raise ValueError("the deck must name mol_set_file and dpi_file")
+    species = []
+    for mk, kv in _read_meta_file(fp(ctl["mol_set_file"])):
+        if mk != "mol_def":
+            continue
+        species.append(dict(file=kv["mol_parm_file"], therm=kv.get("mol_therm_opt", "none").lower(),
+                            nmol=int(kv["nmol"]), index=int(kv.get("mol_index", len(species) + 1))))
+    species.sort(key=lambda s: s["index"])
+    db = _read_meta_file(fp(ctl["inter_file"]))
+    if ctl["bond_file"] != ctl["inter_file"]:
+        db += _read_meta_file(fp(ctl["bond_file"]))
+    if ctl["bend_file"] not in (ctl["inter_file"], ctl["bond_file"]):
+        db += _read_meta_file(fp(ctl["bend_file"]))
+
+    types, itype, amass, qe, ithm = [], [], [], [], []
+    mol_start, bonds, bends = [0], [], []
+    for sp in species:
+        top = _read_meta_file(fp(sp["file"]))
+        atoms = sorted([kv for mk, kv in top if mk == "atom_def"], key=lambda a: int(a["atom_ind"]))
+        bdef = [kv for mk, kv in top if mk == "bond_def"]
+        adef = [kv for mk, kv in top if mk == "bend_def"]
+        if sp["therm"] not in ("none", "global"):
+            raise ValueError(f"thermostat option {sp['therm']!r}: only none and global")
+        for _ in range(sp["nmol"]):
+            a0 = len(itype)
+            for a in atoms:
+                t = a["atom_typ"]
+                if t not in types:
+                    types.append(t)
+                itype.append(types.index(t))
+                amass.append(float(a["mass"]))
+                qe.append(float(a.get("charge", 0.0)))
+                ithm.append(0 if sp["therm"] == "global" else -1)
+            for b in bdef:
+                bonds.append((a0 + int(b["atom1"]) - 1, a0 + int(b["atom2"]) - 1))
+            for b in adef:
+                bends.append((a0 + int(b["atom1"]) - 1, a0 + int(b["atom2"]) - 1,
+                              a0 + int(b["atom3"]) - 1))
+            mol_start.append(len(itype))
+    n = len(itype)
+    nt = len(types)
+    ni, ist, pos, hmat = read_init(fp(ctl["dpi_file"]))
+    if ni != n:
+        raise ValueError(f"{ctl['dpi_file']}: {ni} atoms, the set file gives {n}")
+
+    # pair interactions by type pair (input/md_searchbase.c)
+    inter_db = {}
+    for mk, kv in db:
+        if mk == "inter_parm":
+            key = frozenset((kv["atom1"], kv["atom2"]))
+            inter_db[key] = kv
+    inter, mp = [], -np.ones((nt, nt), np.int32)
+    for i in range(nt):
+        for j in range(i, nt):
+            kv = inter_db.get(frozenset((types[i], types[j])))
+            if kv is None:
+                raise ValueError(f"no inter_parm for {types[i]}-{types[j]}")
+            pt = kv["pot_type"].lower()
+            if pt == "null":
+                continue
+            if pt != "lennard-jones":
+                raise ValueError(f"pot_type {pt!r}: only lennard-jones and null")
+            rmax = float(kv["max_dist"])
+            rres = float(kv.get("res_dist", rmax))
+            inter.append((nb.POT_LJ, [float(kv["eps"]) * ctl["scaleeps"], float(kv["sig"])],
+                          float(kv["min_dist"]), rmax, rres))
+            mp[i, j] = mp[j, i] = len(inter) - 1
+    s = S._base(n, nt, ctl["ntable"], ctl["skin"], ctl["rheal_res"])
+    s.update(iperd=ctl["iperd"], itype=np.array(itype, np.int32), map=mp.reshape(-1),
+             qch=np.array(qe) * S.SQRT_CCONV, hmat=hmat, pos=pos,
+             amass=np.array(amass) * S.MCONV)
+    S._pair_tables(s, inter, ctl["ntable"], ctl["skin"], ctl["rheal_res"])
+    S._lrc(s)
+    if ctl["long_rc"].lower() == "off":
+        s["vlrc"] = s["wlrc"] = 0.0
+
+    # bonded terms (harmonic; fk in K/A^2 and K/rad^2, bend eq in degrees)
+    def parm(mk, atyps):
+        for m, kv in db:
+            if m != mk:
+                continue
+            names = [kv.get(f"atom{k + 1}") for k in range(len(atyps))]
+            if names == atyps or names == atyps[::-1]:
+                if kv.get("pot_type", "harm").lower() not in ("harm", "harmonic"):
+                    raise ValueError(f"{mk} {atyps}: only harmonic")
+                return float(kv["fk"]), float(kv["eq"])
+        raise ValueError(f"no {mk} for {atyps}")
+    if bonds:
+        pk = [parm("bond_parm", [types[itype[i]], types[itype[j]]]) for i, j in bonds]
+        s.update(bond_i=np.array([b[0] for b in bonds], np.int32),
+                 bond_j=np.array([b[1] for b in bonds], np.int32),
+                 bond_k=np.array([p[0] for p in pk]), bond_r0=np.array([p[1] for p in pk]))
+    if bends:
+        pk = [parm("bend_parm", [types[itype[i]], types[itype[j]], types[itype[k]]])
+              for i, j, k in bends]
+        s.update(bend_i=np.array([b[0] for b in bends], np.int32),
+                 bend_j=np.array([b[1] for b in bends], np.int32),
+                 bend_k=np.array([b[2] for b in bends], np.int32),
+                 bend_fk=np.array([p[0] for p in pk]),
+                 bend_th0=np.array([math.radians(p[1]) for p in pk]))
+    if bonds or bends or len(mol_start) - 1 != n:
+        s["mol_start"] = np.array(mol_start, np.int32)
+    excl = [(i, j) for i, j in bonds] + [(i, k) for i, _, k in bends]
+    S._exclusions(s, excl)
+    if np.count_nonzero(s["qch"]):
+        S._coulomb(s, ctl["ntable"], ctl["alp_ewald"], ctl["rcute_min"], ctl["rcute_resp"],
+                   ctl["rcute_max"], ctl["rheal_res"], ctl["skin"], ctl["kmax"])
+    else:
+        S._coulomb(s, ctl["ntable"], ctl["alp_ewald"], ctl["rcute_min"], ctl["rcute_resp"],
+                   ctl["rcute_max"], ctl["rheal_res"], ctl["skin"], 0)
+        s.update(ktot=0, ewald_self=0.0, ewald_bkg=0.0)
+
+    # cold start: cm3d's random stream (seeded by the deck, md_readsim.c:257,
+    # 505), samvel, then the chain velocities
+    nvt = ctl["ensemble"].lower() == "nvt"
+    ndof = 3 * n if nvt else (3 * n - 3 if ctl["iperd"] != 0 else 3 * n - 6)
+    temp = ctl["t_ext"] * ctl["scale_temp"]
+    rng = Ran1(ctl["iseed"] if ctl["iseed"] is not None else 0)
+    vel = samvel(rng, s["amass"], temp, max(ndof, 1)) if ist == 1 else np.zeros(3 * n)
+    run = dict(ctl=ctl, nsteps=ctl["ntime"], dt_ps=ctl["dt_ps"], nres_intra=ctl["nres_intra"],
+               ensemble=ctl["ensemble"].lower(), ndof=ndof, temp=temp, vel=vel,
+               iwr_screen=ctl["iwr_screen"], iwr_inst=ctl["iwr_inst"], base=base)
+    if nvt:
+        nthm = sum(1 for t in ithm if t == 0)
+        nchain = ctl["len_nhc"]
+        # set_therm (md_therm.new.c:158-169): gkt[0] = T DIM n, gkt[i>0] = T,
+        # mnh = gkt tau^2
+        gkt = np.array([temp * 3 * nthm] + [temp] * (nchain - 1))
+        mnh = gkt * ctl["tau_nhc_ps"] * ctl["tau_nhc_ps"]  # (gkt tau) tau, as cm3d rounds
+        run.update(nchain=nchain, ithm=np.array(ithm, np.int32), gkt=gkt, mnh=mnh,
+                   veta=samveta(rng, nchain, mnh, temp))
+    return s, run
+
+
+def write_inst_headers(fham, feng, natoms, ntherm, nchain, ninst, dt_ps):
+    """The two header lines of the .ham and .eng files (output/md_headers.c:
+    146-163; no barostat)."""
+    fham.write("# %d %d %d %d %d %g\n" % (natoms, ntherm, nchain, 0, ninst, dt_ps))
+    fham.write("# time ham inter intra ke potn ken potv kev inter\n")
+    feng.write("# %d %d %9g\n" % (natoms, ninst, dt_ps))
+    feng.write("# time Ke inter bond bend tors onfo onfoe elec recip temp prs\n")
+
+
+def write_inst(fham, feng, istep, dt_ps, e):
+    """One record of each (save_inst, output/md_saveinst.c:28-39); e is an
+    NBMD.energies() dict.  Terms this engine does not have (torsions,
+    one-fours, barostat) are 0, as cm3d writes them for such systems."""
+    fham.write("%6g %.10g %g %g %g %g %g %g %g %g\n" % (
+        istep * dt_ps, e["ham"], e["poth"], e["potra"], e["zke"], e["potn"], e["zken"],
+        0., 0., e["poter"]))
+    feng.write("%6g %10g %10g %10g %10g %10g %10g %10g %10g %10g\n" % (
+        istep * dt_ps, e["zke"], e["vinter"], e["vbond"], e["vbend"], 0., 0., 0., e["velec"],
+        e["vewald"]))

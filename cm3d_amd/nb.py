@@ -83,8 +83,10 @@ class NBMDParams(C.Structure):
 
 
 class NBMDEnergy(C.Structure):
-    _fields_ = [(k, C.c_double) for k in ("vinter", "velec", "vewald", "vbond", "vbend",
-                                          "zke", "poter", "potra", "ham")] + \
+    _fields_ = [(k, C.c_double) for k in ("vinter", "vhinter", "velec", "vewald", "vbond",
+                                          "vbend", "zke", "poter", "poth", "potra", "potn",
+                                          "zken", "ham", "winter", "welec", "wewald", "wbond",
+                                          "prsi", "tiout")] + \
                [("overflow", C.c_int)]
 
     def as_dict(self):
@@ -98,6 +100,7 @@ EXPORTS = ["nb_create", "nb_destroy", "nb_last_error", "nb_build_lists",
            "nb_footprint",
            "nb_md_create", "nb_md_destroy", "nb_md_set_state", "nb_md_get_state",
            "nb_md_init", "nb_md_step", "nb_md_energies", "nb_md_rebuilds",
+           "nb_md_set_thermostat",
            "nb_batch_create", "nb_batch_destroy", "nb_batch_step", "nb_batch_launches",
            "nb_table_pair", "nb_table_coul", "nb_kvectors", "nb_ewald_consts"]
 
@@ -139,8 +142,10 @@ def lib():
     L.nb_md_get_state.argtypes = [C.c_void_p, _dp, _dp]
     L.nb_md_init.argtypes = [C.c_void_p]
     L.nb_md_step.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int]
-    L.nb_md_energies.argtypes = [C.c_void_p, C.c_double, C.POINTER(NBMDEnergy)]
+    L.nb_md_energies.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int,
+                                  C.POINTER(NBMDEnergy)]
     L.nb_md_rebuilds.argtypes = [C.c_void_p, _ip]
+    L.nb_md_set_thermostat.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _ip]
     L.nb_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]
     L.nb_batch_destroy.argtypes = [C.c_void_p]
     L.nb_batch_destroy.restype = None
@@ -385,6 +390,10 @@ class NBMD:
         check(lib().nb_md_create(ctx.handle, C.byref(p), C.byref(h)), "nb_md_create")
         self._h = h
         self.vlrc = float(np.asarray(sysd.get("vlrc", 0.0)).reshape(-1)[0])
+        self.wlrc = float(np.asarray(sysd.get("wlrc", 0.0)).reshape(-1)[0])
+        # cm3d's ndof for NVE with periodic boundaries (3N - 3); an NVT run
+        # sets 3N (set_thermostat does)
+        self.ndof = 3 * self.n - 3
 
     def close(self):
         if getattr(self, "_h", None):
@@ -419,8 +428,20 @@ class NBMD:
 
     def energies(self):
         e = NBMDEnergy()
-        check(lib().nb_md_energies(self._h, self.vlrc, C.byref(e)), "nb_md_energies")
+        check(lib().nb_md_energies(self._h, self.vlrc, self.wlrc, int(self.ndof), C.byref(e)),
+              "nb_md_energies")
         return e.as_dict()
+
+    def set_thermostat(self, nchain, gkt, mnh, veta0, ithm):
+        """One Nose-Hoover chain (cm3d's global thermostat) over atoms with
+        ithm == 0; nchain 0 returns to NVE.  Call before init()."""
+        g, gp = _d(gkt)
+        m, mp = _d(mnh)
+        v, vp = _d(veta0)
+        t, tp = _i(ithm)
+        check(lib().nb_md_set_thermostat(self._h, int(nchain), gp, mp, vp, tp),
+              "nb_md_set_thermostat")
+        self.ndof = 3 * self.n if nchain else 3 * self.n - 3
 
     def rebuilds(self):
         c = C.c_int(0)

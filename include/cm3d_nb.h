@@ -169,7 +169,13 @@ typedef struct nb_md_params {
   int nres_lrf;                                 /* must be 1               */
 } nb_md_params;
 typedef struct nb_md_energy {
-  double vinter, velec, vewald, vbond, vbend, zke, poter, potra, ham;
+  double vinter, vhinter, velec, vewald, vbond, vbend;
+  double zke, poter, poth, potra;               /* geteng, md_fcntr.c:259  */
+  double potn, zken;                            /* chain, getengn          */
+  double ham;                                   /* poth+potra+zke+potn+zken */
+  double winter, welec, wewald, wbond;          /* virials (getvireal)     */
+  double prsi;                                  /* (p2m + W + WI) / (3 V)  */
+  double tiout;                                 /* p2m / ndof              */
   int overflow;                                 /* a list build overflowed */
 } nb_md_energy;
 int nb_md_create(nb_ctx *ctx, const nb_md_params *p, nb_md **md);
@@ -182,8 +188,19 @@ int nb_md_init(nb_md *md);
 /* nsteps outer steps on the stream (NULL = the context's); use_graph
  * captures one step into a hipGraph once and replays it.  Asynchronous. */
 int nb_md_step(nb_md *md, int nsteps, void *stream, int use_graph);
-/* Energies of the current state; vlrc is cm3d's simparms->vlrc. */
-int nb_md_energies(nb_md *md, double vlrc, nb_md_energy *e);
+/* NVT: one Nose-Hoover chain of nchain links (cm3d's intnhc,
+ * integrate/md_therm.new.c:278-332, applied at every inner RESPA substep as
+ * md_step.c:113 does) over the atoms with ithm[i] == 0 (cm3d's "global"
+ * option; -1 = not thermostatted).  gkt, mnh, veta0: [nchain] as set_therm /
+ * samveta leave them (md_therm.new.c:158-169, 248-275); eta starts at 0.
+ * nchain 0 returns to NVE.  Takes effect at the next nb_md_init. */
+int nb_md_set_thermostat(nb_md *md, int nchain, const double *gkt, const double *mnh,
+                         const double *veta0, const int *ithm);
+/* Energies, virials, pressure and temperature of the current state, as
+ * getvireal + geteng leave them (md_fcntr.c:136-231, 259-306): vlrc, wlrc
+ * are cm3d's simparms->vlrc / wlrc (applied when iperd == 3), ndof its
+ * simparms->ndof (for tiout). */
+int nb_md_energies(nb_md *md, double vlrc, double wlrc, int ndof, nb_md_energy *e);
 /* Lists built so far by the device-side rebuild test (synchronises). */
 int nb_md_rebuilds(nb_md *md, int *count);
 

@@ -21,6 +21,7 @@
  *   force        energy/control/md_fcntr.c:44
  *   getvireal    energy/control/md_fcntr.c:136
  *   step_init    integrate/md_step.c:32,  step integrate/md_step.c:62
+ *   geteng       energy/control/md_fcntr.c:259, getengn integrate/md_therm.c
  *
  * usage: cm3d_ref_dump deck out.bin [nsteps]
  */
@@ -114,6 +115,17 @@ int main(int argc, char *argv[])
   put("hmat", 'd', 9, coords.hmat);
   put("pos", 'd', n3, coords.px);
   put("vel", 'd', n3, coords.vx);
+  puti("ndof", simparms.ndof);
+  puti("ntherm", simparms.ntherm);
+  if (simparms.ntherm > 0) {
+    /* the chain after set_therm + samveta (md_therm.new.c:158-169, 248) */
+    puti("nchain", simparms.nchain);
+    put("ithm", 'i', simparms.natoms, coords.ithm);
+    put("gkt", 'd', simparms.nchain, coords.gkt[0]);
+    put("mnh", 'd', simparms.nchain, coords.mnh[0]);
+    put("veta0", 'd', simparms.nchain, coords.veta[0]);
+    put("eta0", 'd', simparms.nchain, coords.eta[0]);
+  }
   put("qch", 'd', simparms.natoms, coords.qch);
   put("amass", 'd', simparms.natoms, coords.amass);
   put("itype", 'i', simparms.natoms, inter.itype);
@@ -243,6 +255,10 @@ int main(int argc, char *argv[])
     geteng(&simparms, &coords, &energies);
     putd("traj_poter0", energies.poter);
     putd("traj_zke0", energies.zke);
+    putd("traj_poth0", energies.poth);
+    putd("traj_potra0", energies.potra);
+    putd("traj_prsi0", energies.prsi);
+    putd("traj_tiout0", energies.tiout);
     for (s = 1; s <= nsteps; s++) {
       simparms.istep = s;
       step(&simparms, &coords, &energies, &inter, &ngbr);
@@ -253,6 +269,17 @@ int main(int argc, char *argv[])
       sprintf(nm, "traj_potra%d", s); putd(nm, energies.potra);
       sprintf(nm, "traj_zke%d", s); putd(nm, energies.zke);
       sprintf(nm, "traj_nupdate%d", s); puti(nm, ngbr.update);
+      sprintf(nm, "traj_poth%d", s); putd(nm, energies.poth);
+      sprintf(nm, "traj_prsi%d", s); putd(nm, energies.prsi);
+      sprintf(nm, "traj_tiout%d", s); putd(nm, energies.tiout);
+      if (simparms.ntherm > 0) {
+        double zken, potn;
+        getengn(&simparms, &coords, &zken, &potn);
+        sprintf(nm, "traj_veta%d", s); put(nm, 'd', simparms.nchain, coords.veta[0]);
+        sprintf(nm, "traj_eta%d", s); put(nm, 'd', simparms.nchain, coords.eta[0]);
+        sprintf(nm, "traj_zken%d", s); putd(nm, zken);
+        sprintf(nm, "traj_potn%d", s); putd(nm, potn);
+      }
     }
     puti("traj_nsteps", nsteps);
   }

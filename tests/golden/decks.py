@@ -25,15 +25,21 @@ def _init_file(path, sysd):
 
 
 def write_deck(d, sysd, kind, nstep=1, dt=2.0, nres_intra=1, temp=300.0, seed=12345,
-               extra=None):
+               extra=None, nhc=0, tau_nhc=1000.0, therm=None):
     """kind: 'lj' (one species, one atom), 'water' (3-site flexible water) or
-    'mixture' (two one-atom species, species-major).  Returns the deck path."""
+    'mixture' (two one-atom species, species-major).  nhc > 0 makes it an
+    NVT deck with an nhc-link Nose-Hoover chain (tau_nhc in fs); therm gives
+    each species' mol_therm_opt ('global' or 'none', default all global
+    under NVT).  Returns the deck path."""
     os.makedirs(d, exist_ok=True)
     name = "sys"
     ntable = int(sysd["ntable"])
-    keys = [r"\calculation{md}", r"\istart{1}", rf"\ntime{{{nstep}}}", r"\ensemble{nve}",
+    ens = "nvt" if nhc else "nve"
+    th = list(therm) if therm else ["global" if nhc else "none"] * 2
+    keys = [r"\calculation{md}", r"\istart{1}", rf"\ntime{{{nstep}}}", rf"\ensemble{{{ens}}}",
             r"\nres_lrf{1}", rf"\nres_intra{{{nres_intra}}}", r"\nres_tors{1}",
-            r"\len_nhc{0}", r"\nbar{0}", rf"\t_ext{{{temp}}}", r"\p_ext{0.0}",
+            rf"\len_nhc{{{nhc}}}", rf"\tau_nhc{{{tau_nhc!r}}}", r"\nbar{0}",
+            rf"\t_ext{{{temp}}}", r"\p_ext{0.0}",
             rf"\dt{{{dt}}}", r"\iperd{3}", r"\iwr_screen{1000000}", r"\iwr_dump{1000000}",
             r"\iwr_inst{1000000}", rf"\dpi_file{{{name}.init}}", rf"\dpo_file{{{name}.save}}",
             rf"\ham_file{{{name}.ham}}", rf"\eng_file{{{name}.eng}}",
@@ -53,7 +59,7 @@ def write_deck(d, sysd, kind, nstep=1, dt=2.0, nres_intra=1, temp=300.0, seed=12
             open(os.path.join(d, "data_base"), "w") as fdb:
         if kind == "lj":
             n = int(sysd["natoms"])
-            fset.write(rf"~mol_def[\mol_parm_file{{a.top}} \mol_therm_opt{{none}} \nmol{{{n}}}\mol_index{{1}}]" + "\n")
+            fset.write(rf"~mol_def[\mol_parm_file{{a.top}} \mol_therm_opt{{{th[0]}}} \nmol{{{n}}}\mol_index{{1}}]" + "\n")
             with open(os.path.join(d, "a.top"), "w") as t:
                 t.write(r"~mol_name_def[\mol_name{atom}\natom{1}\nbond{0}]" + "\n")
                 t.write(rf"~atom_def[\atom_typ{{a}}\atom_ind{{1}}\mass{{{float(m[0])!r}}} \charge{{{float(qe[0])!r}}}]" + "\n")
@@ -61,8 +67,8 @@ def write_deck(d, sysd, kind, nstep=1, dt=2.0, nres_intra=1, temp=300.0, seed=12
         elif kind == "mixture":
             n = int(sysd["natoms"])
             h = n // 2
-            fset.write(rf"~mol_def[\mol_parm_file{{a.top}} \mol_therm_opt{{none}} \nmol{{{h}}}\mol_index{{1}}]" + "\n")
-            fset.write(rf"~mol_def[\mol_parm_file{{b.top}} \mol_therm_opt{{none}} \nmol{{{n - h}}}\mol_index{{2}}]" + "\n")
+            fset.write(rf"~mol_def[\mol_parm_file{{a.top}} \mol_therm_opt{{{th[0]}}} \nmol{{{h}}}\mol_index{{1}}]" + "\n")
+            fset.write(rf"~mol_def[\mol_parm_file{{b.top}} \mol_therm_opt{{{th[1]}}} \nmol{{{n - h}}}\mol_index{{2}}]" + "\n")
             for nm, idx in (("a", 0), ("b", h)):
                 with open(os.path.join(d, f"{nm}.top"), "w") as t:
                     t.write(rf"~mol_name_def[\mol_name{{{nm}}}\natom{{1}}\nbond{{0}}]" + "\n")
@@ -70,7 +76,7 @@ def write_deck(d, sysd, kind, nstep=1, dt=2.0, nres_intra=1, temp=300.0, seed=12
             types = ["a", "b"]
         elif kind == "water":
             nmol = int(sysd["natoms"]) // 3
-            fset.write(rf"~mol_def[\mol_parm_file{{w.top}} \mol_therm_opt{{none}} \nmol{{{nmol}}}\mol_index{{1}}]" + "\n")
+            fset.write(rf"~mol_def[\mol_parm_file{{w.top}} \mol_therm_opt{{{th[0]}}} \nmol{{{nmol}}}\mol_index{{1}}]" + "\n")
             with open(os.path.join(d, "w.top"), "w") as t:
                 t.write(r"~mol_name_def[\mol_name{water}\natom{3}\nbond{2}\nbend{1}]" + "\n")
                 t.write(rf"~atom_def[\atom_typ{{ow}}\atom_ind{{1}}\mass{{{float(m[0])!r}}} \charge{{{float(qe[0])!r}}}]" + "\n")

@@ -12,11 +12,18 @@ trajectory of cm3d's own RESPA step.  Large lists are stored as their length
 and SHA-256 digests (ordered list, and sorted (min,max) pair set).
 Cases:
   ref_lj2, ref_lj256   the reference's test/lj decks (2 and 256 LJ atoms)
+  ref_ljnvt            the same 256 atoms under the test/lj NVT deck (a
+                       3-link Nose-Hoover chain), with the chain state
+                       per step
   ref_abutane          the reference's test/abutane deck (Williams, erfc
                        real space, Ewald kmax 15, 64 ecorr pairs)
   ar864                the 864-atom argon FCC workload
   water512             512 flexible waters (rc 10, alpha 0.35, kmax 9, RESPA 4)
   mix600               600-atom LJ + charge binary mixture (3 interactions)
+  water512_nvt         water512 under a 4-link chain (tau 50 fs), RESPA 4
+  mix600_nvt           mix600 with only the first species thermostatted
+                       (2-link chain, tau 100 fs)
+NVT decks are kept beside their fixture (decks_<name>/) for the deck reader.
 """
 import hashlib
 import os
@@ -96,6 +103,10 @@ def synth_case(name, sysd, kind, nsteps, **kw):
     with tempfile.TemporaryDirectory() as tmp:
         deck = write_deck(tmp, sysd, kind, nstep=nsteps, **kw)
         d = run_ref(tmp, deck, nsteps)
+        if kw.get("nhc"):  # the deck itself, for the product's deck reader
+            keep = os.path.join(HERE, "decks_" + name)
+            shutil.rmtree(keep, ignore_errors=True)
+            shutil.copytree(tmp, keep, ignore=shutil.ignore_patterns("dump.bin"))
     # the synthetic positions must round-trip the deck exactly
     assert np.array_equal(d["pos"], sysd["pos"]), name
     save(name, d)
@@ -114,12 +125,17 @@ def main():
     ref_case("ref_lj2", "lj", "lj.input_ljnve.2", 5)
     ref_case("ref_lj256", "lj", "lj.input_ljnve", 3)
     ref_case("ref_abutane", "abutane", "abutane.input", 2)
+    ref_case("ref_ljnvt", "lj", "lj.input_ljnvt", 5)
     synth_case("ar864", S.lj_fcc(), "lj", 3, dt=2.0, temp=120.0)
     w = S.water(nside=8)
     synth_case("water512", w, "water", 2, dt=2.0, nres_intra=4, temp=300.0,
                extra=coul_keys(w))
     m = S.mixture(600)
     synth_case("mix600", m, "mixture", 2, dt=2.0, temp=300.0, extra=coul_keys(m))
+    synth_case("water512_nvt", w, "water", 3, dt=2.0, nres_intra=4, temp=300.0,
+               extra=coul_keys(w), nhc=4, tau_nhc=50.0)
+    synth_case("mix600_nvt", m, "mixture", 3, dt=2.0, temp=300.0, extra=coul_keys(m),
+               nhc=2, tau_nhc=100.0, therm=("global", "none"))
 
 
 if __name__ == "__main__":

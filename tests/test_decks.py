@@ -1,0 +1,151 @@
+"""cm3d's file formats: the deck reader builds, from the reference's own
+test/lj files (copied under tests/golden/cm3d_lj), the system cm3d's input
+layer builds -- checked against the compiled reference's dump of the same
+decks (ref_lj256: NVE, ref_ljnvt: NVT) -- and cm3d's random stream draws the
+same cold-start velocities and chain velocities.  The initial report of
+cm3d's own screen output (lj.out_nve.t / lj.out_nvt.t) is rebuilt from the
+dumped energies with the report formulas.  CPU only."""
+import io
+import math
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, load_fixture
+from cm3d_amd import decks, run as R
+
+LJ = os.path.join(ROOT, "tests", "golden", "cm3d_lj")
+
+
+def screen_reports(path):
+    """The instantaneous column of every report block of a cm3d screen
+    output: [{name: value}], the initial report first."""
+    out, cur = [], None
+    for line in open(path):
+        line = line.rstrip("\n")
+        if line.startswith("nprs="):
+            cur = {}
+            out.append(cur)
+            continue
+        if cur is None or not line.startswith("<"):
+            continue
+        name = line[1:line.index(">")].strip()
+        val = float(line.split("~")[1].split()[0])
+        if name == "pres" and "atm" in line:
+            name = "pres_atm"
+        cur[name] = val
+    return out
+
+
+GOLD = os.path.join(ROOT, "tests", "golden")
+DECKS = [(os.path.join(LJ, "lj.input_ljnve"), "ref_lj256"),
+         (os.path.join(LJ, "lj.input_ljnvt"), "ref_ljnvt"),
+         (os.path.join(GOLD, "decks_water512_nvt", "sys.input"), "water512_nvt"),
+         (os.path.join(GOLD, "decks_mix600_nvt", "sys.input"), "mix600_nvt")]
+
+
+@pytest.mark.parametrize("deck,fix", DECKS, ids=[f for _, f in DECKS])
+def test_deck_matches_cm3d_setup(deck, fix):
+    d = load_fixture(fix)
+    s, run = decks.load(deck)
+    assert s["natoms"] == int(d["natoms"][0])
+    assert np.array_equal(s["pos"], d["pos"]) and np.array_equal(s["hmat"], d["hmat"])
+    assert np.array_equal(s["qch"], d["qch"])
+    if int(d["ktot"][0]) > 0:
+        for k in ("coultab", "dcoultab", "switchse", "ka", "kb", "kc"):
+            assert np.array_equal(np.asarray(s[k]), d[k]), k
+        for k in ("kappa", "ewald_self", "ewald_bkg"):
+            assert float(s[k]) == float(d[k][0]), k
+    if "bond_i" in s:
+        assert "mol_start" in s and len(s["bond_i"]) > 0
+    assert np.array_equal(s["amass"], d["amass"])
+    assert np.array_equal(s["itype"], d["itype"]) and np.array_equal(s["map"], d["map"])
+    for k in ("vtab", "dvtab", "switchs", "rmins_cut2", "rmaxs_cut2", "rminl_cut2",
+              "rmaxl_cut2", "rmaxs_skin2", "rmaxl_skin2", "dx2tab"):
+        assert np.array_equal(np.asarray(s[k]).ravel(), d[k].ravel()), k
+    assert s["vlrc"] == d["vlrc"][0] and s["wlrc"] == d["wlrc"][0]
+    assert run["ndof"] == int(d["ndof"][0])
+    # cold start: samvel (and samveta) on cm3d's ran1 stream, bit for bit
+    assert np.array_equal(run["vel"], d["traj_vel0"])
+    if int(d["ntherm"][0]) > 0:
+        assert run["ensemble"] == "nvt"
+        assert run["nchain"] == int(d["nchain"][0])
+        assert np.array_equal(run["ithm"], d["ithm"])
+        assert np.array_equal(run["gkt"], d["gkt"]) and np.array_equal(run["mnh"], d["mnh"])
+        assert np.array_equal(run["veta"], d["veta0"])
+        assert not np.any(d["eta0"])
+
+
+def test_ran1_minimal_standard():
+    """ran1's generator step is Schrage's form of the Park-Miller minimal
+    standard s -> 16807 s mod (2^31 - 1): equal to the direct product, and
+    10000 steps from 1 reach 1043618065 (Park & Miller's published check)."""
+    r = decks.Ran1(1)
+    rng = np.random.default_rng(5)
+    for s0 in list(rng.integers(1, 2 ** 31 - 1, 2000)) + [1, 2 ** 31 - 2, 127773, 127774]:
+        assert r._step(int(s0)) == 16807 * int(s0) % (2 ** 31 - 1)
+    s0 = 1
+    for _ in range(10000):
+        s0 = r._step(s0)
+    assert s0 == 1043618065
+    u = [r.next() for _ in range(1000)]
+    assert 0.0 < min(u) and max(u) < 1.0
+
+
+@pytest.mark.parametrize("deck,fix,out", [("lj.input_ljnve", "ref_lj256", "lj.out_nve.t"),
+                                          ("lj.input_ljnvt", "ref_ljnvt", "lj.out_nvt.t")])
+def test_initial_report_from_dumped_energies(deck, fix, out):
+    """cm3d's initial report, rebuilt from cm3d's own dumped energies by the
+    runner's report formulas (Ham with poth and the chain terms, the chain
+    normalisations, temperature p2m/ndof, pressure in atm)."""
+    d = load_fixture(fix)
+    _, run = decks.load(os.path.join(LJ, deck))
+    n = int(d["natoms"][0])
+    nvt = fix == "ref_ljnvt"
+    nc = int(d["nchain"][0]) if nvt else 0
+    potn = 0.0
+    zken = 0.5 * float(np.sum(d["mnh"] * d["veta0"] ** 2)) if nvt else 0.0
+    e = dict(poter=d["traj_poter0"][0], poth=d["traj_poth0"][0], potra=d["traj_potra0"][0],
+             zke=d["traj_zke0"][0], vinter=np.nan, potn=potn, zken=zken,
+             tiout=d["traj_tiout0"][0], prsi=d["traj_prsi0"][0])
+    e["ham"] = e["poth"] + e["potra"] + e["zke"] + e["potn"] + e["zken"]
+    r = R.report(e, n, nc, 1 if nvt else 0)
+    ref = screen_reports(os.path.join(LJ, out))[0]
+    for k in ("Uer/N", "Ke /N", "Ham/N", "temp", "pres", "pres_atm") + \
+            (("Ueta/NC", "Keta/NC") if nvt else ()):
+        assert abs(r[k] - ref[k]) <= 5e-7 + 1e-12 * abs(ref[k]), (k, r[k], ref[k])
+
+
+def test_screen_parser_reads_every_block():
+    for out, nblk in (("lj.out_nve.t", 11), ("lj.out_nvt.t", 11)):
+        reps = screen_reports(os.path.join(LJ, out))
+        assert len(reps) == nblk, (out, len(reps))
+        assert all("Ham/N" in r and "temp" in r for r in reps)
+
+
+def test_inst_file_format():
+    """.ham / .eng headers and records in save_inst's printf formats."""
+    fh, fe = io.StringIO(), io.StringIO()
+    decks.write_inst_headers(fh, fe, 256, 1, 3, 100, 0.001)
+    e = dict(ham=-800.123456789012, poth=-1233.25, potra=0.0, zke=384.0, potn=0.5, zken=1.5,
+             poter=-1348.6, vinter=-1232.1, vbond=0.0, vbend=0.0, velec=0.0, vewald=0.0)
+    decks.write_inst(fh, fe, 100, 0.001, e)
+    h = fh.getvalue().splitlines()
+    assert h[0] == "# 256 1 3 0 100 0.001"
+    assert h[1] == "# time ham inter intra ke potn ken potv kev inter"
+    assert h[2] == "   0.1 -800.1234568 -1233.25 0 384 0.5 1.5 0 0 -1348.6"
+    g = fe.getvalue().splitlines()
+    assert g[0] == "# 256 100     0.001"
+    assert g[2].split() == ["0.1", "384", "-1232.1", "0", "0", "0", "0", "0", "0", "0"]
+    assert math.isclose(float(h[2].split()[1]), e["ham"], rel_tol=1e-9)
+
+
+def test_inst_headers_match_cm3d_files():
+    """The header lines cm3d itself wrote for an NVT deck (kept with the
+    water512_nvt deck) are the ones write_inst_headers writes."""
+    d = os.path.join(GOLD, "decks_water512_nvt")
+    fh, fe = io.StringIO(), io.StringIO()
+    decks.write_inst_headers(fh, fe, 1536, 1, 4, 1000000, 0.002)
+    assert fh.getvalue() == open(os.path.join(d, "sys.ham")).read()
+    assert fe.getvalue() == open(os.path.join(d, "sys.eng")).read()

@@ -44,6 +44,12 @@ def test_md_trajectory_matches_cm3d_step(name, graph):
     assert abs(e["poter"] - d[f"traj_poter{s}"][0]) <= 1e-9 * max(1.0, abs(d[f"traj_poter{s}"][0]))
     assert abs(e["zke"] - d[f"traj_zke{s}"][0]) <= 1e-9 * abs(d[f"traj_zke{s}"][0])
     assert abs(e["potra"] - d[f"traj_potra{s}"][0]) <= 1e-9 * max(1.0, abs(d[f"traj_potra{s}"][0]))
+    # geteng's other outputs: poth (shifted pair energy), pressure, temperature
+    md.ndof = int(d["ndof"][0])
+    e = md.energies()
+    for k in ("poth", "prsi", "tiout"):
+        ref = d[f"traj_{k}{s}"][0]
+        assert abs(e[k] - ref) <= 1e-9 * max(1.0, abs(ref)), (name, k, e[k], ref)
     assert md.rebuilds() >= 0
     md.close()
     ctx.close()
