@@ -272,24 +272,24 @@ def main():
     ew_flop = 26.0 * ktot * nq
     pair_gbs = pair_bytes / (tm["pair"] * 1e-3) / 1e9
     ew_tfs = ew_flop / (tm["ewald"] * 1e-3) / 1e12
-    if tm["pair"] >= tm["ewald"]:
-        traffic, tsrc = None, None
-        tpath = os.path.join(ROOT, "profiles", "pair_kernel_water4096_traffic.json")
-        if os.path.exists(tpath) and a.nside == 16:
-            with open(tpath) as f:
-                tj = json.load(f)
-            traffic = round(float(tj["traffic_bytes_per_launch"]), 0)
-            tsrc = "profiles/pair_kernel_water4096_traffic.md (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated)"
-        roof = {"kernel": "k_pair64_1f", "bound": "hbm", "achieved": round(pair_gbs, 2),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(pair_gbs / HBM_PEAK_GBS, 5),
-                "traffic": traffic, "traffic_source": tsrc,
-                "ms_per_launch": round(tm["pair"], 5), "bytes_per_launch": pair_bytes}
-    else:
-        roof = {"kernel": "Ewald (k_ewchg+k_ewsfac+k_ewkred+k_red_ew+k_ewforce)",
-                "bound": "valu-fp64", "achieved": round(ew_tfs, 3), "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(ew_tfs / FP64_PEAK_TFLOPS, 5),
-                "traffic": None, "ms_per_launch": round(tm["ewald"], 5),
-                "flop_per_launch": ew_flop}
+    # the pair kernel is the longest single kernel of the step and is bound
+    # by HBM (a gather over the list); the Ewald kernels together are bound
+    # by FP64 VALU throughput and are reported beside it against that roof
+    traffic, tsrc = None, None
+    tpath = os.path.join(ROOT, "profiles", "pair_kernel_water4096_traffic.json")
+    if os.path.exists(tpath) and a.nside == 16:
+        with open(tpath) as f:
+            tj = json.load(f)
+        traffic = round(float(tj["traffic_bytes_per_launch"]), 0)
+        tsrc = "profiles/pair_kernel_water4096_traffic.md (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, copy-calibrated)"
+    roof = {"kernel": "k_pair64_1f", "bound": "hbm", "achieved": round(pair_gbs, 2),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(pair_gbs / HBM_PEAK_GBS, 5),
+            "traffic": traffic, "traffic_source": tsrc,
+            "ms_per_launch": round(tm["pair"], 5), "bytes_per_launch": pair_bytes}
+    roof_ew = {"kernels": "k_ewchg+k_ewsfac+k_ewkred+k_red+k_ewforce+k_ewfin",
+               "bound": "valu-fp64", "achieved": round(ew_tfs, 3), "peak": FP64_PEAK_TFLOPS,
+               "unit": "TFLOP/s", "frac": round(ew_tfs / FP64_PEAK_TFLOPS, 5),
+               "ms_per_launch": round(tm["ewald"], 5), "flop_per_launch": ew_flop}
     out = {
         "metric": "atom-steps/s", "value": round(atom_steps, 1), "unit": "atom-steps/s",
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -313,6 +313,7 @@ def main():
         "force_phase_ms": {k: round(v, 5) for k, v in tm.items() if k != "calls"},
         "pair_interactions_per_s": round(counts["nprte"] * a.steps / wall, 1),
         "roofline": roof,
+        "roofline_ewald": roof_ew,
         "pair_kernel_gbs": round(pair_gbs, 2),
         "ewald_tflops": round(ew_tfs, 4),
     }

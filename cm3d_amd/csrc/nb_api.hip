@@ -619,6 +619,34 @@ extern "C" int nb_force_device(nb_ctx *c, int iflg, const double *d_pxyz, const 
   return force_on(c, iflg, hmat, d_fs, iflg ? d_fl : nullptr, e, s);
 }
 
+// getvireal(...,0)'s pair sums (getvirter / getvirter_e over the t and te
+// lists, md_finter.c:314-477, md_finter_e.c:235-366) plus the reciprocal
+// and ecorr energies and virials, with no force output.  The pair kernels
+// are the force path's (their force stores go to scratch: 24 B per atom
+// beside 4 B per listed pair they gather), so the values are bit for bit
+// force(...,1)'s; the reciprocal sum skips its force pass.
+extern "C" int nb_energy_only(nb_ctx *c, const double *pxyz, const double hmat[9],
+                              nb_energy *e) {
+  if (!c || !pxyz || !hmat || !e) {
+    nb_set_error("nb_energy_only: null argument");
+    return NB_ERR_ARG;
+  }
+  if (!c->have_lists) {
+    nb_set_error("nb_energy_only: lists not built (call nb_build_lists)");
+    return NB_ERR_ARG;
+  }
+  HCHK(hipSetDevice(c->dev));
+  HCHK(hipMemcpyAsync(c->d_pos, pxyz, sizeof(double) * 3 * c->natoms, hipMemcpyHostToDevice,
+                      c->stream));
+  Box b = make_box(hmat);
+  RC(nb_launch_force(c, 1, b, nullptr, c->d_fs, c->stream, NB_FORCE_EONLY));
+  double E[64];
+  HCHK(hipMemcpyAsync(E, c->d_eout, sizeof(E), hipMemcpyDeviceToHost, c->stream));
+  HCHK(hipStreamSynchronize(c->stream));
+  nb_fill_energy(c, 1, E, hmat, e);
+  return NB_OK;
+}
+
 extern "C" int nb_footprint(nb_ctx *c, size_t *bytes) {
   if (!c || !bytes) { nb_set_error("nb_footprint: null argument"); return NB_ERR_ARG; }
   *bytes = c->dev_bytes;

@@ -214,6 +214,7 @@ int nb_launch_build_cond(nb_ctx *c, const Box &b, hipStream_t s);
 int nb_build_now(nb_ctx *c, const double hmat[9], hipStream_t s);
 #define NB_FORCE_PACKED 1  // sorted records / Ewald charge data already current
 #define NB_FORCE_NORED 2   // leave the energy columns unreduced
+#define NB_FORCE_EONLY 4   // energies and virials only: no reciprocal-space force pass
 int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs,
                     double *d_fl, hipStream_t s, int flags);
 int nb_launch_rebuild_check(nb_ctx *c, const double *d_pos, hipStream_t s);

@@ -995,6 +995,7 @@ int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_f
   nb_time_mark(c, 3, s);
   const bool ec = iflg == 1 && c->iperd == 3 && c->necorr > 0;
   const bool ew = iflg == 1 && c->iperd == 3 && c->ktot > 0 && c->ncharge > 0;
+  const bool ewf = ew && !(flags & NB_FORCE_EONLY);  // reciprocal-space forces
   const int nq = c->ncharge;
   EwChgArgs ch = {c->d_pos, c->d_qch, c->d_qidx, nq, n, b, (double4 *)c->d_chg,
                   (double2 *)c->d_stp};
@@ -1013,11 +1014,11 @@ int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_f
   EwForceArgs ef = {c->d_seg, c->d_rtile, c->d_tseg, (const KData *)c->d_kd,
                     (const double4 *)c->d_chg, (const double2 *)c->d_stp, nq, c->nqb, c->d_fpart,
                     b};
-  NB_GO(c, K_EWFORCE, k_ewforce, ew ? c->nqb * c->nrange : 0, 256, 0, ef, s);
-  EwFinArgs fa = {c->d_fpart, (const double4 *)c->d_chg, c->d_qidx, nq, n, c->nrange, ew, ec,
+  NB_GO(c, K_EWFORCE, k_ewforce, ewf ? c->nqb * c->nrange : 0, 256, 0, ef, s);
+  EwFinArgs fa = {c->d_fpart, (const double4 *)c->d_chg, c->d_qidx, nq, n, c->nrange, ewf, ec,
                   c->d_eoff, c->d_elist, c->d_pos, c->d_qch, c->kappa, d_fl, c->nfin_blocks,
                   c->d_eec};
-  NB_GO(c, K_EWFIN, k_ewfin, (ew || ec) ? c->nfin_blocks : 0, 256, 0, fa, s);
+  NB_GO(c, K_EWFIN, k_ewfin, (ewf || ec) ? c->nfin_blocks : 0, 256, 0, fa, s);
   if (!(flags & NB_FORCE_NORED)) {
     RedSet R = nb_red_set(c, iflg);
     NB_GO(c, K_RED, k_red, R.ncols, 256, 0, R, s);

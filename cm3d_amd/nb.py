@@ -95,7 +95,7 @@ class NBMDEnergy(C.Structure):
 
 # every symbol include/cm3d_nb.h declares
 EXPORTS = ["nb_create", "nb_destroy", "nb_last_error", "nb_build_lists",
-           "nb_need_rebuild", "nb_force", "nb_force_device",
+           "nb_need_rebuild", "nb_force", "nb_force_device", "nb_energy_only",
            "nb_build_lists_device", "nb_list_counts", "nb_get_pairs", "nb_timing",
            "nb_footprint",
            "nb_md_create", "nb_md_destroy", "nb_md_set_state", "nb_md_get_state",
@@ -131,6 +131,7 @@ def lib():
                                   C.c_void_p, C.c_void_p, C.POINTER(NBEnergy),
                                   C.c_void_p]
     L.nb_build_lists_device.argtypes = [C.c_void_p, C.c_void_p, _dp, C.c_void_p]
+    L.nb_energy_only.argtypes = [C.c_void_p, _dp, _dp, C.POINTER(NBEnergy)]
     L.nb_list_counts.argtypes = [C.c_void_p, _ip]
     L.nb_footprint.argtypes = [C.c_void_p, C.POINTER(C.c_size_t)]
     L.nb_get_pairs.argtypes = [C.c_void_p, C.c_int, _ip, _ip, C.c_int, _ip]
@@ -251,6 +252,15 @@ class NBContext:
         check(lib().nb_force(self._h, int(iflg), pp, hp, fs.ctypes.data_as(_dp),
                              fl.ctypes.data_as(_dp), C.byref(e)), "nb_force")
         return fs, fl, e.as_dict()
+
+    def energy_only(self, pos, hmat):
+        """Energies and virials of force(...,1) without the forces (the
+        getvireal(...,0) pair sums plus the Ewald and ecorr terms)."""
+        p, pp = _d(pos)
+        h, hp = _d(hmat)
+        e = NBEnergy()
+        check(lib().nb_energy_only(self._h, pp, hp, C.byref(e)), "nb_energy_only")
+        return e.as_dict()
 
     def force_device(self, iflg, d_pos, hmat, d_fs, d_fl, stream=None, energy=False):
         """Device-pointer variant (ints from torch.Tensor.data_ptr())."""

@@ -116,6 +116,16 @@ int nb_need_rebuild(nb_ctx *ctx, const double *pxyz, int *flag);
 int nb_force(nb_ctx *ctx, int iflg, const double *pxyz, const double hmat[9],
              double *fs, double *fl, nb_energy *e);
 
+/* Energy-only evaluation: the energies and virials getvireal(...,0) sums
+ * through getvirter / getvirter_e (vinter_npol, md_fnopol.c:257-383;
+ * vinter_e, finter_e.c:260-388; over the current t / te lists), plus, for
+ * iperd 3, the reciprocal-space and ecorr energies and virials fk_ewald /
+ * ecorr would leave in COUL -- all of e's fields, no forces.  Values are
+ * those of nb_force(...,1), bit for bit.  (cm3d's own vinter_e indexes its
+ * table from rmaxl_coul2, so every pair is clamped to the first entry; the
+ * engine returns the force path's real-space coulomb energy instead.) */
+int nb_energy_only(nb_ctx *ctx, const double *pxyz, const double hmat[9], nb_energy *e);
+
 /* Same on device-resident arrays (3N doubles in HBM) on the given HIP stream
  * (NULL = the context's stream).  fs may be NULL to skip the switched force. */
 int nb_force_device(nb_ctx *ctx, int iflg, const double *d_pxyz,

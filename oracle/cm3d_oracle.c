@@ -823,7 +823,9 @@ void orc_force(const orc_sys *s, const orc_lists *L, int iflg,
   }
   for (i = 0; i < n3; i++) fx[i] += scr[i];
   if (iflg == 1) for (i = 0; i < n3; i++) fs[i] += rec[i];
-  if (iflg == 1 && s->iperd == 3 && s->ktot > 0) {
+  /* md_fcntr.c:65-68: with iperd 3 fk_ewald (background and self even with
+     no k-vectors) and ecorr always run */
+  if (iflg == 1 && s->iperd == 3) {
     orc_fk_ewald(s, pos, h, fl, e);
     orc_ecorr(s, pos, fl, e);
   }

@@ -19,7 +19,8 @@
  *   readpos      input/md_readpos.c:32
  *   get_ngbr     energy/control/ngbr_list.c:297
  *   force        energy/control/md_fcntr.c:44
- *   getvireal    energy/control/md_fcntr.c:136
+ *   getvireal    energy/control/md_fcntr.c:136 (iflg 1, and iflg 0 for the
+ *                energy-only sums)
  *   step_init    integrate/md_step.c:32,  step integrate/md_step.c:62
  *   geteng       energy/control/md_fcntr.c:259, getengn integrate/md_therm.c
  *
@@ -240,6 +241,11 @@ int main(int argc, char *argv[])
   putd("pot_ecorr", coords.coul.pot_ecorr);
   putd("prs_ecorr", coords.coul.prs_ecorr);
   put("prs_ecorr_tensor", 'd', 9, coords.coul.prs_ecorr_tensor);
+  /* the energy-only sums (getvirter / getvirter_e, vinter_npol / vinter_e)
+     of getvireal(...,0) on the same lists and positions */
+  puti("ilist", ngbr.ilist);
+  getvireal(&simparms, &coords, &energies, &inter, &ngbr, 0);
+  dump_energies("e0_", &energies);
 
   /* ---- optional trajectory through cm3d's own integrator ---- */
   if (nsteps > 0) {

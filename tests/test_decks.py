@@ -149,3 +149,53 @@ def test_inst_headers_match_cm3d_files():
     decks.write_inst_headers(fh, fe, 1536, 1, 4, 1000000, 0.002)
     assert fh.getvalue() == open(os.path.join(d, "sys.ham")).read()
     assert fe.getvalue() == open(os.path.join(d, "sys.eng")).read()
+
+
+# ---------------------------------------------------------------------------
+# the energy-only path (getvireal(...,0): getvirter / getvirter_e)
+
+@pytest.mark.parametrize("name", ["ar864", "mix600", "ref_abutane", "ref_lj256", "water512"])
+def test_reference_energy_only_vdw_equals_force_path(name):
+    """cm3d's vinter_npol sums, for the vdW pairs, what force_npol leaves in
+    its statics (same list, table, clamp and order: the energies bit for
+    bit, the virials to rounding, as force_npol forms them from the force),
+    which is why the engine's energy-only entry reuses the force path's pair
+    kernels."""
+    d = load_fixture(name)
+    for k in ("vinter", "vhinter"):
+        assert d[f"e0_{k}"][0] == d[f"e_{k}"][0], (name, k)
+    w0, w = d["e0_winter"][0], d["e_winter"][0]
+    assert abs(w0 - w) <= 1e-12 * abs(w), (w0, w)
+    assert np.allclose(d["e0_wintertensor"], d["e_wintertensor"], rtol=1e-12,
+                       atol=1e-12 * abs(w))
+
+
+def _min_image(d, dx):
+    """cm3d's truncating minimum image (period_new case 3, anint under
+    -DLINUX: (int)(x +- 0.5)) for rows of displacements."""
+    h = d["hmat"].reshape(3, 3)
+    hi = np.linalg.inv(h)
+    s = dx @ hi.T
+    s = s - np.trunc(s + np.where(s >= 0, .5, -.5))
+    return s @ h.T
+
+
+@pytest.mark.parametrize("name", ["mix600", "ref_abutane"])
+def test_reference_energy_only_coulomb_quirk(name):
+    """cm3d's vinter_e takes its table origin from rmin2, which the missing
+    break in its cutoff switch (finter_e.c:296-302) leaves at rmaxl_coul2:
+    every listed pair inside the cutoff then indexes below the table start
+    and is clamped to k = 1, p = -1, i.e. coultab[0].  The dumped
+    getvireal(...,0) coulomb energy is exactly that sum, not the force
+    path's; the engine's nb_energy_only returns the force path's value."""
+    d = load_fixture(name)
+    n = int(d["natoms"][0])
+    pos = d["pos"].reshape(3, n).T
+    i, j = d["inpte"], d["jnpte"]
+    r = _min_image(d, pos[i] - pos[j])
+    r2 = np.einsum("ij,ij->i", r, r)
+    inside = r2 < float(d["rmaxl_coul2"][0])
+    qq = d["qch"][i] * d["qch"][j]
+    quirk = float(np.sum(qq[inside])) * float(d["coultab"][0])
+    assert abs(d["e0_velec"][0] - quirk) <= 1e-9 * abs(quirk), (d["e0_velec"][0], quirk)
+    assert abs(d["e0_velec"][0] - d["e_velec"][0]) > 1e-3 * abs(d["e_velec"][0])

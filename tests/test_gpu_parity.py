@@ -134,3 +134,23 @@ def test_mixed_bitwise_reproducible():
     b = c.force(d["pos"], d["hmat"], 1)
     assert np.array_equal(a[1], b[1]) and np.array_equal(a[0], b[0])
     c.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("name", ["ar864", "water512", "mix600", "ref_abutane"])
+def test_energy_only_equals_force_energies(name, prec):
+    """nb_energy_only: every energy / virial field of force(...,1), bit for
+    bit, without the forces; the vdW sums equal cm3d's own energy-only
+    getvireal(...,0) (vinter_npol) within the FP64 bar."""
+    d = load_fixture(name)
+    print(f"[eonly] {name} {prec}", flush=True)
+    c = _ctx(d, prec)
+    c.build_lists(d["pos"], d["hmat"])
+    _, _, ef = c.force(d["pos"], d["hmat"], iflg=1)
+    eo = c.energy_only(d["pos"], d["hmat"])
+    for k, v in ef.items():
+        assert np.array_equal(np.asarray(eo[k]), np.asarray(v)), (k, eo[k], v)
+    if prec == "fp64":
+        for mine, ref in (("spotl", "e0_vinter"), ("spothl", "e0_vhinter")):
+            assert _rel(eo[mine], d[ref][0]) < FP64_TOL or abs(eo[mine] - d[ref][0]) < 1e-9
+    c.close()

@@ -114,3 +114,20 @@ def test_newton_third_law(name):
     tot = fl.reshape(3, n).sum(axis=1)
     scale = max(np.abs(fl).sum() / n, 1.0)  # lattice starts have ~zero forces
     assert np.abs(tot).max() < 1e-9 * scale * n, (tot, scale)
+
+
+def test_ecorr_and_background_without_kvectors():
+    """cm3d runs fk_ewald and ecorr whenever iperd is 3 (md_fcntr.c:65-68),
+    also with no k-vectors: the background and self constants and the
+    excluded-pair correction do not depend on the k list."""
+    from oracle import oracle as O
+    d = load_fixture("water512")
+    L = O.build_lists(d, d["pos"], d["hmat"])
+    _, _, full = O.force(d, L, d["pos"], d["hmat"], 1)
+    s = dict(d)
+    s["ktot"] = np.array([0])
+    _, _, none = O.force(s, L, d["pos"], d["hmat"], 1)
+    assert none["pot_ecorr"] == full["pot_ecorr"] != 0.0
+    vol = abs(np.linalg.det(d["hmat"].reshape(3, 3)))
+    const = float(d["ewald_bkg"][0]) / vol + float(d["ewald_self"][0])
+    assert abs(none["vk_ewald"] - const) <= 1e-12 * abs(const)
