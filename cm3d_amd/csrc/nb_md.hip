@@ -547,28 +547,44 @@ static __device__ __forceinline__ void k_mdkick_body(const KickArgs &A, int bid)
 }
 NB_KERNEL_PAIR(k_mdkick, KickArgs, 256)
 
-// Kinetic, bond and bend energies and the bond virial (getvbond / getvbend
-// harmonic cases, md_fbond.c:165-190, md_fbend.c:318-338 -- the bend virial
-// is compiled only with -DPRESSURE, which cm3d's Linux builds leave off;
-// p2m trace, md_fcntr.c:338-356).
-#define MD_NE 4
+// Kinetic tensor, bond and bend energies and the bond virial (get_p2mt,
+// md_fcntr.c:332-354; getvbond / getvbend harmonic cases, md_fbond.c:165-207,
+// md_fbend.c:318-338 -- the bend virial is compiled only with -DPRESSURE,
+// which cm3d's Linux builds leave off).  Columns: kinetic xx yy zz xy xz yz,
+// vbond, vbend, wbond, wbond tensor [9].
+#define MD_NE 18
 __global__ __launch_bounds__(256) void k_md_energy(InnerArgs A, const double *__restrict__ amass,
                                                    double *__restrict__ epart) {
-  double acc[MD_NE] = {0., 0., 0., 0.};
+  double acc[MD_NE];
+#pragma unroll
+  for (int k = 0; k < MD_NE; k++) acc[k] = 0.;
   int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m < A.nmol) {
     const int n = A.n, a0 = A.moff[m], a1 = A.moff[m + 1];
-    for (int g = a0; g < a1; g++)
-      for (int d = 0; d < 3; d++) acc[0] += amass[g] * (A.vel[d * n + g] * A.vel[d * n + g]);
+    for (int g = a0; g < a1; g++) {
+      const double am = amass[g];
+      const double vx = A.vel[g], vy = A.vel[n + g], vz = A.vel[2 * n + g];
+      acc[0] += am * (vx * vx);
+      acc[1] += am * (vy * vy);
+      acc[2] += am * (vz * vz);
+      acc[3] += am * (vy * vx);
+      acc[4] += am * (vz * vx);
+      acc[5] += am * (vz * vy);
+    }
     for (int b = A.boff[m]; b < A.boff[m + 1]; b++) {
       int i = A.bi[b], j = A.bj[b];
       double dx = A.pos[i] - A.pos[j], dy = A.pos[n + i] - A.pos[n + j];
       double dz = A.pos[2 * n + i] - A.pos[2 * n + j];
       const double r = sqrt(dx * dx + dy * dy + dz * dz);
       double dr = r - A.br0[b];
-      acc[1] += .5 * A.bk[b] * dr * dr;
+      acc[6] += .5 * A.bk[b] * dr * dr;
       const double fk = A.bk[b] * dr / r;
-      acc[3] -= fk * r * r;
+      acc[8] -= fk * r * r;
+      const double d[3] = {dx, dy, dz};
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) acc[9 + 3 * a + c] -= fk * d[c] * d[a];
     }
     for (int e = A.aoff[m]; e < A.aoff[m + 1]; e++) {
       int i = A.ai[e], j = A.aj[e], k = A.ak[e];
@@ -578,7 +594,7 @@ __global__ __launch_bounds__(256) void k_md_energy(InnerArgs A, const double *__
       c = c < 1. ? c : 1.;
       c = c > -1. ? c : -1.;
       double th = fabs(c) >= 1. ? 1. : acos(c), dth = th - A.ath0[e];
-      acc[2] += .5 * A.afk[e] * dth * dth;
+      acc[7] += .5 * A.afk[e] * dth * dth;
     }
   }
   __shared__ double red[4][MD_NE];
@@ -794,7 +810,7 @@ extern "C" int nb_md_get_state(nb_md *md, double *pos, double *vel) {
 // set_therm + samveta's result (md_therm.new.c:158-169, 248-275) for one
 // "global" thermostat: eta starts at 0 and veta at veta0.
 extern "C" int nb_md_set_thermostat(nb_md *md, int nchain, const double *gkt, const double *mnh,
-                                    const double *veta0, const int *ithm) {
+                                    const double *veta0, const double *eta0, const int *ithm) {
   if (!md || nchain < 0 || nchain > MD_MAXCHAIN) {
     nb_set_error("nb_md_set_thermostat: bad argument (nchain 0..%d)", MD_MAXCHAIN);
     return NB_ERR_ARG;
@@ -823,6 +839,7 @@ extern "C" int nb_md_set_thermostat(nb_md *md, int nchain, const double *gkt, co
   std::vector<double> h(4 * nchain, 0.);
   for (int i = 0; i < nchain; i++) {
     h[i] = veta0[i];
+    h[nchain + i] = eta0 ? eta0[i] : 0.;
     h[2 * nchain + i] = gkt[i];
     h[3 * nchain + i] = mnh[i];
   }
@@ -839,6 +856,22 @@ extern "C" int nb_md_set_thermostat(nb_md *md, int nchain, const double *gkt, co
   }
   md->nhc_bytes = c->dev_bytes - b0;
   md->nchain = nchain;
+  return NB_OK;
+}
+
+extern "C" int nb_md_get_thermostat(nb_md *md, double *veta, double *eta) {
+  if (!md) { nb_set_error("nb_md_get_thermostat: null argument"); return NB_ERR_ARG; }
+  const int nc = md->nchain;
+  if (nc == 0) return NB_OK;
+  nb_ctx *c = md->c;
+  MCHK(hipSetDevice(c->dev));
+  MCHK(hipStreamSynchronize(c->stream));
+  std::vector<double> h(2 * nc);
+  MCHK(hipMemcpy(h.data(), md->d_nhc, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int i = 0; i < nc; i++) {
+    if (veta) veta[i] = h[i];
+    if (eta) eta[i] = h[nc + i];
+  }
   return NB_OK;
 }
 
@@ -957,10 +990,13 @@ extern "C" int nb_md_energies(nb_md *md, double vlrc, double wlrc, int ndof, nb_
   MCHK(hipMemcpyAsync(E, c->d_eout, sizeof(E), hipMemcpyDeviceToHost, s));
   MCHK(hipMemcpyAsync(c->h_flags, c->d_flags, 8 * sizeof(int), hipMemcpyDeviceToHost, s));
   MCHK(hipStreamSynchronize(s));
-  double p2m = 0., vb = 0., va = 0., wb = 0.;
-  for (int b = 0; b < md->ep_blocks; b++) {
-    p2m += ep[MD_NE * b]; vb += ep[MD_NE * b + 1]; va += ep[MD_NE * b + 2]; wb += ep[MD_NE * b + 3];
-  }
+  double col[MD_NE] = {0.};
+  for (int b = 0; b < md->ep_blocks; b++)
+    for (int k = 0; k < MD_NE; k++) col[k] += ep[MD_NE * (size_t)b + k];
+  // p2m_tensor (get_p2mt) and its trace, as geteng forms it
+  const double p2mt[9] = {col[0], col[3], col[4], col[3], col[1], col[5], col[4], col[5], col[2]};
+  const double p2m = p2mt[0] + p2mt[4] + p2mt[8];
+  const double vb = col[6], va = col[7], wb = col[8];
   const int nc = md->nchain;
   std::vector<double> nhc(4 * nc);
   if (nc) MCHK(hipMemcpy(nhc.data(), md->d_nhc, nhc.size() * sizeof(double), hipMemcpyDeviceToHost));
@@ -984,11 +1020,21 @@ extern "C" int nb_md_energies(nb_md *md, double vlrc, double wlrc, int ndof, nb_
   e->poter = e->vinter + e->velec + e->vewald;
   e->poth = e->vhinter + e->velec + e->vewald;
   double W = e->winter + e->welec + e->wewald;
+  double Wt[9];
+  for (int k = 0; k < 9; k++) {
+    Wt[k] = ne.sprslt[k] + ne.sprset[k];
+    if (c->iperd == 3) Wt[k] += ne.vp_ewald_tensor[k] + ne.prs_ecorr_tensor[k];
+  }
   if (c->iperd == 3) {
     e->poter += vlrc / vol;
     e->poth += vlrc / vol;
-    W -= 3. * wlrc * (1. / vol);
+    const double rvol = 1. / vol;
+    W -= 3. * wlrc * rvol;
+    Wt[0] -= wlrc * rvol;
+    Wt[4] -= wlrc * rvol;
+    Wt[8] -= wlrc * rvol;
   }
+  for (int k = 0; k < 9; k++) e->prs_tensor[k] = (p2mt[k] + Wt[k] + col[9 + k]) / vol;
   e->potra = vb + va;
   // getengn_new (md_therm.new.c:200-222)
   for (int i = 0; i < nc; i++) e->potn += nhc[2 * nc + i] * nhc[nc + i];

@@ -35,18 +35,20 @@ import numpy as np
 from . import nb
 from . import systems as S
 
+PCONV_ATM = 138.0622 / 1.01325  # K/A^3 -> atm (include/md_numbers.h:33)
+
 # input/md_setsimdef.c
 DEFAULTS = dict(calculation="md", istart=1, ntime=1, ensemble="nve", dt=1.0, t_ext=300.0,
                 p_ext=0.0, iperd=3, nres_lrf=1, nres_intra=1, nres_tors=1, len_nhc=0,
                 nbar=0, skin=0.5, rheal_res=0.5, ntable=500, tau_nhc=1000.0,
                 alp_ewald=0.15, kmax=7, ishift=0, rcute_max=10.0, rcute_min=1.0,
-                rcute_resp=7.0, iwr_screen=0, iwr_inst=0, iseed=None, scaleeps=1.0,
+                rcute_resp=7.0, iwr_screen=0, iwr_inst=0, iwr_dump=0, iseed=None, scaleeps=1.0,
                 scale_temp=1.0, inter_file="data_base", bond_file="data_base",
                 bend_file="data_base", mol_set_file=None, dpi_file=None, ham_file=None,
                 eng_file=None, long_rc="on")
 
 _INT_KEYS = {"istart", "ntime", "iperd", "nres_lrf", "nres_intra", "nres_tors", "len_nhc",
-             "nbar", "ntable", "kmax", "ishift", "iwr_screen", "iwr_inst", "iseed"}
+             "nbar", "ntable", "kmax", "ishift", "iwr_screen", "iwr_inst", "iwr_dump", "iseed"}
 _FLOAT_KEYS = {"dt", "t_ext", "p_ext", "skin", "rheal_res", "tau_nhc", "alp_ewald",
                "rcute_max", "rcute_min", "rcute_resp", "scaleeps"}
 
@@ -189,8 +191,14 @@ class Ran1:
 def samvel(rng, amass, temp, ndof):
     """integrate/md_zeroam.c:33-46: gaussian velocities (vx|vy|vz contiguous),
     zero total momentum, rescale to temp over ndof degrees of freedom."""
+    return rescale_vel(rng.ggauss(3 * len(amass)), amass, temp, ndof)
+
+
+def rescale_vel(v, amass, temp, ndof):
+    """samvel without resampling (irandom 0): zerotm, then scale to temp
+    over ndof (md_zeroam.c:33-46, 48-66, 84-111)."""
     n = len(amass)
-    v = rng.ggauss(3 * n)
+    v = np.array(v, dtype=np.float64)
     vx, vy, vz = v[:n], v[n:2 * n], v[2 * n:]
     tm = xv = yv = zv = 0.0
     if n > 1:
@@ -288,7 +296,17 @@ def load(deck_path):
             mol_start.append(len(itype))
     n = len(itype)
     nt = len(types)
-    ni, ist, pos, hmat = read_init(fp(ctl["dpi_file"]))
+    istart = ctl["istart"]
+    if istart == 1:
+        ni, ist, pos, hmat = read_init(fp(ctl["dpi_file"]))
+        if ist != istart:  # md_readpos.c:74-78
+            raise ValueError(f"{ctl['dpi_file']}: istart {ist} in the file, {istart} in the deck")
+        rst = None
+    elif istart in (2, 3):
+        rst = read_restart(fp(ctl["dpi_file"]))
+        ni, pos, hmat = rst["natoms"], rst["pos"], rst["hmat"]
+    else:
+        raise ValueError("istart must be 1, 2 or 3")
     if ni != n:
         raise ValueError(f"{ctl['dpi_file']}: {ni} atoms, the set file gives {n}")
 
@@ -359,26 +377,134 @@ def load(deck_path):
                    ctl["rcute_max"], ctl["rheal_res"], ctl["skin"], 0)
         s.update(ktot=0, ewald_self=0.0, ewald_bkg=0.0)
 
-    # cold start: cm3d's random stream (seeded by the deck, md_readsim.c:257,
-    # 505), samvel, then the chain velocities
+    # velocities (main/md_mdcntr.new.c:88-107): a cold start draws them from
+    # cm3d's random stream (seeded by the deck, md_readsim.c:257, 505) with
+    # samvel and then the chain velocities with samveta; a warm start takes
+    # them from the restart file, rescaled (istart 2) when the stored
+    # temperature differs (md_readpos.c:227-232)
     nvt = ctl["ensemble"].lower() == "nvt"
     ndof = 3 * n if nvt else (3 * n - 3 if ctl["iperd"] != 0 else 3 * n - 6)
-    temp = ctl["t_ext"] * ctl["scale_temp"]
+    t_set = ctl["t_ext"]                     # simparms->temp at set-up
+    temp = t_set * ctl["scale_temp"]         # after samvel's scaletemp
     rng = Ran1(ctl["iseed"] if ctl["iseed"] is not None else 0)
-    vel = samvel(rng, s["amass"], temp, max(ndof, 1)) if ist == 1 else np.zeros(3 * n)
-    run = dict(ctl=ctl, nsteps=ctl["ntime"], dt_ps=ctl["dt_ps"], nres_intra=ctl["nres_intra"],
-               ensemble=ctl["ensemble"].lower(), ndof=ndof, temp=temp, vel=vel,
-               iwr_screen=ctl["iwr_screen"], iwr_inst=ctl["iwr_inst"], base=base)
+    istep0 = 0
+    if rst is None:
+        vel = samvel(rng, s["amass"], temp, max(ndof, 1))
+    else:
+        vel = rst["vel"]
+        istep0 = rst["istep"]
+        if istart == 2 and abs((rst["temp"] - t_set) / (rst["temp"] or 1.0)) > 1e-10:
+            vel = rescale_vel(vel, s["amass"], temp, max(ndof, 1))
+    run = dict(ctl=ctl, istart=istart, istep0=istep0, dt_ps=ctl["dt_ps"],
+               nres_intra=ctl["nres_intra"], ensemble=ctl["ensemble"].lower(), ndof=ndof,
+               temp=temp, vel=vel, iwr_screen=ctl["iwr_screen"], iwr_inst=ctl["iwr_inst"],
+               iwr_dump=ctl["iwr_dump"], base=base, restart=rst,
+               pext=ctl["p_ext"] / PCONV_ATM)
+    # steps to run: ntime from istep 1 (istart 1, 2), or from the stored step
+    # + 1 up to ntime (istart 3; md_mdcntr.new.c:157, 171)
+    run["nsteps"] = ctl["ntime"] - istep0 if istart == 3 else ctl["ntime"]
     if nvt:
         nthm = sum(1 for t in ithm if t == 0)
         nchain = ctl["len_nhc"]
         # set_therm (md_therm.new.c:158-169): gkt[0] = T DIM n, gkt[i>0] = T,
-        # mnh = gkt tau^2
-        gkt = np.array([temp * 3 * nthm] + [temp] * (nchain - 1))
+        # mnh = gkt tau^2, with the deck's temperature
+        gkt = np.array([t_set * 3 * nthm] + [t_set] * (nchain - 1))
         mnh = gkt * ctl["tau_nhc_ps"] * ctl["tau_nhc_ps"]  # (gkt tau) tau, as cm3d rounds
+        if rst is None:
+            veta, eta = samveta(rng, nchain, mnh, temp), np.zeros(nchain)
+        elif istart == 3 and rst["nchain"] == nchain and rst["ntherm"] == 1:
+            veta, eta = rst["veta"][0].copy(), rst["eta"][0].copy()
+        else:  # istart 2, or a different chain: zeroed (md_readpos.c:243-262)
+            veta, eta = np.zeros(nchain), np.zeros(nchain)
         run.update(nchain=nchain, ithm=np.array(ithm, np.int32), gkt=gkt, mnh=mnh,
-                   veta=samveta(rng, nchain, mnh, temp))
+                   veta=veta, eta=eta)
     return s, run
+
+
+AVERAGE_KEYS = (["avham", "avpotra", "avpoter", "avzke", "avpotn", "avzken", "avpotv",
+                 "avzkev"],
+                ["avpot_inter", "avpot_bond", "avpot_bend", "avpot_tors", "avpot_onfo",
+                 "avpot_onfo_e", "avpot_elec", "avpot_recip", "avpot_extern",
+                 "avpot_extern_e"],
+                ["avtemp", "avprs", "avvol", "econv", "econ2"])
+
+
+def read_restart(path):
+    """A restart file as writestart writes it and readpos reads it for
+    istart 2 / 3 (output/md_writestart.c:31-90, input/md_readpos.c:124-262):
+    header, running sums, positions + velocities, hmat, vvol9, chain."""
+    with open(path) as f:
+        lines = f.read().split("\n")
+    h = lines[0].split()
+    r = dict(istep=int(h[0]), natoms=int(h[1]), ntherm=int(h[2]), nchain=int(h[3]),
+             nbar=int(h[4]), temp=float(h[5]), pext=float(h[6]), dt=float(h[7]))
+    for row, keys in zip(lines[1:4], AVERAGE_KEYS):
+        vals = [float(x) for x in row.split()]
+        if len(vals) != len(keys):
+            raise ValueError(f"{path}: averages line has {len(vals)} numbers, {len(keys)} expected")
+        r.update(zip(keys, vals))
+    r["avprs_tensor"] = np.array([float(x) for x in lines[4].split()])
+    if len(r["avprs_tensor"]) != 9:
+        raise ValueError(f"{path}: pressure tensor averages")
+    n = r["natoms"]
+    pv = np.array([[float(x) for x in ln.split()[:6]] for ln in lines[5:5 + n]])
+    if pv.shape != (n, 6):
+        raise ValueError(f"{path}: positions and velocities")
+    r["pos"] = pv[:, :3].T.reshape(-1).copy()
+    r["vel"] = pv[:, 3:].T.reshape(-1).copy()
+    r["hmat"] = np.array(" ".join(lines[5 + n:8 + n]).split(), float)
+    r["vvol9"] = np.array(" ".join(lines[8 + n:11 + n]).split(), float)
+    tok = " ".join(lines[11 + n:]).split()
+    nt, nc = r["ntherm"], r["nchain"]
+    ch = np.array(tok[:2 * nt * nc], float).reshape(nt, nc, 2) if nt * nc else \
+        np.zeros((nt, nc, 2))
+    r["eta"], r["veta"] = ch[:, :, 0].copy(), ch[:, :, 1].copy()
+    return r
+
+
+def write_restart(path, istep, natoms, ntherm, nchain, temp, pext, dt_ps, av, pos, vel,
+                  hmat, eta=None, veta=None):
+    """writestart (output/md_writestart.c:31-90) with no barostat: av holds
+    the running sums (AVERAGE_KEYS and avprs_tensor); eta / veta are
+    [ntherm][nchain]."""
+    n = natoms
+    p = np.asarray(pos).reshape(3, n)
+    v = np.asarray(vel).reshape(3, n)
+    g = lambda k: av.get(k, 0.0)  # noqa: E731
+    with open(path, "w") as f:
+        f.write("%d %d %d %d %d %12g %12g %12g\n" % (istep, n, ntherm, nchain, 0, temp, pext,
+                                                      dt_ps))
+        f.write("%12g %12g %12g %12g %12g %12g %12g %12g\n" % tuple(g(k) for k in AVERAGE_KEYS[0]))
+        f.write("%12g %12g %12g %12g %12g %12g %12g %12g %12g %12g\n"
+                % tuple(g(k) for k in AVERAGE_KEYS[1]))
+        f.write("%12g %12g %12g %12g %12g\n" % tuple(g(k) for k in AVERAGE_KEYS[2]))
+        f.write("".join("%12g " % x for x in av.get("avprs_tensor", np.zeros(9))) + "\n")
+        for i in range(n):
+            f.write("%.18g %.18g %.18g %.18g %.18g %.18g\n" % (p[0, i], p[1, i], p[2, i],
+                                                              v[0, i], v[1, i], v[2, i]))
+        h = np.asarray(hmat).reshape(-1)
+        for i in range(3):
+            f.write("%16.10g %16.10g %16.10g\n" % tuple(h[3 * i:3 * i + 3]))
+        for i in range(3):
+            f.write("%16.10g %16.10g %16.10g\n" % (0., 0., 0.))
+        for j in range(ntherm):
+            f.write("".join(" %.12g %.12g" % (eta[j][i], veta[j][i]) for i in range(nchain)))
+            f.write("\n")
+        f.write("\n")
+
+
+def write_coord(path, pos, hmat):
+    """write_coord (output/md_writestart.c:118-142): a cold-start file of the
+    final positions, written as <initfile>_N."""
+    p = np.asarray(pos).reshape(3, -1)
+    n = p.shape[1]
+    h = np.asarray(hmat).reshape(-1)
+    with open(path, "w") as f:
+        f.write("%d %d\n" % (n, 1))
+        for i in range(n):
+            f.write("%10g %10g %10g\n" % (p[0, i], p[1, i], p[2, i]))
+        for i in range(3):
+            f.write("%g %g %g\n" % tuple(h[3 * i:3 * i + 3]))
 
 
 def write_inst_headers(fham, feng, natoms, ntherm, nchain, ninst, dt_ps):

@@ -86,11 +86,15 @@ class NBMDEnergy(C.Structure):
     _fields_ = [(k, C.c_double) for k in ("vinter", "vhinter", "velec", "vewald", "vbond",
                                           "vbend", "zke", "poter", "poth", "potra", "potn",
                                           "zken", "ham", "winter", "welec", "wewald", "wbond",
-                                          "prsi", "tiout")] + \
-               [("overflow", C.c_int)]
+                                          "prsi")] + \
+               [("prs_tensor", C.c_double * 9), ("tiout", C.c_double), ("overflow", C.c_int)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        out = {}
+        for k, _ in self._fields_:
+            v = getattr(self, k)
+            out[k] = np.array(v[:]) if hasattr(v, "__len__") else v
+        return out
 
 
 # every symbol include/cm3d_nb.h declares
@@ -100,7 +104,7 @@ EXPORTS = ["nb_create", "nb_destroy", "nb_last_error", "nb_build_lists",
            "nb_footprint",
            "nb_md_create", "nb_md_destroy", "nb_md_set_state", "nb_md_get_state",
            "nb_md_init", "nb_md_step", "nb_md_energies", "nb_md_rebuilds",
-           "nb_md_set_thermostat",
+           "nb_md_set_thermostat", "nb_md_get_thermostat",
            "nb_batch_create", "nb_batch_destroy", "nb_batch_step", "nb_batch_launches",
            "nb_table_pair", "nb_table_coul", "nb_kvectors", "nb_ewald_consts"]
 
@@ -146,7 +150,8 @@ def lib():
     L.nb_md_energies.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int,
                                   C.POINTER(NBMDEnergy)]
     L.nb_md_rebuilds.argtypes = [C.c_void_p, _ip]
-    L.nb_md_set_thermostat.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _ip]
+    L.nb_md_set_thermostat.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _ip]
+    L.nb_md_get_thermostat.argtypes = [C.c_void_p, _dp, _dp]
     L.nb_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]
     L.nb_batch_destroy.argtypes = [C.c_void_p]
     L.nb_batch_destroy.restype = None
@@ -442,16 +447,27 @@ class NBMD:
               "nb_md_energies")
         return e.as_dict()
 
-    def set_thermostat(self, nchain, gkt, mnh, veta0, ithm):
+    def set_thermostat(self, nchain, gkt, mnh, veta0, ithm, eta0=None):
         """One Nose-Hoover chain (cm3d's global thermostat) over atoms with
-        ithm == 0; nchain 0 returns to NVE.  Call before init()."""
+        ithm == 0; nchain 0 returns to NVE.  eta0 None starts eta at 0."""
         g, gp = _d(gkt)
         m, mp = _d(mnh)
         v, vp = _d(veta0)
         t, tp = _i(ithm)
-        check(lib().nb_md_set_thermostat(self._h, int(nchain), gp, mp, vp, tp),
+        e, ep = _d(eta0) if eta0 is not None else (None, None)
+        check(lib().nb_md_set_thermostat(self._h, int(nchain), gp, mp, vp, ep, tp),
               "nb_md_set_thermostat")
+        self.nchain = int(nchain)
         self.ndof = 3 * self.n if nchain else 3 * self.n - 3
+
+    def get_thermostat(self):
+        """(veta, eta) of the chain now."""
+        nc = getattr(self, "nchain", 0)
+        veta, eta = np.zeros(nc), np.zeros(nc)
+        if nc:
+            check(lib().nb_md_get_thermostat(self._h, veta.ctypes.data_as(_dp),
+                                             eta.ctypes.data_as(_dp)), "nb_md_get_thermostat")
+        return veta, eta
 
     def rebuilds(self):
         c = C.c_int(0)

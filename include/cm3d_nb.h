@@ -185,6 +185,7 @@ typedef struct nb_md_energy {
   double ham;                                   /* poth+potra+zke+potn+zken */
   double winter, welec, wewald, wbond;          /* virials (getvireal)     */
   double prsi;                                  /* (p2m + W + WI) / (3 V)  */
+  double prs_tensor[9];                         /* (p2m_t + W_t + WI_t) / V */
   double tiout;                                 /* p2m / ndof              */
   int overflow;                                 /* a list build overflowed */
 } nb_md_energy;
@@ -202,10 +203,14 @@ int nb_md_step(nb_md *md, int nsteps, void *stream, int use_graph);
  * integrate/md_therm.new.c:278-332, applied at every inner RESPA substep as
  * md_step.c:113 does) over the atoms with ithm[i] == 0 (cm3d's "global"
  * option; -1 = not thermostatted).  gkt, mnh, veta0: [nchain] as set_therm /
- * samveta leave them (md_therm.new.c:158-169, 248-275); eta starts at 0.
- * nchain 0 returns to NVE.  Takes effect at the next nb_md_init. */
+ * samveta leave them (md_therm.new.c:158-169, 248-275); eta0 [nchain] or
+ * NULL for zeros (a warm start reads both from the restart file,
+ * md_readpos.c:243-262).  nchain 0 returns to NVE. */
 int nb_md_set_thermostat(nb_md *md, int nchain, const double *gkt, const double *mnh,
-                         const double *veta0, const int *ithm);
+                         const double *veta0, const double *eta0, const int *ithm);
+/* The chain's current veta, eta ([nchain] each, either may be NULL), for
+ * the restart file (writestart, output/md_writestart.c:67-73). */
+int nb_md_get_thermostat(nb_md *md, double *veta, double *eta);
 /* Energies, virials, pressure and temperature of the current state, as
  * getvireal + geteng leave them (md_fcntr.c:136-231, 259-306): vlrc, wlrc
  * are cm3d's simparms->vlrc / wlrc (applied when iperd == 3), ndof its

@@ -19,22 +19,39 @@ LJ = os.path.join(ROOT, "tests", "golden", "cm3d_lj")
 
 
 def screen_reports(path):
-    """The instantaneous column of every report block of a cm3d screen
-    output: [{name: value}], the initial report first."""
-    out, cur = [], None
+    """Every report block of a cm3d screen output (md_writescreen.c) as
+    {name: (average, instantaneous)}, the initial report first; the pressure
+    tensor rows as "ptensor_atm": (average[9], instantaneous[9])."""
+    out, cur, prow = [], None, None
     for line in open(path):
         line = line.rstrip("\n")
         if line.startswith("nprs="):
             cur = {}
             out.append(cur)
+            prow = None
             continue
-        if cur is None or not line.startswith("<"):
+        if cur is None:
+            continue
+        if line.startswith("Pressure Tensor"):
+            prow = []
+            continue
+        if prow is not None and "~" in line and line.rstrip().endswith("atm") \
+                and not line.startswith("<"):
+            a, b = line.split("~")
+            prow.append(([float(x) for x in a.split()], [float(x) for x in b.split()[:3]]))
+            if len(prow) == 3:
+                cur["ptensor_atm"] = (np.array(sum((r[0] for r in prow), [])),
+                                      np.array(sum((r[1] for r in prow), [])))
+                prow = None
+            continue
+        if not line.startswith("<"):
             continue
         name = line[1:line.index(">")].strip()
+        avg = float(line.split("=")[1].split("~")[0])
         val = float(line.split("~")[1].split()[0])
         if name == "pres" and "atm" in line:
             name = "pres_atm"
-        cur[name] = val
+        cur[name] = (avg, val)
     return out
 
 
@@ -110,18 +127,21 @@ def test_initial_report_from_dumped_energies(deck, fix, out):
              zke=d["traj_zke0"][0], vinter=np.nan, potn=potn, zken=zken,
              tiout=d["traj_tiout0"][0], prsi=d["traj_prsi0"][0])
     e["ham"] = e["poth"] + e["potra"] + e["zke"] + e["potn"] + e["zken"]
+    e["prs_tensor"] = np.zeros(9)
     r = R.report(e, n, nc, 1 if nvt else 0)
     ref = screen_reports(os.path.join(LJ, out))[0]
     for k in ("Uer/N", "Ke /N", "Ham/N", "temp", "pres", "pres_atm") + \
             (("Ueta/NC", "Keta/NC") if nvt else ()):
-        assert abs(r[k] - ref[k]) <= 5e-7 + 1e-12 * abs(ref[k]), (k, r[k], ref[k])
+        assert abs(r[k][1] - ref[k][1]) <= 5e-7 + 1e-12 * abs(ref[k][1]), (k, r[k], ref[k])
+        assert ref[k][0] == 0.0
 
 
 def test_screen_parser_reads_every_block():
     for out, nblk in (("lj.out_nve.t", 11), ("lj.out_nvt.t", 11)):
         reps = screen_reports(os.path.join(LJ, out))
         assert len(reps) == nblk, (out, len(reps))
-        assert all("Ham/N" in r and "temp" in r for r in reps)
+        assert all("Ham/N" in r and "temp" in r and "ptensor_atm" in r for r in reps)
+        assert all("dE" in r for r in reps[1:])
 
 
 def test_inst_file_format():
@@ -199,3 +219,41 @@ def test_reference_energy_only_coulomb_quirk(name):
     quirk = float(np.sum(qq[inside])) * float(d["coultab"][0])
     assert abs(d["e0_velec"][0] - quirk) <= 1e-9 * abs(quirk), (d["e0_velec"][0], quirk)
     assert abs(d["e0_velec"][0] - d["e_velec"][0]) > 1e-3 * abs(d["e_velec"][0])
+
+
+def test_restart_round_trip(tmp_path):
+    """writestart's format read back by readpos's rules: positions and
+    velocities to the last bit (%.18g), the running sums at %12g, hmat at
+    %16.10g, the chain at %.12g; a deck with istart 3 continues from it
+    and istart 2 zeroes the chain."""
+    import shutil
+    src = os.path.join(GOLD, "decks_water512_nvt")
+    w = tmp_path / "w"
+    shutil.copytree(src, w)
+    s, run = decks.load(str(w / "sys.input"))
+    n = s["natoms"]
+    rng = np.random.default_rng(3)
+    vel = rng.standard_normal(3 * n)
+    av = {k: float(rng.standard_normal()) for row in decks.AVERAGE_KEYS for k in row}
+    av["avprs_tensor"] = rng.standard_normal(9)
+    eta, veta = rng.standard_normal((1, 4)), rng.standard_normal((1, 4))
+    decks.write_restart(str(w / "sys.rst"), 250, n, 1, 4, run["temp"], run["pext"],
+                        run["dt_ps"], av, s["pos"], vel, s["hmat"], eta, veta)
+    r = decks.read_restart(str(w / "sys.rst"))
+    assert r["istep"] == 250 and r["natoms"] == n and (r["ntherm"], r["nchain"]) == (1, 4)
+    assert np.array_equal(r["pos"], s["pos"]) and np.array_equal(r["vel"], vel)
+    assert np.allclose(r["hmat"], s["hmat"], rtol=1e-10)
+    assert np.allclose(r["eta"], eta, rtol=1e-11) and np.allclose(r["veta"], veta, rtol=1e-11)
+    for k, v in av.items():
+        assert np.allclose(r[k], v, rtol=1e-5), k
+    deck = open(w / "sys.input").read().replace(r"\istart{1}", r"\istart{3}")
+    deck = deck.replace(r"\dpi_file{sys.init}", r"\dpi_file{sys.rst}")
+    deck = deck.replace(r"\ntime{3}", r"\ntime{300}")
+    open(w / "warm.input", "w").write(deck)
+    s3, run3 = decks.load(str(w / "warm.input"))
+    assert run3["istep0"] == 250 and run3["nsteps"] == 50
+    assert np.array_equal(run3["vel"], vel) and np.array_equal(s3["pos"], s["pos"])
+    assert np.allclose(run3["veta"], veta[0], rtol=1e-11)
+    open(w / "warm2.input", "w").write(deck.replace(r"\istart{3}", r"\istart{2}"))
+    s2, run2 = decks.load(str(w / "warm2.input"))
+    assert not np.any(run2["veta"]) and not np.any(run2["eta"]) and run2["nsteps"] == 300
