@@ -379,25 +379,23 @@ enum {
 
 #define NB_IP_LDS 16    // interactions staged in LDS
 #define NB_MAP_LDS 256  // type-map entries staged in LDS
-template <int IFLG, bool WANT_FS>
-static __device__ __forceinline__ void pair64_body(const PairArgs &A, int bid) {
-  extern __shared__ double tabs[];
-  __shared__ InterPar sip[NB_IP_LDS];
-  __shared__ int smap[NB_MAP_LDS];
-  const double *vt = A.vtab3, *ct = A.ctab3;
-  if (A.tab_lds) {
-    int nv = A.ninter * A.ntable * 3, nc = A.ntable * 3;
-    for (int t = threadIdx.x; t < nv; t += blockDim.x) tabs[t] = A.vtab3[t];
-    for (int t = threadIdx.x; t < nc; t += blockDim.x) tabs[A.ctab_off + t] = A.ctab3[t];
-    vt = tabs;
-    ct = tabs + A.ctab_off;
-  }
-  const bool ip_lds = A.ninter <= NB_IP_LDS && A.ntypes * A.ntypes <= NB_MAP_LDS;
-  if (ip_lds) {
-    for (int t = threadIdx.x; t < A.ninter; t += blockDim.x) sip[t] = A.ip[t];
-    for (int t = threadIdx.x; t < A.ntypes * A.ntypes; t += blockDim.x) smap[t] = A.map[t];
-  }
-  __syncthreads();
+// Table interpolation and the force / energy / virial sums may contract to
+// FMA; the minimum image, r^2, the cutoff tests and the table index stay in
+// cm3d's operation order (no contraction), so that which pairs enter and at
+// which table node is decided exactly as cm3d decides it.
+static __device__ __forceinline__ double interp3c(const double *t, int k, int c, double p) {
+#pragma clang fp contract(fast)
+  const double fm1 = t[3 * (k - 1) + c], f0 = t[3 * k + c], f1 = t[3 * (k + 1) + c];
+  return f0 + .5 * p * (f1 - fm1 + p * (fm1 - 2. * f0 + f1));
+}
+
+// The atom loop.  TLDS: the tables (vt / ct) and the interaction parameters
+// (sip / smap) are all in LDS, so the reads compile to LDS reads rather than
+// generic (flat) ones; otherwise any of them may be in HBM.
+template <int IFLG, bool WANT_FS, bool TLDS>
+static __device__ __forceinline__ void pair64_run(const PairArgs &A, int bid, const double *vt,
+                                                  const double *ct, const InterPar *sip,
+                                                  const int *smap, bool ip_lds) {
   double acc[NB_NE_PAIR];
 #pragma unroll
   for (int k = 0; k < NB_NE_PAIR; k++) acc[k] = 0.;
@@ -439,8 +437,9 @@ static __device__ __forceinline__ void pair64_body(const PairArgs &A, int bid) {
         nb_minimg_box(dx, dy, dz, A.b);
         double d2 = dx * dx + dy * dy + dz * dz;
         if (fl & fv) {
-          const int in = ip_lds ? smap[ti * A.ntypes + tjc] : A.map[ti * A.ntypes + tjc];
-          const InterPar &P = ip_lds ? sip[in] : A.ip[in];
+          const int in = TLDS ? smap[ti * A.ntypes + tjc]
+                              : (ip_lds ? smap[ti * A.ntypes + tjc] : A.map[ti * A.ntypes + tjc]);
+          const InterPar &P = TLDS ? sip[in] : (ip_lds ? sip[in] : A.ip[in]);
           double rmax2 = IFLG ? P.rmaxl2 : P.rmaxs2;
           if (d2 < rmax2) {
             double p = (d2 - P.rmins2) * P.dx2;
@@ -448,26 +447,28 @@ static __device__ __forceinline__ void pair64_body(const PairArgs &A, int bid) {
             if (kk < 1) { acc[E_CLAMP] += 1.; kk = 1; p = 1.; }
             p = p - (double)kk;
             const double *T = vt + (size_t)in * A.ntable * 3;
-            double f = nb_interp3(T, kk, 0, p);
+            double f = interp3c(T, kk, 0, p);
             double swv = 1.;
-            if ((WANT_FS || !IFLG) && d2 > P.rminl2) swv = nb_interp3(T, kk, 1, p);
-            if (IFLG) {
-              acc[E_SPRSL] -= d2 * f;
-              acc[E_SL0] -= dx * dx * f;
-              acc[E_SL1] -= dy * dx * f;
-              acc[E_SL2] -= dz * dx * f;
-              acc[E_SL4] -= dy * dy * f;
-              acc[E_SL5] -= dz * dy * f;
-              acc[E_SL8] -= dz * dz * f;
-              double gx = dx * f, gy = dy * f, gz = dz * f;
-              fx -= gx; fy -= gy; fz -= gz;
-              if (WANT_FS) { sx -= gx * swv; sy -= gy * swv; sz -= gz * swv; }
-              double v = nb_interp3(T, kk, 2, p);
-              acc[E_SPOTL] += v;
-              acc[E_SPOTHL] += v - T[3 * ntm4 + 2];
-            } else {
-              f *= swv;
-              sx -= dx * f; sy -= dy * f; sz -= dz * f;
+            if ((WANT_FS || !IFLG) && d2 > P.rminl2) swv = interp3c(T, kk, 1, p);
+            {
+#pragma clang fp contract(fast)
+              if (IFLG) {
+                acc[E_SPRSL] -= d2 * f;
+                acc[E_SL0] -= dx * dx * f;
+                acc[E_SL1] -= dy * dx * f;
+                acc[E_SL2] -= dz * dx * f;
+                acc[E_SL4] -= dy * dy * f;
+                acc[E_SL5] -= dz * dy * f;
+                acc[E_SL8] -= dz * dz * f;
+                fx -= dx * f; fy -= dy * f; fz -= dz * f;
+                if (WANT_FS) { sx -= dx * f * swv; sy -= dy * f * swv; sz -= dz * f * swv; }
+                double v = interp3c(T, kk, 2, p);
+                acc[E_SPOTL] += v;
+                acc[E_SPOTHL] += v - T[3 * ntm4 + 2];
+              } else {
+                f *= swv;
+                sx -= dx * f; sy -= dy * f; sz -= dz * f;
+              }
             }
           }
         }
@@ -476,23 +477,25 @@ static __device__ __forceinline__ void pair64_body(const PairArgs &A, int bid) {
           int kk = (int)p;
           if (kk < 1) { acc[E_CLAMP] += 1.; kk = 1; p = 1; }
           p = p - (double)kk;
-          double qij = ri.w * rj.w;
-          double fcc = qij * nb_interp3(ct, kk, 0, p);
-          double gx = dx * fcc, gy = dy * fcc, gz = dz * fcc;
-          if (IFLG) {
-            acc[E_SPRSE] -= d2 * fcc;
-            acc[E_SE0] -= dx * dx * fcc;
-            acc[E_SE1] -= dy * dx * fcc;
-            acc[E_SE2] -= dz * dx * fcc;
-            acc[E_SE4] -= dy * dy * fcc;
-            acc[E_SE5] -= dz * dy * fcc;
-            acc[E_SE8] -= dz * dz * fcc;
-            fx -= gx; fy -= gy; fz -= gz;
-            double v = nb_interp3(ct, kk, 2, p);
-            acc[E_SPOTHE] += qij * (v - A.clast);
-            acc[E_SPOTE] += qij * v;
+          {
+#pragma clang fp contract(fast)
+            double qij = ri.w * rj.w;
+            double fcc = qij * interp3c(ct, kk, 0, p);
+            if (IFLG) {
+              acc[E_SPRSE] -= d2 * fcc;
+              acc[E_SE0] -= dx * dx * fcc;
+              acc[E_SE1] -= dy * dx * fcc;
+              acc[E_SE2] -= dz * dx * fcc;
+              acc[E_SE4] -= dy * dy * fcc;
+              acc[E_SE5] -= dz * dy * fcc;
+              acc[E_SE8] -= dz * dz * fcc;
+              fx -= dx * fcc; fy -= dy * fcc; fz -= dz * fcc;
+              double v = interp3c(ct, kk, 2, p);
+              acc[E_SPOTHE] += qij * (v - A.clast);
+              acc[E_SPOTE] += qij * v;
+            }
+            if (WANT_FS || !IFLG) { sx -= dx * fcc; sy -= dy * fcc; sz -= dz * fcc; }
           }
-          if (WANT_FS || !IFLG) { sx -= gx; sy -= gy; sz -= gz; }
         }
       }
       ec = en;
@@ -509,6 +512,30 @@ static __device__ __forceinline__ void pair64_body(const PairArgs &A, int bid) {
     }
   }
   block_reduce_store<NB_NE_PAIR>(acc, A.epart + bid, A.nblocks);
+}
+
+template <int IFLG, bool WANT_FS>
+static __device__ __forceinline__ void pair64_body(const PairArgs &A, int bid) {
+  extern __shared__ double tabs[];
+  __shared__ InterPar sip[NB_IP_LDS];
+  __shared__ int smap[NB_MAP_LDS];
+  if (A.tab_lds) {
+    int nv = A.ninter * A.ntable * 3, nc = A.ntable * 3;
+    for (int t = threadIdx.x; t < nv; t += blockDim.x) tabs[t] = A.vtab3[t];
+    for (int t = threadIdx.x; t < nc; t += blockDim.x) tabs[A.ctab_off + t] = A.ctab3[t];
+  }
+  const bool ip_lds = A.ninter <= NB_IP_LDS && A.ntypes * A.ntypes <= NB_MAP_LDS;
+  if (ip_lds) {
+    for (int t = threadIdx.x; t < A.ninter; t += blockDim.x) sip[t] = A.ip[t];
+    for (int t = threadIdx.x; t < A.ntypes * A.ntypes; t += blockDim.x) smap[t] = A.map[t];
+  }
+  __syncthreads();
+  if (A.tab_lds && ip_lds)
+    pair64_run<IFLG, WANT_FS, true>(A, bid, tabs, tabs + A.ctab_off, sip, smap, ip_lds);
+  else if (A.tab_lds)
+    pair64_run<IFLG, WANT_FS, false>(A, bid, tabs, tabs + A.ctab_off, sip, smap, ip_lds);
+  else
+    pair64_run<IFLG, WANT_FS, false>(A, bid, A.vtab3, A.ctab3, sip, smap, ip_lds);
 }
 #define k_pair64_0t_body pair64_body<0, true>
 #define k_pair64_1t_body pair64_body<1, true>

@@ -88,25 +88,13 @@ static __device__ __forceinline__ long long to_fix(double v) {
 // per lane over all its pairs (a few tens), reduced in FP64 once per
 // workgroup.  The next chunk's list entries and partner records are loaded
 // before the current chunk is evaluated.
-template <int IFLG, bool WANT_FS>
-static __device__ __forceinline__ void pair_mixed_body(const MixArgs &A, int bid) {
-  // FP32 pair arithmetic may contract to FMA (the FP64 path may not)
+// The atom loop.  TLDS: tables (vt / ct) and type-pair parameters (stp) are
+// all in LDS, so their reads compile to LDS reads, not generic (flat) ones.
+template <int IFLG, bool WANT_FS, bool TLDS>
+static __device__ __forceinline__ void pair_mixed_run(const MixArgs &A, int bid, const float *vt,
+                                                      const float *ct, const TPairF *stp,
+                                                      bool tp_lds) {
 #pragma clang fp contract(fast)
-  extern __shared__ float tabsf[];
-  __shared__ TPairF stp[NB_TPF_LDS];
-  const float *vt = A.vtab3, *ct = A.ctab3;
-  if (A.tab_lds) {
-    int nv = A.ninter * A.ntable * 3, nc = A.ntable * 3;
-    for (int t = threadIdx.x; t < nv; t += blockDim.x) tabsf[t] = A.vtab3[t];
-    for (int t = threadIdx.x; t < nc; t += blockDim.x) tabsf[A.ctab_off + t] = A.ctab3[t];
-    vt = tabsf;
-    ct = tabsf + A.ctab_off;
-  }
-  const int nt2 = A.ntypes * A.ntypes;
-  const bool tp_lds = nt2 <= NB_TPF_LDS;
-  if (tp_lds)
-    for (int t = threadIdx.x; t < nt2; t += blockDim.x) stp[t] = A.tpf[t];
-  __syncthreads();
   float e[NB_NE_PAIR];  // this lane's energy / virial sums over its pairs
 #pragma unroll
   for (int k = 0; k < NB_NE_PAIR; k++) e[k] = 0.f;
@@ -121,7 +109,7 @@ static __device__ __forceinline__ void pair_mixed_body(const MixArgs &A, int bid
     const int ti = A.stype[s];
     const int nb = A.nnb[s];
     const uint32_t *lst = A.nbr + (size_t)s * A.cap;
-    const TPairF *tprow = (tp_lds ? stp : A.tpf) + ti * A.ntypes;
+    const TPairF *tprow = (TLDS ? stp : (tp_lds ? stp : A.tpf)) + ti * A.ntypes;
     float fx = 0.f, fy = 0.f, fz = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
     uint32_t ec = lane < nb ? lst[lane] : 0u;
     uint4 rjc = make_uint4(0u, 0u, 0u, 0u);
@@ -236,6 +224,28 @@ static __device__ __forceinline__ void pair_mixed_body(const MixArgs &A, int bid
 #pragma unroll
   for (int k = 0; k < NB_NE_PAIR; k++) acc[k] = (double)e[k];
   block_reduce_store_m<NB_NE_PAIR>(acc, A.epart + bid, A.nblocks);
+}
+
+template <int IFLG, bool WANT_FS>
+static __device__ __forceinline__ void pair_mixed_body(const MixArgs &A, int bid) {
+  extern __shared__ float tabsf[];
+  __shared__ TPairF stp[NB_TPF_LDS];
+  if (A.tab_lds) {
+    int nv = A.ninter * A.ntable * 3, nc = A.ntable * 3;
+    for (int t = threadIdx.x; t < nv; t += blockDim.x) tabsf[t] = A.vtab3[t];
+    for (int t = threadIdx.x; t < nc; t += blockDim.x) tabsf[A.ctab_off + t] = A.ctab3[t];
+  }
+  const int nt2 = A.ntypes * A.ntypes;
+  const bool tp_lds = nt2 <= NB_TPF_LDS;
+  if (tp_lds)
+    for (int t = threadIdx.x; t < nt2; t += blockDim.x) stp[t] = A.tpf[t];
+  __syncthreads();
+  if (A.tab_lds && tp_lds)
+    pair_mixed_run<IFLG, WANT_FS, true>(A, bid, tabsf, tabsf + A.ctab_off, stp, tp_lds);
+  else if (A.tab_lds)
+    pair_mixed_run<IFLG, WANT_FS, false>(A, bid, tabsf, tabsf + A.ctab_off, stp, tp_lds);
+  else
+    pair_mixed_run<IFLG, WANT_FS, false>(A, bid, A.vtab3, A.ctab3, stp, tp_lds);
 }
 #define k_mix_0t_body pair_mixed_body<0, true>
 #define k_mix_1t_body pair_mixed_body<1, true>
