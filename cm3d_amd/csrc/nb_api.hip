@@ -81,6 +81,7 @@ static double det9(const double h[9]) { return nb_det9(h); }
 extern "C" void nb_destroy(nb_ctx *c) {
   if (!c) return;
   hipSetDevice(c->dev);
+  hipDeviceSynchronize();
   void *ptrs[] = {c->d_itype, c->d_map, c->d_ip, c->d_vtab3, c->d_ctab3, c->d_vtab3f,
                   c->d_ctab3f, c->d_qch, c->d_xoff, c->d_xlist, c->d_eoff, c->d_elist,
                   c->d_qidx, c->d_qpos, c->d_seg, c->d_sseg, c->d_tseg, c->d_rtile, c->d_fpart, c->d_kabc, c->d_pos, c->d_posb, c->d_fs, c->d_fl,
@@ -97,6 +98,9 @@ extern "C" void nb_destroy(nb_ctx *c) {
     delete[] c->tev;
   }
   if (c->stream) hipStreamDestroy(c->stream);
+  if (c->stream2) hipStreamDestroy(c->stream2);
+  for (int k = 0; k < 2; k++)
+    if (c->fork_ev[k]) hipEventDestroy(c->fork_ev[k]);
   delete c;
 }
 
@@ -150,6 +154,13 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
   }
   int rc = NB_OK;
   auto fail = [&](int r) { nb_destroy(c); return r; };
+  // default-priority second stream and the two events of the MD step's fork
+  if (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->fork_ev[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->fork_ev[1], hipEventDisableTiming) != hipSuccess) {
+    nb_set_error("hipStreamCreate / hipEventCreate failed");
+    return fail(NB_ERR_HIP);
+  }
   c->natoms = n; c->ntypes = p->ntypes; c->ninter = p->ninter; c->ntable = p->ntable;
   c->iperd = p->iperd; c->prec = p->precision; c->skin = p->skin;
   c->kappa = p->kappa; c->ewald_self = p->ewald_self; c->ewald_bkg = p->ewald_bkg;
@@ -276,25 +287,29 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
   }
   c->nseg = (int)seg.size();
   // structure factor: one lane per segment, longest first; a workgroup of 4
-  // waves takes 64 segments and one chunk of charges; chunks sized for
-  // >= ~512 workgroups
+  // waves takes 64 segments and one chunk of charges (>= 32); chunks sized
+  // for ~NB_SFAC_WG workgroups, ~16 a CU, which the long charge loops need
+  // to hide FP64 latency (measured on the 98,304-atom water: 4096 -> 7.3 ms,
+  // 512 -> 11.5 ms for the whole reciprocal sum)
   std::vector<EwSeg> sseg(seg);
   std::stable_sort(sseg.begin(), sseg.end(),
                    [](const EwSeg &a, const EwSeg &b) { return a.cnt > b.cnt; });
   c->nsw = std::max(1, (c->nseg + NB_WAVE - 1) / NB_WAVE);
   {
     int maxch = std::max(1, (ncharge + 31) / 32);  // >= 32 charges a workgroup
-    int nch = std::min(std::max(1, (512 + c->nsw - 1) / c->nsw), maxch);
+    int nch = std::min(std::max(1, (NB_SFAC_WG + c->nsw - 1) / c->nsw), maxch);
     c->qc = std::max(1, (ncharge + nch - 1) / nch);
     c->nchunk = std::max(1, (ncharge + c->qc - 1) / c->qc);
   }
-  // force: two charges per lane, a workgroup per (512 charges, tile range);
-  // tiles are runs of whole segments small enough for LDS and for enough
-  // ranges to give >= ~1024 workgroups; ranges balanced by k count
-  c->nqb = std::max(1, (ncharge + 511) / 512);  // 256 lanes x 2 charges
+  // force: NB_EWF_Q charges per lane, a workgroup per (256 NB_EWF_Q
+  // charges, tile range); tiles are runs of whole segments small enough for
+  // LDS and for enough ranges to give ~NB_EWF_WG workgroups (each range's
+  // partial forces are 24 B a charge to write and to read back in k_ewfin);
+  // ranges balanced by k count
+  c->nqb = std::max(1, (ncharge + 256 * NB_EWF_Q - 1) / (256 * NB_EWF_Q));
   std::vector<int> tseg(1, 0), rtile(1, 0);
   {
-    int nrt = std::min(std::max(1, (1024 + c->nqb - 1) / c->nqb), std::max(1, c->nseg));
+    int nrt = std::min(std::max(1, (NB_EWF_WG + c->nqb - 1) / c->nqb), std::max(1, c->nseg));
     int lim = std::min(NB_EW_KT, std::max(NB_EW_SEG, (c->ktot + nrt - 1) / std::max(1, nrt)));
     int nk = 0;
     std::vector<int> tk;  // k per tile
@@ -305,14 +320,15 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
     tseg.push_back(c->nseg);
     tk.push_back(nk);
     c->ntile = (int)tseg.size() - 1;
-    c->nrange = std::min(nrt, c->ntile);
+    // close range r after the tile where the running k count reaches r/nr of
+    // the total: every range gets at least one tile, and uneven tiles only
+    // ever give fewer ranges, never one range for everything
+    const int nr = std::min(nrt, c->ntile);
     long long cum = 0;
-    for (int t = 0; t < c->ntile; t++) {
+    for (int t = 0; t + 1 < c->ntile; t++) {
       cum += tk[t];
-      int r = (int)rtile.size();
-      if (r < c->nrange && cum * c->nrange >= (long long)r * c->ktot && t + 1 < c->ntile &&
-          c->ntile - (t + 1) >= c->nrange - r)
-        rtile.push_back(t + 1);
+      const int r = (int)rtile.size();
+      if (r < nr && cum * nr >= (long long)r * c->ktot) rtile.push_back(t + 1);
     }
     rtile.push_back(c->ntile);
     c->nrange = (int)rtile.size() - 1;
@@ -360,7 +376,7 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
   c->npair_blocks = p->precision == NB_PREC_MIXED ? std::max(1, std::min((n + 15) / 16, 2048))
                                                   : std::min((n + 3) / 4, 2048);
   c->new_blocks = std::max(1, (c->ktot + NB_WAVE - 1) / NB_WAVE);
-  c->nfin_blocks = std::max(1, (ncharge + 255) / 256);
+  c->nfin_blocks = std::max(1, (ncharge + NB_WAVE - 1) / NB_WAVE);  // 64 charges a workgroup
   if ((rc = dalloc(c, &c->d_epair, (size_t)c->npair_blocks * NB_NE_PAIR))) return fail(rc);
   if ((rc = dalloc(c, &c->d_eec, (size_t)c->nfin_blocks * NB_NE_ECORR))) return fail(rc);
   if ((rc = dalloc(c, &c->d_eew, (size_t)c->new_blocks * NB_NE_EW))) return fail(rc);

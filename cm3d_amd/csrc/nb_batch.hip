@@ -46,6 +46,7 @@ struct nb_batch {
 extern "C" void nb_batch_destroy(nb_batch *b) {
   if (!b) return;
   hipSetDevice(b->dev);
+  hipDeviceSynchronize();  // the batch's graph may still be running
   if (b->have_graph) { hipGraphExecDestroy(b->gexec); hipGraphDestroy(b->graph); }
   for (auto &k : b->ks) {
     if (k.d_args) hipFree(k.d_args);

@@ -35,6 +35,15 @@ struct LaunchRec;
 #define NB_MAX_ATOMS (1 << 28)
 #define NB_EW_SEG 16     // k vectors per Ewald segment (one c-axis run)
 #define NB_EW_KT 512     // k vectors per LDS tile of the Ewald force pass
+#ifndef NB_EWF_Q
+#define NB_EWF_Q 2       // charges per lane in the Ewald force pass
+#endif
+#ifndef NB_EWF_WG
+#define NB_EWF_WG 1024   // workgroups the Ewald force pass aims for
+#endif
+#ifndef NB_SFAC_WG
+#define NB_SFAC_WG 4096  // workgroups the structure-factor pass aims for
+#endif
 #define NB_NE_PAIR 19    // pair energy/virial accumulators + clamp count
 #define NB_NE_ECORR 11   // ecorr energy/virial accumulators
 #define NB_NE_EW 11      // Ewald accumulators
@@ -152,6 +161,8 @@ __device__ __forceinline__ void nb_minimg_box(double &dx, double &dy, double &dz
 struct nb_ctx {
   int dev;
   hipStream_t stream;
+  hipStream_t stream2;        // the reciprocal sum's branch of the MD step
+  hipEvent_t fork_ev[2];      // fork / join of that branch
   int natoms, ntypes, ninter, ntable, iperd, prec;
   int ncharge, ktot, nseg, necorr;
   double skin, kappa, ewald_self, ewald_bkg;
@@ -215,6 +226,9 @@ int nb_build_now(nb_ctx *c, const double hmat[9], hipStream_t s);
 #define NB_FORCE_PACKED 1  // sorted records / Ewald charge data already current
 #define NB_FORCE_NORED 2   // leave the energy columns unreduced
 #define NB_FORCE_EONLY 4   // energies and virials only: no reciprocal-space force pass
+#define NB_FORCE_NO_PAIR 8    // leave out the pair part (pack + pair kernel)
+#define NB_FORCE_NO_RECIP 16  // leave out the reciprocal sum (charge records .. k_ewforce)
+#define NB_FORCE_NO_FIN 32    // leave out k_ewfin (ecorr + reciprocal forces into fl) and k_red
 int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs,
                     double *d_fl, hipStream_t s, int flags);
 int nb_launch_rebuild_check(nb_ctx *c, const double *d_pos, hipStream_t s);

@@ -760,7 +760,7 @@ struct EwForceArgs {
   const EwSeg *seg; const int *rtile, *tseg; const KData *kd; const double4 *chg;
   const double2 *stp; int nq, nqb; double *fpart; Box b;
 };
-#define EWF_Q 2  // charges per lane
+#define EWF_Q NB_EWF_Q
 static __device__ __forceinline__ void k_ewforce_body(const EwForceArgs &A, int bid) {
 #pragma clang fp contract(fast)
   __shared__ double2 kl[NB_EW_KT];  // {2 preg Re S, 2 preg Im S} per k of the tile
@@ -844,29 +844,45 @@ static __device__ __forceinline__ void k_ewforce_body(const EwForceArgs &A, int 
 }
 NB_KERNEL_PAIR(k_ewforce, EwForceArgs, 256)
 
-// k_ewfin, one thread per charge: ecorr (md_ewald.c:385-453) over the
-// atom's excluded charged partners (both directions, so energies and
+// k_ewfin, a workgroup per 64 charges (lane = charge): the four waves sum
+// interleaved quarters of the range partials (ranges w, w+4, ...), combined
+// in LDS in a fixed order; wave 0 then does ecorr (md_ewald.c:385-453) over
+// the atom's excluded charged partners (both directions, so energies and
 // virials are halved later), on the caller's unwrapped positions with no
-// minimum image, added to fl; then the reciprocal force, the range partials
-// summed in fixed order and times q, added to fl.  The ecorr energy / virial
-// partials go to a column-major table, one row per workgroup.
+// minimum image, and adds it and q times the reciprocal force to fl.  The
+// ecorr energy / virial partials go to a column-major table, one row per
+// workgroup.
 struct EwFinArgs {
   const double *fpart; const double4 *chg; const int *qidx; int nq, n, nrange, ew, ec;
   const int *eoff, *elist; const double *pos, *q; double kappa; double *fl;
   int nblocks; double *epart;
 };
 static __device__ __forceinline__ void k_ewfin_body(const EwFinArgs &A, int bid) {
+  __shared__ double rs[3][3][NB_WAVE];
   double acc[NB_NE_ECORR];
 #pragma unroll
   for (int k = 0; k < NB_NE_ECORR; k++) acc[k] = 0.;
-  const int c = bid * blockDim.x + threadIdx.x;
-  if (c < A.nq) {
+  const int lane = lane_id(), w = threadIdx.x >> 6;
+  const int c = bid * NB_WAVE + lane;
+  double fx = 0., fy = 0., fz = 0.;
+  if (A.ew && c < A.nq) {
+    const size_t nq = A.nq;
+    for (int r = w; r < A.nrange; r += 4) {
+      const double *p = A.fpart + 3 * (size_t)r * nq + c;
+      fx += p[0];
+      fy += p[nq];
+      fz += p[2 * nq];
+    }
+  }
+  if (w > 0) { rs[w - 1][0][lane] = fx; rs[w - 1][1][lane] = fy; rs[w - 1][2][lane] = fz; }
+  __syncthreads();
+  if (w == 0 && c < A.nq) {
     const int a = A.qidx[c], n = A.n;
     double gx = A.fl[a], gy = A.fl[n + a], gz = A.fl[2 * n + a];
     if (A.ec && A.eoff[a + 1] > A.eoff[a]) {
       const double spi = 1. / sqrt(M_PI), kappa = A.kappa, ka2 = kappa * kappa;
       const double *pos = A.pos, *q = A.q;
-      double fx = 0., fy = 0., fz = 0.;
+      double ex = 0., ey = 0., ez = 0.;
       for (int e = A.eoff[a]; e < A.eoff[a + 1]; e++) {
         int b = A.elist[e];
         double dx = pos[a] - pos[b], dy = pos[n + a] - pos[n + b], dz = pos[2 * n + a] - pos[2 * n + b];
@@ -879,19 +895,14 @@ static __device__ __forceinline__ void k_ewfin_body(const EwFinArgs &A, int bid)
         acc[2] += dv * dx * dx; acc[3] += dv * dx * dy; acc[4] += dv * dx * dz;
         acc[5] += dv * dy * dx; acc[6] += dv * dy * dy; acc[7] += dv * dy * dz;
         acc[8] += dv * dz * dx; acc[9] += dv * dz * dy; acc[10] += dv * dz * dz;
-        fx += dx * dv; fy += dy * dv; fz += dz * dv;
+        ex += dx * dv; ey += dy * dv; ez += dz * dv;
       }
-      gx += fx; gy += fy; gz += fz;
+      gx += ex; gy += ey; gz += ez;
     }
     if (A.ew) {
-      double fx = 0., fy = 0., fz = 0.;
-      const size_t nq = A.nq;
-      for (int r = 0; r < A.nrange; r++) {
-        const double *p = A.fpart + 3 * (size_t)r * nq + c;
-        fx += p[0];
-        fy += p[nq];
-        fz += p[2 * nq];
-      }
+      fx = (fx + rs[0][0][lane]) + (rs[1][0][lane] + rs[2][0][lane]);
+      fy = (fy + rs[0][1][lane]) + (rs[1][1][lane] + rs[2][1][lane]);
+      fz = (fz + rs[0][2][lane]) + (rs[1][2][lane] + rs[2][2][lane]);
       const double qc = A.chg[c].w;
       gx += qc * fx; gy += qc * fy; gz += qc * fz;
     }
@@ -991,7 +1002,9 @@ int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_f
   const int n = c->natoms;
   const bool packed = flags & NB_FORCE_PACKED;
   nb_time_mark(c, 0, s);
-  if (c->prec == NB_PREC_MIXED) {
+  if (flags & NB_FORCE_NO_PAIR) {
+    nb_time_mark(c, 1, s);
+  } else if (c->prec == NB_PREC_MIXED) {
     nb_time_mark(c, 1, s);
     int rc = nb_launch_pair_mixed(c, iflg, b, d_fs, d_fl, s, packed);
     if (rc) return rc;
@@ -1023,13 +1036,14 @@ int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_f
   const bool ec = iflg == 1 && c->iperd == 3 && c->necorr > 0;
   const bool ew = iflg == 1 && c->iperd == 3 && c->ktot > 0 && c->ncharge > 0;
   const bool ewf = ew && !(flags & NB_FORCE_EONLY);  // reciprocal-space forces
+  const bool rp = !(flags & NB_FORCE_NO_RECIP), fin = !(flags & NB_FORCE_NO_FIN);
   const int nq = c->ncharge;
   EwChgArgs ch = {c->d_pos, c->d_qch, c->d_qidx, nq, n, b, (double4 *)c->d_chg,
                   (double2 *)c->d_stp};
-  NB_GO(c, K_EWCHG, k_ewchg, ew && !packed ? blocks(nq, 256) : 0, 256, 0, ch, s);
+  NB_GO(c, K_EWCHG, k_ewchg, rp && ew && !packed ? blocks(nq, 256) : 0, 256, 0, ch, s);
   EwSfacArgs sf = {c->d_sseg, (const double4 *)c->d_chg, (const double2 *)c->d_stp, nq, c->qc,
                    c->nseg, c->nsw, c->ktot, (double2 *)c->d_spart};
-  NB_GO(c, K_EWSFAC, k_ewsfac, ew ? c->nsw * c->nchunk : 0, 256, 0, sf, s);
+  NB_GO(c, K_EWSFAC, k_ewsfac, rp && ew ? c->nsw * c->nchunk : 0, 256, 0, sf, s);
   double det = b.h[0] * (b.h[4] * b.h[8] - b.h[5] * b.h[7]) +
                b.h[1] * (b.h[5] * b.h[6] - b.h[3] * b.h[8]) +
                b.h[2] * (b.h[3] * b.h[7] - b.h[4] * b.h[6]);
@@ -1037,16 +1051,16 @@ int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_f
   // one workgroup per 64 k
   EwKredArgs kr = {c->d_kabc, c->ktot, (const double2 *)c->d_spart, c->nchunk, c->new_blocks,
                    b, 1. / det, falp2, (KData *)c->d_kd, c->d_eew};
-  NB_GO(c, K_EWKRED, k_ewkred, ew ? c->new_blocks : 0, 256, 0, kr, s);
+  NB_GO(c, K_EWKRED, k_ewkred, rp && ew ? c->new_blocks : 0, 256, 0, kr, s);
   EwForceArgs ef = {c->d_seg, c->d_rtile, c->d_tseg, (const KData *)c->d_kd,
                     (const double4 *)c->d_chg, (const double2 *)c->d_stp, nq, c->nqb, c->d_fpart,
                     b};
-  NB_GO(c, K_EWFORCE, k_ewforce, ewf ? c->nqb * c->nrange : 0, 256, 0, ef, s);
+  NB_GO(c, K_EWFORCE, k_ewforce, rp && ewf ? c->nqb * c->nrange : 0, 256, 0, ef, s);
   EwFinArgs fa = {c->d_fpart, (const double4 *)c->d_chg, c->d_qidx, nq, n, c->nrange, ewf, ec,
                   c->d_eoff, c->d_elist, c->d_pos, c->d_qch, c->kappa, d_fl, c->nfin_blocks,
                   c->d_eec};
-  NB_GO(c, K_EWFIN, k_ewfin, (ewf || ec) ? c->nfin_blocks : 0, 256, 0, fa, s);
-  if (!(flags & NB_FORCE_NORED)) {
+  NB_GO(c, K_EWFIN, k_ewfin, fin && (ewf || ec) ? c->nfin_blocks : 0, 256, 0, fa, s);
+  if (fin && !(flags & NB_FORCE_NORED)) {
     RedSet R = nb_red_set(c, iflg);
     NB_GO(c, K_RED, k_red, R.ncols, 256, 0, R, s);
   }

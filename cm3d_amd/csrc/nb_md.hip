@@ -22,6 +22,7 @@
 // NVT (a Nose-Hoover chain, cm3d's intnhc at every inner substep) splits the
 // inner loop at each chain update, which needs the kinetic sum of the whole
 // system: k_md_nhc launches 0..nres_intra replace k_md_inner (see there).
+#include <stdlib.h>
 #include <string.h>
 
 #include <vector>
@@ -53,6 +54,7 @@ struct nb_md {
   // Nose-Hoover chain (nchain 0: NVE).  d_nhc: veta[nc], eta[nc], gkt[nc],
   // mnh[nc]; d_ppart: per-workgroup kinetic sums; d_ticket: last-workgroup
   // counter (zero between launches)
+  int fork;      // run the reciprocal sum on the context's second stream
   int nchain;
   double *d_nhc, *d_ppart;
   int *d_ithm, *d_ticket;
@@ -674,6 +676,7 @@ static void md_free_nhc(nb_md *md) {
 extern "C" void nb_md_destroy(nb_md *md) {
   if (!md) return;
   hipSetDevice(md->c->dev);
+  hipDeviceSynchronize();  // no step of this state may still be running
   if (md->have_graph) { hipGraphExecDestroy(md->gexec); hipGraphDestroy(md->graph); }
   md->c->dev_bytes -= md->bytes;
   md->c->dev_bytes -= md->nhc_bytes;
@@ -750,6 +753,7 @@ extern "C" int nb_md_create(nb_ctx *c, const nb_md_params *p, nb_md **out) {
   memset(md, 0, sizeof(*md));
   md->c = c; md->n = n; md->nmol = nm; md->nbond = p->nbond; md->nbend = p->nbend;
   md->maxa = maxa;
+  md->fork = getenv("NB_MD_NOFORK") ? 0 : 1;
   md->ninnra = p->nres_intra > 0 ? p->nres_intra : 1;
   // md_step.c:73-78 with ninter = ninrra = 1
   md->dt = p->dt;
@@ -790,6 +794,9 @@ extern "C" int nb_md_set_state(nb_md *md, const double *pos, const double *vel,
   if (!md || !pos || !vel || !hmat) { nb_set_error("nb_md_set_state: null argument"); return NB_ERR_ARG; }
   nb_ctx *c = md->c;
   MCHK(hipSetDevice(c->dev));
+  // steps may still run on any stream (a batch's, a caller's): wait for the
+  // device before overwriting the state and dropping the captured graph
+  MCHK(hipDeviceSynchronize());
   MCHK(hipMemcpy(c->d_pos, pos, sizeof(double) * 3 * md->n, hipMemcpyHostToDevice));
   MCHK(hipMemcpy(md->d_vel, vel, sizeof(double) * 3 * md->n, hipMemcpyHostToDevice));
   memcpy(md->hmat, hmat, sizeof(md->hmat));
@@ -801,7 +808,7 @@ extern "C" int nb_md_get_state(nb_md *md, double *pos, double *vel) {
   if (!md) { nb_set_error("nb_md_get_state: null argument"); return NB_ERR_ARG; }
   nb_ctx *c = md->c;
   MCHK(hipSetDevice(c->dev));
-  MCHK(hipStreamSynchronize(c->stream));
+  MCHK(hipDeviceSynchronize());  // the step may have run on another stream
   if (pos) MCHK(hipMemcpy(pos, c->d_pos, sizeof(double) * 3 * md->n, hipMemcpyDeviceToHost));
   if (vel) MCHK(hipMemcpy(vel, md->d_vel, sizeof(double) * 3 * md->n, hipMemcpyDeviceToHost));
   return NB_OK;
@@ -916,10 +923,34 @@ int md_enqueue_step(nb_md *md, hipStream_t s) {
       NB_GO(c, K_MDNHC, k_mdnhc, nblk, MD_TPB, lds, N, s);
     }
   }
-  int rc = nb_launch_build_cond(c, md->box, s);
-  if (rc) return rc;
-  rc = nb_launch_force(c, 1, md->box, nullptr, c->d_fl, s, NB_FORCE_PACKED | NB_FORCE_NORED);
-  if (rc) return rc;
+  // the reciprocal sum needs only the charge records the inner kernel just
+  // wrote, not the lists: it runs on a second stream beside the conditional
+  // list build and the pair kernel (a fork / join in the captured graph),
+  // and k_ewfin joins the two.  A batch records the launches in one
+  // sequence (its reciprocal sum runs after the pair kernel).
+  const int fp = NB_FORCE_PACKED | NB_FORCE_NORED;
+  int rc;
+  if (md->fork && !c->rec) {
+    MCHK(hipEventRecord(c->fork_ev[0], s));
+    MCHK(hipStreamWaitEvent(c->stream2, c->fork_ev[0], 0));
+    rc = nb_launch_force(c, 1, md->box, nullptr, c->d_fl, c->stream2,
+                         fp | NB_FORCE_NO_PAIR | NB_FORCE_NO_FIN);
+    if (rc) return rc;
+    MCHK(hipEventRecord(c->fork_ev[1], c->stream2));
+    rc = nb_launch_build_cond(c, md->box, s);
+    if (rc) return rc;
+    rc = nb_launch_force(c, 1, md->box, nullptr, c->d_fl, s,
+                         fp | NB_FORCE_NO_RECIP | NB_FORCE_NO_FIN);
+    if (rc) return rc;
+    MCHK(hipStreamWaitEvent(s, c->fork_ev[1], 0));
+    rc = nb_launch_force(c, 1, md->box, nullptr, c->d_fl, s, fp | NB_FORCE_NO_PAIR | NB_FORCE_NO_RECIP);
+    if (rc) return rc;
+  } else {
+    rc = nb_launch_build_cond(c, md->box, s);
+    if (rc) return rc;
+    rc = nb_launch_force(c, 1, md->box, nullptr, c->d_fl, s, fp);
+    if (rc) return rc;
+  }
   // kick + energy-column reductions + flag reset
   const int nkick = (3 * md->n + 255) / 256;
   KickArgs K = {md->d_vel, c->d_fl, md->d_rm, md->n, nkick, md->dter, c->d_flags + 1,

@@ -73,7 +73,7 @@ def estimate_bytes(sysd):
                 nseg += 1
                 run = 1
     nsw = max(1, _ceil(nseg, 64))
-    nch = max(1, min(_ceil(512, nsw), _ceil(nq, 32)))
+    nch = max(1, min(_ceil(4096, nsw), _ceil(nq, 32)))
     nrange = max(1, min(_ceil(1024, max(1, _ceil(nq, 512))), max(1, nseg)))
     per_atom = 4 * cap + 420           # lists + positions, forces, records, MD state
     ewald = nq * (64 + 24 * nrange) + ktot * (16 * nch + 96) + nseg * 48
