@@ -179,7 +179,7 @@ def run_batch(nb, S, a, torch):
     ctxs, mds, syss = [], [], []
     for k, nat in enumerate(sizes):
         s = S.mixture(int(nat), seed=100 + k, rc=9.0, kmax=7)
-        c = nb.NBContext(s, precision="mixed")
+        c = nb.NBContext(s, precision="mixed", sharing=a.nsys)
         m = nb.NBMD(c, s, DT_PS, nres_intra=1)
         m.set_state(s["pos"], s["vel"], s["hmat"])
         m.init()

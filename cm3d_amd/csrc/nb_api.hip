@@ -291,13 +291,17 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
   // for ~NB_SFAC_WG workgroups, ~16 a CU, which the long charge loops need
   // to hide FP64 latency (measured on the 98,304-atom water: 4096 -> 7.3 ms,
   // 512 -> 11.5 ms for the whole reciprocal sum)
+  // a system sharing the GPU with others gets a proportionally coarser split
+  const int share = std::max(1, p->sharing);
+  const int sfac_wg = std::max(1, NB_SFAC_WG / share);
+  const int ewf_wg = std::max(1, NB_EWF_WG / share);
   std::vector<EwSeg> sseg(seg);
   std::stable_sort(sseg.begin(), sseg.end(),
                    [](const EwSeg &a, const EwSeg &b) { return a.cnt > b.cnt; });
   c->nsw = std::max(1, (c->nseg + NB_WAVE - 1) / NB_WAVE);
   {
     int maxch = std::max(1, (ncharge + 31) / 32);  // >= 32 charges a workgroup
-    int nch = std::min(std::max(1, (NB_SFAC_WG + c->nsw - 1) / c->nsw), maxch);
+    int nch = std::min(std::max(1, (sfac_wg + c->nsw - 1) / c->nsw), maxch);
     c->qc = std::max(1, (ncharge + nch - 1) / nch);
     c->nchunk = std::max(1, (ncharge + c->qc - 1) / c->qc);
   }
@@ -309,7 +313,7 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
   c->nqb = std::max(1, (ncharge + 256 * NB_EWF_Q - 1) / (256 * NB_EWF_Q));
   std::vector<int> tseg(1, 0), rtile(1, 0);
   {
-    int nrt = std::min(std::max(1, (NB_EWF_WG + c->nqb - 1) / c->nqb), std::max(1, c->nseg));
+    int nrt = std::min(std::max(1, (ewf_wg + c->nqb - 1) / c->nqb), std::max(1, c->nseg));
     int lim = std::min(NB_EW_KT, std::max(NB_EW_SEG, (c->ktot + nrt - 1) / std::max(1, nrt)));
     int nk = 0;
     std::vector<int> tk;  // k per tile

@@ -51,7 +51,7 @@ class NBParams(C.Structure):
         ("ktot", C.c_int), ("ka", _ip), ("kb", _ip), ("kc", _ip),
         ("kappa", C.c_double), ("ewald_self", C.c_double),
         ("ewald_bkg", C.c_double),
-        ("precision", C.c_int), ("device", C.c_int),
+        ("precision", C.c_int), ("device", C.c_int), ("sharing", C.c_int),
     ]
 
 
@@ -190,12 +190,12 @@ def _scalar(sysd, key, default=0):
     return np.asarray(v).reshape(-1)[0].item() if np.size(v) else default
 
 
-def make_params(sysd, precision="fp64", device=0):
+def make_params(sysd, precision="fp64", device=0, sharing=1):
     """NBParams from a system dict; returns (params, keep-alive list)."""
     p = NBParams()
     keep = []
     for name, typ in NBParams._fields_:
-        if name in ("precision", "device"):
+        if name in ("precision", "device", "sharing"):
             continue
         if typ is C.c_int:
             setattr(p, name, int(_scalar(sysd, name, 0)))
@@ -207,15 +207,19 @@ def make_params(sysd, precision="fp64", device=0):
             setattr(p, name, ptr)
     p.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
     p.device = int(device)
+    p.sharing = int(sharing)
     return p, keep
 
 
 class NBContext:
     """One device context of the nonbonded engine (a cm3d system)."""
 
-    def __init__(self, sysd, precision="fp64", device=0):
+    def __init__(self, sysd, precision="fp64", device=0, sharing=1):
+        """sharing: systems expected to run on the GPU together with this one
+        (a batch's or a server's size); it only sizes the reciprocal sum's
+        work split."""
         self.natoms = int(_scalar(sysd, "natoms"))
-        params, keep = make_params(sysd, precision, device)
+        params, keep = make_params(sysd, precision, device, sharing)
         h = C.c_void_p()
         check(lib().nb_create(C.byref(params), C.byref(h)), "nb_create")
         self._h = h

@@ -45,7 +45,7 @@ def _ceil(a, b):
     return -(-a // b)
 
 
-def estimate_bytes(sysd):
+def estimate_bytes(sysd, sharing=1):
     """Device bytes a job will need, before its context exists, following
     the allocation sizes of nb_create / nb_md_create: the neighbour lists
     (capacity 1.6x the mean full-list length + 64 entries, grown by 1.25x on
@@ -73,8 +73,9 @@ def estimate_bytes(sysd):
                 nseg += 1
                 run = 1
     nsw = max(1, _ceil(nseg, 64))
-    nch = max(1, min(_ceil(4096, nsw), _ceil(nq, 32)))
-    nrange = max(1, min(_ceil(1024, max(1, _ceil(nq, 512))), max(1, nseg)))
+    share = max(1, int(sharing))
+    nch = max(1, min(_ceil(max(1, 4096 // share), nsw), _ceil(nq, 32)))
+    nrange = max(1, min(_ceil(max(1, 1024 // share), max(1, _ceil(nq, 512))), max(1, nseg)))
     per_atom = 4 * cap + 420           # lists + positions, forces, records, MD state
     ewald = nq * (64 + 24 * nrange) + ktot * (16 * nch + 96) + nseg * 48
     ncells = (int(2 * (vol ** (1. / 3.)) / math.sqrt(max(r2, 1e-12))) + 1) ** 3
@@ -85,7 +86,8 @@ class Job:
     """One MD job: a system dict (cm3d_amd.systems), its initial velocities
     and step count, its arrival time (s from the start of the run)."""
 
-    def __init__(self, jid, sysd, nsteps, arrival, nres_intra=1, dt_ps=0.002, vel=None):
+    def __init__(self, jid, sysd, nsteps, arrival, nres_intra=1, dt_ps=0.002, vel=None,
+                 sharing=16):
         self.id = jid
         self.sys = sysd
         self.nsteps = int(nsteps)
@@ -94,7 +96,8 @@ class Job:
         self.dt_ps = float(dt_ps)
         self.vel = sysd["vel"] if vel is None else vel
         self.natoms = int(sysd["natoms"])
-        self.est_bytes = estimate_bytes(sysd)
+        self.sharing = int(sharing)
+        self.est_bytes = estimate_bytes(sysd, self.sharing)
         self.ctx = self.md = None
         self.done = 0
         self.bytes = 0
@@ -102,7 +105,7 @@ class Job:
         self.result = None
 
     def start(self, precision):
-        self.ctx = nb.NBContext(self.sys, precision=precision)
+        self.ctx = nb.NBContext(self.sys, precision=precision, sharing=self.sharing)
         self.md = nb.NBMD(self.ctx, self.sys, self.dt_ps, nres_intra=self.nres_intra)
         self.md.set_state(self.sys["pos"], self.vel, self.sys["hmat"])
         self.md.init()

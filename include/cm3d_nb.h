@@ -87,6 +87,13 @@ typedef struct nb_params {
   double kappa, ewald_self, ewald_bkg;
   int precision;                     /* NB_PREC_*                        */
   int device;                        /* HIP device ordinal               */
+  /* No cm3d counterpart: how many systems are expected to share the GPU
+   * with this one (the members of a batch, a server's slots); 0 or 1 =
+   * alone.  It sizes the work split of the reciprocal sum (fewer, larger
+   * pieces when many systems run together); results change only at
+   * rounding level and stay fixed for the context's lifetime, so a system
+   * steps bit for bit the same alone and in a batch. */
+  int sharing;
 } nb_params;
 
 /* By-products of force(...,1) that getvireal(...,1) consumes. */

@@ -154,3 +154,20 @@ def test_energy_only_equals_force_energies(name, prec):
         for mine, ref in (("spotl", "e0_vinter"), ("spothl", "e0_vhinter")):
             assert _rel(eo[mine], d[ref][0]) < FP64_TOL or abs(eo[mine] - d[ref][0]) < 1e-9
     c.close()
+
+
+def test_sharing_changes_only_the_work_split():
+    """A context expecting to share the GPU (sharing=64) splits the
+    reciprocal sum into fewer, larger pieces; the forces and energies equal
+    the alone context's to rounding and cm3d's to the FP64 bar."""
+    d = load_fixture("water512")
+    from cm3d_amd import nb
+    out = []
+    for share in (1, 64):
+        c = nb.NBContext(d, precision="fp64", sharing=share)
+        c.build_lists(d["pos"], d["hmat"])
+        out.append(c.force(d["pos"], d["hmat"], iflg=1))
+        c.close()
+    assert rel_rms(out[1][1], out[0][1]) < 1e-13
+    assert rel_rms(out[1][1], d["fxl"]) < FP64_TOL
+    assert _rel(out[1][2]["vk_ewald"], out[0][2]["vk_ewald"]) < 1e-13

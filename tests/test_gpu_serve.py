@@ -28,7 +28,7 @@ def _jobs():
 
 def _alone(job, precision):
     from cm3d_amd import nb
-    ctx = nb.NBContext(job.sys, precision=precision)
+    ctx = nb.NBContext(job.sys, precision=precision, sharing=job.sharing)
     md = nb.NBMD(ctx, job.sys, job.dt_ps, nres_intra=job.nres_intra)
     md.set_state(job.sys["pos"], job.vel, job.sys["hmat"])
     md.init()
