@@ -216,6 +216,12 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
   for (int k = 0; k < p->ninter; k++) r2 = std::max(r2, p->rmaxl_skin2[k]);
   if (ncharge > 0) r2 = std::max(r2, p->rmaxl_skin2_e);
   c->rlist = sqrt(r2);
+  {
+    double rp2 = r2;
+    for (int k = 0; k < p->ninter; k++) rp2 = std::max(rp2, p->rmaxs_skin2[k]);
+    if (ncharge > 0) rp2 = std::max(rp2, p->rmaxs_skin2_e);
+    c->rprune2 = rp2 * (1. + 2e-6);
+  }
 
   // exclusions, both directions: partner | bit30 (vdW) | bit31 (coulomb)
   std::vector<std::vector<unsigned>> ex(n);

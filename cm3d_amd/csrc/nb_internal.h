@@ -168,6 +168,7 @@ struct nb_ctx {
   double skin, kappa, ewald_self, ewald_bkg;
   double dx2coul, rmins_coul2, rmaxs_coul2, rminl_coul2, rmaxl_coul2;
   double rmaxs_skin2_e, rmaxl_skin2_e, rlist, clast;
+  double rprune2;  // every list cutoff^2, widened by 2e-6: the column-pruning bound
   // parameters on device
   int *d_itype, *d_map;
   InterPar *d_ip;

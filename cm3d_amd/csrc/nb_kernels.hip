@@ -193,7 +193,8 @@ NB_KERNEL_PAIR(k_pack64, Pack64Args, 256)
 // ---------------------------------------------------------------------------
 #define NB_PCUT_LDS 1024  // type pairs whose list cutoffs are staged in LDS
 struct ListArgs {
-  int n, ntypes, cap, ncx, ncy, ncz, rx, ry, rz, pcut_lds;
+  int n, ntypes, cap, ncx, ncy, ncz, rx, ry, rz, pcut_lds, prune;
+  double rprune2, wcx, wcy, wcz;  // pruning bound^2, cell widths (A) of an orthorhombic box
   const double4 *rec;
   const int *stype, *perm, *scell, *cstart, *ccount;
   const int *xoff, *xlist;
@@ -237,18 +238,48 @@ static __device__ __forceinline__ void k_list_body(const ListArgs &A, int bid) {
   const int wy = 2 * A.ry + 1, ncol = (2 * A.rx + 1) * wy;
   int rs0 = 0, rl0 = 0, rs1 = 0, rl1 = 0;
   if (lane < ncol) {
-    const int xx = wrapc(cx + lane / wy - A.rx, A.ncx), yy = wrapc(cy + lane % wy - A.ry, A.ncy);
+    const int ox = lane / wy - A.rx, oy = lane % wy - A.ry;
+    const int xx = wrapc(cx + ox, A.ncx), yy = wrapc(cy + oy, A.ncy);
     const int base = (xx * A.ncy + yy) * A.ncz;
-    const int zlo = cz - A.rz, zhi = cz + A.rz;
-    const int a0 = zlo < 0 ? 0 : zlo, a1 = zhi > A.ncz - 1 ? A.ncz - 1 : zhi;
-    rs0 = A.cstart[base + a0];
-    rl0 = A.cstart[base + a1] + A.ccount[base + a1] - rs0;
-    int b0 = -1, b1 = -1;
-    if (zlo < 0) { b0 = A.ncz + zlo; b1 = A.ncz - 1; }
-    else if (zhi > A.ncz - 1) { b0 = 0; b1 = zhi - A.ncz; }
-    if (b0 >= 0) {
-      rs1 = A.cstart[base + b0];
-      rl1 = A.cstart[base + b1] + A.ccount[base + b1] - rs1;
+    int zlo = cz - A.rz, zhi = cz + A.rz;
+    if (A.prune) {
+      // orthorhombic box: the column's nearest point lies gx, gy away in x,
+      // y; columns beyond every list cutoff are dropped and the rest keep
+      // only the cells within reach in z.  The bound is the list cutoff
+      // widened by 1e-6, so no pair the exact test below would keep is lost.
+      const double *hi = A.b.hi;
+      double sx = ri.x * hi[0], sy = ri.y * hi[4], sz = ri.z * hi[8];
+      sx -= nb_anint(sx);
+      sy -= nb_anint(sy);
+      sz -= nb_anint(sz);
+      const double ux = (sx + .5) * A.ncx, uy = (sy + .5) * A.ncy, uz = (sz + .5) * A.ncz;
+      double gx = ox > 0 ? (double)(cx + ox) - ux : (ox < 0 ? ux - (double)(cx + ox + 1) : 0.);
+      double gy = oy > 0 ? (double)(cy + oy) - uy : (oy < 0 ? uy - (double)(cy + oy + 1) : 0.);
+      gx = gx > 0. ? gx * A.wcx : 0.;
+      gy = gy > 0. ? gy * A.wcy : 0.;
+      const double rest = A.rprune2 - (gx * gx + gy * gy);
+      if (rest < 0.) {
+        zhi = zlo - 1;
+      } else {
+        const double dz = sqrt(rest) / A.wcz;
+        const int z0 = (int)floor(uz - dz), z1 = (int)floor(uz + dz);
+        zlo = z0 > zlo ? z0 : zlo;
+        zhi = z1 < zhi ? z1 : zhi;
+      }
+    }
+    if (zlo <= zhi) {
+      const int a0 = zlo < 0 ? 0 : zlo, a1 = zhi > A.ncz - 1 ? A.ncz - 1 : zhi;
+      if (a0 <= a1) {
+        rs0 = A.cstart[base + a0];
+        rl0 = A.cstart[base + a1] + A.ccount[base + a1] - rs0;
+      }
+      int b0 = -1, b1 = -1;
+      if (zlo < 0) { b0 = A.ncz + zlo; b1 = A.ncz - 1; }
+      else if (zhi > A.ncz - 1) { b0 = 0; b1 = zhi - A.ncz; }
+      if (b0 >= 0) {
+        rs1 = A.cstart[base + b0];
+        rl1 = A.cstart[base + b1] + A.ccount[base + b1] - rs1;
+      }
     }
   }
   const double2 *pcut = A.pcut + ti * A.ntypes;
@@ -949,6 +980,10 @@ static int launch_build_impl(nb_ctx *c, const Box &b, hipStream_t s, const int *
   A.n = n; A.ntypes = c->ntypes; A.cap = c->cap;
   A.ncx = c->ncx; A.ncy = c->ncy; A.ncz = c->ncz; A.rx = c->crx; A.ry = c->cry; A.rz = c->crz;
   A.pcut_lds = c->ntypes * c->ntypes <= NB_PCUT_LDS;
+  // column pruning needs cells that are boxes in Cartesian space
+  A.prune = b.ortho && c->ncx > 1 && c->ncy > 1 && c->ncz > 1;
+  A.rprune2 = c->rprune2;
+  A.wcx = b.h[0] / c->ncx; A.wcy = b.h[4] / c->ncy; A.wcz = b.h[8] / c->ncz;
   A.rec = (const double4 *)c->d_rec64; A.stype = c->d_stype; A.perm = c->d_perm;
   A.scell = c->d_cell; A.cstart = c->d_cstart; A.ccount = c->d_ccount;
   A.xoff = c->d_xoff; A.xlist = c->d_xlist; A.pcut = (const double2 *)c->d_pcut;
