@@ -628,44 +628,36 @@ struct EwSfacArgs {
 #define EW_SFAC_LDS (2 * 2 * NB_EW_SEG * NB_WAVE)
 #define EW_QT (EW_SFAC_LDS / 6)  // charges per tile
 // one charge's contribution to a lane's segment: start phase by sincos, then
-// NB_EW_SEG steps of the c-axis recursion (steps past the lane's own count
-// accumulate into slots that are never stored)
-#define EW_SFAC_CHARGE(T)                                                     \
-  {                                                                           \
-    const double *cq = lds + 6 * (T);                                         \
-    const double q = cq[3], stc = cq[4], sts = cq[5];                         \
-    double sn, cs;                                                            \
-    nb_sincos(ka * cq[0] + kb * cq[1] + kc0 * cq[2], &sn, &cs);                \
-    _Pragma("unroll") for (int m = 0; m < NB_EW_SEG; m++) {                   \
-      if (m >= mmax) break;                                                   \
-      re[m] += q * cs;                                                        \
-      im[m] += q * sn;                                                        \
-      const double c2 = cs * stc - sn * sts;                                  \
-      sn = sn * stc + cs * sts;                                               \
-      cs = c2;                                                                \
-    }                                                                         \
+// M steps of the c-axis recursion (M = the wave's longest segment; steps past
+// the lane's own count accumulate into slots that are never stored).  M is a
+// template parameter so that every accumulator index is static and re / im
+// stay in registers.
+template <int M>
+static __device__ __forceinline__ void sfac_charge(const double *cq, double ka, double kb,
+                                                   double kc0, double (&re)[NB_EW_SEG],
+                                                   double (&im)[NB_EW_SEG]) {
+  const double q = cq[3], stc = cq[4], sts = cq[5];
+  double sn, cs;
+  nb_sincos(ka * cq[0] + kb * cq[1] + kc0 * cq[2], &sn, &cs);
+#pragma unroll
+  for (int m = 0; m < M; m++) {
+    re[m] = __builtin_fma(q, cs, re[m]);
+    im[m] = __builtin_fma(q, sn, im[m]);
+    if (m + 1 < M) {
+      const double c2 = __builtin_fma(cs, stc, -(sn * sts));
+      sn = __builtin_fma(sn, stc, cs * sts);
+      cs = c2;
+    }
   }
-static __device__ __forceinline__ void k_ewsfac_body(const EwSfacArgs &A, int bid) {
-#pragma clang fp contract(fast)
-  // block = 4 waves on the same 64 segments; the block's chunk of charges is
-  // staged through LDS in tiles and each wave takes every 4th charge; the 4
-  // waves' sums are then combined in LDS in a fixed order
-  __shared__ double lds[EW_SFAC_LDS];
-  static_assert(6 * EW_QT <= EW_SFAC_LDS, "charge tile exceeds the LDS buffer");
+}
+template <int M>
+static __device__ __forceinline__ void sfac_run(const EwSfacArgs &A, double *lds, int chunk,
+                                                int k0, int cnt, double ka, double kb,
+                                                double kc0) {
   const int lane = lane_id(), w = threadIdx.x >> 6;
-  const int sw = bid % A.nsw, chunk = bid / A.nsw;
-  const int g = sw * NB_WAVE + lane;
-  int k0 = 0, cnt = 0;
-  double ka = 0., kb = 0., kc0 = 0.;
-  if (g < A.nseg) {
-    const EwSeg sg = A.sseg[g];
-    k0 = sg.k0; cnt = sg.cnt; ka = sg.ka; kb = sg.kb; kc0 = sg.kc0;
-  }
-  // lane 0 (always a real segment) has the wave's longest
-  const int mmax = __builtin_amdgcn_readfirstlane(cnt);
   double re[NB_EW_SEG], im[NB_EW_SEG];
 #pragma unroll
-  for (int m = 0; m < NB_EW_SEG; m++) { re[m] = 0.; im[m] = 0.; }
+  for (int m = 0; m < M; m++) { re[m] = 0.; im[m] = 0.; }
   const int c0 = chunk * A.qc, c1 = min(A.nq, c0 + A.qc);
   for (int t0 = c0; t0 < c1; t0 += EW_QT) {
     const int nt = min(EW_QT, c1 - t0);
@@ -679,18 +671,17 @@ static __device__ __forceinline__ void k_ewsfac_body(const EwSfacArgs &A, int bi
     __syncthreads();
     int t = w;
     for (; t + 4 < nt; t += 8) {  // two charges per trip for ILP
-      EW_SFAC_CHARGE(t)
-      EW_SFAC_CHARGE(t + 4)
+      sfac_charge<M>(lds + 6 * t, ka, kb, kc0, re, im);
+      sfac_charge<M>(lds + 6 * (t + 4), ka, kb, kc0, re, im);
     }
-    if (t < nt) EW_SFAC_CHARGE(t)
+    if (t < nt) sfac_charge<M>(lds + 6 * t, ka, kb, kc0, re, im);
   }
   // (w0 + w2) + (w1 + w3), through the same LDS
   double (*red)[2 * NB_EW_SEG][NB_WAVE] = (double (*)[2 * NB_EW_SEG][NB_WAVE])lds;
   __syncthreads();
   if (w >= 2) {
 #pragma unroll
-    for (int m = 0; m < NB_EW_SEG; m++) {
-      if (m >= mmax) break;
+    for (int m = 0; m < M; m++) {
       red[w - 2][2 * m][lane] = re[m];
       red[w - 2][2 * m + 1][lane] = im[m];
     }
@@ -698,8 +689,7 @@ static __device__ __forceinline__ void k_ewsfac_body(const EwSfacArgs &A, int bi
   __syncthreads();
   if (w < 2) {
 #pragma unroll
-    for (int m = 0; m < NB_EW_SEG; m++) {
-      if (m >= mmax) break;
+    for (int m = 0; m < M; m++) {
       re[m] += red[w][2 * m][lane];
       im[m] += red[w][2 * m + 1][lane];
     }
@@ -707,8 +697,7 @@ static __device__ __forceinline__ void k_ewsfac_body(const EwSfacArgs &A, int bi
   __syncthreads();
   if (w == 1) {
 #pragma unroll
-    for (int m = 0; m < NB_EW_SEG; m++) {
-      if (m >= mmax) break;
+    for (int m = 0; m < M; m++) {
       red[0][2 * m][lane] = re[m];
       red[0][2 * m + 1][lane] = im[m];
     }
@@ -717,11 +706,39 @@ static __device__ __forceinline__ void k_ewsfac_body(const EwSfacArgs &A, int bi
   if (w == 0) {
     double2 *out = A.part + (size_t)chunk * A.ktot + k0;
 #pragma unroll
-    for (int m = 0; m < NB_EW_SEG; m++)
+    for (int m = 0; m < M; m++)
       if (m < cnt) out[m] = make_double2(re[m] + red[0][2 * m][lane], im[m] + red[0][2 * m + 1][lane]);
   }
 }
-#undef EW_SFAC_CHARGE
+static __device__ __forceinline__ void k_ewsfac_body(const EwSfacArgs &A, int bid) {
+  // block = 4 waves on the same 64 segments; the block's chunk of charges is
+  // staged through LDS in tiles and each wave takes every 4th charge; the 4
+  // waves' sums are then combined in LDS in a fixed order
+  __shared__ double lds[EW_SFAC_LDS];
+  static_assert(6 * EW_QT <= EW_SFAC_LDS, "charge tile exceeds the LDS buffer");
+  static_assert(NB_EW_SEG == 16, "the dispatch below covers segments of 1..16 k vectors");
+  const int lane = lane_id();
+  const int sw = bid % A.nsw, chunk = bid / A.nsw;
+  const int g = sw * NB_WAVE + lane;
+  int k0 = 0, cnt = 0;
+  double ka = 0., kb = 0., kc0 = 0.;
+  if (g < A.nseg) {
+    const EwSeg sg = A.sseg[g];
+    k0 = sg.k0; cnt = sg.cnt; ka = sg.ka; kb = sg.kb; kc0 = sg.kc0;
+  }
+  // lane 0 (always a real segment) has the longest; the 4 waves of the block
+  // share the segments, so the trip count is uniform over the block
+  switch (__builtin_amdgcn_readfirstlane(cnt)) {
+#define EW_SFAC_CASE(M) \
+  case M: sfac_run<M>(A, lds, chunk, k0, cnt, ka, kb, kc0); break;
+    EW_SFAC_CASE(1) EW_SFAC_CASE(2) EW_SFAC_CASE(3) EW_SFAC_CASE(4)
+    EW_SFAC_CASE(5) EW_SFAC_CASE(6) EW_SFAC_CASE(7) EW_SFAC_CASE(8)
+    EW_SFAC_CASE(9) EW_SFAC_CASE(10) EW_SFAC_CASE(11) EW_SFAC_CASE(12)
+    EW_SFAC_CASE(13) EW_SFAC_CASE(14) EW_SFAC_CASE(15) EW_SFAC_CASE(16)
+#undef EW_SFAC_CASE
+    default: break;
+  }
+}
 NB_KERNEL_PAIR(k_ewsfac, EwSfacArgs, 256)
 
 // per k: 2 preg S(k), the amplitude the force pass needs
