@@ -3,9 +3,12 @@
     python -m cm3d_amd.build
 
 No GPU is needed to build: hipcc cross-compiles for gfx950.  Flags:
-  -ffp-contract=off  no FMA contraction, so the FP64 minimum image, r^2 and
-                     table interpolation round exactly as cm3d's x86-64 build
-                     (list membership is decided bit for bit on them);
+  -ffp-contract=off  no FMA contraction by default, so the FP64 minimum
+                     image, r^2, cut-off tests and table index round exactly
+                     as cm3d's x86-64 build (list membership and the table
+                     node of a pair are decided bit for bit on them); the
+                     interpolation, the sums and the Ewald kernels opt in to
+                     FMA with `#pragma clang fp contract(fast)`;
   -Xclang -target-feature -Xclang -packed-fp32-ops   no packed FP32 ops.
 """
 import os

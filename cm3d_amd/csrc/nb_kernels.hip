@@ -597,8 +597,8 @@ NB_KERNEL_PAIR_W(k_pair64_1f, PairArgs, 256, 4)
 //   order, tile by tile (a segment that continues its row continues the
 //   recursion), summing k (Re S' sin - Im S' cos) (md_ewald.c:303-323) into
 //   a per-range partial.
-// k_ewfadd: per charge, the range partials in fixed order, times q, added
-//   to fl.
+// k_ewfin: per charge, the range partials in fixed order, times q, added
+//   to fl, with the excluded-pair correction (ecorr).
 // ---------------------------------------------------------------------------
 template <typename T>
 using cptr4 = __attribute__((address_space(4))) const T *;
@@ -809,6 +809,31 @@ struct EwForceArgs {
   const double2 *stp; int nq, nqb; double *fpart; Box b;
 };
 #define EWF_Q NB_EWF_Q
+// the M k vectors of one segment against the lane's charges: s0 = sum tt,
+// s1 = sum m tt, and the phases advanced M steps (a segment that continues
+// its row picks them up).  M is a template parameter: the loop unrolls fully,
+// the k data loads are issued ahead and the phases rotate without copies.
+template <int M>
+static __device__ __forceinline__ void ewf_segment(const double2 *K, const double2 (&st)[EWF_Q],
+                                                   double (&sn)[EWF_Q], double (&cs)[EWF_Q],
+                                                   double (&s0)[EWF_Q], double (&s1)[EWF_Q]) {
+#pragma clang fp contract(fast)
+  static_assert(NB_EW_SEG == 16, "the dispatch covers segments of 1..16 k vectors");
+#pragma unroll
+  for (int m = 0; m < M; m++) {
+    const double2 kk = K[m];
+    const double fm = m;
+#pragma unroll
+    for (int u = 0; u < EWF_Q; u++) {
+      const double tt = kk.x * sn[u] - kk.y * cs[u];
+      s0[u] += tt;
+      s1[u] += fm * tt;
+      const double c2 = cs[u] * st[u].x - sn[u] * st[u].y;
+      sn[u] = sn[u] * st[u].x + cs[u] * st[u].y;
+      cs[u] = c2;
+    }
+  }
+}
 static __device__ __forceinline__ void k_ewforce_body(const EwForceArgs &A, int bid) {
 #pragma clang fp contract(fast)
   __shared__ double2 kl[NB_EW_KT];  // {2 preg Re S, 2 preg Im S} per k of the tile
@@ -853,18 +878,15 @@ static __device__ __forceinline__ void k_ewforce_body(const EwForceArgs &A, int 
 #pragma unroll
       for (int u = 0; u < EWF_Q; u++) { s0[u] = 0.; s1[u] = 0.; }
       const double2 *K = kl + (seg[g].k0 - kb0);
-      for (int m = 0; m < cnt; m++) {
-        const double2 kk = K[m];
-        const double fm = m;
-#pragma unroll
-        for (int u = 0; u < EWF_Q; u++) {
-          const double tt = kk.x * sn[u] - kk.y * cs[u];
-          s0[u] += tt;
-          s1[u] += fm * tt;
-          const double c2 = cs[u] * st[u].x - sn[u] * st[u].y;
-          sn[u] = sn[u] * st[u].x + cs[u] * st[u].y;
-          cs[u] = c2;
-        }
+      switch (cnt) {  // uniform: the segment is the workgroup's
+#define EWF_CASE(M) \
+  case M: ewf_segment<M>(K, st, sn, cs, s0, s1); break;
+        EWF_CASE(1) EWF_CASE(2) EWF_CASE(3) EWF_CASE(4)
+        EWF_CASE(5) EWF_CASE(6) EWF_CASE(7) EWF_CASE(8)
+        EWF_CASE(9) EWF_CASE(10) EWF_CASE(11) EWF_CASE(12)
+        EWF_CASE(13) EWF_CASE(14) EWF_CASE(15) EWF_CASE(16)
+#undef EWF_CASE
+        default: break;
       }
       // k = A + kc Gc with kc = kc0 + m: sum k tt = A s0 + Gc (kc0 s0 + s1)
       const double ax = tpi * (ka * hi[0] + kb * hi[3]);

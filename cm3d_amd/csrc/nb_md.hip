@@ -753,7 +753,7 @@ extern "C" int nb_md_create(nb_ctx *c, const nb_md_params *p, nb_md **out) {
   memset(md, 0, sizeof(*md));
   md->c = c; md->n = n; md->nmol = nm; md->nbond = p->nbond; md->nbend = p->nbend;
   md->maxa = maxa;
-  md->fork = getenv("NB_MD_NOFORK") ? 0 : 1;
+  md->fork = 1;
   md->ninnra = p->nres_intra > 0 ? p->nres_intra : 1;
   // md_step.c:73-78 with ninter = ninrra = 1
   md->dt = p->dt;
