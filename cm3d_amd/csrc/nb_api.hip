@@ -307,7 +307,13 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
   c->nsw = std::max(1, (c->nseg + NB_WAVE - 1) / NB_WAVE);
   {
     int maxch = std::max(1, (ncharge + 31) / 32);  // >= 32 charges a workgroup
-    int nch = std::min(std::max(1, (sfac_wg + c->nsw - 1) / c->nsw), maxch);
+    int nch = std::max(1, (sfac_wg + c->nsw - 1) / c->nsw);
+    // in a batch, workgroups of every system should take about the same
+    // time: cap a workgroup's k x charge products
+    if (share > 1)
+      nch = std::max(nch, (int)(((long long)c->ktot * ncharge + (long long)NB_SFAC_WORK * c->nsw - 1) /
+                                ((long long)NB_SFAC_WORK * c->nsw)));
+    nch = std::min(nch, maxch);
     c->qc = std::max(1, (ncharge + nch - 1) / nch);
     c->nchunk = std::max(1, (ncharge + c->qc - 1) / c->qc);
   }
@@ -319,7 +325,10 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
   c->nqb = std::max(1, (ncharge + 256 * NB_EWF_Q - 1) / (256 * NB_EWF_Q));
   std::vector<int> tseg(1, 0), rtile(1, 0);
   {
-    int nrt = std::min(std::max(1, (ewf_wg + c->nqb - 1) / c->nqb), std::max(1, c->nseg));
+    int nrt = std::max(1, (ewf_wg + c->nqb - 1) / c->nqb);
+    if (share > 1)  // as for the structure factor: a bounded k x charge product per workgroup
+      nrt = std::max(nrt, (int)(((long long)c->ktot * 256 * NB_EWF_Q + NB_EWF_WORK - 1) / NB_EWF_WORK));
+    nrt = std::min(nrt, std::max(1, c->nseg));
     int lim = std::min(NB_EW_KT, std::max(NB_EW_SEG, (c->ktot + nrt - 1) / std::max(1, nrt)));
     int nk = 0;
     std::vector<int> tk;  // k per tile
@@ -383,8 +392,9 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
   if ((rc = dalloc(c, &c->d_fpart, 3 * (size_t)c->nrange * std::max(ncharge, 1)))) return fail(rc);
   // pair kernels: a wave per atom in turn; the FP32 kernel gives each wave
   // about 4 atoms so its once-per-wave energy reduction is amortised
-  c->npair_blocks = p->precision == NB_PREC_MIXED ? std::max(1, std::min((n + 15) / 16, 2048))
-                                                  : std::min((n + 3) / 4, 2048);
+  const int apw = share >= 4 ? NB_MIX_ATOMS_WG_SHARED : NB_MIX_ATOMS_WG;
+  c->npair_blocks = p->precision == NB_PREC_MIXED ? std::max(1, std::min((n + apw - 1) / apw, 2048))
+                                                  : std::min((n + 3) / 4, NB_PAIR64_MAX_WG);
   c->new_blocks = std::max(1, (c->ktot + NB_WAVE - 1) / NB_WAVE);
   c->nfin_blocks = std::max(1, (ncharge + NB_WAVE - 1) / NB_WAVE);  // 64 charges a workgroup
   if ((rc = dalloc(c, &c->d_epair, (size_t)c->npair_blocks * NB_NE_PAIR))) return fail(rc);
@@ -505,6 +515,7 @@ int nb_build_now(nb_ctx *c, const double hmat[9], hipStream_t s) {
     double nexp = 4.18879 * c->rlist * c->rlist * c->rlist * c->natoms / vol;
     int want = (int)(1.6 * nexp) + 64;
     if (want > c->natoms) want = c->natoms + 63;
+    c->mix_g16 = nexp < NB_MIX_G16_MAX;
     RC(ensure_capacity(c, b, want));
   }
   for (int attempt = 0; attempt < 3; attempt++) {

@@ -44,6 +44,24 @@ struct LaunchRec;
 #ifndef NB_SFAC_WG
 #define NB_SFAC_WG 4096  // workgroups the structure-factor pass aims for
 #endif
+#ifndef NB_EWF_WORK
+#define NB_EWF_WORK 65536  // k x charge products per force workgroup in a shared context
+#endif
+#ifndef NB_SFAC_WORK
+#define NB_SFAC_WORK 65536  // the same for a structure-factor workgroup
+#endif
+#ifndef NB_MIX_G16_MAX
+#define NB_MIX_G16_MAX 256  // mixed pair kernel: 16 lanes per atom below this mean list length
+#endif
+#ifndef NB_MIX_ATOMS_WG
+#define NB_MIX_ATOMS_WG 16   // atoms per workgroup of the mixed pair kernel
+#endif
+#ifndef NB_MIX_ATOMS_WG_SHARED
+#define NB_MIX_ATOMS_WG_SHARED 64  // the same for a context with sharing >= 4
+#endif
+#ifndef NB_PAIR64_MAX_WG
+#define NB_PAIR64_MAX_WG 2048  // workgroup cap of the FP64 pair kernel
+#endif
 #define NB_NE_PAIR 19    // pair energy/virial accumulators + clamp count
 #define NB_NE_ECORR 11   // ecorr energy/virial accumulators
 #define NB_NE_EW 11      // Ewald accumulators
@@ -126,6 +144,15 @@ __device__ __forceinline__ double nb_readlane(double v, int l) {
 __device__ __forceinline__ float nb_readlane(float v, int l) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
+// sum over the 16 lanes of a DPP row; every lane of the row gets it
+template <typename T>
+__device__ __forceinline__ T nb_row_sum(T v) {
+  v += nb_dpp<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += nb_dpp<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += nb_dpp<0x124>(v);  // row_ror:4
+  v += nb_dpp<0x128>(v);  // row_ror:8
+  return v;
+}
 template <typename T>
 __device__ __forceinline__ T nb_wave_sum(T v) {
   v += nb_dpp<0xB1>(v);   // quad_perm [1,0,3,2]
@@ -202,6 +229,7 @@ struct nb_ctx {
   int nrange, nqb, ntile;      // force: tile ranges, charge workgroups, tiles
   double *d_epair, *d_eew, *d_eec, *d_eout;
   int npair_blocks, new_blocks, nfin_blocks;
+  int mix_g16;     // mixed pair kernel: 16 lanes per atom (short lists)
   long long *d_fxp;            // fixed-point force accumulators (mixed)
   float4 *d_recf;              // mixed: {sx, sy, sz (u32 bits), q}
   TPairF *d_tpf;               // mixed: per type pair parameters

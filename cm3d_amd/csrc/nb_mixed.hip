@@ -36,6 +36,7 @@ NB_KERNEL_PAIR(k_packmix, PackMixArgs, 256)
 
 struct MixArgs {
   int n, ntypes, ninter, ntable, cap, ctab_off, tab_lds, nblocks;
+  int g16;  // 16 lanes per atom (four atoms a wavefront) instead of 64
   const uint32_t *nbr;
   const int *nnb, *stype, *perm;
   const uint4 *rec;
@@ -81,37 +82,51 @@ static __device__ __forceinline__ long long to_fix(double v) {
   return __double2ll_rn(v * NB_FX_SCALE);
 }
 
-// One wavefront per sorted atom at a time (a wave takes several atoms in
-// turn); each lane takes every 64th list entry.  A lane's pair forces for the
-// atom are summed in FP32 (a handful of terms), then go to fixed point and
-// are reduced across the wave in integers.  Energies and virials: FP32 sums
-// per lane over all its pairs (a few tens), reduced in FP64 once per
-// workgroup.  The next chunk's list entries and partner records are loaded
-// before the current chunk is evaluated.
-// The atom loop.  TLDS: tables (vt / ct) and type-pair parameters (stp) are
-// all in LDS, so their reads compile to LDS reads, not generic (flat) ones.
-template <int IFLG, bool WANT_FS, bool TLDS>
+// G lanes per sorted atom (G = 64: one atom per wavefront; G = 16: four
+// atoms per wavefront, one per DPP row), each lane taking every G-th list
+// entry; a wave takes several atoms (groups of 64 / G) in turn.  A lane's
+// pair forces for the atom are summed in FP32 (a handful of terms), then go
+// to fixed point and are reduced across the G lanes in integers, so the sum
+// does not depend on the order.  Energies and virials: FP32 sums per lane
+// over all its pairs (a few tens), reduced in FP64 once per workgroup.  The
+// next chunk's list entries and partner records are loaded before the
+// current chunk is evaluated.  G = 16 suits short lists (dense LJ-like
+// mixtures, ~100 entries): the last, partly filled chunk wastes fewer lanes
+// and one reduction serves four atoms.
+// TLDS: tables (vt / ct) and type-pair parameters (stp) are all in LDS, so
+// their reads compile to LDS reads, not generic (flat) ones.
+template <int G>
+static __device__ __forceinline__ long long group_sum_ll(long long v) {
+  if constexpr (G == NB_WAVE) return wave_sum_ll(v);
+  else return nb_row_sum(v);
+}
+template <int IFLG, bool WANT_FS, bool TLDS, int G>
 static __device__ __forceinline__ void pair_mixed_run(const MixArgs &A, int bid, const float *vt,
                                                       const float *ct, const TPairF *stp,
                                                       bool tp_lds) {
 #pragma clang fp contract(fast)
+  static_assert(G == 16 || G == NB_WAVE, "16 or 64 lanes per atom");
+  constexpr int APW = NB_WAVE / G;  // atoms a wavefront evaluates together
   float e[NB_NE_PAIR];  // this lane's energy / virial sums over its pairs
 #pragma unroll
   for (int k = 0; k < NB_NE_PAIR; k++) e[k] = 0.f;
-  const int lane = lane_id();
+  const int lane = lane_id(), gl = lane % G, sub = lane / G;
   const int nw = A.nblocks * 4;
   constexpr unsigned fv = IFLG ? NB_FLAG_VT : NB_FLAG_VS;
   constexpr unsigned fe = IFLG ? NB_FLAG_ET : NB_FLAG_ES;
   const float s32 = 2.3283064365386963e-10f;  // 2^-32
-  for (int s = bid * 4 + (threadIdx.x >> 6); s < A.n; s += nw) {
-    const uint4 ri = A.rec[s];
+  for (int s0 = (bid * 4 + (threadIdx.x >> 6)) * APW; s0 < A.n; s0 += nw * APW) {
+    const int s = s0 + sub;
+    const bool live = APW == 1 || s < A.n;
+    const int sa = live ? s : s0;  // a dead group reads its wave's first atom
+    const uint4 ri = A.rec[sa];
     const float qi = __uint_as_float(ri.w);
-    const int ti = A.stype[s];
-    const int nb = A.nnb[s];
-    const uint32_t *lst = A.nbr + (size_t)s * A.cap;
+    const int ti = A.stype[sa];
+    const int nb = live ? A.nnb[sa] : 0;
+    const uint32_t *lst = A.nbr + (size_t)sa * A.cap;
     const TPairF *tprow = (TLDS ? stp : (tp_lds ? stp : A.tpf)) + ti * A.ntypes;
     float fx = 0.f, fy = 0.f, fz = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
-    uint32_t ec = lane < nb ? lst[lane] : 0u;
+    uint32_t ec = gl < nb ? lst[gl] : 0u;
     uint4 rjc = make_uint4(0u, 0u, 0u, 0u);
     int tjc = 0;
     if ((ec >> 28) & (fv | fe)) {
@@ -119,8 +134,8 @@ static __device__ __forceinline__ void pair_mixed_run(const MixArgs &A, int bid,
       rjc = A.rec[j];
       tjc = A.stype[j];
     }
-    for (int k0 = 0; k0 < nb; k0 += NB_WAVE) {
-      const int kn = k0 + NB_WAVE + lane;
+    for (int k0 = 0; APW == 1 ? k0 < nb : __ballot(k0 < nb) != 0ull; k0 += G) {
+      const int kn = k0 + G + gl;
       const uint32_t en = kn < nb ? lst[kn] : 0u;
       uint4 rjn = make_uint4(0u, 0u, 0u, 0u);
       int tjn = 0;
@@ -201,19 +216,22 @@ static __device__ __forceinline__ void pair_mixed_run(const MixArgs &A, int bid,
       rjc = rjn;
       tjc = tjn;
     }
-    const int ai = A.perm[s];
+    const int ai = A.perm[sa];
     const int n = A.n;
+    const bool wr = gl == 0 && live;
     if (IFLG) {
-      long long X = wave_sum_ll(to_fix(fx)), Y = wave_sum_ll(to_fix(fy)), Z = wave_sum_ll(to_fix(fz));
-      if (lane == 0) {
+      long long X = group_sum_ll<G>(to_fix(fx)), Y = group_sum_ll<G>(to_fix(fy));
+      long long Z = group_sum_ll<G>(to_fix(fz));
+      if (wr) {
         A.fl[ai] = (double)X / NB_FX_SCALE;
         A.fl[n + ai] = (double)Y / NB_FX_SCALE;
         A.fl[2 * n + ai] = (double)Z / NB_FX_SCALE;
       }
     }
     if (WANT_FS || !IFLG) {
-      long long X = wave_sum_ll(to_fix(sx)), Y = wave_sum_ll(to_fix(sy)), Z = wave_sum_ll(to_fix(sz));
-      if (lane == 0) {
+      long long X = group_sum_ll<G>(to_fix(sx)), Y = group_sum_ll<G>(to_fix(sy));
+      long long Z = group_sum_ll<G>(to_fix(sz));
+      if (wr) {
         A.fs[ai] = (double)X / NB_FX_SCALE;
         A.fs[n + ai] = (double)Y / NB_FX_SCALE;
         A.fs[2 * n + ai] = (double)Z / NB_FX_SCALE;
@@ -224,6 +242,17 @@ static __device__ __forceinline__ void pair_mixed_run(const MixArgs &A, int bid,
 #pragma unroll
   for (int k = 0; k < NB_NE_PAIR; k++) acc[k] = (double)e[k];
   block_reduce_store_m<NB_NE_PAIR>(acc, A.epart + bid, A.nblocks);
+}
+
+template <int IFLG, bool WANT_FS, int G>
+static __device__ __forceinline__ void pair_mixed_tabs(const MixArgs &A, int bid, const float *tabsf,
+                                                       const TPairF *stp, bool tp_lds) {
+  if (A.tab_lds && tp_lds)
+    pair_mixed_run<IFLG, WANT_FS, true, G>(A, bid, tabsf, tabsf + A.ctab_off, stp, tp_lds);
+  else if (A.tab_lds)
+    pair_mixed_run<IFLG, WANT_FS, false, G>(A, bid, tabsf, tabsf + A.ctab_off, stp, tp_lds);
+  else
+    pair_mixed_run<IFLG, WANT_FS, false, G>(A, bid, A.vtab3, A.ctab3, stp, tp_lds);
 }
 
 template <int IFLG, bool WANT_FS>
@@ -240,12 +269,10 @@ static __device__ __forceinline__ void pair_mixed_body(const MixArgs &A, int bid
   if (tp_lds)
     for (int t = threadIdx.x; t < nt2; t += blockDim.x) stp[t] = A.tpf[t];
   __syncthreads();
-  if (A.tab_lds && tp_lds)
-    pair_mixed_run<IFLG, WANT_FS, true>(A, bid, tabsf, tabsf + A.ctab_off, stp, tp_lds);
-  else if (A.tab_lds)
-    pair_mixed_run<IFLG, WANT_FS, false>(A, bid, tabsf, tabsf + A.ctab_off, stp, tp_lds);
+  if (A.g16)
+    pair_mixed_tabs<IFLG, WANT_FS, 16>(A, bid, tabsf, stp, tp_lds);
   else
-    pair_mixed_run<IFLG, WANT_FS, false>(A, bid, A.vtab3, A.ctab3, stp, tp_lds);
+    pair_mixed_tabs<IFLG, WANT_FS, NB_WAVE>(A, bid, tabsf, stp, tp_lds);
 }
 #define k_mix_0t_body pair_mixed_body<0, true>
 #define k_mix_1t_body pair_mixed_body<1, true>
@@ -263,6 +290,7 @@ int nb_launch_pair_mixed(nb_ctx *c, int iflg, const Box &b, double *d_fs, double
   M.n = n; M.ntypes = c->ntypes; M.ninter = c->ninter; M.ntable = c->ntable;
   M.cap = c->cap; M.ctab_off = c->ctab_off; M.tab_lds = c->tab_lds_bytes_f > 0;
   M.nblocks = c->npair_blocks;
+  M.g16 = c->mix_g16;
   M.nbr = c->d_nbr; M.nnb = c->d_nnb; M.stype = c->d_stype; M.perm = c->d_perm;
   M.rec = (const uint4 *)c->d_recf; M.tpf = (const TPairF *)c->d_tpf;
   M.vtab3 = c->d_vtab3f; M.ctab3 = c->d_ctab3f;

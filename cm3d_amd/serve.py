@@ -74,8 +74,13 @@ def estimate_bytes(sysd, sharing=1):
                 run = 1
     nsw = max(1, _ceil(nseg, 64))
     share = max(1, int(sharing))
-    nch = max(1, min(_ceil(max(1, 4096 // share), nsw), _ceil(nq, 32)))
-    nrange = max(1, min(_ceil(max(1, 1024 // share), max(1, _ceil(nq, 512))), max(1, nseg)))
+    nch = _ceil(max(1, 4096 // share), nsw)
+    nrange = _ceil(max(1, 1024 // share), max(1, _ceil(nq, 512)))
+    if share > 1:  # a bounded k x charge product per workgroup (65536 each pass)
+        nch = max(nch, _ceil(ktot * nq, 65536 * nsw))
+        nrange = max(nrange, _ceil(ktot * 512, 65536))
+    nch = max(1, min(nch, _ceil(nq, 32)))
+    nrange = max(1, min(nrange, max(1, nseg)))
     per_atom = 4 * cap + 420           # lists + positions, forces, records, MD state
     ewald = nq * (64 + 24 * nrange) + ktot * (16 * nch + 96) + nseg * 48
     ncells = (int(2 * (vol ** (1. / 3.)) / math.sqrt(max(r2, 1e-12))) + 1) ** 3
