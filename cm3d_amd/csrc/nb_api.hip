@@ -460,13 +460,30 @@ static void cell_grid(nb_ctx *c, const Box &b, int *nx, int *ny, int *nz) {
   cross(cc, a, w);  widths[1] = vol / sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
   cross(a, bb, w);  widths[2] = vol / sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
   int ncs[3], reach[3];
+  // third-cutoff cells only where they hold a few atoms each on average:
+  // sparser systems pay more in per-cell work than they save in candidates
+  bool third = NB_CELL_REACH_MAX >= 3 && c->rlist > 0;
+  double cells3 = 1.;
+  for (int d = 0; d < 3 && third; d++) {
+    const int m3 = (int)floor(3. * widths[d] / (c->rlist * (1. + 1e-9)));
+    third = m3 >= 7;
+    cells3 *= std::min(m3, 128);
+  }
+  third = third && c->natoms >= NB_CELL3_MIN_ATOMS * cells3;
   for (int d = 0; d < 3; d++) {
     double r = c->rlist > 0 ? c->rlist * (1. + 1e-9) : widths[d];
-    // half-cutoff cells with a reach of 2 when at least 5 fit (so the 5
-    // cells around are distinct), else cutoff cells with a reach of 1 when
-    // at least 3 fit, else the whole axis as one cell
+    // third-cutoff cells with a reach of 3 when at least 7 fit on every
+    // axis (so the 7 cells around are distinct; 7 x 7 columns still fit one
+    // wavefront's lanes) and the cells are dense enough (above), else
+    // half-cutoff cells with a reach of 2 when at least 5 fit,
+    // else cutoff cells with a reach of 1 when at least 3 fit, else the
+    // whole axis as one cell.  Finer cells hug the list sphere more closely:
+    // fewer candidates per listed pair.
+    int m3 = (int)floor(3. * widths[d] / r);
     int m2 = (int)floor(2. * widths[d] / r), m1 = (int)floor(widths[d] / r);
-    if (m2 >= 5) {
+    if (third) {
+      ncs[d] = std::min(m3, 128); reach[d] = 3;
+    } else if (m2 >= 5) {
       ncs[d] = std::min(m2, 128); reach[d] = 2;
     } else if (m1 >= 3) {
       ncs[d] = m1; reach[d] = 1;

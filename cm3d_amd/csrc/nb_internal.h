@@ -44,6 +44,12 @@ struct LaunchRec;
 #ifndef NB_SFAC_WG
 #define NB_SFAC_WG 4096  // workgroups the structure-factor pass aims for
 #endif
+#ifndef NB_CELL_REACH_MAX
+#define NB_CELL_REACH_MAX 3  // list stencil: cells of >= rlist / reach, reach <= 3
+#endif
+#ifndef NB_CELL3_MIN_ATOMS
+#define NB_CELL3_MIN_ATOMS 2.0  // mean atoms per cell needed for reach-3 cells
+#endif
 #ifndef NB_EWF_WORK
 #define NB_EWF_WORK 65536  // k x charge products per force workgroup in a shared context
 #endif
@@ -223,7 +229,7 @@ struct nb_ctx {
   int *d_flags;                // [0] overflow max count, [1] rebuild flag,
                                // [6] builds done, [7] k_cell's ticket
   int ncx, ncy, ncz, ncells, cells_alloc;
-  int crx, cry, crz;           // cell reach of the list stencil per axis (0..2)
+  int crx, cry, crz;           // cell reach of the list stencil per axis (0..3)
   double *d_chg, *d_stp, *d_spart, *d_kd, *d_fpart;
   int nchunk, qc, nsw;         // structure factor: charge chunks, segment waves
   int nrange, nqb, ntile;      // force: tile ranges, charge workgroups, tiles

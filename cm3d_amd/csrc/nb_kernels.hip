@@ -45,9 +45,9 @@ NB_KERNEL_PAIR(k_zero, ZeroArgs, 256)
 
 // ---------------------------------------------------------------------------
 // Cell sort.  Cells tile the box in fractional coordinates; along each box
-// vector they are either no narrower than half the longest list cutoff
-// (reach 2: the 5 cells around a cell's own), no narrower than the cutoff
-// (reach 1) or one cell for the whole box (reach 0), so every pair within
+// vector they are either no narrower than a third of the longest list
+// cutoff (reach 3: the 7 cells around a cell's own), half of it (reach 2),
+// the cutoff (reach 1) or one cell for the whole box (reach 0), so every pair within
 // the cutoff under cm3d's minimum image lies in the (2rx+1)(2ry+1)(2rz+1)
 // cells around.  Binning follows get_lnklist (set_link.c:218-236): wrap
 // s -= anint(s), then (int)((s+.5)*n) clamped.
@@ -214,6 +214,7 @@ static __device__ __forceinline__ int wrapc(int v, int n) {
 static __device__ __forceinline__ void k_list_body(const ListArgs &A, int bid) {
   NB_SKIP(A.cond);
   __shared__ double2 pc[NB_PCUT_LDS];
+  __shared__ int colrun[4][4 * NB_WAVE];  // per wave: column ends, rs0, rl0, rs1
   if (A.pcut_lds) {
     for (int t = threadIdx.x; t < A.ntypes * A.ntypes; t += blockDim.x) pc[t] = A.pcut[t];
     __syncthreads();
@@ -285,41 +286,61 @@ static __device__ __forceinline__ void k_list_body(const ListArgs &A, int bid) {
   const double2 *pcut = A.pcut + ti * A.ntypes;
   int count = 0, hs = 0, ht = 0, hse = 0, hte = 0;
   uint32_t *out = A.nbr + (size_t)s * A.cap;
-  // walk the chunks (run r, offset b) with the next chunk's candidate loads
-  // issued before the current chunk is evaluated
-  const int nrun = 2 * ncol;
-#define NB_RUN_ST(r) __builtin_amdgcn_readlane(((r) & 1) ? rs1 : rs0, (r) >> 1)
-#define NB_RUN_LEN(r) __builtin_amdgcn_readlane(((r) & 1) ? rl1 : rl0, (r) >> 1)
-#define NB_NEXT_CHUNK(r, b, st, len)              \
-  {                                               \
-    b += NB_WAVE;                                 \
-    while (b >= len) {                            \
-      if (++r >= nrun) break;                     \
-      st = NB_RUN_ST(r);                          \
-      len = NB_RUN_LEN(r);                        \
-      b = 0;                                      \
-    }                                             \
+  // The candidates are the concatenation, column by column, of each
+  // column's runs; 64 consecutive candidates make a chunk whatever runs
+  // they come from, so short runs (small cells) leave no lanes idle.  The
+  // order is the concatenation order either way.  Column l's runs and the
+  // end of its candidates in the concatenation go to this wave's LDS rows;
+  // each lane keeps a cursor on the column its candidate falls in.
+  int cend = rl0 + rl1;  // inclusive prefix over the columns (lanes)
+#pragma unroll
+  for (int o = 1; o < NB_WAVE; o <<= 1) {
+    const int y = __shfl_up(cend, o, NB_WAVE);
+    if (lane >= o) cend += y;
   }
-  int r = -1, b = -NB_WAVE, st = 0, len = 0;
-  NB_NEXT_CHUNK(r, b, st, len)
+  const int total = __builtin_amdgcn_readlane(cend, NB_WAVE - 1);
+  int *lc = colrun[threadIdx.x >> 6];
+  lc[lane] = cend;
+  lc[NB_WAVE + lane] = rs0;
+  lc[2 * NB_WAVE + lane] = rl0;
+  lc[3 * NB_WAVE + lane] = rs1;
+  // the wave's LDS rows are read by other lanes of the same wave
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  // a lane's cursor: col, the column of its next candidate, and colend,
+  // that column's end; it only moves forward (it stops at lane 63's entry,
+  // which is the total)
+  int col = 0, colend = lc[0];
+  int cb = 0;
   double4 rjc = make_double4(0., 0., 0., 0.);
   int tjc = 0, ajc = -1, s2c = 0;
   bool okc = false;
-  if (r < nrun) {
-    s2c = st + b + lane;
-    okc = b + lane < len && s2c != s;
-    if (okc) { rjc = A.rec[s2c]; tjc = A.stype[s2c]; ajc = A.perm[s2c]; }
+  {
+    const int c = cb + lane;
+    if (c < total) {
+      while (colend <= c) colend = lc[++col];
+      const int cstart = col > 0 ? lc[col - 1] : 0;
+      const int loc = c - cstart, r0 = lc[2 * NB_WAVE + col];
+      s2c = loc < r0 ? lc[NB_WAVE + col] + loc : lc[3 * NB_WAVE + col] + (loc - r0);
+      okc = s2c != s;
+      if (okc) { rjc = A.rec[s2c]; tjc = A.stype[s2c]; ajc = A.perm[s2c]; }
+    }
   }
-  while (r < nrun) {
-    int rn = r, bn = b, stn = st, lenn = len;
-    NB_NEXT_CHUNK(rn, bn, stn, lenn)
+  for (; cb < total; cb += NB_WAVE) {
     double4 rjn = make_double4(0., 0., 0., 0.);
     int tjn = 0, ajn = -1, s2n = 0;
     bool okn = false;
-    if (rn < nrun) {
-      s2n = stn + bn + lane;
-      okn = bn + lane < lenn && s2n != s;
-      if (okn) { rjn = A.rec[s2n]; tjn = A.stype[s2n]; ajn = A.perm[s2n]; }
+    {
+      const int c = cb + NB_WAVE + lane;
+      if (c < total) {
+        while (colend <= c) colend = lc[++col];
+        const int cstart = col > 0 ? lc[col - 1] : 0;
+        const int loc = c - cstart, r0 = lc[2 * NB_WAVE + col];
+        s2n = loc < r0 ? lc[NB_WAVE + col] + loc : lc[3 * NB_WAVE + col] + (loc - r0);
+        okn = s2n != s;
+        if (okn) { rjn = A.rec[s2n]; tjn = A.stype[s2n]; ajn = A.perm[s2n]; }
+      }
     }
     unsigned fl = 0;
     if (okc) {
@@ -360,12 +381,8 @@ static __device__ __forceinline__ void k_list_body(const ListArgs &A, int bid) {
       }
     }
     count += __popcll(m);
-    r = rn; b = bn; st = stn; len = lenn;
     rjc = rjn; tjc = tjn; ajc = ajn; s2c = s2n; okc = okn;
   }
-#undef NB_NEXT_CHUNK
-#undef NB_RUN_ST
-#undef NB_RUN_LEN
   hs = nb_wave_sum(hs);
   ht = nb_wave_sum(ht);
   hse = nb_wave_sum(hse);
