@@ -306,7 +306,7 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
                    [](const EwSeg &a, const EwSeg &b) { return a.cnt > b.cnt; });
   c->nsw = std::max(1, (c->nseg + NB_WAVE - 1) / NB_WAVE);
   {
-    int maxch = std::max(1, (ncharge + 31) / 32);  // >= 32 charges a workgroup
+    int maxch = std::max(1, (ncharge + NB_SFAC_MIN_Q - 1) / NB_SFAC_MIN_Q);  // charges a workgroup
     int nch = std::max(1, (sfac_wg + c->nsw - 1) / c->nsw);
     // in a batch, workgroups of every system should take about the same
     // time: cap a workgroup's k x charge products

@@ -50,6 +50,9 @@ struct LaunchRec;
 #ifndef NB_CELL3_MIN_ATOMS
 #define NB_CELL3_MIN_ATOMS 2.0  // mean atoms per cell needed for reach-3 cells
 #endif
+#ifndef NB_SFAC_MIN_Q
+#define NB_SFAC_MIN_Q 64  // fewest charges in a structure-factor chunk
+#endif
 #ifndef NB_EWF_WORK
 #define NB_EWF_WORK 65536  // k x charge products per force workgroup in a shared context
 #endif
@@ -87,8 +90,12 @@ struct EwSeg {      // run of <= NB_EW_SEG consecutive kc in one (ka,kb) row
 };
 
 // cm3d's -DLINUX anint: (int)(x +- 0.5), truncating (md_macros.h:31-52).
-__device__ __forceinline__ int nb_anint(double a) {
-  return (int)(a + (a >= 0 ? .5 : -.5));
+__device__ __forceinline__ double nb_anint(double a) {
+  // (int)(a +- .5) as cm3d writes it, kept in FP64: truncating a +- .5
+  // gives the same integer (|a| < 2^31 for any fractional coordinate) in
+  // two instructions instead of a compare, select and two conversions; only
+  // the sign of a zero result can differ
+  return __builtin_trunc(a + __builtin_copysign(.5, a));
 }
 
 // Minimum image in cm3d's operation order (md_period.new.c:39-51).  The

@@ -468,6 +468,7 @@ static __device__ __forceinline__ void pair64_run(const PairArgs &A, int bid, co
       rjc = A.rec[j];
       tjc = A.stype[j];
     }
+#pragma unroll 2
     for (int k0 = 0; k0 < nb; k0 += NB_WAVE) {
       const int kn = k0 + NB_WAVE + lane;
       const uint32_t en = kn < nb ? lst[kn] : 0u;
