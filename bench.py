@@ -38,6 +38,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+VALU_ISSUE_PER_S = 256 * 2.4e9  # wave instructions: one per CU per cycle
 DT_PS = 0.002
 
 
@@ -286,6 +287,18 @@ def main():
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(pair_gbs / HBM_PEAK_GBS, 5),
             "traffic": traffic, "traffic_source": tsrc,
             "ms_per_launch": round(tm["pair"], 5), "bytes_per_launch": pair_bytes}
+    # what does bound the kernel: vector instruction issue.  The launch's
+    # VALU instruction count (rocprofv3 SQ_INSTS_VALU, committed beside the
+    # traffic) over what 256 CUs issue at one wave instruction per cycle at
+    # 2.4 GHz in the launch time measured here
+    vpath = os.path.join(ROOT, "profiles", "pair_kernel_water4096_valu.json")
+    if os.path.exists(vpath) and a.nside == 16:
+        with open(vpath) as f:
+            vj = json.load(f)
+        insts = float(vj["valu_insts_per_launch"])
+        roof["valu_insts_per_launch"] = insts
+        roof["valu_issue_frac"] = round(insts / (VALU_ISSUE_PER_S * tm["pair"] * 1e-3), 4)
+        roof["valu_source"] = "profiles/pair_kernel_water4096_valu.md (rocprofv3 --pmc SQ_INSTS_VALU)"
     roof_ew = {"kernels": "k_ewchg+k_ewsfac+k_ewkred+k_red+k_ewforce+k_ewfin",
                "bound": "valu-fp64", "achieved": round(ew_tfs, 3), "peak": FP64_PEAK_TFLOPS,
                "unit": "TFLOP/s", "frac": round(ew_tfs / FP64_PEAK_TFLOPS, 5),
