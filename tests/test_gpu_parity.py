@@ -171,3 +171,27 @@ def test_sharing_changes_only_the_work_split():
     assert rel_rms(out[1][1], out[0][1]) < 1e-13
     assert rel_rms(out[1][1], d["fxl"]) < FP64_TOL
     assert _rel(out[1][2]["vk_ewald"], out[0][2]["vk_ewald"]) < 1e-13
+
+
+@pytest.mark.parametrize("sharing", [1, 16])
+@pytest.mark.parametrize("natoms", [1001, 1002, 1003])
+def test_mixed_short_lists_partial_groups(natoms, sharing):
+    """LJ + charge mixtures have ~100-entry lists, which the mixed kernel
+    evaluates four atoms to a wavefront (16 lanes an atom).  With N not a
+    multiple of 4 the last wavefront has idle groups; with sharing >= 4 a
+    workgroup takes 64 atoms instead of 16.  Both force kinds (fl at iflg
+    1, fs at iflg 0) stay within the mixed-precision bar of the FP64
+    forces."""
+    from cm3d_amd import nb, systems as S
+    s = S.mixture(natoms, seed=300 + natoms, rc=9.0, kmax=5)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        c = nb.NBContext(s, precision=prec, sharing=sharing)
+        c.build_lists(s["pos"], s["hmat"])
+        _, fl, _ = c.force(s["pos"], s["hmat"], iflg=1)
+        fs, _, _ = c.force(s["pos"], s["hmat"], iflg=0)
+        out[prec] = (fl, fs)
+        c.close()
+    for k in range(2):
+        err = rel_rms(out["mixed"][k], out["fp64"][k])
+        assert err < MIXED_FORCE_TOL, (natoms, sharing, k, err)
