@@ -90,9 +90,11 @@ typedef struct nb_params {
   /* No cm3d counterpart: how many systems are expected to share the GPU
    * with this one (the members of a batch, a server's slots); 0 or 1 =
    * alone.  It sizes the work split of the reciprocal sum (fewer, larger
-   * pieces when many systems run together); results change only at
-   * rounding level and stay fixed for the context's lifetime, so a system
-   * steps bit for bit the same alone and in a batch. */
+   * pieces when many systems run together, each piece capped so that the
+   * pieces of small and large systems take about the same time) and, from
+   * 4 up, the mixed pair kernel's atoms per workgroup; results change only
+   * at rounding level and stay fixed for the context's lifetime, so a
+   * system steps bit for bit the same alone and in a batch. */
   int sharing;
 } nb_params;
 
