@@ -325,7 +325,7 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
   c->nqb = std::max(1, (ncharge + 256 * NB_EWF_Q - 1) / (256 * NB_EWF_Q));
   std::vector<int> tseg(1, 0), rtile(1, 0);
   {
-    int nrt = std::max(1, (ewf_wg + c->nqb - 1) / c->nqb);
+    int nrt = std::max(1, ewf_wg / c->nqb);  // at most ~NB_EWF_WG: one round of waves
     if (share > 1)  // as for the structure factor: a bounded k x charge product per workgroup
       nrt = std::max(nrt, (int)(((long long)c->ktot * 256 * NB_EWF_Q + NB_EWF_WORK - 1) / NB_EWF_WORK));
     nrt = std::min(nrt, std::max(1, c->nseg));

@@ -827,6 +827,7 @@ struct EwForceArgs {
   const double2 *stp; int nq, nqb; double *fpart; Box b;
 };
 #define EWF_Q NB_EWF_Q
+#define NB_EWF_DW 4  // largest kc shift between chained row starts
 // the M k vectors of one segment against the lane's charges: s0 = sum tt,
 // s1 = sum m tt, and the phases advanced M steps (a segment that continues
 // its row picks them up).  M is a template parameter: the loop unrolls fully,
@@ -860,6 +861,10 @@ static __device__ __forceinline__ void k_ewforce_body(const EwForceArgs &A, int 
   double4 q4[EWF_Q];
   double2 st[EWF_Q];
   double fx[EWF_Q], fy[EWF_Q], fz[EWF_Q], sn[EWF_Q], cs[EWF_Q];
+  // row starts: e^{i k.r} at the first k of the current row (ps, pc) and
+  // the step from row kb to row kb + 1, e^{2 pi i s_b} (eb)
+  double ps[EWF_Q], pc[EWF_Q];
+  double2 eb[EWF_Q];
 #pragma unroll
   for (int u = 0; u < EWF_Q; u++) {
     const int c = cbase + u * blockDim.x;
@@ -869,7 +874,11 @@ static __device__ __forceinline__ void k_ewforce_body(const EwForceArgs &A, int 
     fx[u] = fy[u] = fz[u] = 0.;
     sn[u] = 0.;
     cs[u] = 1.;
+    ps[u] = 0.;
+    pc[u] = 1.;
+    nb_sincos(q4[u].y, &eb[u].y, &eb[u].x);
   }
+  int cka = -1, ckb = 0, ckc = 0;  // (ka, kb, kc0) of the last row start
   const double tpi = 2. * M_PI, *hi = A.b.hi;
   const double gcx = tpi * hi[6], gcy = tpi * hi[7], gcz = tpi * hi[8];
   cptr4<EwSeg> seg = as_const4(A.seg);
@@ -888,9 +897,44 @@ static __device__ __forceinline__ void k_ewforce_body(const EwForceArgs &A, int 
       const int cnt = seg[g].cnt;
       const double ka = seg[g].ka, kb = seg[g].kb, kc0 = seg[g].kc0;
       if (g == gfirst || !seg[g].cont) {
+        // a row start: the next row of the same ka plane is one kb step on,
+        // and its first kc a few steps from the last start's, so its phase
+        // follows from the last start's by 1 + |dw| complex products;
+        // otherwise (new plane, first segment of the range, long jumps in
+        // kc) from one sine / cosine
+        const int ika = seg[g].ka, ikb = seg[g].kb, ikc = seg[g].kc0;
+        int dw = ikc - ckc;
+        if (g != gfirst && ika == cka && ikb == ckb + 1 && dw >= -NB_EWF_DW && dw <= NB_EWF_DW) {
 #pragma unroll
-        for (int u = 0; u < EWF_Q; u++)
-          nb_sincos(ka * q4[u].x + kb * q4[u].y + kc0 * q4[u].z, &sn[u], &cs[u]);
+          for (int u = 0; u < EWF_Q; u++) {
+            const double c2 = pc[u] * eb[u].x - ps[u] * eb[u].y;
+            ps[u] = ps[u] * eb[u].x + pc[u] * eb[u].y;
+            pc[u] = c2;
+          }
+          for (; dw > 0; dw--) {
+#pragma unroll
+            for (int u = 0; u < EWF_Q; u++) {
+              const double c2 = pc[u] * st[u].x - ps[u] * st[u].y;
+              ps[u] = ps[u] * st[u].x + pc[u] * st[u].y;
+              pc[u] = c2;
+            }
+          }
+          for (; dw < 0; dw++) {
+#pragma unroll
+            for (int u = 0; u < EWF_Q; u++) {
+              const double c2 = pc[u] * st[u].x + ps[u] * st[u].y;
+              ps[u] = ps[u] * st[u].x - pc[u] * st[u].y;
+              pc[u] = c2;
+            }
+          }
+        } else {
+#pragma unroll
+          for (int u = 0; u < EWF_Q; u++)
+            nb_sincos(ka * q4[u].x + kb * q4[u].y + kc0 * q4[u].z, &ps[u], &pc[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < EWF_Q; u++) { sn[u] = ps[u]; cs[u] = pc[u]; }
+        cka = ika; ckb = ikb; ckc = ikc;
       }
       double s0[EWF_Q], s1[EWF_Q];
 #pragma unroll
