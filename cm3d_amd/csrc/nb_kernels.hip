@@ -183,9 +183,10 @@ NB_KERNEL_PAIR(k_pack64, Pack64Args, 256)
 
 // ---------------------------------------------------------------------------
 // Verlet list build, one wavefront per (sorted) atom.  The candidate cells
-// around the atom's own are walked as runs of consecutive cells along c (at
-// most 2 runs per (a,b) column, consecutive in sorted order), 64 candidates
-// at a time; each lane applies cm3d's filters in FP64 exactly as ngbr_list /
+// around the atom's own are runs of consecutive cells along c (at most 2
+// runs per (a,b) column, consecutive in sorted order); the runs of all
+// columns, concatenated, are walked 64 candidates at a time; each lane
+// applies cm3d's filters in FP64 exactly as ngbr_list /
 // ngbr_liste do (null map, exclusion, minimum image, strict
 // r^2 < (rc+skin)^2, s nested in t), and the survivors are compacted with a
 // ballot + mbcnt prefix into the atom's slots.  Both directions of every pair
