@@ -352,7 +352,7 @@ def main():
         out["serving"] = {
             "workload": f"{sv['jobs']} MD jobs, Poisson arrivals at {a.serve_rate}/s, half LJ+charge "
                         f"mixtures (N log-uniform 500-20000, rc 9 A, kmax 7), half flexible water "
-                        f"(216-8000 molecules, RESPA 4x0.5/2 fs), 200-5000 steps each, mixed "
+                        f"(343-8000 molecules: n^3 for n = 7..20, within 256-8192; RESPA 4x0.5/2 fs), 200-5000 steps each, mixed "
                         f"precision, admitted by slot ({sv['max_slots']}) and HBM footprint",
             "atom_steps_per_s": round(sv["atom_steps_per_s"], 1),
             "wall_s": round(sv["wall_s"], 3),

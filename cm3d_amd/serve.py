@@ -204,11 +204,12 @@ class Server:
                     p50_job_s=float(np.percentile(jl, 50)), p99_job_s=float(np.percentile(jl, 99)))
 
 
-def job_stream(njobs, rate_per_s, seed, water_nside=(6, 20), mix_atoms=(500, 20000),
+def job_stream(njobs, rate_per_s, seed, water_nside=(7, 20), mix_atoms=(500, 20000),
                steps=(200, 5000), ntemplates=24):
     """Poisson arrivals (mean rate_per_s) of jobs drawn half from the LJ +
     charge binary-mixture family (N log-uniform in mix_atoms) and half from
-    flexible water (nside^3 molecules), step counts uniform in steps.  Systems
+    flexible water (nside^3 molecules: 343-8000, the cubes within 256-8192),
+    step counts uniform in steps.  Systems
     come from a pool of ntemplates generated up front; every job gets its own
     Maxwell-Boltzmann velocities at 300 K."""
     from . import systems as S
