@@ -134,7 +134,13 @@ static __device__ __forceinline__ void pair_mixed_run(const MixArgs &A, int bid,
       rjc = A.rec[j];
       tjc = A.stype[j];
     }
-    for (int k0 = 0; APW == 1 ? k0 < nb : __ballot(k0 < nb) != 0ull; k0 += G) {
+    // trip bound: the longest list of the wave's atoms, read from lane 0 of
+    // each group, so the compiler sees a wave-uniform loop
+    int nbw = nb;
+    if constexpr (APW == 4)
+      nbw = max(max(__builtin_amdgcn_readlane(nb, 0), __builtin_amdgcn_readlane(nb, 16)),
+                max(__builtin_amdgcn_readlane(nb, 32), __builtin_amdgcn_readlane(nb, 48)));
+    for (int k0 = 0; k0 < nbw; k0 += G) {
       const int kn = k0 + G + gl;
       const uint32_t en = kn < nb ? lst[kn] : 0u;
       uint4 rjn = make_uint4(0u, 0u, 0u, 0u);
