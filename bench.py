@@ -52,7 +52,7 @@ def parse():
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--equil", type=int, default=600,
                     help="untimed steps with velocity rescaling to 300 K before timing")
-    ap.add_argument("--drift-steps", type=int, default=10000)
+    ap.add_argument("--drift-steps", type=int, default=40000)
     ap.add_argument("--lat-steps", type=int, default=100,
                     help="single synchronised steps timed for the latency percentiles")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -158,7 +158,12 @@ def run_single(nb, S, a, torch):
         ham = np.array(ham) / n
         fit = np.polyfit(t_ns, ham, 1)
         drift[name] = float(fit[0])
-        drift[name + "_rms_about_fit_K_per_atom"] = float(np.sqrt(np.mean((np.polyval(fit, t_ns) - ham) ** 2)))
+        resid = np.polyval(fit, t_ns) - ham
+        drift[name + "_rms_about_fit_K_per_atom"] = float(np.sqrt(np.mean(resid ** 2)))
+        # standard error of the slope: the two precisions' drifts are equal
+        # when they agree within it
+        drift[name + "_stderr"] = float(np.sqrt(np.sum(resid ** 2) / (len(t_ns) - 2)
+                                                / np.sum((t_ns - t_ns.mean()) ** 2)))
         drift[name + "_ham0_K_per_atom"] = float(ham[0])
     t0 = time.perf_counter()
     mdm.step(a.steps, graph=True)
@@ -318,7 +323,8 @@ def main():
         "latency_basis": f"{a.lat_steps} single steps, each launched and waited for",
         "energy_drift_K_per_atom_per_ns": {k: round(v, 6) for k, v in r["drift"].items()},
         "drift_run": f"slope of a linear fit to Ham/N sampled every {max(1, a.drift_steps // 20)} "
-                     f"steps over {a.drift_steps} NVE steps from the same 300 K state, each precision",
+                     f"steps over {a.drift_steps} NVE steps from the same 300 K state, each precision; "
+                     f"*_stderr: the slope's standard error",
         "mixed_single": {"atom_steps_per_s": round(n * a.steps / r["wall_mixed"], 1),
                          "force_rel_rms_vs_fp64": r["mixed_force_err"]},
         "list_rebuilds": r["rebuilds"],
