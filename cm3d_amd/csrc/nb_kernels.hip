@@ -285,7 +285,7 @@ static __device__ __forceinline__ void k_list_body(const ListArgs &A, int bid) {
     }
   }
   const double2 *pcut = A.pcut + ti * A.ntypes;
-  int count = 0, hs = 0, ht = 0, hse = 0, hte = 0;
+  int nv = 0, nc = 0, hs = 0, ht = 0, hse = 0, hte = 0;
   uint32_t *out = A.nbr + (size_t)s * A.cap;
   // The candidates are the concatenation, column by column, of each
   // column's runs; 64 consecutive candidates make a chunk whatever runs
@@ -370,10 +370,18 @@ static __device__ __forceinline__ void k_list_body(const ListArgs &A, int bid) {
         }
       }
     }
-    const unsigned long long m = __ballot(fl != 0);
+    // entries with a vdW flag fill the row from the front, Coulomb-only
+    // entries from the back, each in candidate order
+    const bool isv = (fl & NB_FLAG_VT) != 0;
+    const unsigned long long mv = __ballot(isv), mc = __ballot(fl != 0 && !isv);
     if (fl) {
-      const int slot = count + mbcnt(m);
-      if (slot < A.cap) out[slot] = (uint32_t)s2c | (fl << 28);
+      if (isv) {
+        const int slot = nv + mbcnt(mv);
+        if (slot < A.cap) out[slot] = (uint32_t)s2c | (fl << 28);
+      } else {
+        const int slot = A.cap - 1 - (nc + mbcnt(mc));
+        if (slot >= 0) out[slot] = (uint32_t)s2c | (fl << 28);
+      }
       if (ai < ajc) {
         hs += (fl & NB_FLAG_VS) != 0;
         ht += (fl & NB_FLAG_VT) != 0;
@@ -381,8 +389,26 @@ static __device__ __forceinline__ void k_list_body(const ListArgs &A, int bid) {
         hte += (fl & NB_FLAG_ET) != 0;
       }
     }
-    count += __popcll(m);
+    nv += __popcll(mv);
+    nc += __popcll(mc);
     rjc = rjn; tjc = tjn; ajc = ajn; s2c = s2n; okc = okn;
+  }
+  const int count = nv + nc;
+  // close the row: the Coulomb-only entries move down behind the vdW ones,
+  // so the row is [vdW entries | Coulomb-only entries] and the pair kernels'
+  // chunks past the vdW entries skip the vdW branch as a whole wave.  The
+  // destination lies below the source, and each chunk is read before it is
+  // written, so the move is safe in place.
+  if (count <= A.cap) {
+    // the back entries were stored by other lanes of this wave
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    for (int i0 = 0; i0 < nc; i0 += NB_WAVE) {
+      const int i = i0 + lane;
+      const uint32_t v = i < nc ? out[A.cap - nc + i] : 0u;
+      if (i < nc) out[nv + i] = v;
+    }
   }
   hs = nb_wave_sum(hs);
   ht = nb_wave_sum(ht);
