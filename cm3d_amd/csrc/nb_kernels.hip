@@ -285,6 +285,10 @@ static __device__ __forceinline__ void k_list_body(const ListArgs &A, int bid) {
     }
   }
   const double2 *pcut = A.pcut + ti * A.ntypes;
+  // does this atom's type have any vdW partner type (null pairs carry -1)
+  bool rowv = false;
+  for (int tj = 0; tj < A.ntypes; tj++)
+    rowv |= (A.pcut_lds ? pc[ti * A.ntypes + tj].x : pcut[tj].x) > 0.;
   int nv = 0, nc = 0, hs = 0, ht = 0, hse = 0, hte = 0;
   uint32_t *out = A.nbr + (size_t)s * A.cap;
   // The candidates are the concatenation, column by column, of each
@@ -371,8 +375,9 @@ static __device__ __forceinline__ void k_list_body(const ListArgs &A, int bid) {
       }
     }
     // entries with a vdW flag fill the row from the front, Coulomb-only
-    // entries from the back, each in candidate order
-    const bool isv = (fl & NB_FLAG_VT) != 0;
+    // entries from the back, each in candidate order (all from the front in
+    // a row whose atom type has no vdW partner type)
+    const bool isv = (fl & NB_FLAG_VT) != 0 || (fl != 0 && !rowv);
     const unsigned long long mv = __ballot(isv), mc = __ballot(fl != 0 && !isv);
     if (fl) {
       if (isv) {
