@@ -140,32 +140,46 @@ struct OrderArgs {
   const int *count, *start, *tmp; int *perm, *iperm, *scell;
   const double *pos, *q; const int *itype; double4 *rec; int *stype; double *posb; int n;
   int *builds; uint4 *recf; Box b; const int *cond;
+  int ncells, packed;  // packed: 16 cells a workgroup, 16 lanes a cell
 };
+// atom a of cell c goes to sorted slot st + (its rank by index in the cell)
+static __device__ __forceinline__ void order_atom(const OrderArgs &A, int c, int st, int cnt,
+                                                  const int *src, int a) {
+  const int n = A.n;
+  int r = 0;
+  for (int u = 0; u < cnt; u++) r += src[u] < a;
+  const int sl = st + r;
+  A.perm[sl] = a;
+  A.iperm[a] = sl;
+  A.scell[sl] = c;
+  const double x = A.pos[a], y = A.pos[n + a], z = A.pos[2 * n + a];
+  A.rec[sl] = make_double4(x, y, z, A.q[a]);
+  if (A.recf) A.recf[sl] = nb_mixed_record(x, y, z, A.q[a], A.b.hi);
+  A.stype[sl] = A.itype[a];
+  A.posb[a] = x; A.posb[n + a] = y; A.posb[2 * n + a] = z;
+}
 static __device__ __forceinline__ void k_order_body(const OrderArgs &A, int bid) {
   NB_SKIP(A.cond);
   __shared__ int seg[NB_CELL_LDS];
   if (bid == 0 && threadIdx.x == 0) atomicAdd(A.builds, 1);
-  const int c = bid, cnt = A.count[c], st = A.start[c], n = A.n;
+  if (A.packed) {
+    // many small cells: a workgroup per 16 cells keeps the launch (and the
+    // skipped launch of a step without a rebuild) small
+    const int c = bid * 16 + (threadIdx.x >> 4), l = threadIdx.x & 15;
+    if (c >= A.ncells) return;
+    const int cnt = A.count[c], st = A.start[c];
+    const int *src = A.tmp + st;
+    for (int t = l; t < cnt; t += 16) order_atom(A, c, st, cnt, src, src[t]);
+    return;
+  }
+  const int c = bid, cnt = A.count[c], st = A.start[c];
   const int *src = A.tmp + st;
   if (cnt <= NB_CELL_LDS) {
     for (int t = threadIdx.x; t < cnt; t += blockDim.x) seg[t] = src[t];
     __syncthreads();
     src = seg;
   }
-  for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
-    const int a = src[t];
-    int r = 0;
-    for (int u = 0; u < cnt; u++) r += src[u] < a;
-    const int sl = st + r;
-    A.perm[sl] = a;
-    A.iperm[a] = sl;
-    A.scell[sl] = c;
-    const double x = A.pos[a], y = A.pos[n + a], z = A.pos[2 * n + a];
-    A.rec[sl] = make_double4(x, y, z, A.q[a]);
-    if (A.recf) A.recf[sl] = nb_mixed_record(x, y, z, A.q[a], A.b.hi);
-    A.stype[sl] = A.itype[a];
-    A.posb[a] = x; A.posb[n + a] = y; A.posb[2 * n + a] = z;
-  }
+  for (int t = threadIdx.x; t < cnt; t += blockDim.x) order_atom(A, c, st, cnt, src, src[t]);
 }
 NB_KERNEL_PAIR(k_order, OrderArgs, 256)
 
@@ -1107,8 +1121,9 @@ static int launch_build_impl(nb_ctx *c, const Box &b, hipStream_t s, const int *
   OrderArgs oa = {c->d_ccount, c->d_cstart, c->d_tmp, c->d_perm, c->d_iperm, c->d_cell,
                   c->d_pos, c->d_qch, c->d_itype, (double4 *)c->d_rec64, c->d_stype,
                   c->d_posb, n, c->d_flags + 6,
-                  c->prec == NB_PREC_MIXED ? (uint4 *)c->d_recf : nullptr, b, cond};
-  NB_GO(c, K_ORDER, k_order, c->ncells, 256, 0, oa, s);
+                  c->prec == NB_PREC_MIXED ? (uint4 *)c->d_recf : nullptr, b, cond,
+                  c->ncells, c->ncells >= 64};
+  NB_GO(c, K_ORDER, k_order, oa.packed ? (c->ncells + 15) / 16 : c->ncells, 256, 0, oa, s);
   ListArgs A;
   A.n = n; A.ntypes = c->ntypes; A.cap = c->cap;
   A.ncx = c->ncx; A.ncy = c->ncy; A.ncz = c->ncz; A.rx = c->crx; A.ry = c->cry; A.rz = c->crz;
