@@ -74,3 +74,36 @@ def test_create_without_gpu_fails_loudly():
         pytest.skip("a GPU is present")
     with pytest.raises(nb.NBError):
         nb.NBContext(S.lj_fcc(ncells=3))
+
+
+def test_header_is_plain_c(tmp_path):
+    """cm3d is C: the header compiles as C99 with warnings as errors, every
+    declared entry point links against the built library, and a program
+    that calls nb_create without a GPU gets an error code and a message
+    (no GPU is touched beyond that failing call)."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    hdr = open(os.path.join(ROOT, "include", "cm3d_nb.h")).read()
+    names = sorted(set(re.findall(r"^\s*(?:int|void|const char \*)\s*\*?\s*(nb_\w+)\s*\(", hdr, re.M)))
+    src = tmp_path / "use_header.c"
+    src.write_text(
+        '#include <stdio.h>\n#include <string.h>\n#include "cm3d_nb.h"\n'
+        "int main(void) {\n"
+        "  void *fns[] = {" + ", ".join(f"(void *){n}" for n in names) + "};\n"
+        "  nb_params p; memset(&p, 0, sizeof p);\n"
+        "  nb_ctx *c = 0;\n"
+        "  int rc = nb_create(&p, &c);\n"
+        '  printf("%d %d %s\\n", (int)(sizeof fns / sizeof fns[0]), rc, rc ? nb_last_error() : "");\n'
+        "  return rc == 0;\n}\n")
+    exe = tmp_path / "use_header"
+    libdir = os.path.join(ROOT, "cm3d_amd")
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Wextra", "-Werror", "-I",
+                        os.path.join(ROOT, "include"), str(src), "-o", str(exe), "-L", libdir,
+                        "-lcm3d_nb", f"-Wl,-rpath,{libdir}"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    if os.environ.get("HIP_VISIBLE_DEVICES") is None and not os.path.exists("/dev/kfd"):
+        out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+        n, rc, *msg = out.stdout.split(None, 2)
+        assert int(n) == len(names) and int(rc) != 0 and msg, out.stdout
