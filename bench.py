@@ -63,6 +63,8 @@ def parse():
     ap.add_argument("--serve-rate", type=float, default=20.0, help="mean arrivals per second")
     ap.add_argument("--serve-slots", type=int, default=64)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-c3", action="store_true",
+                    help="skip the full-size (98,304-atom) reciprocal-sum timing")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run only a 256 MiB device copy (counter calibration) and 5 FP64 "
                          "force(...,1) calls on the water system, for rocprofv3 --pmc")
@@ -215,6 +217,37 @@ def run_batch(nb, S, a, torch):
     return out
 
 
+def run_c3_ewald(nb, S, torch):
+    """The reciprocal sum at full size (98,304-atom water, alpha 0.30, kmax
+    32: 68,532 k): force(...,1) phases timed with HIP events on the
+    kernels' stream, the Ewald kernels against the FP64 peak."""
+    s = S.water(nside=32, rc=12.0, alpha=0.30, kmax=32)
+    ctx = nb.NBContext(s, precision="fp64")
+    ctx.build_lists(s["pos"], s["hmat"])
+    d_pos = torch.from_numpy(s["pos"]).cuda()
+    d_fl = torch.zeros_like(d_pos)
+    sp = torch.cuda.current_stream().cuda_stream
+    for _ in range(2):
+        ctx.force_device(1, d_pos.data_ptr(), s["hmat"], 0, d_fl.data_ptr(), stream=sp)
+    torch.cuda.synchronize()
+    ctx.timing_start()
+    for _ in range(5):
+        ctx.force_device(1, d_pos.data_ptr(), s["hmat"], 0, d_fl.data_ptr(), stream=sp)
+    torch.cuda.synchronize()
+    tm = ctx.timing_read()
+    ctx.close()
+    nq = int((s["qch"] != 0).sum())
+    flop = 26.0 * int(s["ktot"]) * nq
+    tf = flop / (tm["ewald"] * 1e-3) / 1e12
+    return {"workload": f"{int(s['natoms'])}-atom water, alpha 0.30, kmax 32 ({int(s['ktot'])} k), "
+                        f"FP64 force(...,1)",
+            "kernels": "k_ewchg+k_ewsfac+k_ewkred+k_red+k_ewforce+k_ewfin", "bound": "valu-fp64",
+            "achieved": round(tf, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(tf / FP64_PEAK_TFLOPS, 5), "ms_per_launch": round(tm["ewald"], 5),
+            "flop_per_launch": flop, "pair_ms": round(tm["pair"], 5),
+            "force_ms": round(tm["total"], 5)}
+
+
 def run_serve(a, torch):
     from cm3d_amd import serve
     jobs = serve.job_stream(a.serve_jobs, a.serve_rate, seed=2027)
@@ -336,6 +369,8 @@ def main():
         "pair_kernel_gbs": round(pair_gbs, 2),
         "ewald_tflops": round(ew_tfs, 4),
     }
+    if not a.no_c3:
+        out["roofline_ewald_c3"] = run_c3_ewald(nb, S, torch)
     if not a.no_batch:
         b = run_batch(nb, S, a, torch)
         bas = b["natoms_total"] * a.batch_steps / b["wall"]
