@@ -926,12 +926,16 @@ double orc_lj_direct(const orc_sys *s, const double *pos, const double *h,
 }
 
 /* Harmonic bonds and bends (energy/intra/md_fbond.c:76-128 case 0,
- * energy/intra/md_fbend.c:126-260 case 0); adds forces, returns energy. */
+ * energy/intra/md_fbend.c:126-260 case 0); adds forces, returns energy.
+ * The energies follow getvbond (md_fbond.c:131-200, case 0: .5*fk*dr*dr)
+ * and getvbend (md_fbend.c:262-340, case 0, with th = 1 when |cos| >= 1),
+ * each in its own running sum, added as geteng adds them
+ * (energy/control/md_fcntr.c:206-207). */
 double orc_bonded_force(int natoms, const orc_bonded *b, const double *pos,
-                        double *f)
+                        double *f, double parts[2])
 {
   const double *x = pos, *y = pos + natoms, *z = pos + 2 * natoms;
-  double *fx = f, *fy = f + natoms, *fz = f + 2 * natoms, v = 0.;
+  double *fx = f, *fy = f + natoms, *fz = f + 2 * natoms, v = 0., va = 0.;
   int nb;
   for (nb = 0; nb < b->nbond; nb++) {
     int i = b->ib[nb], j = b->jb[nb];
@@ -952,11 +956,11 @@ double orc_bonded_force(int natoms, const orc_bonded *b, const double *pos,
     double c = ab * r11, dudc, d1x, d1y, d1z, d3x, d3y, d3z;
     if (fabs(c) >= 1.) {
       dudc = 0.;
-      v += .5 * b->ak[nb] * (1. - b->aeq[nb]) * (1. - b->aeq[nb]);
+      va += .5 * b->ak[nb] * (1. - b->aeq[nb]) * (1. - b->aeq[nb]);
     } else {
       double th = acos(c), dth = th - b->aeq[nb];
       dudc = (b->ak[nb] * dth) * (-1. / sqrt(1. - c * c));
-      v += .5 * b->ak[nb] * dth * dth;
+      va += .5 * b->ak[nb] * dth * dth;
     }
     d1x = c * ax / al - bx * r11; d1y = c * ay / al - by * r11; d1z = c * az / al - bz * r11;
     d3x = c * bx / bl - ax * r11; d3y = c * by / bl - ay * r11; d3z = c * bz / bl - az * r11;
@@ -964,5 +968,6 @@ double orc_bonded_force(int natoms, const orc_bonded *b, const double *pos,
     fx[j] += dudc * (-d1x - d3x); fy[j] += dudc * (-d1y - d3y); fz[j] += dudc * (-d1z - d3z);
     fx[k] += dudc * d3x; fy[k] += dudc * d3y; fz[k] += dudc * d3z;
   }
-  return v;
+  if (parts) { parts[0] = v; parts[1] = va; }
+  return v + va;
 }

@@ -104,8 +104,10 @@ typedef struct orc_bonded {
   const int *ia, *ja, *ka;       /* bends, ja is the centre               */
   const double *ak, *aeq;        /* V = 1/2 k (theta-theta0)^2, radians   */
 } orc_bonded;
+/* parts (may be NULL) gets {vbond, vbend}, summed separately as getvbond /
+ * getvbend do; the return value is their sum, cm3d's potra for this model. */
 double orc_bonded_force(int natoms, const orc_bonded *b, const double *pos,
-                        double *f);
+                        double *f, double parts[2]);
 #ifdef __cplusplus
 }
 #endif

@@ -119,7 +119,7 @@ def lib():
         L.orc_coul_real_direct.restype = C.c_double
         L.orc_lj_direct.argtypes = [C.POINTER(OrcSys), _dp, _dp, _dp, _dp, _dp]
         L.orc_lj_direct.restype = C.c_double
-        L.orc_bonded_force.argtypes = [C.c_int, C.POINTER(OrcBonded), _dp, _dp]
+        L.orc_bonded_force.argtypes = [C.c_int, C.POINTER(OrcBonded), _dp, _dp, _dp]
         L.orc_bonded_force.restype = C.c_double
         _LIB = L
     return _LIB
@@ -241,8 +241,9 @@ def lj_direct(sysd, pos, hmat, eps, sig):
     return v, f
 
 
-def bonded_force(natoms, bonded, pos):
-    """Harmonic bond + bend forces (added to zero) and energy."""
+def bonded_force(natoms, bonded, pos, parts=False):
+    """Harmonic bond + bend forces (added to zero) and energy; with parts,
+    also (vbond, vbend) as cm3d's getvbond / getvbend sum them."""
     B = OrcBonded()
     keep = []
     B.nbond = len(bonded.get("ib", []))
@@ -253,8 +254,10 @@ def bonded_force(natoms, bonded, pos):
         setattr(B, name, ptr)
     p, pp = _d(pos)
     f = np.zeros(3 * natoms)
-    v = lib().orc_bonded_force(int(natoms), C.byref(B), pp, f.ctypes.data_as(_dp))
-    return v, f
+    vp = np.zeros(2)
+    v = lib().orc_bonded_force(int(natoms), C.byref(B), pp, f.ctypes.data_as(_dp),
+                               vp.ctypes.data_as(_dp))
+    return (v, f, (vp[0], vp[1])) if parts else (v, f)
 
 
 # ---------------------------------------------------------------- set-up

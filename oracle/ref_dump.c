@@ -14,6 +14,13 @@
  *   - the tables, cutoffs, k-vectors and exclusion lists built at set-up;
  *   - optionally an nstep trajectory of cm3d's own step() (RESPA + bonded).
  *
+ * With npre > 0 the snapshot is not taken on the deck's own configuration
+ * but after npre steps of cm3d's step() from it (cold-start velocities
+ * included), so that a fixture can hold a thermalised state; the deck's
+ * positions are then dumped as pos_deck and the trajectory's step count
+ * continues from npre (nrefsh keeps the step of the last rebuild in a
+ * static, md_finter.c:48).
+ *
  * Call sites it drives (reference file:line):
  *   md_startup   main/md_startup.new.c:31
  *   readpos      input/md_readpos.c:32
@@ -24,7 +31,7 @@
  *   step_init    integrate/md_step.c:32,  step integrate/md_step.c:62
  *   geteng       energy/control/md_fcntr.c:259, getengn integrate/md_therm.c
  *
- * usage: cm3d_ref_dump deck out.bin [nsteps]
+ * usage: cm3d_ref_dump deck out.bin [nsteps [npre]]
  */
 #include "md.h"
 #include <stdint.h>
@@ -77,14 +84,15 @@ int main(int argc, char *argv[])
   static COORDS coords;
   static ENERGIES energies;
   char *fargv[2];
-  int i, n3, nsteps = 0, ninter;
+  int i, n3, nsteps = 0, npre = 0, ninter;
   double *msave;
 
   if (argc < 3) {
-    fprintf(stderr, "usage: %s deck out.bin [nsteps]\n", argv[0]);
+    fprintf(stderr, "usage: %s deck out.bin [nsteps [npre]]\n", argv[0]);
     return 2;
   }
   if (argc > 3) nsteps = atoi(argv[3]);
+  if (argc > 4) npre = atoi(argv[4]);
   fout = fopen(argv[2], "wb");
   if (!fout) { perror("out"); return 2; }
 
@@ -106,6 +114,16 @@ int main(int argc, char *argv[])
 
   n3 = 3 * simparms.natoms;
   ninter = inter.ntype;
+  if (npre > 0) {
+    int s;
+    put("pos_deck", 'd', n3, coords.px);
+    step_init(&simparms, &coords, &energies, &inter, &ngbr);
+    for (s = 1; s <= npre; s++) {
+      simparms.istep = s;
+      step(&simparms, &coords, &energies, &inter, &ngbr);
+    }
+    puti("npre", npre);
+  }
   puti("natoms", simparms.natoms);
   puti("ntypes", simparms.ntypes);
   puti("ninter", ninter);
@@ -266,7 +284,7 @@ int main(int argc, char *argv[])
     putd("traj_prsi0", energies.prsi);
     putd("traj_tiout0", energies.tiout);
     for (s = 1; s <= nsteps; s++) {
-      simparms.istep = s;
+      simparms.istep = npre + s;
       step(&simparms, &coords, &energies, &inter, &ngbr);
       geteng(&simparms, &coords, &energies);
       sprintf(nm, "traj_pos%d", s); put(nm, 'd', n3, coords.px);

@@ -17,7 +17,10 @@ Cases:
                        per step
   ref_abutane          the reference's test/abutane deck (Williams, erfc
                        real space, Ewald kmax 15, 64 ecorr pairs)
-  ar864                the 864-atom argon FCC workload
+  ar864                the 864-atom argon workload: the FCC deck at 120 K
+                       run 200 steps by cm3d's own step() before the
+                       snapshot, so its forces are those of a thermalised
+                       state, not the lattice's zeros
   water512             512 flexible waters (rc 10, alpha 0.35, kmax 9, RESPA 4)
   mix600               600-atom LJ + charge binary mixture (3 interactions)
   water512_nvt         water512 under a 4-link chain (tau 50 fs), RESPA 4
@@ -60,9 +63,9 @@ def order_digest(i, j):
     return hashlib.sha256(np.stack([i, j], 1).astype(np.int32).tobytes()).hexdigest()
 
 
-def run_ref(workdir, deck, nsteps):
+def run_ref(workdir, deck, nsteps, npre=0):
     out = os.path.join(workdir, "dump.bin")
-    r = subprocess.run([DUMP, deck, out, str(nsteps)], cwd=workdir,
+    r = subprocess.run([DUMP, deck, out, str(nsteps), str(npre)], cwd=workdir,
                        capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(r.stdout[-2000:] + r.stderr[-2000:])
@@ -99,16 +102,16 @@ def ref_case(name, subdir, deck, nsteps):
     save(name, d)
 
 
-def synth_case(name, sysd, kind, nsteps, **kw):
+def synth_case(name, sysd, kind, nsteps, npre=0, **kw):
     with tempfile.TemporaryDirectory() as tmp:
         deck = write_deck(tmp, sysd, kind, nstep=nsteps, **kw)
-        d = run_ref(tmp, deck, nsteps)
+        d = run_ref(tmp, deck, nsteps, npre)
         if kw.get("nhc"):  # the deck itself, for the product's deck reader
             keep = os.path.join(HERE, "decks_" + name)
             shutil.rmtree(keep, ignore_errors=True)
             shutil.copytree(tmp, keep, ignore=shutil.ignore_patterns("dump.bin"))
     # the synthetic positions must round-trip the deck exactly
-    assert np.array_equal(d["pos"], sysd["pos"]), name
+    assert np.array_equal(d["pos_deck" if npre else "pos"], sysd["pos"]), name
     save(name, d)
 
 
@@ -119,24 +122,37 @@ def coul_keys(s):
             rf"\rcute_min{{{float(np.sqrt(s['rmins_coul2']))!r}}}"]
 
 
-def main():
+def cases():
+    w = S.water(nside=8)
+    m = S.mixture(600)
+    return {
+        "ref_lj2": lambda: ref_case("ref_lj2", "lj", "lj.input_ljnve.2", 5),
+        "ref_lj256": lambda: ref_case("ref_lj256", "lj", "lj.input_ljnve", 3),
+        "ref_abutane": lambda: ref_case("ref_abutane", "abutane", "abutane.input", 2),
+        "ref_ljnvt": lambda: ref_case("ref_ljnvt", "lj", "lj.input_ljnvt", 5),
+        "ar864": lambda: synth_case("ar864", S.lj_fcc(), "lj", 3, npre=200, dt=2.0,
+                                    temp=120.0),
+        "water512": lambda: synth_case("water512", w, "water", 2, dt=2.0, nres_intra=4,
+                                       temp=300.0, extra=coul_keys(w)),
+        "mix600": lambda: synth_case("mix600", m, "mixture", 2, dt=2.0, temp=300.0,
+                                     extra=coul_keys(m)),
+        "water512_nvt": lambda: synth_case("water512_nvt", w, "water", 3, dt=2.0,
+                                           nres_intra=4, temp=300.0, extra=coul_keys(w),
+                                           nhc=4, tau_nhc=50.0),
+        "mix600_nvt": lambda: synth_case("mix600_nvt", m, "mixture", 3, dt=2.0, temp=300.0,
+                                         extra=coul_keys(m), nhc=2, tau_nhc=100.0,
+                                         therm=("global", "none")),
+    }
+
+
+def main(only=None):
+    """Write every fixture, or only the named ones."""
     if not os.path.exists(DUMP):
         subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
-    ref_case("ref_lj2", "lj", "lj.input_ljnve.2", 5)
-    ref_case("ref_lj256", "lj", "lj.input_ljnve", 3)
-    ref_case("ref_abutane", "abutane", "abutane.input", 2)
-    ref_case("ref_ljnvt", "lj", "lj.input_ljnvt", 5)
-    synth_case("ar864", S.lj_fcc(), "lj", 3, dt=2.0, temp=120.0)
-    w = S.water(nside=8)
-    synth_case("water512", w, "water", 2, dt=2.0, nres_intra=4, temp=300.0,
-               extra=coul_keys(w))
-    m = S.mixture(600)
-    synth_case("mix600", m, "mixture", 2, dt=2.0, temp=300.0, extra=coul_keys(m))
-    synth_case("water512_nvt", w, "water", 3, dt=2.0, nres_intra=4, temp=300.0,
-               extra=coul_keys(w), nhc=4, tau_nhc=50.0)
-    synth_case("mix600_nvt", m, "mixture", 3, dt=2.0, temp=300.0, extra=coul_keys(m),
-               nhc=2, tau_nhc=100.0, therm=("global", "none"))
+    for name, run in cases().items():
+        if not only or name in only:
+            run()
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

@@ -107,3 +107,26 @@ def test_header_is_plain_c(tmp_path):
         out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
         n, rc, *msg = out.stdout.split(None, 2)
         assert int(n) == len(names) and int(rc) != 0 and msg, out.stdout
+
+
+def test_integration_shim_compiles_against_cm3d_headers(tmp_path):
+    """The binding INTEGRATION.md documents (md_nbgpu.c, the first C block of
+    its section 2) compiles against cm3d's own md.h and this header, so the
+    documented drop-in cannot drift from either side.  Skipped where the
+    reference tree is absent."""
+    import shutil
+    import subprocess
+    ref_inc = "/root/reference/include"
+    if not os.path.isdir(ref_inc) or shutil.which("gcc") is None:
+        pytest.skip("reference headers or gcc not available")
+    doc = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    sec = doc[doc.index("## 2."):]
+    code = sec[sec.index("```c") + 4:]
+    code = code[:code.index("```")]
+    assert "nbgpu_setup" in code and "nb_create" in code
+    src = tmp_path / "md_nbgpu.c"
+    src.write_text(code)
+    r = subprocess.run(["gcc", "-std=gnu99", "-fsyntax-only", "-DLINUX", "-I", ref_inc,
+                        "-I", "/root/reference/minimize", "-I", os.path.join(ROOT, "include"),
+                        str(src)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]

@@ -80,3 +80,76 @@ def test_abutane_setup_matches_printed_golden():
     assert np.array_equal(ka, d["ka"]) and np.array_equal(kc, d["kc"])
     assert np.array_equal(kup[:-1], d["kup"][:-1])
     assert int(d["necorr"][0]) == 64
+
+
+# ---------------------------------------------------------------------------
+# set-up constants and the bonded energies, against compiled cm3d
+
+def _synthetic(name):
+    from cm3d_amd import systems as S
+    return {"ar864": S.lj_fcc, "water512": lambda: S.water(nside=8),
+            "mix600": lambda: S.mixture(600)}[name]()
+
+
+@pytest.mark.parametrize("name", ["water512", "mix600", "ref_abutane", "water512_nvt"])
+def test_ewald_constants_bitwise(name):
+    """orc_ewald_consts gives ewald_setup's self energy and neutralising
+    background (md_ewald.c:59-79) bit for bit.  mix600 and abutane carry a
+    rounding-size net charge, so their background (~1e-22, ~1e-25) pins the
+    -pi (sum q)^2 / (2 alpha^2) form as well."""
+    d = load_fixture(name)
+    self_, bkg = O.ewald_consts(d["qch"], float(d["kappa"][0]))
+    assert self_ == d["ewald_self"][0] and self_ != 0.
+    assert bkg == d["ewald_bkg"][0]
+
+
+@pytest.mark.parametrize("name", ["ar864", "water512", "mix600"])
+def test_lrc_totals_bitwise(name):
+    """orc_lrc_total over the per-interaction tails of orc_table_pair equals
+    simparms->vlrc / wlrc as search_ter_base summed them
+    (md_searchbase.c:212-235), bit for bit (one, one and three LJ
+    interactions; water's O-H and H-H pairs are null)."""
+    d = load_fixture(name)
+    s = _synthetic(name)
+    nt = int(d["ntable"][0])
+    vl, wl = [], []
+    for prm in s["pair_params"]:
+        t = O.table_pair(O.POT_LJ, prm[:2], prm[2], prm[3], prm[4], float(d["rheal"][0]),
+                         float(d["skin"][0]), nt)
+        vl.append(t["vlrc"])
+        wl.append(t["wlrc"])
+    v, w = O.lrc_total(d["itype"], int(d["ntypes"][0]), d["map"], np.array(vl), np.array(wl))
+    assert v == d["vlrc"][0] and w == d["wlrc"][0]
+    assert v < 0. < w
+
+
+@pytest.mark.parametrize("name", ["water512", "water512_nvt"])
+def test_bonded_energies_bitwise(name):
+    """orc_bonded_force's energies are getvbond / getvbend's (V = k/2 dx^2,
+    md_fbond.c:131-200, md_fbend.c:262-340): vbond and vbend at the snapshot,
+    and potra along cm3d's own RESPA trajectory, where the bonds are
+    stretched by ~10^4-10^5 K, bit for bit.  A k-for-k/2 slip doubles them."""
+    d = load_fixture(name)
+    s = _synthetic("water512")
+    b = dict(ib=s["bond_i"], jb=s["bond_j"], bk=s["bond_k"], beq=s["bond_r0"],
+             ia=s["bend_i"], ja=s["bend_j"], ka=s["bend_k"], ak=s["bend_fk"],
+             aeq=s["bend_th0"])
+    n = int(d["natoms"][0])
+    _, _, (vb, va) = O.bonded_force(n, b, d["pos"], parts=True)
+    assert vb == d["e_vbond"][0] and va == d["e_vbend"][0]
+    for st in range(int(d["traj_nsteps"][0]) + 1):
+        v, _, (vb, va) = O.bonded_force(n, b, d[f"traj_pos{st}"], parts=True)
+        assert v == d[f"traj_potra{st}"][0], st
+        if st > 0:
+            assert vb > 1e3 and va > 1e2, (st, vb, va)
+
+
+def test_c1_fixture_is_thermalised():
+    """The 864-atom argon fixture is taken after 200 of cm3d's own steps
+    from the FCC deck at 120 K: its forces are O(10^2) K/A, so force
+    comparisons on it test the pair kernel rather than lattice rounding."""
+    d = load_fixture("ar864")
+    rms = float(np.sqrt(np.mean(d["fxl"] ** 2)))
+    assert 50. < rms < 5e3, rms
+    assert int(d["npre"][0]) == 200
+    assert not np.array_equal(d["pos"], d["pos_deck"])
