@@ -498,21 +498,56 @@ static void cell_grid(nb_ctx *c, const Box &b, int *nx, int *ny, int *nz) {
 static int ensure_capacity(nb_ctx *c, const Box &b, int want_cap) {
   if (want_cap <= c->cap && c->d_nbr) return NB_OK;
   if (c->d_nbr) {
+    // a recorded step of this context may still be running on any stream
+    HCHK(hipDeviceSynchronize());
     hipFree(c->d_nbr);
     c->d_nbr = nullptr;
     c->dev_bytes -= (size_t)c->cap * c->natoms * sizeof(uint32_t);
   }
   c->cap = (want_cap + 63) & ~63;
+  c->gen++;
   return dalloc(c, &c->d_nbr, (size_t)c->cap * c->natoms);
+}
+
+int nb_grow_capacity(nb_ctx *c, int want) {
+  Box b;
+  memset(&b, 0, sizeof(b));
+  return ensure_capacity(c, b, want);
+}
+
+extern "C" int nb_list_reserve(nb_ctx *c, int slots) {
+  if (!c || slots <= 0) { nb_set_error("nb_list_reserve: bad argument"); return NB_ERR_ARG; }
+  HCHK(hipSetDevice(c->dev));
+  HCHK(hipDeviceSynchronize());  // recorded steps may still read the slots
+  if (c->d_nbr) {
+    hipFree(c->d_nbr);
+    c->d_nbr = nullptr;
+    c->dev_bytes -= (size_t)c->cap * c->natoms * sizeof(uint32_t);
+  }
+  c->cap = (slots + 63) & ~63;
+  c->gen++;
+  RC(dalloc(c, &c->d_nbr, (size_t)c->cap * c->natoms));
+  // the old rows are gone: empty rows until the next build, which the
+  // rebuild flag makes the next device step do first
+  HCHK(hipMemset(c->d_nnb, 0, sizeof(int) * c->natoms));
+  const int one = 1;
+  HCHK(hipMemcpy(c->d_flags + 1, &one, sizeof(int), hipMemcpyHostToDevice));
+  c->have_lists = 0;
+  return NB_OK;
 }
 
 int nb_build_now(nb_ctx *c, const double hmat[9], hipStream_t s) {
   Box b = make_box(hmat);
   int nx, ny, nz;
+  const int rx = c->crx, ry = c->cry, rz = c->crz;
   cell_grid(c, b, &nx, &ny, &nz);
+  if (nx != c->ncx || ny != c->ncy || nz != c->ncz || rx != c->crx || ry != c->cry ||
+      rz != c->crz)
+    c->gen++;
   c->ncx = nx; c->ncy = ny; c->ncz = nz;
   c->ncells = nx * ny * nz;
   if (c->ncells > c->cells_alloc) {
+    if (c->d_ccount) HCHK(hipDeviceSynchronize());
     if (c->d_ccount) c->dev_bytes -= 4 * (size_t)c->cells_alloc * sizeof(int);
     if (c->d_ccount) hipFree(c->d_ccount);
     if (c->d_cstart) hipFree(c->d_cstart);
@@ -525,21 +560,31 @@ int nb_build_now(nb_ctx *c, const double hmat[9], hipStream_t s) {
     RC(dalloc(c, &c->d_cwork, c->ncells));
     HCHK(hipMemsetAsync(c->d_cwork, 0, c->ncells * sizeof(int), s));
     c->cells_alloc = c->ncells;
+    c->gen++;
   }
-  if (c->cap == 0) {
-    // expected full-list length from the mean density, with a margin
+  {
+    // expected full-list length from the mean density: the initial slots
+    // (with a margin) and the mixed kernel's lanes per atom
     double vol = fabs(det9(hmat));
     double nexp = 4.18879 * c->rlist * c->rlist * c->rlist * c->natoms / vol;
-    int want = (int)(1.6 * nexp) + 64;
-    if (want > c->natoms) want = c->natoms + 63;
-    c->mix_g16 = nexp < NB_MIX_G16_MAX;
-    RC(ensure_capacity(c, b, want));
+    const int g16 = nexp < NB_MIX_G16_MAX;
+    if (g16 != c->mix_g16) c->gen++;
+    c->mix_g16 = g16;
+    if (c->cap == 0) {
+      int want = (int)(1.6 * nexp) + 64;
+      if (want > c->natoms) want = c->natoms + 63;
+      RC(ensure_capacity(c, b, want));
+    }
   }
+  // a host build grows the slots itself: start the sticky overflow record
+  // afresh (device-side builds from here on are what it reports)
+  HCHK(hipMemsetAsync(c->d_flags + 2, 0, sizeof(int), s));
   for (int attempt = 0; attempt < 3; attempt++) {
     RC(nb_launch_build(c, b, s));
     HCHK(hipMemcpyAsync(c->h_flags, c->d_flags, 8 * sizeof(int), hipMemcpyDeviceToHost, s));
     HCHK(hipStreamSynchronize(s));
     if (c->h_flags[0] <= c->cap) {
+      HCHK(hipMemsetAsync(c->d_flags + 2, 0, sizeof(int), s));
       c->box_built = b;
       c->have_lists = 1;
       return NB_OK;
@@ -621,12 +666,13 @@ void nb_fill_energy(nb_ctx *c, int iflg, const double *E, const double hmat[9],
   }
 }
 
+// want_e: the caller reads the energies (else a forces-only evaluation)
 static int force_on(nb_ctx *c, int iflg, const double hmat[9], double *d_fs, double *d_fl,
-                    nb_energy *e, hipStream_t s) {
+                    nb_energy *e, hipStream_t s, bool want_e) {
   if (iflg != 0 && iflg != 1) { nb_set_error("nb_force: iflg must be 0 or 1"); return NB_ERR_ARG; }
   if (!c->have_lists) { nb_set_error("nb_force: lists not built (call nb_build_lists)"); return NB_ERR_ARG; }
   Box b = make_box(hmat);
-  RC(nb_launch_force(c, iflg, b, d_fs, d_fl, s, 0));
+  RC(nb_launch_force(c, iflg, b, d_fs, d_fl, s, want_e ? 0 : NB_FORCE_NOENER));
   if (e) {
     double E[64];
     HCHK(hipMemcpyAsync(E, c->d_eout, sizeof(E), hipMemcpyDeviceToHost, s));
@@ -645,7 +691,7 @@ extern "C" int nb_force(nb_ctx *c, int iflg, const double *pxyz, const double hm
   HCHK(hipSetDevice(c->dev));
   const size_t bytes = sizeof(double) * 3 * c->natoms;
   HCHK(hipMemcpyAsync(c->d_pos, pxyz, bytes, hipMemcpyHostToDevice, c->stream));
-  RC(force_on(c, iflg, hmat, c->d_fs, iflg ? c->d_fl : nullptr, nullptr, c->stream));
+  RC(force_on(c, iflg, hmat, c->d_fs, iflg ? c->d_fl : nullptr, nullptr, c->stream, e != nullptr));
   HCHK(hipMemcpyAsync(fs, c->d_fs, bytes, hipMemcpyDeviceToHost, c->stream));
   if (iflg == 1) HCHK(hipMemcpyAsync(fl, c->d_fl, bytes, hipMemcpyDeviceToHost, c->stream));
   if (e) {
@@ -670,7 +716,7 @@ extern "C" int nb_force_device(nb_ctx *c, int iflg, const double *d_pxyz, const 
   if (d_pxyz != c->d_pos)
     HCHK(hipMemcpyAsync(c->d_pos, d_pxyz, sizeof(double) * 3 * c->natoms,
                         hipMemcpyDeviceToDevice, s));
-  return force_on(c, iflg, hmat, d_fs, iflg ? d_fl : nullptr, e, s);
+  return force_on(c, iflg, hmat, d_fs, iflg ? d_fl : nullptr, e, s, e != nullptr);
 }
 
 // getvireal(...,0)'s pair sums (getvirter / getvirter_e over the t and te

@@ -233,8 +233,14 @@ struct nb_ctx {
   uint32_t *d_nbr;
   int *d_nnb;
   int cap;
-  int *d_flags;                // [0] overflow max count, [1] rebuild flag,
-                               // [6] builds done, [7] k_cell's ticket
+  int *d_flags;                // [0] overflow max count of the last build,
+                               // [1] rebuild flag, [2] overflow max since the
+                               // host last cleared it (sticky over a batch
+                               // tick), [6] builds done, [7] k_cell's ticket
+  // Bumped whenever a buffer or a launch shape that a recorded step (a
+  // captured graph, a batch's argument arrays) refers to is reallocated or
+  // changed: the neighbour slots, the cell arrays or the cell grid.
+  unsigned gen;
   int ncx, ncy, ncz, ncells, cells_alloc;
   int crx, cry, crz;           // cell reach of the list stencil per axis (0..3)
   double *d_chg, *d_stp, *d_spart, *d_kd, *d_fpart;
@@ -271,6 +277,8 @@ int nb_build_now(nb_ctx *c, const double hmat[9], hipStream_t s);
 #define NB_FORCE_NO_PAIR 8    // leave out the pair part (pack + pair kernel)
 #define NB_FORCE_NO_RECIP 16  // leave out the reciprocal sum (charge records .. k_ewforce)
 #define NB_FORCE_NO_FIN 32    // leave out k_ewfin (ecorr + reciprocal forces into fl) and k_red
+#define NB_FORCE_NOENER 64    // forces only: no pair energies / virials (the MD step's
+                              // force evaluation; its energies are evaluated on request)
 int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs,
                     double *d_fl, hipStream_t s, int flags);
 int nb_launch_rebuild_check(nb_ctx *c, const double *d_pos, hipStream_t s);
@@ -284,4 +292,8 @@ static inline double nb_det9(const double h[9]) {
 void nb_time_mark(nb_ctx *c, int ev, hipStream_t s);
 int nb_launch_pair_mixed(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_fl,
                          hipStream_t s, bool packed);
+struct RedSet;
+// the energy-column reductions of a force evaluation; pair: the pair
+// kernel's columns were written (not a forces-only evaluation)
+RedSet nb_red_set(const nb_ctx *c, int iflg, bool pair = true);
 void nb_set_error(const char *fmt, ...);

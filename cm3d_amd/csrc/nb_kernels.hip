@@ -435,7 +435,10 @@ static __device__ __forceinline__ void k_list_body(const ListArgs &A, int bid) {
   hte = nb_wave_sum(hte);
   if (lane == 0) {
     A.nnb[s] = count < A.cap ? count : A.cap;
-    if (count > A.cap) atomicMax(&A.flags[0], count);
+    if (count > A.cap) {
+      atomicMax(&A.flags[0], count);
+      atomicMax(&A.flags[2], count);  // sticky until the host clears it
+    }
     A.hcnt[s] = make_int4(hs, ht, hse, hte);
   }
 }
@@ -485,8 +488,11 @@ static __device__ __forceinline__ double interp3c(const double *t, int k, int c,
 
 // The atom loop.  TLDS: the tables (vt / ct) and the interaction parameters
 // (sip / smap) are all in LDS, so the reads compile to LDS reads rather than
-// generic (flat) ones; otherwise any of them may be in HBM.
-template <int IFLG, bool WANT_FS, bool TLDS>
+// generic (flat) ones; otherwise any of them may be in HBM.  ENER: energies,
+// virials and the clamp count are summed (else forces only: the MD step,
+// whose energies are evaluated when they are asked for); WF: forces are
+// written (else energies only).
+template <int IFLG, bool WANT_FS, bool TLDS, bool ENER, bool WF>
 static __device__ __forceinline__ void pair64_run(const PairArgs &A, int bid, const double *vt,
                                                   const double *ct, const InterPar *sip,
                                                   const int *smap, bool ip_lds) {
@@ -539,7 +545,7 @@ static __device__ __forceinline__ void pair64_run(const PairArgs &A, int bid, co
           if (d2 < rmax2) {
             double p = (d2 - P.rmins2) * P.dx2;
             int kk = (int)p;
-            if (kk < 1) { acc[E_CLAMP] += 1.; kk = 1; p = 1.; }
+            if (kk < 1) { if (ENER) acc[E_CLAMP] += 1.; kk = 1; p = 1.; }
             p = p - (double)kk;
             const double *T = vt + (size_t)in * A.ntable * 3;
             double f = interp3c(T, kk, 0, p);
@@ -548,18 +554,20 @@ static __device__ __forceinline__ void pair64_run(const PairArgs &A, int bid, co
             {
 #pragma clang fp contract(fast)
               if (IFLG) {
-                acc[E_SPRSL] -= d2 * f;
-                acc[E_SL0] -= dx * dx * f;
-                acc[E_SL1] -= dy * dx * f;
-                acc[E_SL2] -= dz * dx * f;
-                acc[E_SL4] -= dy * dy * f;
-                acc[E_SL5] -= dz * dy * f;
-                acc[E_SL8] -= dz * dz * f;
-                fx -= dx * f; fy -= dy * f; fz -= dz * f;
+                if (ENER) {
+                  acc[E_SPRSL] -= d2 * f;
+                  acc[E_SL0] -= dx * dx * f;
+                  acc[E_SL1] -= dy * dx * f;
+                  acc[E_SL2] -= dz * dx * f;
+                  acc[E_SL4] -= dy * dy * f;
+                  acc[E_SL5] -= dz * dy * f;
+                  acc[E_SL8] -= dz * dz * f;
+                  double v = interp3c(T, kk, 2, p);
+                  acc[E_SPOTL] += v;
+                  acc[E_SPOTHL] += v - T[3 * ntm4 + 2];
+                }
+                if (WF) { fx -= dx * f; fy -= dy * f; fz -= dz * f; }
                 if (WANT_FS) { sx -= dx * f * swv; sy -= dy * f * swv; sz -= dz * f * swv; }
-                double v = interp3c(T, kk, 2, p);
-                acc[E_SPOTL] += v;
-                acc[E_SPOTHL] += v - T[3 * ntm4 + 2];
               } else {
                 f *= swv;
                 sx -= dx * f; sy -= dy * f; sz -= dz * f;
@@ -570,24 +578,26 @@ static __device__ __forceinline__ void pair64_run(const PairArgs &A, int bid, co
         if ((fl & fe) && d2 < A.rmaxl_coul2) {
           double p = (d2 - A.rmins_coul2) * A.dx2coul;
           int kk = (int)p;
-          if (kk < 1) { acc[E_CLAMP] += 1.; kk = 1; p = 1; }
+          if (kk < 1) { if (ENER) acc[E_CLAMP] += 1.; kk = 1; p = 1; }
           p = p - (double)kk;
           {
 #pragma clang fp contract(fast)
             double qij = ri.w * rj.w;
             double fcc = qij * interp3c(ct, kk, 0, p);
             if (IFLG) {
-              acc[E_SPRSE] -= d2 * fcc;
-              acc[E_SE0] -= dx * dx * fcc;
-              acc[E_SE1] -= dy * dx * fcc;
-              acc[E_SE2] -= dz * dx * fcc;
-              acc[E_SE4] -= dy * dy * fcc;
-              acc[E_SE5] -= dz * dy * fcc;
-              acc[E_SE8] -= dz * dz * fcc;
-              fx -= dx * fcc; fy -= dy * fcc; fz -= dz * fcc;
-              double v = interp3c(ct, kk, 2, p);
-              acc[E_SPOTHE] += qij * (v - A.clast);
-              acc[E_SPOTE] += qij * v;
+              if (ENER) {
+                acc[E_SPRSE] -= d2 * fcc;
+                acc[E_SE0] -= dx * dx * fcc;
+                acc[E_SE1] -= dy * dx * fcc;
+                acc[E_SE2] -= dz * dx * fcc;
+                acc[E_SE4] -= dy * dy * fcc;
+                acc[E_SE5] -= dz * dy * fcc;
+                acc[E_SE8] -= dz * dz * fcc;
+                double v = interp3c(ct, kk, 2, p);
+                acc[E_SPOTHE] += qij * (v - A.clast);
+                acc[E_SPOTE] += qij * v;
+              }
+              if (WF) { fx -= dx * fcc; fy -= dy * fcc; fz -= dz * fcc; }
             }
             if (WANT_FS || !IFLG) { sx -= dx * fcc; sy -= dy * fcc; sz -= dz * fcc; }
           }
@@ -597,19 +607,21 @@ static __device__ __forceinline__ void pair64_run(const PairArgs &A, int bid, co
       rjc = rjn;
       tjc = tjn;
     }
-    const int ai = A.perm[s];
-    if (IFLG) { fx = nb_wave_sum(fx); fy = nb_wave_sum(fy); fz = nb_wave_sum(fz); }
-    if (WANT_FS || !IFLG) { sx = nb_wave_sum(sx); sy = nb_wave_sum(sy); sz = nb_wave_sum(sz); }
-    if (lane == 0) {
-      const int n = A.n;
-      if (IFLG) { A.fl[ai] = fx; A.fl[n + ai] = fy; A.fl[2 * n + ai] = fz; }
-      if (WANT_FS || !IFLG) { A.fs[ai] = sx; A.fs[n + ai] = sy; A.fs[2 * n + ai] = sz; }
+    if (WF) {
+      const int ai = A.perm[s];
+      if (IFLG) { fx = nb_wave_sum(fx); fy = nb_wave_sum(fy); fz = nb_wave_sum(fz); }
+      if (WANT_FS || !IFLG) { sx = nb_wave_sum(sx); sy = nb_wave_sum(sy); sz = nb_wave_sum(sz); }
+      if (lane == 0) {
+        const int n = A.n;
+        if (IFLG) { A.fl[ai] = fx; A.fl[n + ai] = fy; A.fl[2 * n + ai] = fz; }
+        if (WANT_FS || !IFLG) { A.fs[ai] = sx; A.fs[n + ai] = sy; A.fs[2 * n + ai] = sz; }
+      }
     }
   }
-  block_reduce_store<NB_NE_PAIR>(acc, A.epart + bid, A.nblocks);
+  if (ENER) block_reduce_store<NB_NE_PAIR>(acc, A.epart + bid, A.nblocks);
 }
 
-template <int IFLG, bool WANT_FS>
+template <int IFLG, bool WANT_FS, bool ENER, bool WF>
 static __device__ __forceinline__ void pair64_body(const PairArgs &A, int bid) {
   extern __shared__ double tabs[];
   __shared__ InterPar sip[NB_IP_LDS];
@@ -626,18 +638,24 @@ static __device__ __forceinline__ void pair64_body(const PairArgs &A, int bid) {
   }
   __syncthreads();
   if (A.tab_lds && ip_lds)
-    pair64_run<IFLG, WANT_FS, true>(A, bid, tabs, tabs + A.ctab_off, sip, smap, ip_lds);
+    pair64_run<IFLG, WANT_FS, true, ENER, WF>(A, bid, tabs, tabs + A.ctab_off, sip, smap, ip_lds);
   else if (A.tab_lds)
-    pair64_run<IFLG, WANT_FS, false>(A, bid, tabs, tabs + A.ctab_off, sip, smap, ip_lds);
+    pair64_run<IFLG, WANT_FS, false, ENER, WF>(A, bid, tabs, tabs + A.ctab_off, sip, smap, ip_lds);
   else
-    pair64_run<IFLG, WANT_FS, false>(A, bid, A.vtab3, A.ctab3, sip, smap, ip_lds);
+    pair64_run<IFLG, WANT_FS, false, ENER, WF>(A, bid, A.vtab3, A.ctab3, sip, smap, ip_lds);
 }
-#define k_pair64_0t_body pair64_body<0, true>
-#define k_pair64_1t_body pair64_body<1, true>
-#define k_pair64_1f_body pair64_body<1, false>
+// 0t: force(...,0) (fs, clamp count); 1t: force(...,1) with fs and energies;
+// 1f: fl and energies; 1n: fl only (the MD step); 1e: energies only
+#define k_pair64_0t_body pair64_body<0, true, true, true>
+#define k_pair64_1t_body pair64_body<1, true, true, true>
+#define k_pair64_1f_body pair64_body<1, false, true, true>
+#define k_pair64_1n_body pair64_body<1, false, false, true>
+#define k_pair64_1e_body pair64_body<1, false, true, false>
 NB_KERNEL_PAIR_W(k_pair64_0t, PairArgs, 256, 4)
 NB_KERNEL_PAIR_W(k_pair64_1t, PairArgs, 256, 3)
 NB_KERNEL_PAIR_W(k_pair64_1f, PairArgs, 256, 4)
+NB_KERNEL_PAIR_W(k_pair64_1n, PairArgs, 256, 4)
+NB_KERNEL_PAIR_W(k_pair64_1e, PairArgs, 256, 4)
 
 // ---------------------------------------------------------------------------
 // Reciprocal-space Ewald, FP64 throughout (fk_ewald, md_ewald.c:185-361).
@@ -1161,12 +1179,12 @@ int nb_launch_rebuild_check(nb_ctx *c, const double *d_pos, hipStream_t s) {
 }
 
 // The energy-column reductions of force(...,1): which tables, how many rows.
-RedSet nb_red_set(const nb_ctx *c, int iflg) {
+RedSet nb_red_set(const nb_ctx *c, int iflg, bool pair) {
   RedSet R;
   memset(&R, 0, sizeof(R));
   const bool ec = iflg == 1 && c->iperd == 3 && c->necorr > 0;
   const bool ew = iflg == 1 && c->iperd == 3 && c->ktot > 0 && c->ncharge > 0;
-  R.part[0] = c->d_epair; R.rows[0] = c->npair_blocks; R.cols[0] = NB_NE_PAIR;
+  R.part[0] = c->d_epair; R.rows[0] = c->npair_blocks; R.cols[0] = pair ? NB_NE_PAIR : 0;
   R.out[0] = c->d_eout;
   R.part[1] = c->d_eec; R.rows[1] = c->nfin_blocks; R.cols[1] = ec ? NB_NE_ECORR : 0;
   R.out[1] = c->d_eout + 20;
@@ -1179,40 +1197,54 @@ RedSet nb_red_set(const nb_ctx *c, int iflg) {
 // force(...,1) / force(...,0).  flags: NB_FORCE_PACKED -- the sorted records
 // and the per-charge Ewald data are already current (the MD step writes
 // them); NB_FORCE_NORED -- leave the energy columns unreduced (the MD kick
-// reduces them).
+// reduces them); NB_FORCE_NOENER -- forces only; NB_FORCE_EONLY -- energies
+// and virials only.  Pair energies and virials are always evaluated in FP64
+// (k_pair64_1f / 1t / 1e): a mixed context computes its forces with the FP32
+// kernel and, when energies are asked for, runs the FP64 energies-only
+// kernel beside it.
 int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_fl,
                     hipStream_t s, int flags) {
   const int n = c->natoms;
   const bool packed = flags & NB_FORCE_PACKED;
+  const bool eonly = flags & NB_FORCE_EONLY, noener = (flags & NB_FORCE_NOENER) && !eonly;
+  const bool mixed = c->prec == NB_PREC_MIXED;
   nb_time_mark(c, 0, s);
   if (flags & NB_FORCE_NO_PAIR) {
     nb_time_mark(c, 1, s);
-  } else if (c->prec == NB_PREC_MIXED) {
-    nb_time_mark(c, 1, s);
-    int rc = nb_launch_pair_mixed(c, iflg, b, d_fs, d_fl, s, packed);
-    if (rc) return rc;
   } else {
-    Pack64Args pa = {c->d_pos, c->d_qch, c->d_perm, n, (double4 *)c->d_rec64};
-    NB_GO(c, K_PACK64, k_pack64, packed ? 0 : blocks(n, 256), 256, 0, pa, s);
-    nb_time_mark(c, 1, s);
-    PairArgs P;
-    P.n = n; P.ntypes = c->ntypes; P.ninter = c->ninter; P.ntable = c->ntable;
-    P.cap = c->cap; P.tab_lds = c->tab_lds_bytes > 0; P.ctab_off = c->ctab_off;
-    P.nblocks = c->npair_blocks;
-    P.nbr = c->d_nbr; P.nnb = c->d_nnb; P.stype = c->d_stype; P.perm = c->d_perm;
-    P.map = c->d_map; P.rec = (const double4 *)c->d_rec64; P.ip = c->d_ip;
-    P.vtab3 = c->d_vtab3; P.ctab3 = c->d_ctab3;
-    P.rmins_coul2 = c->rmins_coul2; P.dx2coul = c->dx2coul; P.rmaxl_coul2 = c->rmaxl_coul2;
-    P.clast = c->clast;
-    P.b = b; P.fl = d_fl; P.fs = d_fs; P.epart = c->d_epair;
-    const int g = c->npair_blocks, lds = c->tab_lds_bytes;
-    if (iflg == 0)
-      NB_GO(c, K_PAIR64_0T, k_pair64_0t, g, 256, lds, P, s);
-    else if (d_fs)
-      NB_GO(c, K_PAIR64_1T, k_pair64_1t, g, 256, lds, P, s);
-    else
-      NB_GO(c, K_PAIR64_1F, k_pair64_1f, g, 256, lds, P, s);
-    if (!c->rec) LCHK(hipGetLastError());
+    if (mixed && !eonly) {
+      nb_time_mark(c, 1, s);
+      int rc = nb_launch_pair_mixed(c, iflg, b, d_fs, d_fl, s, packed);
+      if (rc) return rc;
+    }
+    if (!mixed || (iflg == 1 && !noener)) {
+      // a mixed context's step keeps only the FP32 records current
+      Pack64Args pa = {c->d_pos, c->d_qch, c->d_perm, n, (double4 *)c->d_rec64};
+      NB_GO(c, K_PACK64, k_pack64, packed && !mixed ? 0 : blocks(n, 256), 256, 0, pa, s);
+      if (!mixed) nb_time_mark(c, 1, s);
+      PairArgs P;
+      P.n = n; P.ntypes = c->ntypes; P.ninter = c->ninter; P.ntable = c->ntable;
+      P.cap = c->cap; P.tab_lds = c->tab_lds_bytes > 0; P.ctab_off = c->ctab_off;
+      P.nblocks = c->npair_blocks;
+      P.nbr = c->d_nbr; P.nnb = c->d_nnb; P.stype = c->d_stype; P.perm = c->d_perm;
+      P.map = c->d_map; P.rec = (const double4 *)c->d_rec64; P.ip = c->d_ip;
+      P.vtab3 = c->d_vtab3; P.ctab3 = c->d_ctab3;
+      P.rmins_coul2 = c->rmins_coul2; P.dx2coul = c->dx2coul; P.rmaxl_coul2 = c->rmaxl_coul2;
+      P.clast = c->clast;
+      P.b = b; P.fl = d_fl; P.fs = d_fs; P.epart = c->d_epair;
+      const int g = c->npair_blocks, lds = c->tab_lds_bytes;
+      if (mixed || eonly)
+        NB_GO(c, K_PAIR64_1E, k_pair64_1e, g, 256, lds, P, s);
+      else if (iflg == 0)
+        NB_GO(c, K_PAIR64_0T, k_pair64_0t, g, 256, lds, P, s);
+      else if (d_fs)
+        NB_GO(c, K_PAIR64_1T, k_pair64_1t, g, 256, lds, P, s);
+      else if (noener)
+        NB_GO(c, K_PAIR64_1N, k_pair64_1n, g, 256, lds, P, s);
+      else
+        NB_GO(c, K_PAIR64_1F, k_pair64_1f, g, 256, lds, P, s);
+      if (!c->rec) LCHK(hipGetLastError());
+    }
   }
   nb_time_mark(c, 2, s);
   nb_time_mark(c, 3, s);
@@ -1244,7 +1276,7 @@ int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_f
                   c->d_eec};
   NB_GO(c, K_EWFIN, k_ewfin, fin && (ewf || ec) ? c->nfin_blocks : 0, 256, 0, fa, s);
   if (fin && !(flags & NB_FORCE_NORED)) {
-    RedSet R = nb_red_set(c, iflg);
+    RedSet R = nb_red_set(c, iflg, !noener && !(flags & NB_FORCE_NO_PAIR));
     NB_GO(c, K_RED, k_red, R.ncols, 256, 0, R, s);
   }
   if (!c->rec) LCHK(hipGetLastError());
@@ -1268,6 +1300,8 @@ static struct RegisterKernels {
     g_batch_fn[K_PAIR64_0T] = k_pair64_0t_bl;
     g_batch_fn[K_PAIR64_1T] = k_pair64_1t_bl;
     g_batch_fn[K_PAIR64_1F] = k_pair64_1f_bl;
+    g_batch_fn[K_PAIR64_1N] = k_pair64_1n_bl;
+    g_batch_fn[K_PAIR64_1E] = k_pair64_1e_bl;
     g_batch_fn[K_RED] = k_red_bl;
     g_batch_fn[K_EWCHG] = k_ewchg_bl;
     g_batch_fn[K_EWSFAC] = k_ewsfac_bl;

@@ -55,6 +55,15 @@ struct nb_md {
   // mnh[nc]; d_ppart: per-workgroup kinetic sums; d_ticket: last-workgroup
   // counter (zero between launches)
   int fork;      // run the reciprocal sum on the context's second stream
+  // what the captured graph was recorded against: the context's generation
+  // and this state's (bumped when the thermostat changes the kernel
+  // sequence or reallocates its buffers); a mismatch re-captures
+  unsigned tgen, graph_cgen, graph_tgen;
+  // snapshot of the dynamic state (pos, vel, aa, fl: 12N; then veta, eta)
+  // taken at the start of a batch tick, restored when a list build in the
+  // tick overflowed
+  double *d_save;
+  int used;      // a step has been recorded or launched (it may be in flight)
   int nchain;
   double *d_nhc, *d_ppart;
   int *d_ithm, *d_ticket;
@@ -683,7 +692,7 @@ extern "C" void nb_md_destroy(nb_md *md) {
   void *ptrs[] = {md->d_vel, md->d_rm, md->d_aa, md->d_amass, md->d_moff, md->d_boff,
                   md->d_aoff, md->d_bi, md->d_bj, md->d_ai, md->d_aj, md->d_ak, md->d_bk,
                   md->d_br0, md->d_afk, md->d_ath0, md->d_ep, md->d_eo, md->d_nhc,
-                  md->d_ppart, md->d_ithm, md->d_ticket};
+                  md->d_ppart, md->d_ithm, md->d_ticket, md->d_save};
   for (void *p : ptrs)
     if (p) hipFree(p);
   delete md;
@@ -782,6 +791,8 @@ extern "C" int nb_md_create(nb_ctx *c, const nb_md_params *p, nb_md **out) {
   md->ep_blocks = (nm + 255) / 256;
   MUP(&md->d_ep, (const double *)nullptr, MD_NE * (size_t)md->ep_blocks);
   MUP(&md->d_eo, (const double *)nullptr, 8);
+  // a batch tick's snapshot: pos, vel, aa, fl (12N) and the chain's veta, eta
+  MUP(&md->d_save, (const double *)nullptr, 12 * (size_t)n + 2 * MD_MAXCHAIN);
 #undef MUP
   md->bytes = c->dev_bytes - bytes0;
   (void)g_dummy;
@@ -795,10 +806,15 @@ extern "C" int nb_md_set_state(nb_md *md, const double *pos, const double *vel,
   nb_ctx *c = md->c;
   MCHK(hipSetDevice(c->dev));
   // steps may still run on any stream (a batch's, a caller's): wait for the
-  // device before overwriting the state and dropping the captured graph
-  MCHK(hipDeviceSynchronize());
-  MCHK(hipMemcpy(c->d_pos, pos, sizeof(double) * 3 * md->n, hipMemcpyHostToDevice));
-  MCHK(hipMemcpy(md->d_vel, vel, sizeof(double) * 3 * md->n, hipMemcpyHostToDevice));
+  // device before overwriting the state and dropping the captured graph (a
+  // state that never stepped has nothing in flight, and a server admitting
+  // it must not wait for the running batch)
+  if (md->used) MCHK(hipDeviceSynchronize());
+  MCHK(hipMemcpyAsync(c->d_pos, pos, sizeof(double) * 3 * md->n, hipMemcpyHostToDevice,
+                      c->stream));
+  MCHK(hipMemcpyAsync(md->d_vel, vel, sizeof(double) * 3 * md->n, hipMemcpyHostToDevice,
+                      c->stream));
+  MCHK(hipStreamSynchronize(c->stream));
   memcpy(md->hmat, hmat, sizeof(md->hmat));
   md_drop_graph(md);
   return NB_OK;
@@ -824,9 +840,12 @@ extern "C" int nb_md_set_thermostat(nb_md *md, int nchain, const double *gkt, co
   }
   nb_ctx *c = md->c;
   MCHK(hipSetDevice(c->dev));
-  MCHK(hipStreamSynchronize(c->stream));
+  // a step of this state may be running on any stream (a batch's, a
+  // caller's): wait for the device before freeing the chain's buffers
+  MCHK(hipDeviceSynchronize());
   md_drop_graph(md);
   md_free_nhc(md);
+  md->tgen++;  // the kernel sequence and the chain's buffers change
   if (nchain == 0) return NB_OK;
   if (!gkt || !mnh || !veta0 || !ithm) {
     nb_set_error("nb_md_set_thermostat: null argument");
@@ -882,13 +901,16 @@ extern "C" int nb_md_get_thermostat(nb_md *md, double *veta, double *eta) {
   return NB_OK;
 }
 
-RedSet nb_red_set(const nb_ctx *c, int iflg);
 
 // step_init (md_step.c:32-58): lists, force(...,1), force(...,2).
 extern "C" int nb_md_init(nb_md *md) {
   if (!md) { nb_set_error("nb_md_init: null argument"); return NB_ERR_ARG; }
   nb_ctx *c = md->c;
   MCHK(hipSetDevice(c->dev));
+  if (md->used) {  // a step of this state may be running on any stream
+    MCHK(hipDeviceSynchronize());
+    md_drop_graph(md);
+  }
   int rc = nb_build_now(c, md->hmat, c->stream);
   if (rc) return rc;
   md->box = c->box_built;
@@ -906,6 +928,7 @@ extern "C" int nb_md_init(nb_md *md) {
 
 int md_enqueue_step(nb_md *md, hipStream_t s) {
   nb_ctx *c = md->c;
+  md->used = 1;
   // inner RESPA loop + nrefsh test + records for the force evaluation
   InnerArgs A = inner_args(md);
   const int lds = md->maxa > 3 ? MD_TPB * 9 * MD_MAXA * (int)sizeof(double) : 0;
@@ -928,7 +951,9 @@ int md_enqueue_step(nb_md *md, hipStream_t s) {
   // list build and the pair kernel (a fork / join in the captured graph),
   // and k_ewfin joins the two.  A batch records the launches in one
   // sequence (its reciprocal sum runs after the pair kernel).
-  const int fp = NB_FORCE_PACKED | NB_FORCE_NORED;
+  // forces only: the step's energies are evaluated when asked for
+  // (nb_md_energies), as cm3d's output steps read them
+  const int fp = NB_FORCE_PACKED | NB_FORCE_NORED | NB_FORCE_NOENER;
   int rc;
   if (md->fork && !c->rec) {
     MCHK(hipEventRecord(c->fork_ev[0], s));
@@ -954,10 +979,114 @@ int md_enqueue_step(nb_md *md, hipStream_t s) {
   // kick + energy-column reductions + flag reset
   const int nkick = (3 * md->n + 255) / 256;
   KickArgs K = {md->d_vel, c->d_fl, md->d_rm, md->n, nkick, md->dter, c->d_flags + 1,
-                nb_red_set(c, 1)};
+                nb_red_set(c, 1, false)};
   NB_GO(c, K_MDKICK, k_mdkick, nkick + K.R.ncols, 256, 0, K, s);
   if (!c->rec) MCHK(hipGetLastError());
   return NB_OK;
+}
+
+// Snapshot / restore of the dynamic state around a batch tick: pos, vel,
+// aa (inner accelerations), fl (the long force the next step's first kick
+// uses) and the chain's veta, eta.  The snapshot also clears the sticky
+// overflow flag, so after the tick it says whether any list build of the
+// tick overflowed.
+struct SaveArgs {
+  double *pos, *vel, *aa, *fl, *nhc, *save;
+  int n, nnhc, restore, nblocks;
+  int *flags;
+};
+static __device__ __forceinline__ void k_mdsave_body(const SaveArgs &A, int bid) {
+  if (bid == 0 && threadIdx.x == 0 && !A.restore) A.flags[2] = 0;
+  const int i = bid * blockDim.x + threadIdx.x, n3 = 3 * A.n;
+  double *st;
+  int k;
+  if (i < n3) { st = A.pos; k = i; }
+  else if (i < 2 * n3) { st = A.vel; k = i - n3; }
+  else if (i < 3 * n3) { st = A.aa; k = i - 2 * n3; }
+  else if (i < 4 * n3) { st = A.fl; k = i - 3 * n3; }
+  else if (i < 4 * n3 + A.nnhc) { st = A.nhc; k = i - 4 * n3; }
+  else return;
+  if (A.restore) st[k] = A.save[i];
+  else A.save[i] = st[k];
+}
+NB_KERNEL_PAIR(k_mdsave, SaveArgs, 256)
+
+// Per system of a batch: out[0] = the largest row a list build of the tick
+// wanted, if it exceeded the slots (0 otherwise).
+struct OvfArgs { const int *flags; int cap; int *out; };
+static __device__ __forceinline__ void k_ovfchk_body(const OvfArgs &A, int bid) {
+  if (bid == 0 && threadIdx.x == 0) {
+    const int m = A.flags[2];
+    *A.out = m > A.cap ? m : 0;
+  }
+}
+NB_KERNEL_PAIR(k_ovfchk, OvfArgs, 64)
+
+// The snapshot buffer is allocated with the state (nb_md_create), sized
+// for the longest chain, so a server's footprint accounting sees it.
+int md_ensure_save(nb_md *md) {
+  if (!md->d_save) {
+    nb_set_error("nb_batch: MD state has no snapshot buffer");
+    return NB_ERR_ARG;
+  }
+  return NB_OK;
+}
+
+// Snapshot (restore 0) or restore (1) on stream s; recordable by a batch.
+int md_enqueue_save(nb_md *md, hipStream_t s, int restore) {
+  nb_ctx *c = md->c;
+  SaveArgs A;
+  A.pos = c->d_pos; A.vel = md->d_vel; A.aa = md->d_aa; A.fl = c->d_fl; A.nhc = md->d_nhc;
+  A.save = md->d_save; A.n = md->n; A.nnhc = 2 * md->nchain; A.restore = restore;
+  A.flags = c->d_flags;
+  A.nblocks = (int)((12 * (size_t)md->n + A.nnhc + 255) / 256);
+  NB_GO(c, K_MDSAVE, k_mdsave, A.nblocks, 256, 0, A, s);
+  if (!c->rec) MCHK(hipGetLastError());
+  return NB_OK;
+}
+
+int md_enqueue_ovfchk(nb_md *md, hipStream_t s, int *d_out) {
+  nb_ctx *c = md->c;
+  OvfArgs A = {c->d_flags, c->cap, d_out};
+  NB_GO(c, K_OVFCHK, k_ovfchk, 1, 64, 0, A, s);
+  if (!c->rec) MCHK(hipGetLastError());
+  return NB_OK;
+}
+
+void nb_md_gens(nb_md *md, unsigned *cgen, unsigned *tgen) {
+  *cgen = md->c->gen;
+  *tgen = md->tgen;
+}
+
+int nb_grow_capacity(nb_ctx *c, int want);
+
+// After an overflowed tick: back to the snapshot, neighbour slots for the
+// largest row seen (x1.25 + 64, as a host build grows them), lists rebuilt
+// at the restored positions, and the nsteps redone for this system alone
+// (bitwise what the batch computes for it).  Synchronous.
+int md_recover(nb_md *md, int nsteps, int want, hipStream_t s) {
+  nb_ctx *c = md->c;
+  for (int attempt = 0; attempt < 4; attempt++) {
+    int rc = md_enqueue_save(md, s, 1);
+    if (rc) return rc;
+    MCHK(hipStreamSynchronize(s));
+    rc = nb_grow_capacity(c, (int)(want * 1.25) + 64);
+    if (rc) return rc;
+    rc = nb_build_now(c, md->hmat, s);
+    if (rc) return rc;
+    MCHK(hipMemsetAsync(c->d_flags + 1, 0, 2 * sizeof(int), s));  // rebuild + sticky flags
+    md_drop_graph(md);
+    rc = nb_md_step(md, nsteps, s, 0);
+    if (rc) return rc;
+    int f2 = 0;
+    MCHK(hipMemcpyAsync(&f2, c->d_flags + 2, sizeof(int), hipMemcpyDeviceToHost, s));
+    MCHK(hipStreamSynchronize(s));
+    if (f2 <= c->cap) return NB_OK;
+    want = f2;
+  }
+  nb_set_error("neighbour list overflow persists after regrowing the slots (%d > %d)", want,
+               c->cap);
+  return NB_ERR_CAPACITY;
 }
 
 nb_ctx *nb_md_ctx(nb_md *md) { return md->c; }
@@ -967,6 +1096,8 @@ static struct RegisterMD {
     g_batch_fn[K_MDINNER] = k_mdinner_bl;
     g_batch_fn[K_MDKICK] = k_mdkick_bl;
     g_batch_fn[K_MDNHC] = k_mdnhc_bl;
+    g_batch_fn[K_MDSAVE] = k_mdsave_bl;
+    g_batch_fn[K_OVFCHK] = k_ovfchk_bl;
   }
 } g_register_md;
 
@@ -975,6 +1106,11 @@ extern "C" int nb_md_step(nb_md *md, int nsteps, void *stream, int use_graph) {
   nb_ctx *c = md->c;
   hipStream_t s = stream ? (hipStream_t)stream : c->stream;
   MCHK(hipSetDevice(c->dev));
+  if (md->have_graph && (md->graph_cgen != c->gen || md->graph_tgen != md->tgen)) {
+    // recorded against buffers or a launch shape that have since changed
+    MCHK(hipDeviceSynchronize());
+    md_drop_graph(md);
+  }
   if (use_graph && !md->have_graph) {
     MCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     int rc = md_enqueue_step(md, s);
@@ -985,6 +1121,8 @@ extern "C" int nb_md_step(nb_md *md, int nsteps, void *stream, int use_graph) {
     MCHK(hipGraphInstantiate(&md->gexec, g, nullptr, nullptr, 0));
     md->graph = g;
     md->have_graph = 1;
+    md->graph_cgen = c->gen;
+    md->graph_tgen = md->tgen;
   }
   for (int k = 0; k < nsteps; k++) {
     if (use_graph) {
@@ -1007,10 +1145,13 @@ extern "C" int nb_md_energies(nb_md *md, double vlrc, double wlrc, int ndof, nb_
   nb_ctx *c = md->c;
   hipStream_t s = c->stream;
   MCHK(hipSetDevice(c->dev));
-  MCHK(hipStreamSynchronize(s));
-  // nonbonded energies of the current positions: rerun force(...,1) into
-  // scratch (fs) so the integrator's fl is untouched
-  int rc = nb_launch_force(c, 1, md->box, nullptr, c->d_fs, s, 0);
+  // the last step may have run on another stream (a batch's, a caller's);
+  // the force rerun below rewrites the records and energy columns it uses
+  MCHK(hipDeviceSynchronize());
+  // nonbonded energies and virials of the current positions: force(...,1)'s
+  // energy pass (FP64 pair sums, the reciprocal sum without its force pass,
+  // ecorr) into scratch (fs), so the integrator's fl is untouched
+  int rc = nb_launch_force(c, 1, md->box, nullptr, c->d_fs, s, NB_FORCE_EONLY);
   if (rc) return rc;
   InnerArgs A = inner_args(md);
   k_md_energy<<<md->ep_blocks, 256, 0, s>>>(A, md->d_amass, md->d_ep);
@@ -1075,7 +1216,8 @@ extern "C" int nb_md_energies(nb_md *md, double vlrc, double wlrc, int ndof, nb_
   e->ham = e->poth + e->potra + e->zke + e->potn + e->zken;
   e->prsi = (p2m + W + wb) / (3. * vol);
   e->tiout = ndof > 0 ? p2m / (double)ndof : 0.;
-  e->overflow = c->h_flags[0] > c->cap;
+  // the last build's row maximum, or any since the last batch tick began
+  e->overflow = c->h_flags[0] > c->cap || c->h_flags[2] > c->cap;
   return NB_OK;
 }
 
@@ -1088,8 +1230,9 @@ extern "C" int nb_md_rebuilds(nb_md *md, int *count) {
   int f[8];
   MCHK(hipMemcpy(f, c->d_flags, sizeof(f), hipMemcpyDeviceToHost));
   *count = f[6];
-  if (f[0] > c->cap) {
-    nb_set_error("neighbour list overflow in a device-side rebuild (%d > %d slots)", f[0], c->cap);
+  if (f[0] > c->cap || f[2] > c->cap) {
+    nb_set_error("neighbour list overflow in a device-side rebuild (%d > %d slots)",
+                 f[0] > f[2] ? f[0] : f[2], c->cap);
     return NB_ERR_CAPACITY;
   }
   return NB_OK;

@@ -13,8 +13,10 @@
 //     are converted to 64-bit fixed point (2^-24 K/A resolution) and reduced
 //     across the wavefront in integer arithmetic, so the per-atom force does
 //     not depend on the order in which the lanes' partial sums meet;
-//   - energies and virials: per-lane FP32 sums (a few tens of terms), then
-//     reduced in FP64 in a fixed order.
+//   - no energies or virials: a mixed context's pair energies and virials
+//     are evaluated in FP64 by the energy pass of nb_launch_force when they
+//     are asked for (the MD step never needs them), so the FP32 kernel is
+//     forces only.
 #include "nb_device.h"
 
 #define NB_FX_SCALE 16777216.0  // 2^24 fixed-point units per K/A
@@ -43,39 +45,14 @@ struct MixArgs {
   const TPairF *tpf;
   const float *vtab3, *ctab3;
   float h[9];
-  float rmins_coul2, dx2coul, rmaxl_coul2, clast;
+  float rmins_coul2, dx2coul, rmaxl_coul2;
   double *fl, *fs;
-  double *epart;
-};
-
-enum {
-  M_SPOTL = 0, M_SPOTHL, M_SPRSL, M_SL0, M_SL1, M_SL2, M_SL4, M_SL5, M_SL8,
-  M_SPOTE, M_SPOTHE, M_SPRSE, M_SE0, M_SE1, M_SE2, M_SE4, M_SE5, M_SE8, M_CLAMP
 };
 
 static __device__ __forceinline__ float interp3f(const float *t, int k, int c, float p) {
 #pragma clang fp contract(fast)
   float fm1 = t[3 * (k - 1) + c], f0 = t[3 * k + c], f1 = t[3 * (k + 1) + c];
   return f0 + .5f * p * (f1 - fm1 + p * (fm1 - 2.f * f0 + f1));
-}
-
-// block totals of NACC accumulators stored column-major (out[k * stride])
-template <int NACC>
-static __device__ __forceinline__ void block_reduce_store_m(const double *acc, double *out,
-                                                            int stride) {
-  __shared__ double red[4][NACC];
-  const int w = threadIdx.x >> 6, lane = lane_id();
-#pragma unroll
-  for (int k = 0; k < NACC; k++) {
-    double v = nb_wave_sum(acc[k]);
-    if (lane == 0) red[w][k] = v;
-  }
-  __syncthreads();
-  if (threadIdx.x < NACC) {
-    double v = red[0][threadIdx.x] + red[1][threadIdx.x];
-    v += red[2][threadIdx.x] + red[3][threadIdx.x];
-    out[(size_t)threadIdx.x * stride] = v;
-  }
 }
 
 static __device__ __forceinline__ long long to_fix(double v) {
@@ -87,9 +64,7 @@ static __device__ __forceinline__ long long to_fix(double v) {
 // entry; a wave takes several atoms (groups of 64 / G) in turn.  A lane's
 // pair forces for the atom are summed in FP32 (a handful of terms), then go
 // to fixed point and are reduced across the G lanes in integers, so the sum
-// does not depend on the order.  Energies and virials: FP32 sums per lane
-// over all its pairs (a few tens), reduced in FP64 once per workgroup.  The
-// next chunk's list entries and partner records are loaded before the
+// does not depend on the order.  The next chunk's list entries and partner records are loaded before the
 // current chunk is evaluated.  G = 16 suits short lists (dense LJ-like
 // mixtures, ~100 entries): the last, partly filled chunk wastes fewer lanes
 // and one reduction serves four atoms.
@@ -107,9 +82,6 @@ static __device__ __forceinline__ void pair_mixed_run(const MixArgs &A, int bid,
 #pragma clang fp contract(fast)
   static_assert(G == 16 || G == NB_WAVE, "16 or 64 lanes per atom");
   constexpr int APW = NB_WAVE / G;  // atoms a wavefront evaluates together
-  float e[NB_NE_PAIR];  // this lane's energy / virial sums over its pairs
-#pragma unroll
-  for (int k = 0; k < NB_NE_PAIR; k++) e[k] = 0.f;
   const int lane = lane_id(), gl = lane % G, sub = lane / G;
   const int nw = A.nblocks * 4;
   constexpr unsigned fv = IFLG ? NB_FLAG_VT : NB_FLAG_VS;
@@ -166,7 +138,7 @@ static __device__ __forceinline__ void pair_mixed_run(const MixArgs &A, int bid,
           if (d2 < rmax2) {
             float p = (d2 - P.rmins2) * P.dx2;
             int kk = (int)p;
-            if (kk < 1) { e[M_CLAMP] += 1.f; kk = 1; p = 1.f; }
+            if (kk < 1) { kk = 1; p = 1.f; }
             if (kk > A.ntable - 3) kk = A.ntable - 3;
             p = p - (float)kk;
             const float *T = vt + (size_t)P.in * A.ntable * 3;
@@ -174,19 +146,9 @@ static __device__ __forceinline__ void pair_mixed_run(const MixArgs &A, int bid,
             float swv = 1.f;
             if ((WANT_FS || !IFLG) && d2 > P.rminl2) swv = interp3f(T, kk, 1, p);
             if (IFLG) {
-              e[M_SPRSL] -= d2 * f;
-              e[M_SL0] -= dx * dx * f;
-              e[M_SL1] -= dy * dx * f;
-              e[M_SL2] -= dz * dx * f;
-              e[M_SL4] -= dy * dy * f;
-              e[M_SL5] -= dz * dy * f;
-              e[M_SL8] -= dz * dz * f;
               float gx = dx * f, gy = dy * f, gz = dz * f;
               fx -= gx; fy -= gy; fz -= gz;
               if (WANT_FS) { sx -= gx * swv; sy -= gy * swv; sz -= gz * swv; }
-              float v = interp3f(T, kk, 2, p);
-              e[M_SPOTL] += v;
-              e[M_SPOTHL] += v - P.vlast;
             } else {
               f *= swv;
               sx -= dx * f; sy -= dy * f; sz -= dz * f;
@@ -196,25 +158,13 @@ static __device__ __forceinline__ void pair_mixed_run(const MixArgs &A, int bid,
         if ((fl & fe) && d2 < A.rmaxl_coul2) {
           float p = (d2 - A.rmins_coul2) * A.dx2coul;
           int kk = (int)p;
-          if (kk < 1) { e[M_CLAMP] += 1.f; kk = 1; p = 1.f; }
+          if (kk < 1) { kk = 1; p = 1.f; }
           if (kk > A.ntable - 3) kk = A.ntable - 3;
           p = p - (float)kk;
           float qij = qi * __uint_as_float(rj.w);
           float fcc = qij * interp3f(ct, kk, 0, p);
           float gx = dx * fcc, gy = dy * fcc, gz = dz * fcc;
-          if (IFLG) {
-            e[M_SPRSE] -= d2 * fcc;
-            e[M_SE0] -= dx * dx * fcc;
-            e[M_SE1] -= dy * dx * fcc;
-            e[M_SE2] -= dz * dx * fcc;
-            e[M_SE4] -= dy * dy * fcc;
-            e[M_SE5] -= dz * dy * fcc;
-            e[M_SE8] -= dz * dz * fcc;
-            fx -= gx; fy -= gy; fz -= gz;
-            float v = interp3f(ct, kk, 2, p);
-            e[M_SPOTHE] += qij * (v - A.clast);
-            e[M_SPOTE] += qij * v;
-          }
+          if (IFLG) { fx -= gx; fy -= gy; fz -= gz; }
           if (WANT_FS || !IFLG) { sx -= gx; sy -= gy; sz -= gz; }
         }
       }
@@ -244,10 +194,6 @@ static __device__ __forceinline__ void pair_mixed_run(const MixArgs &A, int bid,
       }
     }
   }
-  double acc[NB_NE_PAIR];
-#pragma unroll
-  for (int k = 0; k < NB_NE_PAIR; k++) acc[k] = (double)e[k];
-  block_reduce_store_m<NB_NE_PAIR>(acc, A.epart + bid, A.nblocks);
 }
 
 template <int IFLG, bool WANT_FS, int G>
@@ -302,8 +248,8 @@ int nb_launch_pair_mixed(nb_ctx *c, int iflg, const Box &b, double *d_fs, double
   M.vtab3 = c->d_vtab3f; M.ctab3 = c->d_ctab3f;
   for (int k = 0; k < 9; k++) M.h[k] = (float)b.h[k];
   M.rmins_coul2 = (float)c->rmins_coul2; M.dx2coul = (float)c->dx2coul;
-  M.rmaxl_coul2 = (float)c->rmaxl_coul2; M.clast = (float)c->clast;
-  M.fl = d_fl; M.fs = d_fs; M.epart = c->d_epair;
+  M.rmaxl_coul2 = (float)c->rmaxl_coul2;
+  M.fl = d_fl; M.fs = d_fs;
   const int g = c->npair_blocks, lds = c->tab_lds_bytes_f;
   if (iflg == 0)
     NB_GO(c, K_MIX_0T, k_mix_0t, g, 256, lds, M, s);

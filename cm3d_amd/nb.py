@@ -101,12 +101,12 @@ class NBMDEnergy(C.Structure):
 EXPORTS = ["nb_create", "nb_destroy", "nb_last_error", "nb_build_lists",
            "nb_need_rebuild", "nb_force", "nb_force_device", "nb_energy_only",
            "nb_build_lists_device", "nb_list_counts", "nb_get_pairs", "nb_timing",
-           "nb_footprint",
+           "nb_footprint", "nb_list_reserve",
            "nb_md_create", "nb_md_destroy", "nb_md_set_state", "nb_md_get_state",
            "nb_md_init", "nb_md_step", "nb_md_energies", "nb_md_rebuilds",
            "nb_md_set_thermostat", "nb_md_get_thermostat",
            "nb_batch_create", "nb_batch_destroy", "nb_batch_step", "nb_batch_launches",
-           "nb_table_pair", "nb_table_coul", "nb_kvectors", "nb_ewald_consts"]
+           "nb_batch_recoveries", "nb_table_pair", "nb_table_coul", "nb_kvectors", "nb_ewald_consts"]
 
 _LIB = None
 
@@ -157,6 +157,8 @@ def lib():
     L.nb_batch_destroy.restype = None
     L.nb_batch_step.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int]
     L.nb_batch_launches.argtypes = [C.c_void_p, _ip]
+    L.nb_batch_recoveries.argtypes = [C.c_void_p, C.POINTER(C.c_longlong)]
+    L.nb_list_reserve.argtypes = [C.c_void_p, C.c_int]
     L.nb_table_pair.argtypes = [C.c_int, _dp, C.c_double, C.c_double,
                                 C.c_double, C.c_double, C.c_double, C.c_double,
                                 C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
@@ -311,6 +313,11 @@ class NBContext:
         b = C.c_size_t(0)
         check(lib().nb_footprint(self._h, C.byref(b)), "nb_footprint")
         return int(b.value)
+
+    def list_reserve(self, slots):
+        """Neighbour slots per atom for the next build (nb_list_reserve);
+        drops the current lists."""
+        check(lib().nb_list_reserve(self._h, int(slots)), "nb_list_reserve")
 
     def get_pairs(self, which):
         """Sorted (i<j) pair arrays of list 's', 't', 'se' or 'te'."""
@@ -503,8 +510,16 @@ class NBBatch:
             pass
 
     def step(self, nsteps=1, stream=None, graph=True):
+        """One tick of nsteps steps; returns when it is done (systems whose
+        list build overflowed are redone with more slots)."""
         check(lib().nb_batch_step(self._h, int(nsteps), C.c_void_p(stream) if stream else None,
                                   int(bool(graph))), "nb_batch_step")
+
+    def recoveries(self):
+        """Systems redone after a neighbour-list overflow so far."""
+        c = C.c_longlong(0)
+        check(lib().nb_batch_recoveries(self._h, C.byref(c)), "nb_batch_recoveries")
+        return c.value
 
     def launches(self):
         c = C.c_int(0)

@@ -152,6 +152,16 @@ int nb_list_counts(nb_ctx *ctx, int counts[4]);
  * accounts with. */
 int nb_footprint(nb_ctx *ctx, size_t *bytes);
 
+/* Neighbour slots per atom for the next list build (rounded up to 64).  No
+ * counterpart in cm3d, whose get_ngbr sizes its lists by counting with a
+ * x1.2 margin (energy/control/ngbr_list.c:331-335); here slots are sized
+ * from the density and grown x1.25 when a build needs more.  A caller that
+ * bounds list memory reserves them; the current lists are dropped (the next
+ * device step rebuilds first, nb_force asks for nb_build_lists).  A host
+ * build that needs more slots grows them; a device-side rebuild that needs
+ * more is recovered by nb_batch_step and reported by nb_md_rebuilds. */
+int nb_list_reserve(nb_ctx *ctx, int slots);
+
 /* Copy one list as half pairs (i<j) in ascending (i,j) order.
  * which: 0 = s (vdW short), 1 = t (vdW), 2 = se, 3 = te.  Fails with
  * NB_ERR_CAPACITY (and sets *n) when cap is too small. */
@@ -237,9 +247,18 @@ int nb_md_rebuilds(nb_md *md, int *count);
 typedef struct nb_batch nb_batch;
 int nb_batch_create(nb_md **mds, int nsys, nb_batch **b);
 void nb_batch_destroy(nb_batch *b);
+/* nsteps steps of every system (one tick); returns when they are done.  The
+ * tick snapshots each system's state first and checks after it whether any
+ * of its device-side list builds wanted more neighbour slots than it has; a
+ * system that did is restored, given the slots (x1.25), rebuilt and stepped
+ * again alone, which is bitwise what the batch computes for it -- no system
+ * continues on truncated lists.  A batch whose members were regrown or
+ * re-thermostatted since it recorded its step records it again. */
 int nb_batch_step(nb_batch *b, int nsteps, void *stream, int use_graph);
 /* Number of kernel launches one batched step issues. */
 int nb_batch_launches(nb_batch *b, int *nkernels);
+/* Systems redone after a list overflow since the batch was created. */
+int nb_batch_recoveries(nb_batch *b, long long *count);
 
 /* ---- set-up helpers (host only) ---- */
 #define NB_POT_LJ 0

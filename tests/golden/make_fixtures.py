@@ -26,6 +26,10 @@ Cases:
   water512_nvt         water512 under a 4-link chain (tau 50 fs), RESPA 4
   mix600_nvt           mix600 with only the first species thermostatted
                        (2-link chain, tau 100 fs)
+  water512_tri,        water512 and mix600 in a triclinic box of the same
+  mix600_tri           volume (every box vector tilted; molecules moved
+                       rigidly with their first atom), for the general-hmat
+                       minimum image and link cells
 NVT decks are kept beside their fixture (decks_<name>/) for the deck reader.
 """
 import hashlib
@@ -122,6 +126,32 @@ def coul_keys(s):
             rf"\rcute_min{{{float(np.sqrt(s['rmins_coul2']))!r}}}"]
 
 
+def sheared(sysd, tilt=(0.2, 0.1, 0.15)):
+    """The system in a triclinic box of the same volume: hmat gains the
+    off-diagonal entries h[0][1] = tx L, h[0][2] = ty L, h[1][2] = tz L (the
+    b and c box vectors lean), and each molecule moves rigidly with its first
+    atom's image under the box's affine map, so bond lengths and angles are
+    unchanged."""
+    t = dict(sysd)
+    n = int(sysd["natoms"])
+    H = np.asarray(sysd["hmat"], float).reshape(3, 3)
+    L = H[0, 0]
+    H2 = H.copy()
+    H2[0, 1], H2[0, 2], H2[1, 2] = tilt[0] * L, tilt[1] * L, tilt[2] * L
+    M = H2 @ np.linalg.inv(H)
+    r = np.asarray(sysd["pos"], float).reshape(3, n)
+    ms = sysd.get("mol_start")
+    ms = np.arange(n + 1) if ms is None else np.asarray(ms)
+    out = r.copy()
+    for m in range(len(ms) - 1):
+        a0, a1 = int(ms[m]), int(ms[m + 1])
+        shift = M @ r[:, a0] - r[:, a0]
+        out[:, a0:a1] = r[:, a0:a1] + shift[:, None]
+    t["pos"] = out.reshape(-1).copy()
+    t["hmat"] = H2.reshape(-1).copy()
+    return t
+
+
 def cases():
     w = S.water(nside=8)
     m = S.mixture(600)
@@ -142,6 +172,10 @@ def cases():
         "mix600_nvt": lambda: synth_case("mix600_nvt", m, "mixture", 3, dt=2.0, temp=300.0,
                                          extra=coul_keys(m), nhc=2, tau_nhc=100.0,
                                          therm=("global", "none")),
+        "water512_tri": lambda: synth_case("water512_tri", sheared(w), "water", 2, dt=2.0,
+                                           nres_intra=4, temp=300.0, extra=coul_keys(w)),
+        "mix600_tri": lambda: synth_case("mix600_tri", sheared(m), "mixture", 2, dt=2.0,
+                                         temp=300.0, extra=coul_keys(m)),
     }
 
 

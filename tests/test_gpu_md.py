@@ -11,13 +11,14 @@ from conftest import load_fixture, rel_rms
 
 pytestmark = pytest.mark.gpu
 
-TRAJ_CASES = ["ref_lj2", "ref_lj256", "ar864", "mix600", "water512"]
+TRAJ_CASES = ["ref_lj2", "ref_lj256", "ar864", "mix600", "water512", "water512_tri",
+              "mix600_tri"]
 
 
 def _system(name):
     from cm3d_amd import systems as S
     d = load_fixture(name)
-    if name == "water512":
+    if name.startswith("water512"):
         w = S.water(nside=8)
         for k in ("mol_start", "bond_i", "bond_j", "bond_k", "bond_r0", "bend_i", "bend_j",
                   "bend_k", "bend_fk", "bend_th0"):
