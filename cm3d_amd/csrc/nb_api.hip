@@ -200,6 +200,13 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
       ct3[(size_t)t * 3 + 2] = p->coultab[t];
     }
     c->clast = p->coultab[nt - 4];
+    // vinter_e's clamp: k = 1, p = 0 - 1 (finter_e.c:325-352)
+    auto at_clamp = [](const double *t) {
+      const double pp = -1., fm1 = t[0], f0 = t[1], f1 = t[2];
+      return f0 + .5 * pp * (f1 - fm1 + pp * (fm1 - 2. * f0 + f1));
+    };
+    c->cq0 = at_clamp(p->coultab);
+    c->dq0 = at_clamp(p->dcoultab);
   }
   // list cutoffs per type pair: null pairs get -1 (no r^2 is below it)
   std::vector<double> pcut(2 * (size_t)p->ntypes * p->ntypes, -1.);
@@ -397,7 +404,7 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
                                                   : std::min((n + 3) / 4, NB_PAIR64_MAX_WG);
   c->new_blocks = std::max(1, (c->ktot + NB_WAVE - 1) / NB_WAVE);
   c->nfin_blocks = std::max(1, (ncharge + NB_WAVE - 1) / NB_WAVE);  // 64 charges a workgroup
-  if ((rc = dalloc(c, &c->d_epair, (size_t)c->npair_blocks * NB_NE_PAIR))) return fail(rc);
+  if ((rc = dalloc(c, &c->d_epair, (size_t)c->npair_blocks * NB_NE_PAIRQ))) return fail(rc);
   if ((rc = dalloc(c, &c->d_eec, (size_t)c->nfin_blocks * NB_NE_ECORR))) return fail(rc);
   if ((rc = dalloc(c, &c->d_eew, (size_t)c->new_blocks * NB_NE_EW))) return fail(rc);
   if ((rc = dalloc(c, &c->d_eout, 64))) return fail(rc);
@@ -630,7 +637,7 @@ extern "C" int nb_need_rebuild(nb_ctx *c, const double *pxyz, int *flag) {
 }
 
 void nb_fill_energy(nb_ctx *c, int iflg, const double *E, const double hmat[9],
-                    nb_energy *e) {
+                    nb_energy *e, bool cm3d) {
   memset(e, 0, sizeof(*e));
   e->nclamp = (int)(E[NB_NE_PAIR - 1] * .5 + .25);
   if (iflg != 1) return;
@@ -643,13 +650,29 @@ void nb_fill_energy(nb_ctx *c, int iflg, const double *E, const double hmat[9],
   e->sprset[0] = hf * E[12]; e->sprset[1] = hf * E[13]; e->sprset[2] = hf * E[14];
   e->sprset[4] = hf * E[15]; e->sprset[5] = hf * E[16]; e->sprset[8] = hf * E[17];
   e->sprset[3] = e->sprset[1]; e->sprset[6] = e->sprset[2]; e->sprset[7] = e->sprset[5];
+  e->spote_phys = e->spote; e->spothe_phys = e->spothe; e->sprse_phys = e->sprse;
+  for (int k = 0; k < 9; k++) e->sprset_phys[k] = e->sprset[k];
+  if (cm3d && c->ncharge > 0) {
+    // getvirter_e -> vinter_e (finter_e.c:260-352): every te pair inside the
+    // cutoff takes the table at its clamp, so the sums are closed forms in
+    // sum q_i q_j (E[19]) and sum q_i q_j r r (E[20..26])
+    const double sq = hf * E[19];
+    e->spote = c->cq0 * sq;
+    e->spothe = (c->cq0 - c->clast) * sq;
+    e->sprse = -c->dq0 * hf * E[20];
+    const int map6[6] = {0, 1, 2, 4, 5, 8};
+    for (int k = 0; k < 9; k++) e->sprset[k] = 0.;
+    for (int k = 0; k < 6; k++) e->sprset[map6[k]] = -c->dq0 * hf * E[21 + k];
+    e->sprset[3] = e->sprset[1]; e->sprset[6] = e->sprset[2]; e->sprset[7] = e->sprset[5];
+  }
   if (c->iperd == 3 && c->necorr > 0) {
-    e->pot_ecorr = hf * E[20];
-    e->prs_ecorr = hf * E[21];
-    for (int k = 0; k < 9; k++) e->prs_ecorr_tensor[k] = hf * E[22 + k];
+    const double *X = E + NB_EOUT_ECORR;
+    e->pot_ecorr = hf * X[0];
+    e->prs_ecorr = hf * X[1];
+    for (int k = 0; k < 9; k++) e->prs_ecorr_tensor[k] = hf * X[2 + k];
   }
   if (c->iperd == 3 && c->ktot > 0) {
-    const double *W = E + 32;
+    const double *W = E + NB_EOUT_EW;
     double bk = c->ewald_bkg / det9(hmat);
     e->vk_ewald = W[0] + bk + c->ewald_self;
     e->vp_ewald = W[1] + 3. * (W[0] + bk);
@@ -739,11 +762,11 @@ extern "C" int nb_energy_only(nb_ctx *c, const double *pxyz, const double hmat[9
   HCHK(hipMemcpyAsync(c->d_pos, pxyz, sizeof(double) * 3 * c->natoms, hipMemcpyHostToDevice,
                       c->stream));
   Box b = make_box(hmat);
-  RC(nb_launch_force(c, 1, b, nullptr, c->d_fs, c->stream, NB_FORCE_EONLY));
+  RC(nb_launch_force(c, 1, b, nullptr, c->d_fs, c->stream, NB_FORCE_EONLY | NB_FORCE_CM3D_EONLY));
   double E[64];
   HCHK(hipMemcpyAsync(E, c->d_eout, sizeof(E), hipMemcpyDeviceToHost, c->stream));
   HCHK(hipStreamSynchronize(c->stream));
-  nb_fill_energy(c, 1, E, hmat, e);
+  nb_fill_energy(c, 1, E, hmat, e, true);
   return NB_OK;
 }
 
@@ -801,7 +824,7 @@ void nb_time_mark(nb_ctx *c, int ev, hipStream_t s) {
   hipEventRecord(c->tev[c->trec * NB_TIME_EV + ev], s);
 }
 
-extern "C" int nb_timing(nb_ctx *c, int mode, double out[6]) {
+extern "C" int nb_timing(nb_ctx *c, int mode, double out[NB_TIMING_N]) {
   if (!c) { nb_set_error("nb_timing: null context"); return NB_ERR_ARG; }
   HCHK(hipSetDevice(c->dev));
   if (mode == 1) {  // start / restart recording
@@ -815,22 +838,24 @@ extern "C" int nb_timing(nb_ctx *c, int mode, double out[6]) {
   }
   if (mode == 0) { c->timing = 0; return NB_OK; }
   // mode 2: averages over the recorded force calls, milliseconds:
-  // out = {calls, pack, pair kernel, ecorr, ewald (all four kernels), total}
+  // out = {calls, pack, pair kernel, charge records, structure factor,
+  //        k reduction, reciprocal force, ecorr + force sum + reductions, total}
   if (!out || !c->tev) { nb_set_error("nb_timing: nothing recorded"); return NB_ERR_ARG; }
   int nrec = c->trec < NB_TIME_REC ? c->trec : NB_TIME_REC;
-  for (int k = 0; k < 6; k++) out[k] = 0.;
+  for (int k = 0; k < NB_TIMING_N; k++) out[k] = 0.;
   out[0] = nrec;
   for (int r = 0; r < nrec; r++) {
     hipEvent_t *e = c->tev + r * NB_TIME_EV;
-    HCHK(hipEventSynchronize(e[5]));
+    HCHK(hipEventSynchronize(e[NB_TIME_EV - 1]));
     float ms;
-    HCHK(hipEventElapsedTime(&ms, e[0], e[1])); out[1] += ms;
-    HCHK(hipEventElapsedTime(&ms, e[1], e[2])); out[2] += ms;
-    HCHK(hipEventElapsedTime(&ms, e[2], e[3])); out[3] += ms;
-    HCHK(hipEventElapsedTime(&ms, e[3], e[4])); out[4] += ms;
-    HCHK(hipEventElapsedTime(&ms, e[0], e[5])); out[5] += ms;
+    for (int k = 1; k < NB_TIME_EV; k++) {
+      HCHK(hipEventElapsedTime(&ms, e[k - 1], e[k]));
+      out[k] += ms;
+    }
+    HCHK(hipEventElapsedTime(&ms, e[0], e[NB_TIME_EV - 1]));
+    out[NB_TIME_EV] += ms;
   }
   if (nrec > 0)
-    for (int k = 1; k < 6; k++) out[k] /= nrec;
+    for (int k = 1; k < NB_TIMING_N; k++) out[k] /= nrec;
   return NB_OK;
 }

@@ -72,6 +72,9 @@ struct LaunchRec;
 #define NB_PAIR64_MAX_WG 2048  // workgroup cap of the FP64 pair kernel
 #endif
 #define NB_NE_PAIR 19    // pair energy/virial accumulators + clamp count
+#define NB_NE_PAIRQ 27   // the same + cm3d's energy-only coulomb sums (see k_pair64_1q)
+#define NB_EOUT_ECORR 28 // reduced energy columns: pair [0, 27), ecorr [28, 39),
+#define NB_EOUT_EW 40    //   Ewald [40, 51)
 #define NB_NE_ECORR 11   // ecorr energy/virial accumulators
 #define NB_NE_EW 11      // Ewald accumulators
 
@@ -208,6 +211,9 @@ struct nb_ctx {
   double skin, kappa, ewald_self, ewald_bkg;
   double dx2coul, rmins_coul2, rmaxs_coul2, rminl_coul2, rmaxl_coul2;
   double rmaxs_skin2_e, rmaxl_skin2_e, rlist, clast;
+  // vinter_e's per-pair table values: the coulomb table and its derivative
+  // interpolated at its clamp (k = 1, p = -1; finter_e.c:330-352)
+  double cq0, dq0;
   double rprune2;  // every list cutoff^2, widened by 2e-6: the column-pruning bound
   // parameters on device
   int *d_itype, *d_map;
@@ -265,7 +271,8 @@ struct nb_ctx {
   std::vector<LaunchRec> *rec;
 };
 #define NB_TIME_REC 2048
-#define NB_TIME_EV 6             // start | pack | pair | ecorr | ewald | end
+#define NB_TIME_EV 8  // start | pack | pair | charge records | structure factor |
+                      // k reduction | reciprocal force | ecorr + forces + reductions (end)
 
 // kernel launchers (nb_kernels.hip)
 int nb_launch_build(nb_ctx *c, const Box &b, hipStream_t s);
@@ -279,11 +286,16 @@ int nb_build_now(nb_ctx *c, const double hmat[9], hipStream_t s);
 #define NB_FORCE_NO_FIN 32    // leave out k_ewfin (ecorr + reciprocal forces into fl) and k_red
 #define NB_FORCE_NOENER 64    // forces only: no pair energies / virials (the MD step's
                               // force evaluation; its energies are evaluated on request)
+#define NB_FORCE_CM3D_EONLY 128  // with EONLY: also the sums cm3d's energy-only
+                                 // coulomb routine (vinter_e) reduces to
 int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs,
                     double *d_fl, hipStream_t s, int flags);
 int nb_launch_rebuild_check(nb_ctx *c, const double *d_pos, hipStream_t s);
 // nb_energy from the reduced columns E of a force(...,iflg) evaluation
-void nb_fill_energy(nb_ctx *c, int iflg, const double *E, const double hmat[9], nb_energy *e);
+// cm3d: the coulomb fields take vinter_e's values (the reduction ran with
+// NB_FORCE_CM3D_EONLY), the *_phys fields the force path's
+void nb_fill_energy(nb_ctx *c, int iflg, const double *E, const double hmat[9], nb_energy *e,
+                    bool cm3d = false);
 // det(h), cm3d's get_deth
 static inline double nb_det9(const double h[9]) {
   return h[0] * (h[4] * h[8] - h[5] * h[7]) + h[1] * (h[5] * h[6] - h[3] * h[8]) +
@@ -295,5 +307,5 @@ int nb_launch_pair_mixed(nb_ctx *c, int iflg, const Box &b, double *d_fs, double
 struct RedSet;
 // the energy-column reductions of a force evaluation; pair: the pair
 // kernel's columns were written (not a forces-only evaluation)
-RedSet nb_red_set(const nb_ctx *c, int iflg, bool pair = true);
+RedSet nb_red_set(const nb_ctx *c, int iflg, bool pair = true, bool quirk = false);
 void nb_set_error(const char *fmt, ...);

@@ -468,10 +468,13 @@ NB_KERNEL_PAIR(k_check, CheckArgs, 256)
 // virials; IFLG 0: fs = short-range force over the s lists, no energies
 // (cm3d computes them only at the long level).
 // ---------------------------------------------------------------------------
-// accumulator slots (NB_NE_PAIR = 19 with the clamp count)
+// accumulator slots (NB_NE_PAIR = 19 with the clamp count; NB_NE_PAIRQ = 27
+// with the sums of cm3d's energy-only coulomb routine: sum q_i q_j, and
+// sum q_i q_j r^2 and its tensor, over the te pairs inside the cutoff)
 enum {
   E_SPOTL = 0, E_SPOTHL, E_SPRSL, E_SL0, E_SL1, E_SL2, E_SL4, E_SL5, E_SL8,
-  E_SPOTE, E_SPOTHE, E_SPRSE, E_SE0, E_SE1, E_SE2, E_SE4, E_SE5, E_SE8, E_CLAMP
+  E_SPOTE, E_SPOTHE, E_SPRSE, E_SE0, E_SE1, E_SE2, E_SE4, E_SE5, E_SE8, E_CLAMP,
+  E_QQ, E_QR2, E_Q0, E_Q1, E_Q2, E_Q4, E_Q5, E_Q8
 };
 
 #define NB_IP_LDS 16    // interactions staged in LDS
@@ -492,13 +495,15 @@ static __device__ __forceinline__ double interp3c(const double *t, int k, int c,
 // virials and the clamp count are summed (else forces only: the MD step,
 // whose energies are evaluated when they are asked for); WF: forces are
 // written (else energies only).
-template <int IFLG, bool WANT_FS, bool TLDS, bool ENER, bool WF>
+// QK: also the energy-only coulomb sums of cm3d's vinter_e.
+template <int IFLG, bool WANT_FS, bool TLDS, bool ENER, bool WF, bool QK = false>
 static __device__ __forceinline__ void pair64_run(const PairArgs &A, int bid, const double *vt,
                                                   const double *ct, const InterPar *sip,
                                                   const int *smap, bool ip_lds) {
-  double acc[NB_NE_PAIR];
+  constexpr int NACC = QK ? NB_NE_PAIRQ : NB_NE_PAIR;
+  double acc[NACC];
 #pragma unroll
-  for (int k = 0; k < NB_NE_PAIR; k++) acc[k] = 0.;
+  for (int k = 0; k < NACC; k++) acc[k] = 0.;
   const int lane = lane_id();
   const int nw = A.nblocks * 4;
   constexpr unsigned fv = IFLG ? NB_FLAG_VT : NB_FLAG_VS;
@@ -596,6 +601,16 @@ static __device__ __forceinline__ void pair64_run(const PairArgs &A, int bid, co
                 double v = interp3c(ct, kk, 2, p);
                 acc[E_SPOTHE] += qij * (v - A.clast);
                 acc[E_SPOTE] += qij * v;
+                if (QK) {
+                  acc[E_QQ] += qij;
+                  acc[E_QR2] += qij * d2;
+                  acc[E_Q0] += qij * dx * dx;
+                  acc[E_Q1] += qij * dy * dx;
+                  acc[E_Q2] += qij * dz * dx;
+                  acc[E_Q4] += qij * dy * dy;
+                  acc[E_Q5] += qij * dz * dy;
+                  acc[E_Q8] += qij * dz * dz;
+                }
               }
               if (WF) { fx -= dx * fcc; fy -= dy * fcc; fz -= dz * fcc; }
             }
@@ -618,10 +633,10 @@ static __device__ __forceinline__ void pair64_run(const PairArgs &A, int bid, co
       }
     }
   }
-  if (ENER) block_reduce_store<NB_NE_PAIR>(acc, A.epart + bid, A.nblocks);
+  if (ENER) block_reduce_store<NACC>(acc, A.epart + bid, A.nblocks);
 }
 
-template <int IFLG, bool WANT_FS, bool ENER, bool WF>
+template <int IFLG, bool WANT_FS, bool ENER, bool WF, bool QK = false>
 static __device__ __forceinline__ void pair64_body(const PairArgs &A, int bid) {
   extern __shared__ double tabs[];
   __shared__ InterPar sip[NB_IP_LDS];
@@ -638,11 +653,13 @@ static __device__ __forceinline__ void pair64_body(const PairArgs &A, int bid) {
   }
   __syncthreads();
   if (A.tab_lds && ip_lds)
-    pair64_run<IFLG, WANT_FS, true, ENER, WF>(A, bid, tabs, tabs + A.ctab_off, sip, smap, ip_lds);
+    pair64_run<IFLG, WANT_FS, true, ENER, WF, QK>(A, bid, tabs, tabs + A.ctab_off, sip, smap,
+                                                  ip_lds);
   else if (A.tab_lds)
-    pair64_run<IFLG, WANT_FS, false, ENER, WF>(A, bid, tabs, tabs + A.ctab_off, sip, smap, ip_lds);
+    pair64_run<IFLG, WANT_FS, false, ENER, WF, QK>(A, bid, tabs, tabs + A.ctab_off, sip, smap,
+                                                   ip_lds);
   else
-    pair64_run<IFLG, WANT_FS, false, ENER, WF>(A, bid, A.vtab3, A.ctab3, sip, smap, ip_lds);
+    pair64_run<IFLG, WANT_FS, false, ENER, WF, QK>(A, bid, A.vtab3, A.ctab3, sip, smap, ip_lds);
 }
 // 0t: force(...,0) (fs, clamp count); 1t: force(...,1) with fs and energies;
 // 1f: fl and energies; 1n: fl only (the MD step); 1e: energies only
@@ -651,11 +668,14 @@ static __device__ __forceinline__ void pair64_body(const PairArgs &A, int bid) {
 #define k_pair64_1f_body pair64_body<1, false, true, true>
 #define k_pair64_1n_body pair64_body<1, false, false, true>
 #define k_pair64_1e_body pair64_body<1, false, true, false>
+// 1q: 1e + the sums of cm3d's energy-only coulomb routine (nb_energy_only)
+#define k_pair64_1q_body pair64_body<1, false, true, false, true>
 NB_KERNEL_PAIR_W(k_pair64_0t, PairArgs, 256, 4)
 NB_KERNEL_PAIR_W(k_pair64_1t, PairArgs, 256, 3)
 NB_KERNEL_PAIR_W(k_pair64_1f, PairArgs, 256, 4)
 NB_KERNEL_PAIR_W(k_pair64_1n, PairArgs, 256, 4)
 NB_KERNEL_PAIR_W(k_pair64_1e, PairArgs, 256, 4)
+NB_KERNEL_PAIR_W(k_pair64_1q, PairArgs, 256, 4)
 
 // ---------------------------------------------------------------------------
 // Reciprocal-space Ewald, FP64 throughout (fk_ewald, md_ewald.c:185-361).
@@ -1179,17 +1199,19 @@ int nb_launch_rebuild_check(nb_ctx *c, const double *d_pos, hipStream_t s) {
 }
 
 // The energy-column reductions of force(...,1): which tables, how many rows.
-RedSet nb_red_set(const nb_ctx *c, int iflg, bool pair) {
+RedSet nb_red_set(const nb_ctx *c, int iflg, bool pair, bool quirk) {
   RedSet R;
   memset(&R, 0, sizeof(R));
   const bool ec = iflg == 1 && c->iperd == 3 && c->necorr > 0;
   const bool ew = iflg == 1 && c->iperd == 3 && c->ktot > 0 && c->ncharge > 0;
-  R.part[0] = c->d_epair; R.rows[0] = c->npair_blocks; R.cols[0] = pair ? NB_NE_PAIR : 0;
+  // no pair energy pass ran: its columns reduce over no rows, i.e. to zero
+  R.part[0] = c->d_epair; R.rows[0] = pair ? c->npair_blocks : 0;
+  R.cols[0] = quirk ? NB_NE_PAIRQ : NB_NE_PAIR;
   R.out[0] = c->d_eout;
   R.part[1] = c->d_eec; R.rows[1] = c->nfin_blocks; R.cols[1] = ec ? NB_NE_ECORR : 0;
-  R.out[1] = c->d_eout + 20;
+  R.out[1] = c->d_eout + NB_EOUT_ECORR;
   R.part[2] = c->d_eew; R.rows[2] = c->new_blocks; R.cols[2] = ew ? NB_NE_EW : 0;
-  R.out[2] = c->d_eout + 32;
+  R.out[2] = c->d_eout + NB_EOUT_EW;
   R.ncols = R.cols[0] + R.cols[1] + R.cols[2];
   return R;
 }
@@ -1233,7 +1255,9 @@ int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_f
       P.clast = c->clast;
       P.b = b; P.fl = d_fl; P.fs = d_fs; P.epart = c->d_epair;
       const int g = c->npair_blocks, lds = c->tab_lds_bytes;
-      if (mixed || eonly)
+      if (eonly && (flags & NB_FORCE_CM3D_EONLY))
+        NB_GO(c, K_PAIR64_1Q, k_pair64_1q, g, 256, lds, P, s);
+      else if (mixed || eonly)
         NB_GO(c, K_PAIR64_1E, k_pair64_1e, g, 256, lds, P, s);
       else if (iflg == 0)
         NB_GO(c, K_PAIR64_0T, k_pair64_0t, g, 256, lds, P, s);
@@ -1247,7 +1271,6 @@ int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_f
     }
   }
   nb_time_mark(c, 2, s);
-  nb_time_mark(c, 3, s);
   const bool ec = iflg == 1 && c->iperd == 3 && c->necorr > 0;
   const bool ew = iflg == 1 && c->iperd == 3 && c->ktot > 0 && c->ncharge > 0;
   const bool ewf = ew && !(flags & NB_FORCE_EONLY);  // reciprocal-space forces
@@ -1256,9 +1279,11 @@ int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_f
   EwChgArgs ch = {c->d_pos, c->d_qch, c->d_qidx, nq, n, b, (double4 *)c->d_chg,
                   (double2 *)c->d_stp};
   NB_GO(c, K_EWCHG, k_ewchg, rp && ew && !packed ? blocks(nq, 256) : 0, 256, 0, ch, s);
+  nb_time_mark(c, 3, s);
   EwSfacArgs sf = {c->d_sseg, (const double4 *)c->d_chg, (const double2 *)c->d_stp, nq, c->qc,
                    c->nseg, c->nsw, c->ktot, (double2 *)c->d_spart};
   NB_GO(c, K_EWSFAC, k_ewsfac, rp && ew ? c->nsw * c->nchunk : 0, 256, 0, sf, s);
+  nb_time_mark(c, 4, s);
   double det = b.h[0] * (b.h[4] * b.h[8] - b.h[5] * b.h[7]) +
                b.h[1] * (b.h[5] * b.h[6] - b.h[3] * b.h[8]) +
                b.h[2] * (b.h[3] * b.h[7] - b.h[4] * b.h[6]);
@@ -1267,21 +1292,26 @@ int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_f
   EwKredArgs kr = {c->d_kabc, c->ktot, (const double2 *)c->d_spart, c->nchunk, c->new_blocks,
                    b, 1. / det, falp2, (KData *)c->d_kd, c->d_eew};
   NB_GO(c, K_EWKRED, k_ewkred, rp && ew ? c->new_blocks : 0, 256, 0, kr, s);
+  nb_time_mark(c, 5, s);
   EwForceArgs ef = {c->d_seg, c->d_rtile, c->d_tseg, (const KData *)c->d_kd,
                     (const double4 *)c->d_chg, (const double2 *)c->d_stp, nq, c->nqb, c->d_fpart,
                     b};
   NB_GO(c, K_EWFORCE, k_ewforce, rp && ewf ? c->nqb * c->nrange : 0, 256, 0, ef, s);
+  nb_time_mark(c, 6, s);
   EwFinArgs fa = {c->d_fpart, (const double4 *)c->d_chg, c->d_qidx, nq, n, c->nrange, ewf, ec,
                   c->d_eoff, c->d_elist, c->d_pos, c->d_qch, c->kappa, d_fl, c->nfin_blocks,
                   c->d_eec};
   NB_GO(c, K_EWFIN, k_ewfin, fin && (ewf || ec) ? c->nfin_blocks : 0, 256, 0, fa, s);
   if (fin && !(flags & NB_FORCE_NORED)) {
-    RedSet R = nb_red_set(c, iflg, !noener && !(flags & NB_FORCE_NO_PAIR));
+    // the pair energy columns were written by an FP64 pair kernel with
+    // energies (0t counts clamps; a mixed context's only one is 1e)
+    const bool pcols = !(flags & NB_FORCE_NO_PAIR) &&
+                       (mixed ? iflg == 1 && !noener : iflg == 0 || !noener || d_fs);
+    RedSet R = nb_red_set(c, iflg, pcols, eonly && (flags & NB_FORCE_CM3D_EONLY));
     NB_GO(c, K_RED, k_red, R.ncols, 256, 0, R, s);
   }
   if (!c->rec) LCHK(hipGetLastError());
-  nb_time_mark(c, 4, s);
-  nb_time_mark(c, 5, s);
+  nb_time_mark(c, 7, s);
   if (c->timing && !c->rec) c->trec++;
   return NB_OK;
 }
@@ -1302,6 +1332,7 @@ static struct RegisterKernels {
     g_batch_fn[K_PAIR64_1F] = k_pair64_1f_bl;
     g_batch_fn[K_PAIR64_1N] = k_pair64_1n_bl;
     g_batch_fn[K_PAIR64_1E] = k_pair64_1e_bl;
+    g_batch_fn[K_PAIR64_1Q] = k_pair64_1q_bl;
     g_batch_fn[K_RED] = k_red_bl;
     g_batch_fn[K_EWCHG] = k_ewchg_bl;
     g_batch_fn[K_EWSFAC] = k_ewsfac_bl;

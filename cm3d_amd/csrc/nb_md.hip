@@ -830,6 +830,19 @@ extern "C" int nb_md_get_state(nb_md *md, double *pos, double *vel) {
   return NB_OK;
 }
 
+// The forces save_force prints (output/md_saveout.c:65-90): fl, the long
+// (nonbonded) force of the current state before rmass, and aa, the
+// intramolecular accelerations (cm3d's fxl * amass and fxa).
+extern "C" int nb_md_get_forces(nb_md *md, double *fl, double *aa) {
+  if (!md) { nb_set_error("nb_md_get_forces: null argument"); return NB_ERR_ARG; }
+  nb_ctx *c = md->c;
+  MCHK(hipSetDevice(c->dev));
+  MCHK(hipDeviceSynchronize());  // the step may have run on another stream
+  if (fl) MCHK(hipMemcpy(fl, c->d_fl, sizeof(double) * 3 * md->n, hipMemcpyDeviceToHost));
+  if (aa) MCHK(hipMemcpy(aa, md->d_aa, sizeof(double) * 3 * md->n, hipMemcpyDeviceToHost));
+  return NB_OK;
+}
+
 // set_therm + samveta's result (md_therm.new.c:158-169, 248-275) for one
 // "global" thermostat: eta starts at 0 and veta at veta0.
 extern "C" int nb_md_set_thermostat(nb_md *md, int nchain, const double *gkt, const double *mnh,

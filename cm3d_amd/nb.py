@@ -63,7 +63,9 @@ class NBEnergy(C.Structure):
                 ("vk_ewald", C.c_double), ("vp_ewald", C.c_double),
                 ("vp_ewald_tensor", C.c_double * 9),
                 ("pot_ecorr", C.c_double), ("prs_ecorr", C.c_double),
-                ("prs_ecorr_tensor", C.c_double * 9), ("nclamp", C.c_int)]
+                ("prs_ecorr_tensor", C.c_double * 9), ("nclamp", C.c_int),
+                ("spote_phys", C.c_double), ("spothe_phys", C.c_double),
+                ("sprse_phys", C.c_double), ("sprset_phys", C.c_double * 9)]
 
     def as_dict(self):
         out = {}
@@ -103,7 +105,7 @@ EXPORTS = ["nb_create", "nb_destroy", "nb_last_error", "nb_build_lists",
            "nb_build_lists_device", "nb_list_counts", "nb_get_pairs", "nb_timing",
            "nb_footprint", "nb_list_reserve",
            "nb_md_create", "nb_md_destroy", "nb_md_set_state", "nb_md_get_state",
-           "nb_md_init", "nb_md_step", "nb_md_energies", "nb_md_rebuilds",
+           "nb_md_init", "nb_md_step", "nb_md_energies", "nb_md_rebuilds", "nb_md_get_forces",
            "nb_md_set_thermostat", "nb_md_get_thermostat",
            "nb_batch_create", "nb_batch_destroy", "nb_batch_step", "nb_batch_launches",
            "nb_batch_recoveries", "nb_table_pair", "nb_table_coul", "nb_kvectors", "nb_ewald_consts"]
@@ -145,6 +147,7 @@ def lib():
     L.nb_md_destroy.restype = None
     L.nb_md_set_state.argtypes = [C.c_void_p, _dp, _dp, _dp]
     L.nb_md_get_state.argtypes = [C.c_void_p, _dp, _dp]
+    L.nb_md_get_forces.argtypes = [C.c_void_p, _dp, _dp]
     L.nb_md_init.argtypes = [C.c_void_p]
     L.nb_md_step.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int]
     L.nb_md_energies.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int,
@@ -295,12 +298,17 @@ class NBContext:
         check(lib().nb_timing(self._h, 1, None), "nb_timing")
 
     def timing_read(self):
-        """Average ms per force call: pack, pair, ecorr, ewald, total."""
-        out = np.zeros(6)
+        """Average ms per force call by phase: pack, pair, ewchg (charge
+        records), sfac (structure factor), kred (k reduction), ewforce
+        (reciprocal force), fin (ecorr + force sum + energy reductions),
+        ewald (ewchg .. fin), total."""
+        out = np.zeros(9)
         check(lib().nb_timing(self._h, 2, out.ctypes.data_as(_dp)), "nb_timing")
         check(lib().nb_timing(self._h, 0, None), "nb_timing")
-        return dict(calls=int(out[0]), pack=out[1], pair=out[2], ecorr=out[3],
-                    ewald=out[4], total=out[5])
+        d = dict(calls=int(out[0]), pack=out[1], pair=out[2], ewchg=out[3], sfac=out[4],
+                 kred=out[5], ewforce=out[6], fin=out[7], total=out[8])
+        d["ewald"] = d["ewchg"] + d["sfac"] + d["kred"] + d["ewforce"] + d["fin"]
+        return d
 
     def list_counts(self):
         """Half-pair counts as cm3d reports them: nprs, nprt, nprse, nprte."""
@@ -444,6 +452,15 @@ class NBMD:
         check(lib().nb_md_get_state(self._h, pos.ctypes.data_as(_dp),
                                     vel.ctypes.data_as(_dp)), "nb_md_get_state")
         return pos, vel
+
+    def get_forces(self):
+        """(fl, aa): the nonbonded force before rmass and the bonded
+        accelerations of the current state (what save_force prints)."""
+        fl = np.zeros(3 * self.n)
+        aa = np.zeros(3 * self.n)
+        check(lib().nb_md_get_forces(self._h, fl.ctypes.data_as(_dp), aa.ctypes.data_as(_dp)),
+              "nb_md_get_forces")
+        return fl, aa
 
     def init(self):
         check(lib().nb_md_init(self._h), "nb_md_init")

@@ -105,6 +105,10 @@ typedef struct nb_energy {
   double vk_ewald, vp_ewald, vp_ewald_tensor[9];
   double pot_ecorr, prs_ecorr, prs_ecorr_tensor[9];
   int nclamp;                              /* pairs below table min    */
+  /* No cm3d counterpart: the real-space coulomb energies and virials of the
+   * force path.  Equal to spote .. sprset except after nb_energy_only, whose
+   * spote .. sprset are cm3d's vinter_e values (see there). */
+  double spote_phys, spothe_phys, sprse_phys, sprset_phys[9];
 } nb_energy;
 
 int nb_create(const nb_params *p, nb_ctx **ctx);
@@ -129,10 +133,13 @@ int nb_force(nb_ctx *ctx, int iflg, const double *pxyz, const double hmat[9],
  * through getvirter / getvirter_e (vinter_npol, md_fnopol.c:257-383;
  * vinter_e, finter_e.c:260-388; over the current t / te lists), plus, for
  * iperd 3, the reciprocal-space and ecorr energies and virials fk_ewald /
- * ecorr would leave in COUL -- all of e's fields, no forces.  Values are
- * those of nb_force(...,1), bit for bit.  (cm3d's own vinter_e indexes its
- * table from rmaxl_coul2, so every pair is clamped to the first entry; the
- * engine returns the force path's real-space coulomb energy instead.) */
+ * ecorr would leave in COUL -- all of e's fields, no forces.  The vdW,
+ * Ewald and ecorr values are those of nb_force(...,1), bit for bit.  The
+ * coulomb fields spote .. sprset are cm3d's vinter_e values: a missing break
+ * in its cutoff switch (finter_e.c:296-302) sets the table origin to
+ * rmaxl_coul2, so every te pair inside the cutoff is clamped to the table
+ * at k = 1, p = -1 (about coultab[0]); the force path's values are in
+ * spote_phys .. sprset_phys. */
 int nb_energy_only(nb_ctx *ctx, const double *pxyz, const double hmat[9], nb_energy *e);
 
 /* Same on device-resident arrays (3N doubles in HBM) on the given HIP stream
@@ -170,9 +177,11 @@ int nb_get_pairs(nb_ctx *ctx, int which, int *ip, int *jp, int cap, int *n);
 /* Per-phase timing of nb_force / nb_force_device with HIP events on the
  * stream the kernels run on (no counterpart in cm3d; diagnostics).
  * mode 1: start recording (up to 2048 calls), 0: stop, 2: read the averages
- * in ms: out = {calls, position pack, pair kernel, (gap between pair and
- * Ewald phases), Ewald + ecorr + energy reductions, whole call}. */
-int nb_timing(nb_ctx *ctx, int mode, double out[6]);
+ * in ms: out = {calls, position pack, pair kernel(s), Ewald charge records,
+ * structure factor, k reduction, reciprocal force, ecorr + force sum +
+ * energy reductions, whole call}. */
+#define NB_TIMING_N 9
+int nb_timing(nb_ctx *ctx, int mode, double out[NB_TIMING_N]);
 
 /* ---- device-resident MD step (cm3d's step(), integrate/md_step.c:62-159,
  * for nres_lrf == 1) ----
@@ -213,6 +222,11 @@ void nb_md_destroy(nb_md *md);
 int nb_md_set_state(nb_md *md, const double *pos, const double *vel,
                     const double hmat[9]);
 int nb_md_get_state(nb_md *md, double *pos, double *vel);
+/* The forces of the current state as save_force reads them
+ * (output/md_saveout.c:65-90): fl [3N] the nonbonded force before rmass
+ * (cm3d's fxl times amass), aa [3N] the bonded accelerations (fxa); either
+ * may be NULL. */
+int nb_md_get_forces(nb_md *md, double *fl, double *aa);
 /* step_init (md_step.c:32-58): list build, force(...,1), bonded forces. */
 int nb_md_init(nb_md *md);
 /* nsteps outer steps on the stream (NULL = the context's); use_graph

@@ -144,22 +144,37 @@ def test_mixed_bitwise_reproducible():
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
-@pytest.mark.parametrize("name", ["ar864", "water512", "mix600", "ref_abutane"])
+@pytest.mark.parametrize("name", ["ar864", "water512", "mix600", "ref_abutane", "water512_tri"])
 def test_energy_only_equals_force_energies(name, prec):
-    """nb_energy_only: every energy / virial field of force(...,1), bit for
-    bit, without the forces; the vdW sums equal cm3d's own energy-only
-    getvireal(...,0) (vinter_npol) within the FP64 bar."""
+    """nb_energy_only returns what cm3d's getvireal(...,0) sums: the vdW
+    fields (vinter_npol) equal force(...,1)'s bit for bit and cm3d's within
+    the FP64 bar; the coulomb fields are cm3d's vinter_e values -- every te
+    pair inside the cutoff clamped to the table origin -- equal to the
+    fixture's dumped e0_velec / e0_welec / tensor within the FP64 bar, and
+    the force path's coulomb values are in the *_phys fields, bit for bit
+    force(...,1)'s; Ewald and ecorr bit for bit."""
     d = load_fixture(name)
     print(f"[eonly] {name} {prec}", flush=True)
     c = _ctx(d, prec)
     c.build_lists(d["pos"], d["hmat"])
     _, _, ef = c.force(d["pos"], d["hmat"], iflg=1)
     eo = c.energy_only(d["pos"], d["hmat"])
+    coul = ("spote", "spothe", "sprse", "sprset")
     for k, v in ef.items():
-        assert np.array_equal(np.asarray(eo[k]), np.asarray(v)), (k, eo[k], v)
-    if prec == "fp64":
-        for mine, ref in (("spotl", "e0_vinter"), ("spothl", "e0_vhinter")):
-            assert _rel(eo[mine], d[ref][0]) < FP64_TOL or abs(eo[mine] - d[ref][0]) < 1e-9
+        if k in coul:
+            continue
+        if k.endswith("_phys"):
+            assert np.array_equal(np.asarray(eo[k]), np.asarray(ef[k[:-5]])), (k, eo[k])
+        else:
+            assert np.array_equal(np.asarray(eo[k]), np.asarray(v)), (k, eo[k], v)
+    for mine, ref in (("spotl", "e0_vinter"), ("spothl", "e0_vhinter"), ("spote", "e0_velec"),
+                      ("sprse", "e0_welec")):
+        assert _rel(eo[mine], d[ref][0]) < FP64_TOL or abs(eo[mine] - d[ref][0]) < 1e-9, \
+            (mine, eo[mine], d[ref][0])
+    assert np.allclose(eo["sprset"], d["e0_welectensor"], rtol=FP64_TOL, atol=1e-9)
+    if float(np.abs(d["qch"]).max()) > 0:
+        # cm3d's energy-only coulomb value is not the physical one
+        assert abs(eo["spote"] - eo["spote_phys"]) > 1e-6 * abs(eo["spote_phys"])
     c.close()
 
 
