@@ -157,10 +157,11 @@ def test_c4_batch_mixed_forces_and_batch_equals_alone():
 
 def test_c5_stream_slice_served_equals_alone():
     from cm3d_amd import nb, serve
-    # 20 jobs of the serving families at the full size ranges, arriving at
-    # 200/s so that they queue behind 6 slots and join running batches;
-    # short runs (the test checks the mix, not the run length)
-    jobs = serve.job_stream(20, 200.0, seed=77, steps=(10, 40))
+    # 20 jobs of the serving families at the full size ranges, all arriving
+    # within a fraction of a millisecond so that they queue behind 6 slots
+    # and join running batches as slots free up; short runs (the test checks
+    # the mix, not the run length)
+    jobs = serve.job_stream(20, 1e5, seed=77, steps=(10, 40))
     for j in jobs:
         j.prepare()
     sizes = sorted(j.natoms for j in jobs)
