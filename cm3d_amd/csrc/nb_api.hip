@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "nb_internal.h"
@@ -20,6 +21,42 @@ void nb_set_error(const char *fmt, ...) {
 }
 
 extern "C" const char *nb_last_error(void) { return g_err; }
+
+// ---------------------------------------------------------------------------
+// The static scratch area of nb_force_device_noalloc: one per GPU, held by
+// the loaded code object until the process ends.  Host code never uses the
+// symbol's name as an address: each device's address is looked up once with
+// that device current (at nb_create, else on the entry point's first call
+// there) and kept below.
+#define NB_SCRATCH_BYTES (32u << 20)
+#define NB_MAX_DEVICES 64
+static __device__ __align__(256) char scratch[NB_SCRATCH_BYTES];
+static char *g_scratch_ptr[NB_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+static int scratch_ptr(int dev, char **out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (dev < 0 || dev >= NB_MAX_DEVICES) {
+    nb_set_error("scratch: device %d out of range", dev);
+    return NB_ERR_ARG;
+  }
+  if (!g_scratch_ptr[dev]) {
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e == hipSuccess && cur != dev) e = hipSetDevice(dev);
+    void *p = nullptr;
+    if (e == hipSuccess) e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur >= 0 && cur != dev) hipSetDevice(cur);
+    if (e != hipSuccess || !p) {
+      nb_set_error("hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                   hipGetErrorString(e != hipSuccess ? e : hipErrorInvalidSymbol));
+      return NB_ERR_HIP;
+    }
+    g_scratch_ptr[dev] = (char *)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return NB_OK;
+}
 
 #define HCHK(x)                                                        \
   do {                                                                 \
@@ -447,6 +484,10 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
   }
   HCHK(hipMemset(c->d_eout, 0, 64 * sizeof(double)));
   HCHK(hipMemset(c->d_flags, 0, 8 * sizeof(int)));
+  {
+    char *sp;
+    if ((rc = scratch_ptr(c->dev, &sp))) return fail(rc);
+  }
   *out = c;
   return NB_OK;
 }
@@ -740,6 +781,80 @@ extern "C" int nb_force_device(nb_ctx *c, int iflg, const double *d_pxyz, const 
     HCHK(hipMemcpyAsync(c->d_pos, d_pxyz, sizeof(double) * 3 * c->natoms,
                         hipMemcpyDeviceToDevice, s));
   return force_on(c, iflg, hmat, d_fs, iflg ? d_fl : nullptr, e, s, e != nullptr);
+}
+
+// nb_force_device with the per-evaluation scratch (Ewald charge records,
+// structure-factor partials, per-k data, reciprocal-force partials, energy
+// columns) carved from the static scratch area instead of the context's own
+// buffers; allocates nothing.  Every kernel overwrites the scratch it reads
+// before reading it, so nothing in it has to start at zero.
+static size_t scratch_layout(const nb_ctx *c, size_t off[10]) {
+  const size_t nq = std::max(c->ncharge, 1), kt = std::max(c->ktot, 1);
+  const size_t bytes[9] = {32 * nq, 16 * nq, 16 * (size_t)c->nchunk * kt, nb_kdata_bytes() * kt,
+                           24 * (size_t)c->nrange * nq,
+                           8 * (size_t)c->npair_blocks * NB_NE_PAIRQ,
+                           8 * (size_t)c->nfin_blocks * NB_NE_ECORR,
+                           8 * (size_t)c->new_blocks * NB_NE_EW, 8 * 64};
+  size_t o = 0;
+  for (int k = 0; k < 9; k++) {
+    off[k] = o;
+    o += (bytes[k] + 255) & ~(size_t)255;
+  }
+  off[9] = o;
+  return o;
+}
+
+extern "C" int nb_force_device_noalloc(nb_ctx *c, int iflg, const double *d_pxyz,
+                                       const double hmat[9], double *d_fs, double *d_fl,
+                                       nb_energy *e, void *stream) {
+  if (!c || !d_pxyz || !hmat || (iflg == 1 && !d_fl) || (iflg == 0 && !d_fs)) {
+    nb_set_error("nb_force_device_noalloc: null argument");
+    return NB_ERR_ARG;
+  }
+  if (iflg != 0 && iflg != 1) {
+    nb_set_error("nb_force_device_noalloc: iflg must be 0 or 1");
+    return NB_ERR_ARG;
+  }
+  size_t off[10];
+  const size_t need = scratch_layout(c, off);
+  if (need > NB_SCRATCH_BYTES) {
+    nb_set_error("nb_force_device_noalloc: this system needs %zu bytes of scratch, the static "
+                 "area holds %u (use nb_force_device)", need, NB_SCRATCH_BYTES);
+    return NB_ERR_CAPACITY;
+  }
+  if (!c->have_lists) {
+    nb_set_error("nb_force_device_noalloc: lists not built (call nb_build_lists)");
+    return NB_ERR_ARG;
+  }
+  char *base;
+  RC(scratch_ptr(c->dev, &base));
+  hipStream_t s = stream ? (hipStream_t)stream : c->stream;
+  HCHK(hipSetDevice(c->dev));
+  if (d_pxyz != c->d_pos)
+    HCHK(hipMemcpyAsync(c->d_pos, d_pxyz, sizeof(double) * 3 * c->natoms,
+                        hipMemcpyDeviceToDevice, s));
+  // the context's tables, lists and records; the scratch from the static area
+  nb_ctx v = *c;
+  v.d_chg = (double *)(base + off[0]);
+  v.d_stp = (double *)(base + off[1]);
+  v.d_spart = (double *)(base + off[2]);
+  v.d_kd = (double *)(base + off[3]);
+  v.d_fpart = (double *)(base + off[4]);
+  v.d_epair = (double *)(base + off[5]);
+  v.d_eec = (double *)(base + off[6]);
+  v.d_eew = (double *)(base + off[7]);
+  v.d_eout = (double *)(base + off[8]);
+  v.timing = 0;
+  v.rec = nullptr;
+  Box b = make_box(hmat);
+  RC(nb_launch_force(&v, iflg, b, d_fs, iflg ? d_fl : nullptr, s, e ? 0 : NB_FORCE_NOENER));
+  if (e) {
+    double E[64];
+    HCHK(hipMemcpyAsync(E, v.d_eout, sizeof(E), hipMemcpyDeviceToHost, s));
+    HCHK(hipStreamSynchronize(s));
+    nb_fill_energy(c, iflg, E, hmat, e);
+  }
+  return NB_OK;
 }
 
 // getvireal(...,0)'s pair sums (getvirter / getvirter_e over the t and te

@@ -103,7 +103,7 @@ class NBMDEnergy(C.Structure):
 EXPORTS = ["nb_create", "nb_destroy", "nb_last_error", "nb_build_lists",
            "nb_need_rebuild", "nb_force", "nb_force_device", "nb_energy_only",
            "nb_build_lists_device", "nb_list_counts", "nb_get_pairs", "nb_timing",
-           "nb_footprint", "nb_list_reserve",
+           "nb_footprint", "nb_list_reserve", "nb_force_device_noalloc",
            "nb_md_create", "nb_md_destroy", "nb_md_set_state", "nb_md_get_state",
            "nb_md_init", "nb_md_step", "nb_md_energies", "nb_md_rebuilds", "nb_md_get_forces",
            "nb_md_set_thermostat", "nb_md_get_thermostat",
@@ -137,6 +137,9 @@ def lib():
                                   C.c_void_p, C.c_void_p, C.POINTER(NBEnergy),
                                   C.c_void_p]
     L.nb_build_lists_device.argtypes = [C.c_void_p, C.c_void_p, _dp, C.c_void_p]
+    L.nb_force_device_noalloc.argtypes = [C.c_void_p, C.c_int, C.c_void_p, _dp,
+                                          C.c_void_p, C.c_void_p, C.POINTER(NBEnergy),
+                                          C.c_void_p]
     L.nb_energy_only.argtypes = [C.c_void_p, _dp, _dp, C.POINTER(NBEnergy)]
     L.nb_list_counts.argtypes = [C.c_void_p, _ip]
     L.nb_footprint.argtypes = [C.c_void_p, C.POINTER(C.c_size_t)]
@@ -286,6 +289,19 @@ class NBContext:
                                     C.byref(e) if e is not None else None,
                                     C.c_void_p(stream) if stream else None),
               "nb_force_device")
+        return e.as_dict() if e is not None else None
+
+    def force_device_noalloc(self, iflg, d_pos, hmat, d_fs, d_fl, stream=None, energy=False):
+        """force_device with its scratch in the library's static area: no
+        device allocation (nb_force_device_noalloc)."""
+        h, hp = _d(hmat)
+        e = NBEnergy() if energy else None
+        check(lib().nb_force_device_noalloc(self._h, int(iflg), C.c_void_p(d_pos), hp,
+                                            C.c_void_p(d_fs) if d_fs else None,
+                                            C.c_void_p(d_fl) if d_fl else None,
+                                            C.byref(e) if e is not None else None,
+                                            C.c_void_p(stream) if stream else None),
+              "nb_force_device_noalloc")
         return e.as_dict() if e is not None else None
 
     def build_lists_device(self, d_pos, hmat, stream=None):

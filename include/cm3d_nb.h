@@ -150,6 +150,21 @@ int nb_force_device(nb_ctx *ctx, int iflg, const double *d_pxyz,
 int nb_build_lists_device(nb_ctx *ctx, const double *d_pxyz,
                           const double hmat[9], void *stream);
 
+/* nb_force_device that allocates nothing.  No counterpart in cm3d: it is for
+ * callers that cannot allocate device memory where they evaluate forces
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The context (created beforehand) supplies the tables, lists
+ * and records; the per-evaluation scratch comes from one static area per GPU
+ * of 32 MiB, held until the process ends, shared by every call on that GPU.
+ * A system whose scratch would not fit is refused with NB_ERR_CAPACITY
+ * (about 17 MiB for 12,288 charges and 1535 k vectors; nb_force_device has
+ * no such limit).  Calls that overlap on one GPU race on the shared area:
+ * the caller serialises them (one stream, or waits between streams).
+ * Results are bit for bit nb_force_device's. */
+int nb_force_device_noalloc(nb_ctx *ctx, int iflg, const double *d_pxyz,
+                            const double hmat[9], double *d_fs, double *d_fl,
+                            nb_energy *e, void *stream);
+
 /* Pair counts of the current lists as cm3d counts them (half pairs):
  * counts[0..3] = nprs, nprt, nprse, nprte. */
 int nb_list_counts(nb_ctx *ctx, int counts[4]);
