@@ -75,9 +75,11 @@ def parse():
     ap.add_argument("--serve-slots", type=int, default=64)
     for blk in ("cpu", "c2", "batch", "serve"):
         ap.add_argument(f"--no-{blk}", action="store_true")
-    ap.add_argument("--traffic-probe", choices=["c2", "c3"], default=None,
+    ap.add_argument("--traffic-probe", choices=["c2", "c2m", "c3"], default=None,
                     help="run only a 256 MiB device copy (counter calibration) and 5 force(...,1) "
-                         "calls on the 4096- (c2) or 32768-water (c3) system, for rocprofv3 --pmc")
+                         "calls as the MD step makes them (forces only) on the 4096-water "
+                         "system in FP64 (c2) or mixed (c2m), or the 32768-water system mixed "
+                         "(c3), for rocprofv3 --pmc")
     ap.add_argument("--only-batch", action="store_true",
                     help="run the batched form alone (profiling); prints its own JSON")
     return ap.parse_args()
@@ -442,7 +444,7 @@ def traffic_probe(nb, S, a, torch):
     if a.traffic_probe == "c3":
         s, prec = water_c3(S), "mixed"
     else:
-        s, prec = S.water(nside=16), "fp64"
+        s, prec = S.water(nside=16), "mixed" if a.traffic_probe == "c2m" else "fp64"
     ctx = nb.NBContext(s, precision=prec)
     ctx.build_lists(s["pos"], s["hmat"])
     d_pos = torch.from_numpy(s["pos"]).cuda()

@@ -45,10 +45,12 @@ DEFAULTS = dict(calculation="md", istart=1, ntime=1, ensemble="nve", dt=1.0, t_e
                 rcute_resp=7.0, iwr_screen=0, iwr_inst=0, iwr_dump=0, iseed=None, scaleeps=1.0,
                 scale_temp=1.0, inter_file="data_base", bond_file="data_base",
                 bend_file="data_base", mol_set_file=None, dpi_file=None, ham_file=None,
-                eng_file=None, long_rc="on")
+                eng_file=None, long_rc="on", iwr_confp=0, iwr_confv=0, iwr_conff=0,
+                pos_file=None, vel_file=None, force_file=None)
 
 _INT_KEYS = {"istart", "ntime", "iperd", "nres_lrf", "nres_intra", "nres_tors", "len_nhc",
-             "nbar", "ntable", "kmax", "ishift", "iwr_screen", "iwr_inst", "iwr_dump", "iseed"}
+             "nbar", "ntable", "kmax", "ishift", "iwr_screen", "iwr_inst", "iwr_dump", "iseed",
+             "iwr_confp", "iwr_confv", "iwr_conff"}
 _FLOAT_KEYS = {"dt", "t_ext", "p_ext", "skin", "rheal_res", "tau_nhc", "alp_ewald",
                "rcute_max", "rcute_min", "rcute_resp", "scaleeps"}
 
@@ -398,7 +400,8 @@ def load(deck_path):
     run = dict(ctl=ctl, istart=istart, istep0=istep0, dt_ps=ctl["dt_ps"],
                nres_intra=ctl["nres_intra"], ensemble=ctl["ensemble"].lower(), ndof=ndof,
                temp=temp, vel=vel, iwr_screen=ctl["iwr_screen"], iwr_inst=ctl["iwr_inst"],
-               iwr_dump=ctl["iwr_dump"], base=base, restart=rst,
+               iwr_dump=ctl["iwr_dump"], iwr_confp=ctl["iwr_confp"],
+               iwr_confv=ctl["iwr_confv"], iwr_conff=ctl["iwr_conff"], base=base, restart=rst,
                pext=ctl["p_ext"] / PCONV_ATM)
     # steps to run: ntime from istep 1 (istart 1, 2), or from the stored step
     # + 1 up to ntime (istart 3; md_mdcntr.new.c:157, 171)
@@ -505,6 +508,43 @@ def write_coord(path, pos, hmat):
             f.write("%10g %10g %10g\n" % (p[0, i], p[1, i], p[2, i]))
         for i in range(3):
             f.write("%g %g %g\n" % tuple(h[3 * i:3 * i + 3]))
+
+
+def write_traj_header(f, natoms, nwrite, dt_ps):
+    """The header line of a .conf / .vel / .force file (output/md_headers.c:
+    91-124).  cm3d writes the velocity interval into the force file's header
+    (md_headers.c:122-123); the caller passes it so."""
+    f.write("# %d %d %g\n" % (natoms, nwrite, dt_ps))
+
+
+def write_conf(f, pos, hmat):
+    """save_conf (output/md_saveout.c:30-47): positions, then hmat."""
+    p = np.asarray(pos).reshape(3, -1)
+    f.write("".join("%g %g %g\n" % (p[0, i], p[1, i], p[2, i]) for i in range(p.shape[1])))
+    f.write("%g %g %g %g %g %g %g %g %g\n" % tuple(np.asarray(hmat).reshape(-1)))
+    f.flush()
+
+
+def write_vel(f, vel):
+    """save_vel (output/md_saveout.c:49-62)."""
+    v = np.asarray(vel).reshape(3, -1)
+    f.write("".join("%g %g %g\n" % (v[0, i], v[1, i], v[2, i]) for i in range(v.shape[1])))
+    f.flush()
+
+
+def write_force(f, fl, aa, amass):
+    """save_force (output/md_saveout.c:65-90): (fxl + fxa + fxr) * amass with
+    cm3d's accelerations fxl = fl / m (rmass multiplies by 1/m,
+    utils/md_util.new.c:219-229), fxa the bonded ones and fxr (torsions,
+    1-4 terms) zero for the models here."""
+    n = len(amass)
+    fl = np.asarray(fl).reshape(3, n)
+    aa = np.asarray(aa).reshape(3, n)
+    am = np.asarray(amass, float)
+    rm = 1. / am
+    t = (fl * rm + aa + 0.) * am
+    f.write("".join("%g %g %g\n" % (t[0, i], t[1, i], t[2, i]) for i in range(n)))
+    f.flush()
 
 
 def write_inst_headers(fham, feng, natoms, ntherm, nchain, ninst, dt_ps):

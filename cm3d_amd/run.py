@@ -140,12 +140,41 @@ def run_deck(deck, precision="fp64", nsteps=None, files=True, out=None, graph=Tr
     ntherm = 1 if run["ensemble"] == "nvt" else 0
     d = out or run["base"]
     fham = feng = None
-    if inst > 0:  # a continued run (istart 3) appends (md_headers.c:145, 158)
-        mode = "a" if run["istart"] == 3 else "w"
+    mode = "a" if run["istart"] == 3 else "w"  # a continued run appends (md_headers.c:70-84)
+    if inst > 0:
         fham = open(os.path.join(d, ctl["ham_file"] or "md.ham"), mode)
         feng = open(os.path.join(d, ctl["eng_file"] or "md.eng"), mode)
         if run["istart"] != 3:
             decks.write_inst_headers(fham, feng, n, ntherm, nchain, inst, run["dt_ps"])
+    # .conf / .vel / .force every iwr_confp / iwr_confv / iwr_conff steps
+    traj = {}
+    for kind, key, fkey, hdr in (("conf", "iwr_confp", "pos_file", "iwr_confp"),
+                                 ("vel", "iwr_confv", "vel_file", "iwr_confv"),
+                                 ("force", "iwr_conff", "force_file", "iwr_confv")):
+        k = run[key] if files else 0
+        if k > 0:
+            fh = open(os.path.join(d, ctl[fkey] or f"md.{kind}"), mode)
+            if run["istart"] != 3:
+                decks.write_traj_header(fh, n, run[hdr], run["dt_ps"])
+            traj[kind] = (k, fh)
+
+    def save_traj(istep):
+        """cm3d's record order after a step: conf, vel, force
+        (main/md_mdcntr.new.c:222-231)."""
+        want = [kind for kind in ("conf", "vel", "force")
+                if kind in traj and istep % traj[kind][0] == 0]
+        if not want:
+            return
+        pos, vel = md.get_state()
+        for kind in want:
+            fh = traj[kind][1]
+            if kind == "conf":
+                decks.write_conf(fh, pos, sysd["hmat"])
+            elif kind == "vel":
+                decks.write_vel(fh, vel)
+            else:
+                fl, aa = md.get_forces()
+                decks.write_force(fh, fl, aa, sysd["amass"])
 
     def restart(istep, av):
         pos, vel = md.get_state()
@@ -174,11 +203,12 @@ def run_deck(deck, precision="fp64", nsteps=None, files=True, out=None, graph=Tr
                     recs.append((istep, report(e, n, nchain, ntherm, av, istep), e))
                 if dump and istep % dump == 0:
                     restart(istep, av)
+                save_traj(istep)
                 if inst and istep % inst == 0:
                     decks.write_inst(fham, feng, istep, run["dt_ps"], e)
         else:
             marks = set()
-            for k in (every, inst, dump):
+            for k in (every, inst, dump) + tuple(v[0] for v in traj.values()):
                 if k:
                     marks |= {m for m in range(first, last + 1) if m % k == 0}
             marks = sorted(marks | {last})
@@ -193,6 +223,7 @@ def run_deck(deck, precision="fp64", nsteps=None, files=True, out=None, graph=Tr
                     recs.append((istep, report(e, n, nchain, ntherm), e))
                 if dump and istep % dump == 0:
                     restart(istep, None)
+                save_traj(istep)
                 if inst and istep % inst == 0:
                     decks.write_inst(fham, feng, istep, run["dt_ps"], e)
         if files:  # finish(): restart + final coordinates
@@ -203,6 +234,8 @@ def run_deck(deck, precision="fp64", nsteps=None, files=True, out=None, graph=Tr
         if fham:
             fham.close()
             feng.close()
+        for _, fh in traj.values():
+            fh.close()
         md.close()
         ctx.close()
     return recs
