@@ -121,7 +121,7 @@ extern "C" void nb_destroy(nb_ctx *c) {
   hipDeviceSynchronize();
   void *ptrs[] = {c->d_itype, c->d_map, c->d_ip, c->d_vtab3, c->d_ctab3, c->d_vtab3f,
                   c->d_ctab3f, c->d_qch, c->d_xoff, c->d_xlist, c->d_eoff, c->d_elist,
-                  c->d_qidx, c->d_qpos, c->d_seg, c->d_sseg, c->d_tseg, c->d_rtile, c->d_fpart, c->d_kabc, c->d_pos, c->d_posb, c->d_fs, c->d_fl,
+                  c->d_qidx, c->d_qpos, c->d_row, c->d_sym, c->d_ent, c->d_trow, c->d_rtile, c->d_fpart, c->d_kabc, c->d_pos, c->d_posb, c->d_fs, c->d_fl,
                   c->d_perm, c->d_iperm, c->d_cell, c->d_ccount, c->d_cstart, c->d_cfill, c->d_cwork,
                   c->d_hcnt, c->d_pcut,
                   c->d_tmp, c->d_rec64, c->d_stype, c->d_nbr, c->d_nnb, c->d_flags,
@@ -320,35 +320,61 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
     for (int v : ec[i]) elist.push_back(v);
     eoff[i + 1] = (int)elist.size();
   }
-  // Ewald k-vectors and segments (runs of consecutive kc in one (ka,kb) row)
+  // Ewald k vectors as rows (maximal runs of consecutive kc at one (ka, kb)
+  // in k order), each walked by m = |kc| with kc and -kc paired
   c->ktot = (ncharge > 0) ? p->ktot : 0;
-  std::vector<EwSeg> seg;
+  std::vector<EwRow> row;
   std::vector<int> kabc(3 * (size_t)std::max(c->ktot, 1), 0);
   for (int k = 0; k < c->ktot; k++) {
     kabc[3 * k] = p->ka[k]; kabc[3 * k + 1] = p->kb[k]; kabc[3 * k + 2] = p->kc[k];
-    bool row = !seg.empty() && seg.back().ka == p->ka[k] && seg.back().kb == p->kb[k] &&
-               seg.back().kc0 + seg.back().cnt == p->kc[k];
-    if (row && seg.back().cnt < NB_EW_SEG) {
-      seg.back().cnt++;
+    if (!row.empty() && row.back().ka == p->ka[k] && row.back().kb == p->kb[k] &&
+        row.back().kc_hi + 1 == p->kc[k]) {
+      row.back().kc_hi++;
     } else {
-      EwSeg s = {k, 1, p->ka[k], p->kb[k], p->kc[k], row ? 1 : 0};
-      seg.push_back(s);
+      EwRow r = {k, p->kc[k], p->kc[k], p->ka[k], p->kb[k], 0, 0, 0, 0};
+      row.push_back(r);
     }
   }
-  c->nseg = (int)seg.size();
-  // structure factor: one lane per segment, longest first; a workgroup of 4
-  // waves takes 64 segments and one chunk of charges (>= 32); chunks sized
-  // for ~NB_SFAC_WG workgroups, ~16 a CU, which the long charge loops need
-  // to hide FP64 latency (measured on the 98,304-atom water: 4096 -> 7.3 ms,
-  // 512 -> 11.5 ms for the whole reciprocal sum)
-  // a system sharing the GPU with others gets a proportionally coarser split
+  c->nrow = (int)row.size();
+  std::vector<int4> ent;  // per m of each row: k of +m, k of -m (-1: absent), m
+  std::vector<EwSym> sym;
+  for (int g = 0; g < c->nrow; g++) {
+    EwRow &r = row[g];
+    r.mlo = r.kc_lo > 0 ? r.kc_lo : (r.kc_hi < 0 ? -r.kc_hi : 0);
+    r.mhi = std::max(std::abs(r.kc_lo), std::abs(r.kc_hi));
+    r.e0 = (int)ent.size();
+    r.chain = g > 0 && row[g - 1].ka == r.ka && row[g - 1].kb + 1 == r.kb;
+    if (r.mhi - r.mlo + 1 > NB_EW_KT) {
+      nb_set_error("nb_create: a row of %d k vectors along c exceeds %d", r.mhi - r.mlo + 1,
+                   NB_EW_KT);
+      return fail(NB_ERR_ARG);
+    }
+    for (int m = r.mlo; m <= r.mhi; m++) {
+      const int kp = (m >= r.kc_lo && m <= r.kc_hi) ? r.k_lo + m - r.kc_lo : -1;
+      const int km = (m > 0 && -m >= r.kc_lo && -m <= r.kc_hi) ? r.k_lo - m - r.kc_lo : -1;
+      ent.push_back(make_int4(kp, km, m, 0));
+    }
+    for (int m0 = r.mlo; m0 <= r.mhi; m0 += NB_EW_SYM) {
+      EwSym y = {r.k_lo, r.kc_lo, r.kc_hi, r.ka, r.kb, m0, std::min(NB_EW_SYM, r.mhi - m0 + 1), 0};
+      sym.push_back(y);
+    }
+  }
+  const int nent = (int)ent.size();
+  // structure factor: one lane per run of <= NB_EW_SYM values of m, longest
+  // first (a wave's trip count is its first lane's), then by m0 (lanes of
+  // one wave start their phases alike); a workgroup of 4 waves takes 64
+  // lanes and one chunk of charges (>= NB_SFAC_MIN_Q); chunks sized for ~NB_SFAC_WG
+  // workgroups, ~16 a CU, which the long charge loops need to hide FP64
+  // latency; a system sharing the GPU with others gets a proportionally
+  // coarser split
   const int share = std::max(1, p->sharing);
   const int sfac_wg = std::max(1, NB_SFAC_WG / share);
   const int ewf_wg = std::max(1, NB_EWF_WG / share);
-  std::vector<EwSeg> sseg(seg);
-  std::stable_sort(sseg.begin(), sseg.end(),
-                   [](const EwSeg &a, const EwSeg &b) { return a.cnt > b.cnt; });
-  c->nsw = std::max(1, (c->nseg + NB_WAVE - 1) / NB_WAVE);
+  std::stable_sort(sym.begin(), sym.end(), [](const EwSym &a, const EwSym &b) {
+    return a.cnt != b.cnt ? a.cnt > b.cnt : a.m0 < b.m0;
+  });
+  c->nsym = (int)sym.size();
+  c->nsw = std::max(1, (c->nsym + NB_WAVE - 1) / NB_WAVE);
   {
     int maxch = std::max(1, (ncharge + NB_SFAC_MIN_Q - 1) / NB_SFAC_MIN_Q);  // charges a workgroup
     int nch = std::max(1, (sfac_wg + c->nsw - 1) / c->nsw);
@@ -362,36 +388,39 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
     c->nchunk = std::max(1, (ncharge + c->qc - 1) / c->qc);
   }
   // force: NB_EWF_Q charges per lane, a workgroup per (256 NB_EWF_Q
-  // charges, tile range); tiles are runs of whole segments small enough for
-  // LDS and for enough ranges to give ~NB_EWF_WG workgroups (each range's
+  // charges, tile range); tiles are runs of whole rows small enough for LDS
+  // and for enough ranges to give ~NB_EWF_WG workgroups (each range's
   // partial forces are 24 B a charge to write and to read back in k_ewfin);
-  // ranges balanced by k count
+  // ranges balanced by entry count
   c->nqb = std::max(1, (ncharge + 256 * NB_EWF_Q - 1) / (256 * NB_EWF_Q));
-  std::vector<int> tseg(1, 0), rtile(1, 0);
+  std::vector<int> trow(1, 0), rtile(1, 0);
   {
     int nrt = std::max(1, ewf_wg / c->nqb);  // at most ~NB_EWF_WG: one round of waves
     if (share > 1)  // as for the structure factor: a bounded k x charge product per workgroup
       nrt = std::max(nrt, (int)(((long long)c->ktot * 256 * NB_EWF_Q + NB_EWF_WORK - 1) / NB_EWF_WORK));
-    nrt = std::min(nrt, std::max(1, c->nseg));
-    int lim = std::min(NB_EW_KT, std::max(NB_EW_SEG, (c->ktot + nrt - 1) / std::max(1, nrt)));
-    int nk = 0;
-    std::vector<int> tk;  // k per tile
-    for (int g = 0; g < c->nseg; g++) {
-      if (nk > 0 && nk + seg[g].cnt > lim) { tseg.push_back(g); tk.push_back(nk); nk = 0; }
-      nk += seg[g].cnt;
+    nrt = std::min(nrt, std::max(1, c->nrow));
+    int rmax = 1;
+    for (const EwRow &r : row) rmax = std::max(rmax, r.mhi - r.mlo + 1);
+    const int lim = std::min(NB_EW_KT, std::max(rmax, (nent + nrt - 1) / std::max(1, nrt)));
+    int ne = 0;
+    std::vector<int> te;  // entries per tile
+    for (int g = 0; g < c->nrow; g++) {
+      const int cnt = row[g].mhi - row[g].mlo + 1;
+      if (ne > 0 && ne + cnt > lim) { trow.push_back(g); te.push_back(ne); ne = 0; }
+      ne += cnt;
     }
-    tseg.push_back(c->nseg);
-    tk.push_back(nk);
-    c->ntile = (int)tseg.size() - 1;
-    // close range r after the tile where the running k count reaches r/nr of
-    // the total: every range gets at least one tile, and uneven tiles only
-    // ever give fewer ranges, never one range for everything
+    trow.push_back(c->nrow);
+    te.push_back(ne);
+    c->ntile = (int)trow.size() - 1;
+    // close range r after the tile where the running entry count reaches
+    // r/nr of the total: every range gets at least one tile, and uneven tiles
+    // only ever give fewer ranges, never one range for everything
     const int nr = std::min(nrt, c->ntile);
     long long cum = 0;
     for (int t = 0; t + 1 < c->ntile; t++) {
-      cum += tk[t];
+      cum += te[t];
       const int r = (int)rtile.size();
-      if (r < nr && cum * nr >= (long long)r * c->ktot) rtile.push_back(t + 1);
+      if (r < nr && cum * nr >= (long long)r * nent) rtile.push_back(t + 1);
     }
     rtile.push_back(c->ntile);
     c->nrange = (int)rtile.size() - 1;
@@ -411,9 +440,10 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
   if ((rc = upload(c, &c->d_elist, elist.data(), elist.size()))) return fail(rc);
   if ((rc = upload(c, &c->d_qidx, qidx.data(), qidx.size()))) return fail(rc);
   if ((rc = upload(c, &c->d_qpos, qpos.data(), qpos.size()))) return fail(rc);
-  if ((rc = upload(c, &c->d_seg, seg.data(), seg.size()))) return fail(rc);
-  if ((rc = upload(c, &c->d_sseg, sseg.data(), sseg.size()))) return fail(rc);
-  if ((rc = upload(c, &c->d_tseg, tseg.data(), tseg.size()))) return fail(rc);
+  if ((rc = upload(c, &c->d_row, row.data(), row.size()))) return fail(rc);
+  if ((rc = upload(c, &c->d_sym, sym.data(), sym.size()))) return fail(rc);
+  if ((rc = upload(c, &c->d_ent, ent.data(), ent.size()))) return fail(rc);
+  if ((rc = upload(c, &c->d_trow, trow.data(), trow.size()))) return fail(rc);
   if ((rc = upload(c, &c->d_rtile, rtile.data(), rtile.size()))) return fail(rc);
   if ((rc = upload(c, &c->d_kabc, kabc.data(), kabc.size()))) return fail(rc);
   if ((rc = dalloc(c, &c->d_pos, 3 * (size_t)n))) return fail(rc);
@@ -430,7 +460,7 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
   if ((rc = dalloc(c, &c->d_hcnt, 4 * (size_t)n))) return fail(rc);
   if ((rc = dalloc(c, &c->d_flags, 8))) return fail(rc);
   if ((rc = dalloc(c, &c->d_chg, 4 * (size_t)std::max(ncharge, 1)))) return fail(rc);
-  if ((rc = dalloc(c, &c->d_stp, 2 * (size_t)std::max(ncharge, 1)))) return fail(rc);
+  if ((rc = dalloc(c, &c->d_stp, 4 * (size_t)std::max(ncharge, 1)))) return fail(rc);
   if ((rc = dalloc(c, &c->d_spart, 2 * (size_t)c->nchunk * std::max(c->ktot, 1)))) return fail(rc);
   if ((rc = dalloc(c, (char **)&c->d_kd, nb_kdata_bytes() * std::max(c->ktot, 1)))) return fail(rc);
   if ((rc = dalloc(c, &c->d_fpart, 3 * (size_t)c->nrange * std::max(ncharge, 1)))) return fail(rc);
@@ -790,7 +820,7 @@ extern "C" int nb_force_device(nb_ctx *c, int iflg, const double *d_pxyz, const 
 // before reading it, so nothing in it has to start at zero.
 static size_t scratch_layout(const nb_ctx *c, size_t off[10]) {
   const size_t nq = std::max(c->ncharge, 1), kt = std::max(c->ktot, 1);
-  const size_t bytes[9] = {32 * nq, 16 * nq, 16 * (size_t)c->nchunk * kt, nb_kdata_bytes() * kt,
+  const size_t bytes[9] = {32 * nq, 32 * nq, 16 * (size_t)c->nchunk * kt, nb_kdata_bytes() * kt,
                            24 * (size_t)c->nrange * nq,
                            8 * (size_t)c->npair_blocks * NB_NE_PAIRQ,
                            8 * (size_t)c->nfin_blocks * NB_NE_ECORR,

@@ -73,11 +73,11 @@ static __device__ __forceinline__ uint4 nb_mixed_record(double x, double y, doub
 
 // Per-charge Ewald data: 2 pi s with s = h^-1 r wrapped to [-1/2,1/2) (exact
 // for e^{2 pi i k.s} with integer k), the charge times sqrt(4 pi)
-// (md_ewald.c:220-224), and e^{2 pi i s_c}, the c-axis step of the
-// recursion.
+// (md_ewald.c:220-224), and e^{2 pi i s_c} and e^{2 pi i NB_EW_SYM s_c}: the
+// c-axis step of the recursion and the length of a structure-factor lane.
 static __device__ __forceinline__ void nb_charge_record(double x, double y, double z, double q,
                                                         const double *hi, double4 *chg,
-                                                        double2 *stp) {
+                                                        double4 *stp) {
   double sa = x * hi[0] + y * hi[1] + z * hi[2];
   double sb = x * hi[3] + y * hi[4] + z * hi[5];
   double sc = x * hi[6] + y * hi[7] + z * hi[8];
@@ -86,7 +86,8 @@ static __device__ __forceinline__ void nb_charge_record(double x, double y, doub
   sc -= rint(sc);
   const double tpi = 2. * M_PI;
   *chg = make_double4(tpi * sa, tpi * sb, tpi * sc, q * sqrt(4.0 * M_PI));
-  double sn, cs;
+  double sn, cs, sn16, cs16;
   nb_sincos(tpi * sc, &sn, &cs);
-  *stp = make_double2(cs, sn);
+  nb_sincos((double)NB_EW_SYM * (tpi * sc), &sn16, &cs16);
+  *stp = make_double4(cs, sn, cs16, sn16);
 }

@@ -14,8 +14,9 @@
 //                     the list flags (vdW t, vdW s, coul te, coul se)
 //   nnb      N i32    list lengths
 //   chg      Q x {2pi s_a, 2pi s_b, 2pi s_c, q sqrt(4pi)} f64, stp Q x {cos,
-//                     sin}(2pi s_c): per-evaluation Ewald charge records
-//   kd       K x {kx, ky, kz, 2 preg Re S, 2 preg Im S} f64
+//                     sin}(2pi s_c), {cos, sin}(2pi NB_EW_SYM s_c):
+//                     per-evaluation Ewald charge records
+//   kd       K x {2 preg Re S, 2 preg Im S} f64
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -33,13 +34,15 @@ struct LaunchRec;
 #define NB_FLAG_ES 8u  // coulomb se  (rcute_resp+skin)
 #define NB_IDX_MASK 0x0FFFFFFFu
 #define NB_MAX_ATOMS (1 << 28)
-#define NB_EW_SEG 16     // k vectors per Ewald segment (one c-axis run)
-#define NB_EW_KT 512     // k vectors per LDS tile of the Ewald force pass
+#ifndef NB_EW_SYM
+#define NB_EW_SYM 16     // values of m = |kc| per structure-factor lane (2 k vectors each)
+#endif
+#define NB_EW_KT 512     // force-pass entries (values of m) per LDS tile
 #ifndef NB_EWF_Q
-#define NB_EWF_Q 2       // charges per lane in the Ewald force pass
+#define NB_EWF_Q 3       // charges per lane in the Ewald force pass
 #endif
 #ifndef NB_EWF_WG
-#define NB_EWF_WG 1024   // workgroups the Ewald force pass aims for
+#define NB_EWF_WG 2048   // workgroups the Ewald force pass aims for
 #endif
 #ifndef NB_SFAC_WG
 #define NB_SFAC_WG 4096  // workgroups the structure-factor pass aims for
@@ -49,6 +52,9 @@ struct LaunchRec;
 #endif
 #ifndef NB_CELL3_MIN_ATOMS
 #define NB_CELL3_MIN_ATOMS 2.0  // mean atoms per cell needed for reach-3 cells
+#endif
+#ifndef NB_SFAC_PAIR
+#define NB_SFAC_PAIR 1   // structure factor: two charges per trip of the charge loop
 #endif
 #ifndef NB_SFAC_MIN_Q
 #define NB_SFAC_MIN_Q 64  // fewest charges in a structure-factor chunk
@@ -87,9 +93,19 @@ struct TPairF {     // one type pair for the FP32 kernel (null pair: in -1, cuto
   int in, pad;
 };
 
-struct EwSeg {      // run of <= NB_EW_SEG consecutive kc in one (ka,kb) row
-  int k0, cnt, ka, kb, kc0;
-  int cont;         // continues the previous segment's row (kc0 = its end)
+// The k vectors as rows: maximal runs of consecutive kc at one (ka, kb) in
+// k order (cm3d's enumeration gives kc = -n..n, or 1..n for ka = kb = 0).
+// Both reciprocal passes walk a row by m = |kc|, pairing kc with -kc: the
+// phases e^{+-i m theta_c} share one rotation.
+struct EwRow {
+  int k_lo;          // k index of kc = kc_lo
+  int kc_lo, kc_hi, ka, kb;
+  int mlo, mhi;      // values of m the row covers
+  int e0;            // its first force-pass entry (one per m)
+  int chain;         // the previous row is (ka, kb - 1): its phase follows by one kb step
+};
+struct EwSym {       // a structure-factor lane: <= NB_EW_SYM values of m from m0 in one row
+  int k_lo, kc_lo, kc_hi, ka, kb, m0, cnt, pad;
 };
 
 // cm3d's -DLINUX anint: (int)(x +- 0.5), truncating (md_macros.h:31-52).
@@ -207,7 +223,7 @@ struct nb_ctx {
   hipStream_t stream2;        // the reciprocal sum's branch of the MD step
   hipEvent_t fork_ev[2];      // fork / join of that branch
   int natoms, ntypes, ninter, ntable, iperd, prec;
-  int ncharge, ktot, nseg, necorr;
+  int ncharge, ktot, nrow, nsym, necorr;
   double skin, kappa, ewald_self, ewald_bkg;
   double dx2coul, rmins_coul2, rmaxs_coul2, rminl_coul2, rmaxl_coul2;
   double rmaxs_skin2_e, rmaxl_skin2_e, rlist, clast;
@@ -225,9 +241,10 @@ struct nb_ctx {
   int *d_eoff, *d_elist;       // ecorr partners, both directions
   int *d_qidx;                 // charge -> atom
   int *d_qpos;                 // atom -> charge (-1: uncharged)
-  EwSeg *d_seg;                // k order
-  EwSeg *d_sseg;               // sorted longest first (structure-factor lanes)
-  int *d_tseg, *d_rtile;       // force pass: tiles (segment bounds), ranges (tile bounds)
+  EwRow *d_row;                // k order
+  EwSym *d_sym;                // structure-factor lanes, longest first
+  int4 *d_ent;                 // force-pass entries: {k of +m, k of -m (-1: absent), m, 0}
+  int *d_trow, *d_rtile;       // force pass: tiles (row bounds), ranges (tile bounds)
   int *d_kabc;                 // ka,kb,kc per k (3K)
   // state
   double *d_pos, *d_posb, *d_fs, *d_fl;
@@ -250,7 +267,7 @@ struct nb_ctx {
   int ncx, ncy, ncz, ncells, cells_alloc;
   int crx, cry, crz;           // cell reach of the list stencil per axis (0..3)
   double *d_chg, *d_stp, *d_spart, *d_kd, *d_fpart;
-  int nchunk, qc, nsw;         // structure factor: charge chunks, segment waves
+  int nchunk, qc, nsw;         // structure factor: charge chunks, waves of lanes
   int nrange, nqb, ntile;      // force: tile ranges, charge workgroups, tiles
   double *d_epair, *d_eew, *d_eec, *d_eout;
   int npair_blocks, new_blocks, nfin_blocks;

@@ -679,26 +679,30 @@ NB_KERNEL_PAIR_W(k_pair64_1q, PairArgs, 256, 4)
 
 // ---------------------------------------------------------------------------
 // Reciprocal-space Ewald, FP64 throughout (fk_ewald, md_ewald.c:185-361).
-// The k vectors are cut into segments: runs of <= NB_EW_SEG consecutive kc in
-// one (ka,kb) row, along which e^{i k.r} advances by the c-axis step
-// e^{2 pi i s_c} (cm3d's recursion, md_ewald.c:261-279); a segment starts
-// with one sine/cosine evaluation (nb_sincos.h).  What a whole wavefront
+// The k vectors are taken as rows: runs of consecutive kc at one (ka, kb),
+// along which e^{i k.r} advances by the c-axis step e^{2 pi i s_c} (cm3d's
+// recursion, md_ewald.c:261-279).  Both passes walk a row by m = |kc| and
+// take kc = +m and -m together: e^{i k.r} = w e^{+-i m theta_c} with w the
+// row's (ka, kb) phase, so one step of e^{i m theta_c} serves two k
+// vectors.  Sines and cosines come from nb_sincos.h.  What a whole wavefront
 // reads at once (the charges in k_ewsfac, the k data in k_ewforce) is staged
 // in LDS and read as broadcasts.  FMA contraction is allowed in these
 // kernels: the sums only have to agree with cm3d's to rounding.
 // k_ewchg: per charge, fractional coordinates s = h^-1 r wrapped to
 //   [-1/2,1/2) (exact for e^{2 pi i k.s} with integer k), stored as 2 pi s,
-//   the charge times sqrt(4 pi) (md_ewald.c:220-224), and e^{2 pi i s_c}.
-// k_ewsfac: lanes = segments (sorted longest first, so a wave's trip count is
-//   its first lane's); a workgroup walks one chunk of charges and each lane
-//   sums q e^{i k.r} for its segment's k in registers -- no cross-lane
-//   reduction -- then the 4 waves combine and write the chunk's partial S(k).
+//   the charge times sqrt(4 pi) (md_ewald.c:220-224), e^{2 pi i s_c} and
+//   e^{2 pi i NB_EW_SYM s_c}.
+// k_ewsfac: lanes = runs of <= NB_EW_SYM values of m in one row (sorted longest
+//   first, so a wave's trip count is its first lane's); a workgroup walks one
+//   chunk of charges and each lane sums q w e^{i m theta_c} split into its
+//   cosine and sine parts in registers -- no cross-lane reduction -- then
+//   the 4 waves combine and write the chunk's partial S(k) for +m and -m.
 // k_ewkred: per k, the chunk partials summed in fixed order, then preg, prep,
 //   |S|^2 and the energy / virial terms (md_ewald.c:247-302).
-// k_ewforce: lanes = charges; a workgroup walks one range of segments in k
-//   order, tile by tile (a segment that continues its row continues the
-//   recursion), summing k (Re S' sin - Im S' cos) (md_ewald.c:303-323) into
-//   a per-range partial.
+// k_ewforce: lanes = charges; a workgroup walks one range of rows in k
+//   order, tile by tile (the next kb of a ka plane steps w by e^{i theta_b}),
+//   summing k (Re S' sin - Im S' cos) (md_ewald.c:303-323) into a per-range
+//   partial.
 // k_ewfin: per charge, the range partials in fixed order, times q, added
 //   to fl, with the excluded-pair correction (ecorr).
 // ---------------------------------------------------------------------------
@@ -711,7 +715,7 @@ static __device__ __forceinline__ cptr4<T> as_const4(const T *p) {
 }
 
 struct EwChgArgs {
-  const double *pos, *q; const int *qidx; int nq, n; Box b; double4 *chg; double2 *stp;
+  const double *pos, *q; const int *qidx; int nq, n; Box b; double4 *chg, *stp;
 };
 static __device__ __forceinline__ void k_ewchg_body(const EwChgArgs &A, int bid) {
   int c = bid * blockDim.x + threadIdx.x;
@@ -722,117 +726,156 @@ static __device__ __forceinline__ void k_ewchg_body(const EwChgArgs &A, int bid)
 NB_KERNEL_PAIR(k_ewchg, EwChgArgs, 256)
 
 struct EwSfacArgs {
-  const EwSeg *sseg; const double4 *chg; const double2 *stp; int nq, qc, nseg, nsw, ktot;
+  const EwSym *sym; const double4 *chg, *stp; int nq, qc, nsym, nsw, ktot;
   double2 *part;
 };
-// LDS of k_ewsfac: the charge tile (6 doubles a charge) during the sweep,
-// then the cross-wave reduction [2][2 NB_EW_SEG][64]
-#define EW_SFAC_LDS (2 * 2 * NB_EW_SEG * NB_WAVE)
-#define EW_QT (EW_SFAC_LDS / 6)  // charges per tile
-// one charge's contribution to a lane's segment: start phase by sincos, then
-// M steps of the c-axis recursion (M = the wave's longest segment; steps past
-// the lane's own count accumulate into slots that are never stored).  M is a
-// template parameter so that every accumulator index is static and re / im
-// stay in registers.
+// LDS of k_ewsfac: the charge tile (8 doubles a charge) during the sweep,
+// then one wave's sums [4 NB_EW_SYM][64] at a time in the cross-wave
+// reduction
+#define EW_SFAC_LDS 4096
+#define EW_QT (EW_SFAC_LDS / 8)  // charges per tile
+// One charge against a lane's run of M values of m from m0 in one row, with
+// w = q e^{i (ka theta_a + kb theta_b)} given: the phase z^m = e^{i m
+// theta_c} from z^{m0} (1 or z times powers of z^NB_EW_SYM, or a sine /
+// cosine when m0 is neither) by the three-term recurrence of cos(m theta)
+// and sin(m theta) (an error introduced at step j reaches step m at most
+// m - j times larger: |z^m| error <= m^2 eps / 2, 1e-13 at m = 33), and
+// A_m += w Re z^m, B_m += w Im z^m: then S(+m) = A_m + i B_m and
+// S(-m) = A_m - i B_m.  Runs past the lane's own count accumulate into
+// slots that are never stored.
 template <int M>
-static __device__ __forceinline__ void sfac_charge(const double *cq, double ka, double kb,
-                                                   double kc0, double (&re)[NB_EW_SEG],
-                                                   double (&im)[NB_EW_SEG]) {
-  const double q = cq[3], stc = cq[4], sts = cq[5];
-  double sn, cs;
-  nb_sincos(ka * cq[0] + kb * cq[1] + kc0 * cq[2], &sn, &cs);
+static __device__ __forceinline__ void sfac_charge(const double *cq, double wr, double wi,
+                                                   int m0, int m16, int mb,
+                                                   double (&ar)[NB_EW_SYM], double (&ai)[NB_EW_SYM],
+                                                   double (&br)[NB_EW_SYM], double (&bi)[NB_EW_SYM]) {
+  const double zr = cq[4], zi = cq[5];
+  double c = 1., s = 0.;
+  if (mb == 1) { c = zr; s = zi; }
+  if (mb > 1) {
+    nb_sincos(m0 * cq[2], &s, &c);
+  } else {
+    for (int j = 0; j < m16; j++) {
+      const double c2 = c * cq[6] - s * cq[7];
+      s = s * cq[6] + c * cq[7];
+      c = c2;
+    }
+  }
+  // z^{m0 - 1}, and the recurrence z^{m+1} = 2 cos(theta_c) z^m - z^{m-1},
+  // component by component
+  double cp = c * zr + s * zi, sp = s * zr - c * zi;
+  const double t2 = 2. * zr;
 #pragma unroll
   for (int m = 0; m < M; m++) {
-    re[m] = __builtin_fma(q, cs, re[m]);
-    im[m] = __builtin_fma(q, sn, im[m]);
+    ar[m] = __builtin_fma(wr, c, ar[m]);
+    ai[m] = __builtin_fma(wi, c, ai[m]);
+    br[m] = __builtin_fma(wr, s, br[m]);
+    bi[m] = __builtin_fma(wi, s, bi[m]);
     if (m + 1 < M) {
-      const double c2 = __builtin_fma(cs, stc, -(sn * sts));
-      sn = __builtin_fma(sn, stc, cs * sts);
-      cs = c2;
+      const double cn = __builtin_fma(t2, c, -cp), sn = __builtin_fma(t2, s, -sp);
+      cp = c; sp = s;
+      c = cn; s = sn;
     }
   }
 }
+// w = q e^{i (ka theta_a + kb theta_b)} for charge t of the tile, from one
+// sine / cosine
+static __device__ __forceinline__ void sfac_w(const double *lds, int t, double ka, double kb,
+                                              double *wr, double *wi) {
+  const double *cq = lds + 8 * t;
+  double sn, cs;
+  nb_sincos(ka * cq[0] + kb * cq[1], &sn, &cs);
+  *wr = cq[3] * cs;
+  *wi = cq[3] * sn;
+}
 template <int M>
 static __device__ __forceinline__ void sfac_run(const EwSfacArgs &A, double *lds, int chunk,
-                                                int k0, int cnt, double ka, double kb,
-                                                double kc0) {
+                                                const EwSym &y, bool live) {
   const int lane = lane_id(), w = threadIdx.x >> 6;
-  double re[NB_EW_SEG], im[NB_EW_SEG];
+  double ar[NB_EW_SYM], ai[NB_EW_SYM], br[NB_EW_SYM], bi[NB_EW_SYM];
 #pragma unroll
-  for (int m = 0; m < M; m++) { re[m] = 0.; im[m] = 0.; }
+  for (int m = 0; m < M; m++) { ar[m] = 0.; ai[m] = 0.; br[m] = 0.; bi[m] = 0.; }
+  const double ka = y.ka, kb = y.kb;
+  const int mb = y.m0 % NB_EW_SYM, m16 = y.m0 / NB_EW_SYM;
   const int c0 = chunk * A.qc, c1 = min(A.nq, c0 + A.qc);
   for (int t0 = c0; t0 < c1; t0 += EW_QT) {
     const int nt = min(EW_QT, c1 - t0);
     __syncthreads();
     for (int t = threadIdx.x; t < nt; t += blockDim.x) {
       const double4 r = A.chg[t0 + t];
-      const double2 st = A.stp[t0 + t];
-      double *d = lds + 6 * t;
-      d[0] = r.x; d[1] = r.y; d[2] = r.z; d[3] = r.w; d[4] = st.x; d[5] = st.y;
+      const double4 st = A.stp[t0 + t];
+      double *d = lds + 8 * t;
+      d[0] = r.x; d[1] = r.y; d[2] = r.z; d[3] = r.w;
+      d[4] = st.x; d[5] = st.y; d[6] = st.z; d[7] = st.w;
     }
     __syncthreads();
     int t = w;
+    double wr, wi, wr2, wi2;
+#if NB_SFAC_PAIR
     for (; t + 4 < nt; t += 8) {  // two charges per trip for ILP
-      sfac_charge<M>(lds + 6 * t, ka, kb, kc0, re, im);
-      sfac_charge<M>(lds + 6 * (t + 4), ka, kb, kc0, re, im);
+      sfac_w(lds, t, ka, kb, &wr, &wi);
+      sfac_w(lds, t + 4, ka, kb, &wr2, &wi2);
+      sfac_charge<M>(lds + 8 * t, wr, wi, y.m0, m16, mb, ar, ai, br, bi);
+      sfac_charge<M>(lds + 8 * (t + 4), wr2, wi2, y.m0, m16, mb, ar, ai, br, bi);
     }
-    if (t < nt) sfac_charge<M>(lds + 6 * t, ka, kb, kc0, re, im);
+#endif
+    for (; t < nt; t += 4) {
+      sfac_w(lds, t, ka, kb, &wr, &wi);
+      sfac_charge<M>(lds + 8 * t, wr, wi, y.m0, m16, mb, ar, ai, br, bi);
+    }
   }
-  // (w0 + w2) + (w1 + w3), through the same LDS
-  double (*red)[2 * NB_EW_SEG][NB_WAVE] = (double (*)[2 * NB_EW_SEG][NB_WAVE])lds;
+  // ((w3 + w2) + w1) + w0, one wave's sums at a time through the same LDS
+  double (*red)[NB_WAVE] = (double (*)[NB_WAVE])lds;
+#pragma unroll
+  for (int src = 3; src >= 1; src--) {
+    __syncthreads();
+    if (w == src) {
+#pragma unroll
+      for (int m = 0; m < M; m++) {
+        if (src < 3) {
+          ar[m] += red[4 * m][lane]; ai[m] += red[4 * m + 1][lane];
+          br[m] += red[4 * m + 2][lane]; bi[m] += red[4 * m + 3][lane];
+        }
+        red[4 * m][lane] = ar[m]; red[4 * m + 1][lane] = ai[m];
+        red[4 * m + 2][lane] = br[m]; red[4 * m + 3][lane] = bi[m];
+      }
+    }
+  }
   __syncthreads();
-  if (w >= 2) {
+  if (w == 0 && live) {
+    double2 *out = A.part + (size_t)chunk * A.ktot;
 #pragma unroll
     for (int m = 0; m < M; m++) {
-      red[w - 2][2 * m][lane] = re[m];
-      red[w - 2][2 * m + 1][lane] = im[m];
+      if (m < y.cnt) {
+        const double sar = ar[m] + red[4 * m][lane], sai = ai[m] + red[4 * m + 1][lane];
+        const double sbr = br[m] + red[4 * m + 2][lane], sbi = bi[m] + red[4 * m + 3][lane];
+        const int mm = y.m0 + m;
+        if (mm >= y.kc_lo && mm <= y.kc_hi)
+          out[y.k_lo + mm - y.kc_lo] = make_double2(sar - sbi, sai + sbr);
+        if (mm > 0 && -mm >= y.kc_lo && -mm <= y.kc_hi)
+          out[y.k_lo - mm - y.kc_lo] = make_double2(sar + sbi, sai - sbr);
+      }
     }
-  }
-  __syncthreads();
-  if (w < 2) {
-#pragma unroll
-    for (int m = 0; m < M; m++) {
-      re[m] += red[w][2 * m][lane];
-      im[m] += red[w][2 * m + 1][lane];
-    }
-  }
-  __syncthreads();
-  if (w == 1) {
-#pragma unroll
-    for (int m = 0; m < M; m++) {
-      red[0][2 * m][lane] = re[m];
-      red[0][2 * m + 1][lane] = im[m];
-    }
-  }
-  __syncthreads();
-  if (w == 0) {
-    double2 *out = A.part + (size_t)chunk * A.ktot + k0;
-#pragma unroll
-    for (int m = 0; m < M; m++)
-      if (m < cnt) out[m] = make_double2(re[m] + red[0][2 * m][lane], im[m] + red[0][2 * m + 1][lane]);
   }
 }
 static __device__ __forceinline__ void k_ewsfac_body(const EwSfacArgs &A, int bid) {
-  // block = 4 waves on the same 64 segments; the block's chunk of charges is
+  // block = 4 waves on the same 64 lanes; the block's chunk of charges is
   // staged through LDS in tiles and each wave takes every 4th charge; the 4
   // waves' sums are then combined in LDS in a fixed order
   __shared__ double lds[EW_SFAC_LDS];
-  static_assert(6 * EW_QT <= EW_SFAC_LDS, "charge tile exceeds the LDS buffer");
-  static_assert(NB_EW_SEG == 16, "the dispatch below covers segments of 1..16 k vectors");
+  static_assert(8 * EW_QT <= EW_SFAC_LDS && 4 * NB_EW_SYM * NB_WAVE <= EW_SFAC_LDS,
+                "charge tile or reduction exceeds the LDS buffer");
+  static_assert(NB_EW_SYM >= 2 && NB_EW_SYM <= 16, "the dispatch below covers runs of 1..16 values of m");
   const int lane = lane_id();
   const int sw = bid % A.nsw, chunk = bid / A.nsw;
   const int g = sw * NB_WAVE + lane;
-  int k0 = 0, cnt = 0;
-  double ka = 0., kb = 0., kc0 = 0.;
-  if (g < A.nseg) {
-    const EwSeg sg = A.sseg[g];
-    k0 = sg.k0; cnt = sg.cnt; ka = sg.ka; kb = sg.kb; kc0 = sg.kc0;
-  }
-  // lane 0 (always a real segment) has the longest; the 4 waves of the block
-  // share the segments, so the trip count is uniform over the block
-  switch (__builtin_amdgcn_readfirstlane(cnt)) {
+  EwSym y = {0, 1, 0, 0, 0, 0, 0, 0};  // no k: nothing stored
+  const bool live = g < A.nsym;
+  if (live) y = A.sym[g];
+  // lane 0 (always a real lane) has the longest run; the 4 waves of the block
+  // share the lanes, so the trip count is uniform over the block
+  switch (__builtin_amdgcn_readfirstlane(y.cnt)) {
 #define EW_SFAC_CASE(M) \
-  case M: sfac_run<M>(A, lds, chunk, k0, cnt, ka, kb, kc0); break;
+  case M: sfac_run<M>(A, lds, chunk, y, live); break;
     EW_SFAC_CASE(1) EW_SFAC_CASE(2) EW_SFAC_CASE(3) EW_SFAC_CASE(4)
     EW_SFAC_CASE(5) EW_SFAC_CASE(6) EW_SFAC_CASE(7) EW_SFAC_CASE(8)
     EW_SFAC_CASE(9) EW_SFAC_CASE(10) EW_SFAC_CASE(11) EW_SFAC_CASE(12)
@@ -899,151 +942,122 @@ static __device__ __forceinline__ void k_ewkred_body(const EwKredArgs &A, int bi
 }
 NB_KERNEL_PAIR(k_ewkred, EwKredArgs, 256)
 
-// k_ewforce: tiles of whole segments (<= NB_EW_KT k each) are staged in LDS;
-// a range is a run of tiles (rtile[r] .. rtile[r+1]); tile t covers
-// segments tseg[t] .. tseg[t+1].  Each lane takes two charges.  Within a
-// segment (ka, kb fixed) the k vector is A + kc Gc with A = 2 pi (ka h^-T_a +
-// kb h^-T_b) and Gc = 2 pi h^-T_c, so sum_k k tt_k = A sum tt + Gc sum kc tt:
-// per k and charge, tt = Re/Im part of 2 preg S' e^{ik.r} (2 ops), two sums
-// (2) and the recursion (4).
+// k_ewforce: tiles of whole rows (<= NB_EW_KT entries, one per value of m)
+// are staged in LDS; a range is a run of tiles (rtile[r] .. rtile[r+1]);
+// tile t covers rows trow[t] .. trow[t+1].  Each lane takes NB_EWF_Q
+// charges.  In a row (ka, kb fixed) the k vector is A + kc Gc with
+// A = 2 pi (ka h^-T_a + kb h^-T_b) and Gc = 2 pi h^-T_c, and the force sums
+// sum_k k tt_k with tt_k = Im(conj(K_k) e^{i k.r}), K_k = 2 preg S(k).  With
+// w = e^{i (ka theta_a + kb theta_b)}, z = e^{i theta_c} and, per m = |kc|,
+// E = conj(K_{+m}) + conj(K_{-m}), D = conj(K_{+m}) - conj(K_{-m}):
+//   sum_row tt    = Im(w P),  P = sum_m (E Re z^m + i D Im z^m)
+//   sum_row kc tt = Im(w Q),  Q = sum_m m (D Re z^m + i E Im z^m)
+// so a row costs one step of z^m per value of m for two k vectors (the
+// three-term recurrence of cos(m theta_c) and sin(m theta_c), as in
+// k_ewsfac).  An LDS entry holds {Er, Ei, -Di, Dr, m Dr, -m Ei, m Di, m Er}.
 struct EwForceArgs {
-  const EwSeg *seg; const int *rtile, *tseg; const KData *kd; const double4 *chg;
-  const double2 *stp; int nq, nqb; double *fpart; Box b;
+  const EwRow *row; const int4 *ent; const int *rtile, *trow; const KData *kd;
+  const double4 *chg, *stp; int nq, nqb; double *fpart; Box b;
 };
 #define EWF_Q NB_EWF_Q
-#define NB_EWF_DW 4  // largest kc shift between chained row starts
-// the M k vectors of one segment against the lane's charges: s0 = sum tt,
-// s1 = sum m tt, and the phases advanced M steps (a segment that continues
-// its row picks them up).  M is a template parameter: the loop unrolls fully,
-// the k data loads are issued ahead and the phases rotate without copies.
-template <int M>
-static __device__ __forceinline__ void ewf_segment(const double2 *K, const double2 (&st)[EWF_Q],
-                                                   double (&sn)[EWF_Q], double (&cs)[EWF_Q],
-                                                   double (&s0)[EWF_Q], double (&s1)[EWF_Q]) {
-#pragma clang fp contract(fast)
-  static_assert(NB_EW_SEG == 16, "the dispatch covers segments of 1..16 k vectors");
-#pragma unroll
-  for (int m = 0; m < M; m++) {
-    const double2 kk = K[m];
-    const double fm = m;
-#pragma unroll
-    for (int u = 0; u < EWF_Q; u++) {
-      const double tt = kk.x * sn[u] - kk.y * cs[u];
-      s0[u] += tt;
-      s1[u] += fm * tt;
-      const double c2 = cs[u] * st[u].x - sn[u] * st[u].y;
-      sn[u] = sn[u] * st[u].x + cs[u] * st[u].y;
-      cs[u] = c2;
-    }
-  }
-}
 static __device__ __forceinline__ void k_ewforce_body(const EwForceArgs &A, int bid) {
 #pragma clang fp contract(fast)
-  __shared__ double2 kl[NB_EW_KT];  // {2 preg Re S, 2 preg Im S} per k of the tile
+  __shared__ double2 kl[NB_EW_KT][4];
   const int qb = bid % A.nqb, r = bid / A.nqb;
   const int cbase = qb * blockDim.x * EWF_Q + threadIdx.x;
-  double4 q4[EWF_Q];
-  double2 st[EWF_Q];
-  double fx[EWF_Q], fy[EWF_Q], fz[EWF_Q], sn[EWF_Q], cs[EWF_Q];
-  // row starts: e^{i k.r} at the first k of the current row (ps, pc) and
-  // the step from row kb to row kb + 1, e^{2 pi i s_b} (eb)
-  double ps[EWF_Q], pc[EWF_Q];
-  double2 eb[EWF_Q];
+  double ta[EWF_Q], tb[EWF_Q], tc[EWF_Q], zr[EWF_Q], zi[EWF_Q], t2[EWF_Q];
+  double fx[EWF_Q], fy[EWF_Q], fz[EWF_Q];
+  double wr[EWF_Q], wi[EWF_Q];  // e^{i (ka theta_a + kb theta_b)} of the last row
+  double2 eb[EWF_Q];            // e^{i theta_b}: from row kb to row kb + 1
 #pragma unroll
   for (int u = 0; u < EWF_Q; u++) {
     const int c = cbase + u * blockDim.x;
-    q4[u] = make_double4(0., 0., 0., 0.);
-    st[u] = make_double2(1., 0.);
-    if (c < A.nq) { q4[u] = A.chg[c]; st[u] = A.stp[c]; }
+    double4 q4 = make_double4(0., 0., 0., 0.), st = make_double4(1., 0., 1., 0.);
+    if (c < A.nq) { q4 = A.chg[c]; st = A.stp[c]; }
+    ta[u] = q4.x; tb[u] = q4.y; tc[u] = q4.z; zr[u] = st.x; zi[u] = st.y; t2[u] = 2. * st.x;
     fx[u] = fy[u] = fz[u] = 0.;
-    sn[u] = 0.;
-    cs[u] = 1.;
-    ps[u] = 0.;
-    pc[u] = 1.;
-    nb_sincos(q4[u].y, &eb[u].y, &eb[u].x);
+    wr[u] = 1.; wi[u] = 0.;
+    nb_sincos(q4.y, &eb[u].y, &eb[u].x);
   }
-  int cka = -1, ckb = 0, ckc = 0;  // (ka, kb, kc0) of the last row start
   const double tpi = 2. * M_PI, *hi = A.b.hi;
   const double gcx = tpi * hi[6], gcy = tpi * hi[7], gcz = tpi * hi[8];
-  cptr4<EwSeg> seg = as_const4(A.seg);
-  cptr4<int> rt = as_const4(A.rtile), ts = as_const4(A.tseg);
-  const int t0 = rt[r], t1 = rt[r + 1], gfirst = ts[t0];
+  cptr4<EwRow> row = as_const4(A.row);
+  cptr4<int> rt = as_const4(A.rtile), tr = as_const4(A.trow);
+  const int t0 = rt[r], t1 = rt[r + 1], gfirst = tr[t0];
   for (int t = t0; t < t1; t++) {
-    const int g0 = ts[t], g1 = ts[t + 1];
-    const int kb0 = seg[g0].k0, nk = seg[g1 - 1].k0 + seg[g1 - 1].cnt - kb0;
+    const int g0 = tr[t], g1 = tr[t + 1];
+    const int e0 = row[g0].e0, ne = row[g1 - 1].e0 + (row[g1 - 1].mhi - row[g1 - 1].mlo + 1) - e0;
     __syncthreads();
-    for (int u = threadIdx.x; u < nk; u += blockDim.x) {
-      const KData d = A.kd[kb0 + u];
-      kl[u] = make_double2(d.cr, d.ci);
+    for (int u = threadIdx.x; u < ne; u += blockDim.x) {
+      const int4 en = A.ent[e0 + u];
+      const KData p = A.kd[en.x >= 0 ? en.x : 0], q = A.kd[en.y >= 0 ? en.y : 0];
+      const double pr = en.x >= 0 ? p.cr : 0., pi = en.x >= 0 ? p.ci : 0.;
+      const double mr = en.y >= 0 ? q.cr : 0., mi = en.y >= 0 ? q.ci : 0.;
+      const double er = pr + mr, ei = -(pi + mi), dr = pr - mr, di = mi - pi, fm = en.z;
+      kl[u][0] = make_double2(er, ei);
+      kl[u][1] = make_double2(-di, dr);
+      kl[u][2] = make_double2(fm * dr, -(fm * ei));
+      kl[u][3] = make_double2(fm * di, fm * er);
     }
     __syncthreads();
     for (int g = g0; g < g1; g++) {
-      const int cnt = seg[g].cnt;
-      const double ka = seg[g].ka, kb = seg[g].kb, kc0 = seg[g].kc0;
-      if (g == gfirst || !seg[g].cont) {
-        // a row start: the next row of the same ka plane is one kb step on,
-        // and its first kc a few steps from the last start's, so its phase
-        // follows from the last start's by 1 + |dw| complex products;
-        // otherwise (new plane, first segment of the range, long jumps in
-        // kc) from one sine / cosine
-        const int ika = seg[g].ka, ikb = seg[g].kb, ikc = seg[g].kc0;
-        int dw = ikc - ckc;
-        if (g != gfirst && ika == cka && ikb == ckb + 1 && dw >= -NB_EWF_DW && dw <= NB_EWF_DW) {
+      const int ika = row[g].ka, ikb = row[g].kb, mlo = row[g].mlo, mhi = row[g].mhi;
+      const double ka = ika, kb = ikb;
+      if (g != gfirst && row[g].chain) {
+        // the next kb of the same ka: one step of e^{i theta_b}
 #pragma unroll
-          for (int u = 0; u < EWF_Q; u++) {
-            const double c2 = pc[u] * eb[u].x - ps[u] * eb[u].y;
-            ps[u] = ps[u] * eb[u].x + pc[u] * eb[u].y;
-            pc[u] = c2;
-          }
-          for (; dw > 0; dw--) {
-#pragma unroll
-            for (int u = 0; u < EWF_Q; u++) {
-              const double c2 = pc[u] * st[u].x - ps[u] * st[u].y;
-              ps[u] = ps[u] * st[u].x + pc[u] * st[u].y;
-              pc[u] = c2;
-            }
-          }
-          for (; dw < 0; dw++) {
-#pragma unroll
-            for (int u = 0; u < EWF_Q; u++) {
-              const double c2 = pc[u] * st[u].x + ps[u] * st[u].y;
-              ps[u] = ps[u] * st[u].x - pc[u] * st[u].y;
-              pc[u] = c2;
-            }
-          }
-        } else {
-#pragma unroll
-          for (int u = 0; u < EWF_Q; u++)
-            nb_sincos(ka * q4[u].x + kb * q4[u].y + kc0 * q4[u].z, &ps[u], &pc[u]);
+        for (int u = 0; u < EWF_Q; u++) {
+          const double c2 = wr[u] * eb[u].x - wi[u] * eb[u].y;
+          wi[u] = wi[u] * eb[u].x + wr[u] * eb[u].y;
+          wr[u] = c2;
         }
+      } else {
 #pragma unroll
-        for (int u = 0; u < EWF_Q; u++) { sn[u] = ps[u]; cs[u] = pc[u]; }
-        cka = ika; ckb = ikb; ckc = ikc;
+        for (int u = 0; u < EWF_Q; u++) nb_sincos(ka * ta[u] + kb * tb[u], &wi[u], &wr[u]);
       }
-      double s0[EWF_Q], s1[EWF_Q];
+      // z^m from m = mlo (0 or 1 in cm3d's k lists) and z^{m-1}; then the
+      // three-term recurrence z^{m+1} = 2 cos(theta_c) z^m - z^{m-1}
+      double cs[EWF_Q], sn[EWF_Q], cp[EWF_Q], sp[EWF_Q];
 #pragma unroll
-      for (int u = 0; u < EWF_Q; u++) { s0[u] = 0.; s1[u] = 0.; }
-      const double2 *K = kl + (seg[g].k0 - kb0);
-      switch (cnt) {  // uniform: the segment is the workgroup's
-#define EWF_CASE(M) \
-  case M: ewf_segment<M>(K, st, sn, cs, s0, s1); break;
-        EWF_CASE(1) EWF_CASE(2) EWF_CASE(3) EWF_CASE(4)
-        EWF_CASE(5) EWF_CASE(6) EWF_CASE(7) EWF_CASE(8)
-        EWF_CASE(9) EWF_CASE(10) EWF_CASE(11) EWF_CASE(12)
-        EWF_CASE(13) EWF_CASE(14) EWF_CASE(15) EWF_CASE(16)
-#undef EWF_CASE
-        default: break;
+      for (int u = 0; u < EWF_Q; u++) {
+        if (mlo == 0) { cs[u] = 1.; sn[u] = 0.; cp[u] = zr[u]; sp[u] = -zi[u]; }
+        else if (mlo == 1) { cs[u] = zr[u]; sn[u] = zi[u]; cp[u] = 1.; sp[u] = 0.; }
+        else {
+          nb_sincos(mlo * tc[u], &sn[u], &cs[u]);
+          cp[u] = cs[u] * zr[u] + sn[u] * zi[u];
+          sp[u] = sn[u] * zr[u] - cs[u] * zi[u];
+        }
       }
-      // k = A + kc Gc with kc = kc0 + m: sum k tt = A s0 + Gc (kc0 s0 + s1)
+      double pr[EWF_Q], pi[EWF_Q], qr[EWF_Q], qi[EWF_Q];
+#pragma unroll
+      for (int u = 0; u < EWF_Q; u++) { pr[u] = 0.; pi[u] = 0.; qr[u] = 0.; qi[u] = 0.; }
+      const double2 (*K)[4] = kl + (row[g].e0 - e0);
+      const int cnt = mhi - mlo + 1;
+#pragma unroll 2
+      for (int j = 0; j < cnt; j++) {
+        const double2 k0 = K[j][0], k1 = K[j][1], k2 = K[j][2], k3 = K[j][3];
+#pragma unroll
+        for (int u = 0; u < EWF_Q; u++) {
+          pr[u] = __builtin_fma(k1.x, sn[u], __builtin_fma(k0.x, cs[u], pr[u]));
+          pi[u] = __builtin_fma(k1.y, sn[u], __builtin_fma(k0.y, cs[u], pi[u]));
+          qr[u] = __builtin_fma(k2.y, sn[u], __builtin_fma(k2.x, cs[u], qr[u]));
+          qi[u] = __builtin_fma(k3.y, sn[u], __builtin_fma(k3.x, cs[u], qi[u]));
+          const double cn = __builtin_fma(t2[u], cs[u], -cp[u]);
+          const double snn = __builtin_fma(t2[u], sn[u], -sp[u]);
+          cp[u] = cs[u]; sp[u] = sn[u];
+          cs[u] = cn; sn[u] = snn;
+        }
+      }
       const double ax = tpi * (ka * hi[0] + kb * hi[3]);
       const double ay = tpi * (ka * hi[1] + kb * hi[4]);
       const double az = tpi * (ka * hi[2] + kb * hi[5]);
 #pragma unroll
       for (int u = 0; u < EWF_Q; u++) {
-        const double w = kc0 * s0[u] + s1[u];
-        fx[u] += ax * s0[u] + gcx * w;
-        fy[u] += ay * s0[u] + gcy * w;
-        fz[u] += az * s0[u] + gcz * w;
+        const double s0 = wr[u] * pi[u] + wi[u] * pr[u];  // Im(w P)
+        const double s1 = wr[u] * qi[u] + wi[u] * qr[u];  // Im(w Q)
+        fx[u] += ax * s0 + gcx * s1;
+        fy[u] += ay * s0 + gcy * s1;
+        fz[u] += az * s0 + gcz * s1;
       }
     }
   }
@@ -1277,11 +1291,11 @@ int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_f
   const bool rp = !(flags & NB_FORCE_NO_RECIP), fin = !(flags & NB_FORCE_NO_FIN);
   const int nq = c->ncharge;
   EwChgArgs ch = {c->d_pos, c->d_qch, c->d_qidx, nq, n, b, (double4 *)c->d_chg,
-                  (double2 *)c->d_stp};
+                  (double4 *)c->d_stp};
   NB_GO(c, K_EWCHG, k_ewchg, rp && ew && !packed ? blocks(nq, 256) : 0, 256, 0, ch, s);
   nb_time_mark(c, 3, s);
-  EwSfacArgs sf = {c->d_sseg, (const double4 *)c->d_chg, (const double2 *)c->d_stp, nq, c->qc,
-                   c->nseg, c->nsw, c->ktot, (double2 *)c->d_spart};
+  EwSfacArgs sf = {c->d_sym, (const double4 *)c->d_chg, (const double4 *)c->d_stp, nq, c->qc,
+                   c->nsym, c->nsw, c->ktot, (double2 *)c->d_spart};
   NB_GO(c, K_EWSFAC, k_ewsfac, rp && ew ? c->nsw * c->nchunk : 0, 256, 0, sf, s);
   nb_time_mark(c, 4, s);
   double det = b.h[0] * (b.h[4] * b.h[8] - b.h[5] * b.h[7]) +
@@ -1293,8 +1307,8 @@ int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_f
                    b, 1. / det, falp2, (KData *)c->d_kd, c->d_eew};
   NB_GO(c, K_EWKRED, k_ewkred, rp && ew ? c->new_blocks : 0, 256, 0, kr, s);
   nb_time_mark(c, 5, s);
-  EwForceArgs ef = {c->d_seg, c->d_rtile, c->d_tseg, (const KData *)c->d_kd,
-                    (const double4 *)c->d_chg, (const double2 *)c->d_stp, nq, c->nqb, c->d_fpart,
+  EwForceArgs ef = {c->d_row, c->d_ent, c->d_rtile, c->d_trow, (const KData *)c->d_kd,
+                    (const double4 *)c->d_chg, (const double4 *)c->d_stp, nq, c->nqb, c->d_fpart,
                     b};
   NB_GO(c, K_EWFORCE, k_ewforce, rp && ewf ? c->nqb * c->nrange : 0, 256, 0, ef, s);
   nb_time_mark(c, 6, s);

@@ -198,7 +198,7 @@ struct InnerArgs {
   const double *posb; double skin2q; int *flag;
   const int *iperm, *qpos; const double *q;
   double4 *rec64; uint4 *recf;
-  double4 *chg; double2 *stp;
+  double4 *chg, *stp;
   Box b;
 };
 
@@ -658,7 +658,7 @@ static InnerArgs inner_args(nb_md *md) {
   A.recf = c->prec == NB_PREC_MIXED ? (uint4 *)c->d_recf : nullptr;
   const bool ew = c->iperd == 3 && c->ktot > 0 && c->ncharge > 0;
   A.chg = ew ? (double4 *)c->d_chg : nullptr;
-  A.stp = ew ? (double2 *)c->d_stp : nullptr;
+  A.stp = ew ? (double4 *)c->d_stp : nullptr;
   A.b = md->box;
   return A;
 }

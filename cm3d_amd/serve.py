@@ -50,8 +50,9 @@ def estimate_bytes(sysd, sharing=1):
     the allocation sizes of nb_create / nb_md_create: the neighbour lists
     (capacity 1.6x the mean full-list length + 64 entries, grown by 1.25x on
     the rare overflow -- allowed for here), per-atom state, and the Ewald
-    buffers (segments of <= 16 consecutive kc, structure-factor chunks and
-    force ranges sized as nb_create sizes them)."""
+    buffers (rows of consecutive kc, structure-factor lanes of <= 16 values
+    of |kc|, structure-factor chunks and force ranges sized as nb_create
+    sizes them)."""
     n = int(sysd["natoms"])
     h = np.asarray(sysd["hmat"], float).reshape(3, 3)
     vol = abs(np.linalg.det(h))
@@ -61,29 +62,32 @@ def estimate_bytes(sysd, sharing=1):
         r2 = max(r2, float(np.asarray(sysd["rmaxl_skin2_e"]).reshape(-1)[0]))
     cap = int(1.25 * (1.6 * 4.18879 * r2 ** 1.5 * n / vol + 64)) + 128
     ktot = int(np.asarray(sysd.get("ktot", 0)).reshape(-1)[0]) if nq else 0
-    nseg = 0
+    nrow = nsym = nent = 0
     if ktot:
         ka, kb, kc = (np.asarray(sysd[k]) for k in ("ka", "kb", "kc"))
-        run = 0
-        for k in range(ktot):
-            cont = k > 0 and ka[k] == ka[k - 1] and kb[k] == kb[k - 1] and kc[k] == kc[k - 1] + 1
-            if cont and run < 16:
-                run += 1
-            else:
-                nseg += 1
-                run = 1
-    nsw = max(1, _ceil(nseg, 64))
+        lo = 0
+        for k in range(1, ktot + 1):  # rows: runs of consecutive kc at one (ka, kb)
+            if k < ktot and ka[k] == ka[k - 1] and kb[k] == kb[k - 1] and kc[k] == kc[k - 1] + 1:
+                continue
+            a, b = int(kc[lo]), int(kc[k - 1])
+            mlo = a if a > 0 else (-b if b < 0 else 0)
+            cnt = max(abs(a), abs(b)) - mlo + 1  # values of m = |kc|
+            nrow += 1
+            nent += cnt
+            nsym += _ceil(cnt, 16)
+            lo = k
+    nsw = max(1, _ceil(nsym, 64))
     share = max(1, int(sharing))
     nch = _ceil(max(1, 4096 // share), nsw)
-    nrange = _ceil(max(1, 1024 // share), max(1, _ceil(nq, 512)))
+    nrange = _ceil(max(1, 2048 // share), max(1, _ceil(nq, 768)))
     if share > 1:  # a bounded k x charge product per workgroup (65536 each pass)
         nch = max(nch, _ceil(ktot * nq, 65536 * nsw))
-        nrange = max(nrange, _ceil(ktot * 512, 65536))
-    nch = max(1, min(nch, _ceil(nq, 32)))
-    nrange = max(1, min(nrange, max(1, nseg)))
+        nrange = max(nrange, _ceil(ktot * 768, 65536))
+    nch = max(1, min(nch, _ceil(nq, 64)))
+    nrange = max(1, min(nrange, max(1, nrow)))
     per_atom = 4 * cap + 516           # lists + positions, forces, records, MD state
     #                                    (with the batch tick's 96 B snapshot)
-    ewald = nq * (64 + 24 * nrange) + ktot * (16 * nch + 96) + nseg * 48
+    ewald = nq * (64 + 24 * nrange) + ktot * (16 * nch + 32) + nrow * 36 + nent * 16 + nsym * 32
     ncells = (int(2 * (vol ** (1. / 3.)) / math.sqrt(max(r2, 1e-12))) + 1) ** 3
     return n * per_atom + ewald + 16 * ncells + (1 << 20)
 

@@ -76,6 +76,45 @@ def test_fp64_short_level_matches_oracle(name):
     c.close()
 
 
+def _irregular_klist(d):
+    """cm3d's kmax 7 half sphere cut into rows that start away from kc = 0
+    (kc >= 3 only, kc <= -2 only), a row of kc 2..20 and the rest whole:
+    the reciprocal passes then start a row's phase at m = |kc| > 1 (sine /
+    cosine) and split long rows into lanes that start at m = 18.  The
+    oracle evaluates every k directly (kup = 1)."""
+    from cm3d_amd import nb
+    ka, kb, kc = nb.kvectors(7)
+    keep = []
+    for a, b, c in zip(ka, kb, kc):
+        r = (int(a) + int(b)) % 3
+        if (a, b) == (0, 1):
+            continue
+        if (r == 0 and c >= 3) or (r == 1 and c <= -2) or r == 2:
+            keep.append((a, b, c))
+    keep += [(0, 1, c) for c in range(2, 21)]
+    k = np.array(sorted(keep, key=lambda t: (t[0], t[1], t[2])), np.int32)
+    e = dict(d)
+    e.update(ktot=len(k), ka=k[:, 0].copy(), kb=k[:, 1].copy(), kc=k[:, 2].copy(),
+             kup=np.ones(len(k) + 1, np.int32))
+    return e
+
+
+@pytest.mark.parametrize("name", ["water512", "water512_tri"])
+def test_fp64_irregular_k_rows_match_oracle(name):
+    from oracle import oracle as O
+    d = _irregular_klist(load_fixture(name))
+    print(f"[irregular k] {name}: {int(d['ktot'])} k", flush=True)
+    L = O.build_lists(d, d["pos"], d["hmat"])
+    _, ofl, oe = O.force(d, L, d["pos"], d["hmat"], 1)
+    c = _ctx(d)
+    c.build_lists(d["pos"], d["hmat"])
+    _, fl, e = c.force(d["pos"], d["hmat"], iflg=1)
+    assert rel_rms(fl, ofl) < FP64_TOL, rel_rms(fl, ofl)
+    assert _rel(e["vk_ewald"], oe["vk_ewald"]) < 1e-11
+    assert np.allclose(e["vp_ewald_tensor"], oe["vp_ewald_tensor"], rtol=1e-10, atol=1e-8)
+    c.close()
+
+
 def test_fp64_reproducible_run_twice():
     d = load_fixture("water512")
     c = _ctx(d)
