@@ -297,7 +297,11 @@ def ewald_roofline(name, tm_ms, flop):
     tf = flop / (tm_ms * 1e-3) / 1e12
     return {"kernel": name, "bound": "valu-fp64", "achieved": round(tf, 3),
             "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(tf / FP64_PEAK_TFLOPS, 5),
-            "ms_per_launch": round(tm_ms, 5), "flop_per_launch": flop}
+            "ms_per_launch": round(tm_ms, 5), "flop_per_launch": flop,
+            "flop_basis": "algorithmic: per (k vector, charge) 10 FLOP structure factor, 16 FLOP "
+                          "force pass (cm3d's c-axis recursion); the kernels serve kc = +m and -m "
+                          "with one phase step and execute fewer operations -- valu_issue_frac "
+                          "is their executed VALU wave-instructions against the issue peak"}
 
 
 def run_c2(nb, S, a, torch):
@@ -491,9 +495,16 @@ def main():
     roof = ewald_roofline(dom, *phases[dom])
     pj = _profile_json(f"{dom}_water32768_traffic.json")
     roof["traffic"] = round(float(pj["traffic_bytes_per_launch"]), 0) if pj else None
-    roof["traffic_source"] = (f"profiles/{dom}_water32768_traffic.md (rocprofv3 --pmc "
+    roof["traffic_source"] = (f"profiles/ewald_water32768_counters.md (rocprofv3 --pmc "
                               f"FETCH_SIZE / WRITE_SIZE, copy-calibrated)") if pj else None
     other = "k_ewforce" if dom == "k_ewsfac" else "k_ewsfac"
+    roof_o = ewald_roofline(other, *phases[other])
+    for rf in (roof, roof_o):
+        vj = _profile_json(f"{rf['kernel']}_water32768_valu.json")
+        if vj:  # executed wave-instructions against the VALU issue peak
+            insts = float(vj["valu_insts_per_launch"])
+            rf["valu_insts_per_launch"] = insts
+            rf["valu_issue_frac"] = round(insts / (VALU_ISSUE_PER_S * rf["ms_per_launch"] * 1e-3), 4)
     out = {
         "metric": "atom-steps/s", "value": round(atom_steps, 1), "unit": "atom-steps/s",
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -523,7 +534,7 @@ def main():
         "pair_interactions_per_s": round(r["npairs"] * a.steps / wall, 1),
         "force_phase_ms": {k: round(v, 5) for k, v in tm.items() if k != "calls"},
         "roofline": roof,
-        "roofline_other": {other: ewald_roofline(other, *phases[other])},
+        "roofline_other": {other: roof_o},
     }
     print(f"[bench] headline {atom_steps:.4g} atom-steps/s", file=sys.stderr, flush=True)
     if not a.no_c2:

@@ -398,7 +398,9 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
     int nrt = std::max(1, ewf_wg / c->nqb);  // at most ~NB_EWF_WG: one round of waves
     if (share > 1)  // as for the structure factor: a bounded k x charge product per workgroup
       nrt = std::max(nrt, (int)(((long long)c->ktot * 256 * NB_EWF_Q + NB_EWF_WORK - 1) / NB_EWF_WORK));
-    nrt = std::min(nrt, std::max(1, c->nrow));
+    // each range's partial forces cost 24 B a charge each way: no range of
+    // fewer than NB_EWF_MIN_ENT entries
+    nrt = std::min(nrt, std::max(1, std::min(c->nrow, nent / NB_EWF_MIN_ENT)));
     int rmax = 1;
     for (const EwRow &r : row) rmax = std::max(rmax, r.mhi - r.mlo + 1);
     const int lim = std::min(NB_EW_KT, std::max(rmax, (nent + nrt - 1) / std::max(1, nrt)));

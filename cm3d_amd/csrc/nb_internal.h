@@ -44,6 +44,9 @@ struct LaunchRec;
 #ifndef NB_EWF_WG
 #define NB_EWF_WG 2048   // workgroups the Ewald force pass aims for
 #endif
+#ifndef NB_EWF_MIN_ENT
+#define NB_EWF_MIN_ENT 64  // fewest entries (values of |kc| over rows) in a force-pass range
+#endif
 #ifndef NB_SFAC_WG
 #define NB_SFAC_WG 4096  // workgroups the structure-factor pass aims for
 #endif

@@ -5,8 +5,9 @@
 // remainder r = x - n pi/2 formed with fused multiply-adds, then the Taylor
 // series of sin and cos on |r| <= pi/4 through r^17 and r^16 (truncation
 // below 2e-18), evaluated by Horner's rule in r^2; the quadrant n mod 4
-// selects and signs the pair.  About 30 FP64 operations; absolute error a
-// few 1e-16 for |x| up to ~1e5 (the Ewald phases k.r stay below ~3 pi kmax).
+// selects and signs the pair (n taken as a 32-bit integer: |x| < 3e9).
+// About 30 FP64 operations; absolute error a few 1e-16 for |x| up to ~1e5
+// (the Ewald phases k.r stay below ~3 pi kmax).
 // Usable from host C++ (for its tests) and from HIP device code.
 #pragma once
 #include <math.h>
@@ -42,7 +43,7 @@ NB_HD static inline void nb_sincos(double x, double *sp, double *cp) {
   pc = fma(pc, r2, -0.5);
   const double s = fma(r * r2, ps, r);
   const double c = fma(r2, pc, 1.0);
-  const int q = (int)((long long)n & 3);
+  const int q = (int)n & 3;  // n is integral and |n| < 2^31 for |x| < 3e9
   const double a = (q & 1) ? c : s, b = (q & 1) ? s : c;
   *sp = (q & 2) ? -a : a;
   *cp = ((q + 1) & 2) ? -b : b;

@@ -84,7 +84,7 @@ def estimate_bytes(sysd, sharing=1):
         nch = max(nch, _ceil(ktot * nq, 65536 * nsw))
         nrange = max(nrange, _ceil(ktot * 768, 65536))
     nch = max(1, min(nch, _ceil(nq, 64)))
-    nrange = max(1, min(nrange, max(1, nrow)))
+    nrange = max(1, min(nrange, max(1, min(nrow, nent // 64))))
     per_atom = 4 * cap + 516           # lists + positions, forces, records, MD state
     #                                    (with the batch tick's 96 B snapshot)
     ewald = nq * (64 + 24 * nrange) + ktot * (16 * nch + 32) + nrow * 36 + nent * 16 + nsym * 32
