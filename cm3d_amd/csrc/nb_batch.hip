@@ -253,6 +253,8 @@ extern "C" int nb_batch_step(nb_batch *b, int nsteps, void *stream, int use_grap
   BCHK(hipMemcpyAsync(b->h_ovf, b->d_ovf, b->nsys * sizeof(int), hipMemcpyDeviceToHost, s));
   BCHK(hipStreamSynchronize(s));
   for (int i = 0; i < b->nsys; i++) {
+    // each member's last force(...,1) is at the positions the tick leaves
+    nb_md_ctx(b->mds[i])->recip_owner = b->mds[i];
     if (b->h_ovf[i] <= 0) continue;
     int rc = md_recover(b->mds[i], nsteps, b->h_ovf[i], s);
     if (rc) return rc;

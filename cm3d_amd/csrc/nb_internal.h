@@ -267,6 +267,10 @@ struct nb_ctx {
   // captured graph, a batch's argument arrays) refers to is reallocated or
   // changed: the neighbour slots, the cell arrays or the cell grid.
   unsigned gen;
+  // The MD state whose positions the reciprocal-sum and ecorr energy
+  // columns (d_eew, d_eec) were last computed at, by its step's force call;
+  // null after any other force evaluation on this context
+  const void *recip_owner;
   int ncx, ncy, ncz, ncells, cells_alloc;
   int crx, cry, crz;           // cell reach of the list stencil per axis (0..3)
   double *d_chg, *d_stp, *d_spart, *d_kd, *d_fpart;
@@ -306,6 +310,10 @@ int nb_build_now(nb_ctx *c, const double hmat[9], hipStream_t s);
 #define NB_FORCE_NO_FIN 32    // leave out k_ewfin (ecorr + reciprocal forces into fl) and k_red
 #define NB_FORCE_NOENER 64    // forces only: no pair energies / virials (the MD step's
                               // force evaluation; its energies are evaluated on request)
+#define NB_FORCE_REUSE_RECIP 256  // with EONLY: the reciprocal-sum and ecorr energy columns
+                                  // of the last force(...,1) are of these same positions:
+                                  // reduce them as they are (no charge records, structure
+                                  // factor, k reduction or k_ewfin)
 #define NB_FORCE_CM3D_EONLY 128  // with EONLY: also the sums cm3d's energy-only
                                  // coulomb routine (vinter_e) reduces to
 int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs,

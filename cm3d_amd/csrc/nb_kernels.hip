@@ -884,7 +884,10 @@ static __device__ __forceinline__ void k_ewsfac_body(const EwSfacArgs &A, int bi
     default: break;
   }
 }
-NB_KERNEL_PAIR(k_ewsfac, EwSfacArgs, 256)
+#ifndef NB_SFAC_WPE
+#define NB_SFAC_WPE 1
+#endif
+NB_KERNEL_PAIR_W(k_ewsfac, EwSfacArgs, 256, NB_SFAC_WPE)
 
 // per k: 2 preg S(k), the amplitude the force pass needs
 struct KData { double cr, ci; };
@@ -1072,7 +1075,10 @@ static __device__ __forceinline__ void k_ewforce_body(const EwForceArgs &A, int 
     }
   }
 }
-NB_KERNEL_PAIR(k_ewforce, EwForceArgs, 256)
+#ifndef NB_EWF_WPE
+#define NB_EWF_WPE 1
+#endif
+NB_KERNEL_PAIR_W(k_ewforce, EwForceArgs, 256, NB_EWF_WPE)
 
 // k_ewfin, a workgroup per 64 charges (lane = charge): the four waves sum
 // interleaved quarters of the range partials (ranges w, w+4, ...), combined
@@ -1244,6 +1250,8 @@ int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_f
   const bool packed = flags & NB_FORCE_PACKED;
   const bool eonly = flags & NB_FORCE_EONLY, noener = (flags & NB_FORCE_NOENER) && !eonly;
   const bool mixed = c->prec == NB_PREC_MIXED;
+  const bool reuse = eonly && (flags & NB_FORCE_REUSE_RECIP);
+  c->recip_owner = nullptr;  // the caller sets it again when the columns are an MD state's
   nb_time_mark(c, 0, s);
   if (flags & NB_FORCE_NO_PAIR) {
     nb_time_mark(c, 1, s);
@@ -1288,7 +1296,7 @@ int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_f
   const bool ec = iflg == 1 && c->iperd == 3 && c->necorr > 0;
   const bool ew = iflg == 1 && c->iperd == 3 && c->ktot > 0 && c->ncharge > 0;
   const bool ewf = ew && !(flags & NB_FORCE_EONLY);  // reciprocal-space forces
-  const bool rp = !(flags & NB_FORCE_NO_RECIP), fin = !(flags & NB_FORCE_NO_FIN);
+  const bool rp = !(flags & NB_FORCE_NO_RECIP) && !reuse, fin = !(flags & NB_FORCE_NO_FIN);
   const int nq = c->ncharge;
   EwChgArgs ch = {c->d_pos, c->d_qch, c->d_qidx, nq, n, b, (double4 *)c->d_chg,
                   (double4 *)c->d_stp};
@@ -1315,7 +1323,7 @@ int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_f
   EwFinArgs fa = {c->d_fpart, (const double4 *)c->d_chg, c->d_qidx, nq, n, c->nrange, ewf, ec,
                   c->d_eoff, c->d_elist, c->d_pos, c->d_qch, c->kappa, d_fl, c->nfin_blocks,
                   c->d_eec};
-  NB_GO(c, K_EWFIN, k_ewfin, fin && (ewf || ec) ? c->nfin_blocks : 0, 256, 0, fa, s);
+  NB_GO(c, K_EWFIN, k_ewfin, fin && !reuse && (ewf || ec) ? c->nfin_blocks : 0, 256, 0, fa, s);
   if (fin && !(flags & NB_FORCE_NORED)) {
     // the pair energy columns were written by an FP64 pair kernel with
     // energies (0t counts clamps; a mixed context's only one is 1e)

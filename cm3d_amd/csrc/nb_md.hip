@@ -817,6 +817,7 @@ extern "C" int nb_md_set_state(nb_md *md, const double *pos, const double *vel,
   MCHK(hipStreamSynchronize(c->stream));
   memcpy(md->hmat, hmat, sizeof(md->hmat));
   md_drop_graph(md);
+  c->recip_owner = nullptr;
   return NB_OK;
 }
 
@@ -929,6 +930,7 @@ extern "C" int nb_md_init(nb_md *md) {
   md->box = c->box_built;
   rc = nb_launch_force(c, 1, md->box, nullptr, c->d_fl, c->stream, 0);
   if (rc) return rc;
+  c->recip_owner = md;
   InnerArgs A = inner_args(md);
   k_md_bonded<<<(md->nmol + MD_TPB - 1) / MD_TPB, MD_TPB, 0, c->stream>>>(A);
   MCHK(hipGetLastError());
@@ -1145,6 +1147,8 @@ extern "C" int nb_md_step(nb_md *md, int nsteps, void *stream, int use_graph) {
       if (rc) return rc;
     }
   }
+  // the step's last force(...,1) is at the positions it leaves
+  if (nsteps > 0) c->recip_owner = md;
   return NB_OK;
 }
 
@@ -1163,9 +1167,15 @@ extern "C" int nb_md_energies(nb_md *md, double vlrc, double wlrc, int ndof, nb_
   MCHK(hipDeviceSynchronize());
   // nonbonded energies and virials of the current positions: force(...,1)'s
   // energy pass (FP64 pair sums, the reciprocal sum without its force pass,
-  // ecorr) into scratch (fs), so the integrator's fl is untouched
-  int rc = nb_launch_force(c, 1, md->box, nullptr, c->d_fs, s, NB_FORCE_EONLY);
+  // ecorr) into scratch (fs), so the integrator's fl is untouched.  When the
+  // last force(...,1) on this context was this state's, at these positions
+  // (a step's, or init's), its reciprocal-sum and ecorr energy columns are
+  // reduced as they are: the same kernels on the same inputs would
+  // recompute them bit for bit
+  const int reuse = c->recip_owner == md ? NB_FORCE_REUSE_RECIP : 0;
+  int rc = nb_launch_force(c, 1, md->box, nullptr, c->d_fs, s, NB_FORCE_EONLY | reuse);
   if (rc) return rc;
+  c->recip_owner = md;
   InnerArgs A = inner_args(md);
   k_md_energy<<<md->ep_blocks, 256, 0, s>>>(A, md->d_amass, md->d_ep);
   MCHK(hipGetLastError());

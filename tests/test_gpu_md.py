@@ -135,3 +135,28 @@ def test_md_mixed_conserves_ham_like_fp64():
     assert abs(sf) <= 4. * ef and abs(sm) <= 4. * em, fit
     assert abs(sm - sf) <= 3. * np.hypot(ef, em), fit
     assert rm <= 1.5 * rf, fit
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_md_energies_reuse_step_reciprocal_columns(prec):
+    """After a step, nb_md_energies reduces the reciprocal-sum and ecorr
+    energy columns of the step's own force(...,1) instead of recomputing
+    them; after any other force evaluation on the context it recomputes
+    them.  Both give the same energies bit for bit."""
+    from cm3d_amd import nb, systems as S
+    s = S.water(nside=6, kmax=7, seed=5)
+    ctx = nb.NBContext(s, precision=prec)
+    md = nb.NBMD(ctx, s, 0.002, nres_intra=4)
+    md.set_state(s["pos"], s["vel"], s["hmat"])
+    md.init()
+    md.step(7, graph=True)
+    e_reuse = md.energies()
+    pos, _ = md.get_state()
+    # a force evaluation through the API on the same context and positions
+    # (same lists): the columns are no longer marked as the state's
+    ctx.force(pos, s["hmat"], iflg=1)
+    e_full = md.energies()
+    md.close()
+    ctx.close()
+    for k, v in e_full.items():
+        assert np.array_equal(np.asarray(e_reuse[k]), np.asarray(v)), (k, e_reuse[k], v)
