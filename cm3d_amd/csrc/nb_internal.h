@@ -72,7 +72,10 @@ struct LaunchRec;
 #define NB_MIX_G16_MAX 256  // mixed pair kernel: 16 lanes per atom below this mean list length
 #endif
 #ifndef NB_MIX_ATOMS_WG
-#define NB_MIX_ATOMS_WG 16   // atoms per workgroup of the mixed pair kernel
+#define NB_MIX_ATOMS_WG 8    // atoms per workgroup of the mixed pair kernel
+#endif
+#ifndef NB_MIX_MAX_WG
+#define NB_MIX_MAX_WG 8192  // workgroups of the mixed pair kernel at most
 #endif
 #ifndef NB_MIX_ATOMS_WG_SHARED
 #define NB_MIX_ATOMS_WG_SHARED 64  // the same for a context with sharing >= 4

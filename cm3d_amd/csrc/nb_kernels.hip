@@ -515,9 +515,11 @@ static __device__ __forceinline__ void pair64_run(const PairArgs &A, int bid, co
     const int nb = A.nnb[s];
     const uint32_t *lst = A.nbr + (size_t)s * A.cap;
     double fx = 0., fy = 0., fz = 0., sx = 0., sy = 0., sz = 0.;
-    // chunks of 64 list entries; the next chunk's entries and partner
-    // records are loaded before the current chunk is evaluated
+    // chunks of 64 list entries; list entries are loaded two chunks ahead
+    // of the evaluation and partner records one chunk ahead, so a record
+    // gather never waits on the list load it depends on
     uint32_t ec = lane < nb ? lst[lane] : 0u;
+    uint32_t en = NB_WAVE + lane < nb ? lst[NB_WAVE + lane] : 0u;
     double4 rjc = make_double4(0., 0., 0., 0.);
     int tjc = 0;
     if ((ec >> 28) & (fv | fe)) {
@@ -527,8 +529,8 @@ static __device__ __forceinline__ void pair64_run(const PairArgs &A, int bid, co
     }
 #pragma unroll 2
     for (int k0 = 0; k0 < nb; k0 += NB_WAVE) {
-      const int kn = k0 + NB_WAVE + lane;
-      const uint32_t en = kn < nb ? lst[kn] : 0u;
+      const int k2 = k0 + 2 * NB_WAVE + lane;
+      const uint32_t e2 = k2 < nb ? lst[k2] : 0u;
       double4 rjn = make_double4(0., 0., 0., 0.);
       int tjn = 0;
       if ((en >> 28) & (fv | fe)) {
@@ -619,6 +621,7 @@ static __device__ __forceinline__ void pair64_run(const PairArgs &A, int bid, co
         }
       }
       ec = en;
+      en = e2;
       rjc = rjn;
       tjc = tjn;
     }

@@ -64,10 +64,10 @@ static __device__ __forceinline__ long long to_fix(double v) {
 // entry; a wave takes several atoms (groups of 64 / G) in turn.  A lane's
 // pair forces for the atom are summed in FP32 (a handful of terms), then go
 // to fixed point and are reduced across the G lanes in integers, so the sum
-// does not depend on the order.  The next chunk's list entries and partner records are loaded before the
-// current chunk is evaluated.  G = 16 suits short lists (dense LJ-like
-// mixtures, ~100 entries): the last, partly filled chunk wastes fewer lanes
-// and one reduction serves four atoms.
+// does not depend on the order.  List entries are loaded two chunks ahead
+// of the evaluation and partner records one chunk ahead.  G = 16 suits
+// short lists (dense LJ-like mixtures, ~100 entries): the last, partly
+// filled chunk wastes fewer lanes and one reduction serves four atoms.
 // TLDS: tables (vt / ct) and type-pair parameters (stp) are all in LDS, so
 // their reads compile to LDS reads, not generic (flat) ones.
 template <int G>
@@ -98,7 +98,11 @@ static __device__ __forceinline__ void pair_mixed_run(const MixArgs &A, int bid,
     const uint32_t *lst = A.nbr + (size_t)sa * A.cap;
     const TPairF *tprow = (TLDS ? stp : (tp_lds ? stp : A.tpf)) + ti * A.ntypes;
     float fx = 0.f, fy = 0.f, fz = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
+    // list entries run two chunks ahead of the evaluation and partner
+    // records one chunk ahead, so a record gather never waits on the list
+    // load it depends on
     uint32_t ec = gl < nb ? lst[gl] : 0u;
+    uint32_t en = G + gl < nb ? lst[G + gl] : 0u;
     uint4 rjc = make_uint4(0u, 0u, 0u, 0u);
     int tjc = 0;
     if ((ec >> 28) & (fv | fe)) {
@@ -113,8 +117,8 @@ static __device__ __forceinline__ void pair_mixed_run(const MixArgs &A, int bid,
       nbw = max(max(__builtin_amdgcn_readlane(nb, 0), __builtin_amdgcn_readlane(nb, 16)),
                 max(__builtin_amdgcn_readlane(nb, 32), __builtin_amdgcn_readlane(nb, 48)));
     for (int k0 = 0; k0 < nbw; k0 += G) {
-      const int kn = k0 + G + gl;
-      const uint32_t en = kn < nb ? lst[kn] : 0u;
+      const int k2 = k0 + 2 * G + gl;
+      const uint32_t e2 = k2 < nb ? lst[k2] : 0u;
       uint4 rjn = make_uint4(0u, 0u, 0u, 0u);
       int tjn = 0;
       if ((en >> 28) & (fv | fe)) {
@@ -169,6 +173,7 @@ static __device__ __forceinline__ void pair_mixed_run(const MixArgs &A, int bid,
         }
       }
       ec = en;
+      en = e2;
       rjc = rjn;
       tjc = tjn;
     }
