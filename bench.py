@@ -25,9 +25,9 @@ Secondary blocks: the 12,288-atom water (4096 molecules, rc 10, alpha 0.35,
 kmax 9) in both precisions with the pair kernel's roofline and a 40,000-step
 drift fit; the 64-system batch of LJ + charge mixtures (N log-uniform in
 500-20,000) batched in mixed precision against the same 64 systems stepped
-one at a time on the plain FP64 path (the north-star ratio, like for like),
-with force errors and fitted drifts for both precisions; the serving stream
-of 2000 jobs with Poisson arrivals at 20/s.
+one at a time on the plain FP64 path (the batched-vs-plain ratio, like for
+like), with force errors and fitted drifts for both precisions; the serving
+stream of 2000 jobs with Poisson arrivals at 20/s.
 """
 import argparse
 import json
