@@ -126,7 +126,7 @@ extern "C" void nb_destroy(nb_ctx *c) {
                   c->d_hcnt, c->d_pcut,
                   c->d_tmp, c->d_rec64, c->d_stype, c->d_nbr, c->d_nnb, c->d_flags,
                   c->d_chg, c->d_stp, c->d_spart, c->d_kd, c->d_epair, c->d_eew, c->d_eec, c->d_eout,
-                  c->d_fxp, c->d_recf, c->d_tpf};
+                  c->d_fxp, c->d_recf, c->d_tpf, c->d_fcoef, c->d_fcoeff};
   for (void *p : ptrs)
     if (p) hipFree(p);
   if (c->h_flags) hipHostFree(c->h_flags);
@@ -488,6 +488,36 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
   c->ctab_off = (int)vt3.size();
   c->tab_lds_bytes = tab_bytes <= 60 * 1024 ? (int)tab_bytes : 0;
   c->tab_lds_bytes_f = tab_bytes / 2 <= 60 * 1024 ? (int)(tab_bytes / 2) : 0;
+  // force-column coefficients of the forces-only kernels: node t holds
+  // {f_t, (f_t+1 - f_t-1)/2, (f_t-1 - 2 f_t + f_t+1)/2}, so cm3d's quadratic
+  // interpolation f0 + .5 p (f1 - f-1 + p (f-1 - 2 f0 + f1)) is
+  // c0 + p (c1 + p c2); the differences are formed in cm3d's order, and only
+  // nodes 1 .. ntable-2 (those a pair can select) are filled
+  {
+    const int ntab = ni + 1;
+    std::vector<double> fc((size_t)ntab * nt * 3, 0.);
+    for (int k = 0; k < ntab; k++) {
+      const double *col = k < ni ? vt3.data() + (size_t)k * nt * 3 : ct3.data();
+      for (int t = 1; t + 1 < nt; t++) {
+        const double fm1 = col[3 * (t - 1)], f0 = col[3 * t], f1 = col[3 * (t + 1)];
+        double *o = fc.data() + ((size_t)k * nt + t) * 3;
+        o[0] = f0;
+        o[1] = .5 * (f1 - fm1);
+        o[2] = .5 * (fm1 - 2. * f0 + f1);
+      }
+    }
+    if ((rc = upload(c, &c->d_fcoef, fc.data(), fc.size()))) return fail(rc);
+    const size_t fb = fc.size() * sizeof(double);
+    c->fcoef_lds = fb <= 60 * 1024 ? (int)fb : 0;
+    if (p->precision == NB_PREC_MIXED) {
+      std::vector<float4> ff((size_t)ntab * nt);
+      for (size_t t = 0; t < ff.size(); t++)
+        ff[t] = make_float4((float)fc[3 * t], (float)fc[3 * t + 1], (float)fc[3 * t + 2], 0.f);
+      if ((rc = upload(c, &c->d_fcoeff, ff.data(), ff.size()))) return fail(rc);
+      const size_t fbf = ff.size() * sizeof(float4);
+      c->fcoef_lds_f = fbf <= 60 * 1024 ? (int)fbf : 0;
+    }
+  }
   if (c->prec == NB_PREC_MIXED) {
     std::vector<float> vf(vt3.begin(), vt3.end()), cf(ct3.begin(), ct3.end());
     if ((rc = upload(c, &c->d_vtab3f, vf.data(), vf.size()))) return fail(rc);
@@ -584,11 +614,11 @@ static int ensure_capacity(nb_ctx *c, const Box &b, int want_cap) {
     HCHK(hipDeviceSynchronize());
     hipFree(c->d_nbr);
     c->d_nbr = nullptr;
-    c->dev_bytes -= (size_t)c->cap * c->natoms * sizeof(uint32_t);
+    c->dev_bytes -= ((size_t)c->cap * c->natoms + NB_LIST_PAD) * sizeof(uint32_t);
   }
   c->cap = (want_cap + 63) & ~63;
   c->gen++;
-  return dalloc(c, &c->d_nbr, (size_t)c->cap * c->natoms);
+  return dalloc(c, &c->d_nbr, (size_t)c->cap * c->natoms + NB_LIST_PAD);
 }
 
 int nb_grow_capacity(nb_ctx *c, int want) {
@@ -604,11 +634,11 @@ extern "C" int nb_list_reserve(nb_ctx *c, int slots) {
   if (c->d_nbr) {
     hipFree(c->d_nbr);
     c->d_nbr = nullptr;
-    c->dev_bytes -= (size_t)c->cap * c->natoms * sizeof(uint32_t);
+    c->dev_bytes -= ((size_t)c->cap * c->natoms + NB_LIST_PAD) * sizeof(uint32_t);
   }
   c->cap = (slots + 63) & ~63;
   c->gen++;
-  RC(dalloc(c, &c->d_nbr, (size_t)c->cap * c->natoms));
+  RC(dalloc(c, &c->d_nbr, (size_t)c->cap * c->natoms + NB_LIST_PAD));
   // the old rows are gone: empty rows until the next build, which the
   // rebuild flag makes the next device step do first
   HCHK(hipMemset(c->d_nnb, 0, sizeof(int) * c->natoms));

@@ -33,7 +33,9 @@ struct LaunchRec;
 #define NB_FLAG_ET 4u  // coulomb te  (rcute_max+skin)
 #define NB_FLAG_ES 8u  // coulomb se  (rcute_resp+skin)
 #define NB_IDX_MASK 0x0FFFFFFFu
-#define NB_MAX_ATOMS (1 << 28)
+#define NB_LIST_PAD 256  // slots after the last row: the forces-only pair kernels read
+                         // up to two chunks of 64 entries past a row's end unguarded
+#define NB_MAX_ATOMS (1 << 27)  // also keeps a 32-byte record's byte offset in 32 bits
 #ifndef NB_EW_SYM
 #define NB_EW_SYM 16     // values of m = |kc| per structure-factor lane (2 k vectors each)
 #endif
@@ -116,11 +118,12 @@ struct EwSym {       // a structure-factor lane: <= NB_EW_SYM values of m from m
 
 // cm3d's -DLINUX anint: (int)(x +- 0.5), truncating (md_macros.h:31-52).
 __device__ __forceinline__ double nb_anint(double a) {
-  // (int)(a +- .5) as cm3d writes it, kept in FP64: truncating a +- .5
-  // gives the same integer (|a| < 2^31 for any fractional coordinate) in
-  // two instructions instead of a compare, select and two conversions; only
-  // the sign of a zero result can differ
-  return __builtin_trunc(a + __builtin_copysign(.5, a));
+  // (int)(a +- .5) as cm3d writes it, kept in FP64: a +- .5 is -(|a| + .5)
+  // for a < 0, exactly, so the truncated magnitude of |a| + .5 with a's sign
+  // is the same integer (|a| < 2^31 for any fractional coordinate), in three
+  // instructions (add with an abs operand, trunc, sign insert) instead of a
+  // compare, select and two conversions; only the sign of a zero differs
+  return __builtin_copysign(__builtin_trunc(__builtin_fabs(a) + .5), a);
 }
 
 // Minimum image in cm3d's operation order (md_period.new.c:39-51).  The
@@ -285,6 +288,13 @@ struct nb_ctx {
   long long *d_fxp;            // fixed-point force accumulators (mixed)
   float4 *d_recf;              // mixed: {sx, sy, sz (u32 bits), q}
   TPairF *d_tpf;               // mixed: per type pair parameters
+  // forces-only pair kernels: the force column of every table as per-node
+  // coefficients {f0, (f1 - f-1)/2, (f-1 - 2 f0 + f1)/2}, the vdW
+  // interactions' tables first, then the coulomb table's; FP64 (3 doubles
+  // a node) and, in a mixed context, FP32 ({f0, c1, c2, 0})
+  double *d_fcoef;
+  float4 *d_fcoeff;
+  int fcoef_lds, fcoef_lds_f;  // their LDS bytes (0: too large, read from HBM)
   Box box_built;
   int have_lists;
   size_t dev_bytes;            // device memory held by this context (and its MD state)

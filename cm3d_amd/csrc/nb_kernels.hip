@@ -664,12 +664,161 @@ static __device__ __forceinline__ void pair64_body(const PairArgs &A, int bid) {
   else
     pair64_run<IFLG, WANT_FS, false, ENER, WF, QK>(A, bid, A.vtab3, A.ctab3, sip, smap, ip_lds);
 }
+// ---------------------------------------------------------------------------
+// Forces-only pair kernel of the MD step (k_pair64_1n): fl over the t / te
+// lists, no energies.  The same pairs, minimum image, r^2, cut-off tests and
+// table node as pair64_run (cm3d's operation order, no contraction), in
+// fewer instructions per pair:
+//   - a row's entries past its end read the slots that follow it (the
+//     neighbour array ends in NB_LIST_PAD spare slots) and a select zeroes
+//     them; every lane gathers its partner's record (entry 0 reads atom
+//     0's), so no exec-mask change guards a load; partner types are gathered
+//     only for chunks that hold a vdW entry.  The gathers bound the kernel:
+//     a lane's record is a cache line of its own, and the L1 serves about
+//     one line a cycle;
+//   - the coulomb term is evaluated on every lane and kept by a select: a
+//     lane whose entry is not in te, or whose pair lies beyond the cut-off,
+//     adds zero; its table node is clamped into the table so that the read
+//     stays in bounds.  The vdW term runs only where an entry carries the
+//     vdW flag (they come first in a row, so later chunks skip it);
+//   - the force column is interpolated from per-node coefficients
+//     (c0 + p (c1 + p c2): cm3d's quadratic interpolation to rounding);
+//   - an orthorhombic box takes the diagonal minimum image.
+// ---------------------------------------------------------------------------
+// record of sorted atom j (j < NB_MAX_ATOMS: a 32-bit byte offset)
+static __device__ __forceinline__ double4 rec64_at(const double4 *rec, uint32_t j) {
+  return *(const double4 *)((const char *)rec + (j << 5));
+}
+static __device__ __forceinline__ int type_at(const int *t, uint32_t j) {
+  return *(const int *)((const char *)t + (j << 2));
+}
+
+template <bool TLDS, bool ORTHO>
+static __device__ __forceinline__ void pair64_fonly_run(const PairArgs &A, int bid,
+                                                        const double *fc, const InterPar *sip,
+                                                        const int *smap) {
+  const int lane = lane_id();
+  const int nw = A.nblocks * 4;
+  const double *cc = fc + A.ctab_off;
+  const int kmax = A.ntable - 2;
+  for (int s = bid * 4 + (threadIdx.x >> 6); s < A.n; s += nw) {
+    const double4 ri = A.rec[s];
+    const int ti = A.stype[s];
+    const int nb = A.nnb[s];
+    const uint32_t *lst = A.nbr + (size_t)s * A.cap + lane;
+    double fx = 0., fy = 0., fz = 0.;
+    // entries two chunks ahead of the evaluation, partner records and
+    // types one chunk ahead
+    uint32_t ec = lst[0], en = lst[NB_WAVE];
+    ec = lane < nb ? ec : 0u;
+    en = NB_WAVE + lane < nb ? en : 0u;
+    double4 rjc = rec64_at(A.rec, ec & NB_IDX_MASK);
+    int tjc = 0;
+    if (__any((ec >> 28) & NB_FLAG_VT)) tjc = type_at(A.stype, ec & NB_IDX_MASK);
+#pragma unroll 2
+    for (int k0 = 0; k0 < nb; k0 += NB_WAVE) {
+      uint32_t e2 = lst[k0 + 2 * NB_WAVE];
+      e2 = k0 + 2 * NB_WAVE + lane < nb ? e2 : 0u;
+      const double4 rjn = rec64_at(A.rec, en & NB_IDX_MASK);
+      // partner types only for chunks holding vdW entries (the gathers, one
+      // cache line a lane, bound this kernel)
+      int tjn = 0;
+      if (__any((en >> 28) & NB_FLAG_VT)) tjn = type_at(A.stype, en & NB_IDX_MASK);
+      const unsigned fl = ec >> 28;
+      double dx = ri.x - rjc.x, dy = ri.y - rjc.y, dz = ri.z - rjc.z;
+      if (ORTHO) {
+        double sx = dx * A.b.hi[0], sy = dy * A.b.hi[4], sz = dz * A.b.hi[8];
+        sx -= nb_anint(sx);
+        sy -= nb_anint(sy);
+        sz -= nb_anint(sz);
+        dx = sx * A.b.h[0];
+        dy = sy * A.b.h[4];
+        dz = sz * A.b.h[8];
+      } else {
+        nb_minimg(dx, dy, dz, A.b.h, A.b.hi);
+      }
+      const double d2 = dx * dx + dy * dy + dz * dz;
+      if (fl & NB_FLAG_VT) {
+        const int in = TLDS ? smap[ti * A.ntypes + tjc] : A.map[ti * A.ntypes + tjc];
+        const InterPar &P = TLDS ? sip[in] : A.ip[in];
+        if (d2 < P.rmaxl2) {
+          double p = (d2 - P.rmins2) * P.dx2;
+          int kk = (int)p;
+          if (kk < 1) { kk = 1; p = 1.; }
+          kk = min(kk, kmax);
+          p = p - (double)kk;
+          const double *T = fc + ((size_t)in * A.ntable + kk) * 3;
+          {
+#pragma clang fp contract(fast)
+            const double f = T[0] + p * (T[1] + p * T[2]);
+            fx -= dx * f;
+            fy -= dy * f;
+            fz -= dz * f;
+          }
+        }
+      }
+      {
+        double p = (d2 - A.rmins_coul2) * A.dx2coul;
+        int kk = (int)p;
+        const bool lo = kk < 1;
+        p = lo ? 1. : p;
+        kk = lo ? 1 : min(kk, kmax);
+        p = p - (double)kk;
+        const double *T = cc + kk * 3;
+        const bool on = (fl & NB_FLAG_ET) && d2 < A.rmaxl_coul2;
+        {
+#pragma clang fp contract(fast)
+          const double f = T[0] + p * (T[1] + p * T[2]);
+          const double fcc = on ? ri.w * rjc.w * f : 0.;
+          fx -= dx * fcc;
+          fy -= dy * fcc;
+          fz -= dz * fcc;
+        }
+      }
+      ec = en;
+      en = e2;
+      rjc = rjn;
+      tjc = tjn;
+    }
+    fx = nb_wave_sum(fx);
+    fy = nb_wave_sum(fy);
+    fz = nb_wave_sum(fz);
+    if (lane == 0) {
+      const int ai = A.perm[s], n = A.n;
+      A.fl[ai] = fx;
+      A.fl[n + ai] = fy;
+      A.fl[2 * n + ai] = fz;
+    }
+  }
+}
+
+static __device__ __forceinline__ void pair64_fonly_body(const PairArgs &A, int bid) {
+  extern __shared__ double tabs[];
+  __shared__ InterPar sip[NB_IP_LDS];
+  __shared__ int smap[NB_MAP_LDS];
+  const bool lds = A.fc_lds && A.ninter <= NB_IP_LDS && A.ntypes * A.ntypes <= NB_MAP_LDS;
+  if (lds) {
+    const int nd = A.fc_lds / (int)sizeof(double);
+    for (int t = threadIdx.x; t < nd; t += blockDim.x) tabs[t] = A.fcoef[t];
+    for (int t = threadIdx.x; t < A.ninter; t += blockDim.x) sip[t] = A.ip[t];
+    for (int t = threadIdx.x; t < A.ntypes * A.ntypes; t += blockDim.x) smap[t] = A.map[t];
+  }
+  __syncthreads();
+  if (lds) {
+    if (A.b.ortho) pair64_fonly_run<true, true>(A, bid, tabs, sip, smap);
+    else pair64_fonly_run<true, false>(A, bid, tabs, sip, smap);
+  } else {
+    if (A.b.ortho) pair64_fonly_run<false, true>(A, bid, A.fcoef, sip, smap);
+    else pair64_fonly_run<false, false>(A, bid, A.fcoef, sip, smap);
+  }
+}
+
 // 0t: force(...,0) (fs, clamp count); 1t: force(...,1) with fs and energies;
 // 1f: fl and energies; 1n: fl only (the MD step); 1e: energies only
 #define k_pair64_0t_body pair64_body<0, true, true, true>
 #define k_pair64_1t_body pair64_body<1, true, true, true>
 #define k_pair64_1f_body pair64_body<1, false, true, true>
-#define k_pair64_1n_body pair64_body<1, false, false, true>
+#define k_pair64_1n_body pair64_fonly_body
 #define k_pair64_1e_body pair64_body<1, false, true, false>
 // 1q: 1e + the sums of cm3d's energy-only coulomb routine (nb_energy_only)
 #define k_pair64_1q_body pair64_body<1, false, true, false, true>
@@ -1276,6 +1425,7 @@ int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_f
       P.nbr = c->d_nbr; P.nnb = c->d_nnb; P.stype = c->d_stype; P.perm = c->d_perm;
       P.map = c->d_map; P.rec = (const double4 *)c->d_rec64; P.ip = c->d_ip;
       P.vtab3 = c->d_vtab3; P.ctab3 = c->d_ctab3;
+      P.fcoef = c->d_fcoef; P.fc_lds = c->fcoef_lds;
       P.rmins_coul2 = c->rmins_coul2; P.dx2coul = c->dx2coul; P.rmaxl_coul2 = c->rmaxl_coul2;
       P.clast = c->clast;
       P.b = b; P.fl = d_fl; P.fs = d_fs; P.epart = c->d_epair;
@@ -1289,7 +1439,7 @@ int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_f
       else if (d_fs)
         NB_GO(c, K_PAIR64_1T, k_pair64_1t, g, 256, lds, P, s);
       else if (noener)
-        NB_GO(c, K_PAIR64_1N, k_pair64_1n, g, 256, lds, P, s);
+        NB_GO(c, K_PAIR64_1N, k_pair64_1n, g, 256, c->fcoef_lds, P, s);
       else
         NB_GO(c, K_PAIR64_1F, k_pair64_1f, g, 256, lds, P, s);
       if (!c->rec) LCHK(hipGetLastError());

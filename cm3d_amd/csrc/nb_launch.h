@@ -27,6 +27,8 @@ struct PairArgs {
   const double4 *rec;
   const InterPar *ip;
   const double *vtab3, *ctab3;
+  const double *fcoef;  // forces-only kernel: force-column coefficients (vdW tables, then coulomb)
+  int fc_lds;           // ... staged in LDS (bytes > 0) or read from HBM
   double rmins_coul2, dx2coul, rmaxl_coul2, clast;
   Box b;
   double *fl, *fs;

@@ -47,6 +47,12 @@ struct MixArgs {
   float h[9];
   float rmins_coul2, dx2coul, rmaxl_coul2;
   double *fl, *fs;
+  // forces-only kernel (k_mix_1f)
+  const float4 *fcf;  // force-column coefficients {c0, c1, c2, 0}: vdW tables, then coulomb
+  int fc_lds;         // their LDS bytes (0: read from HBM)
+  int fc_coff;        // node offset of the coulomb table
+  int ortho;          // h is diagonal
+  float hs[9];        // h * 2^-32: maps a fixed-point difference to a displacement
 };
 
 static __device__ __forceinline__ float interp3f(const float *t, int k, int c, float p) {
@@ -231,9 +237,164 @@ static __device__ __forceinline__ void pair_mixed_body(const MixArgs &A, int bid
   else
     pair_mixed_tabs<IFLG, WANT_FS, NB_WAVE>(A, bid, tabsf, stp, tp_lds);
 }
+// ---------------------------------------------------------------------------
+// Forces-only mixed kernel of the MD step (k_mix_1f): fl over the t / te
+// lists.  Same pairs and FP32 arithmetic as pair_mixed_run, in fewer
+// instructions per pair:
+//   - a row's entries past its end read the slots that follow it (the
+//     neighbour array ends in NB_LIST_PAD spare slots) and a select zeroes
+//     them; every lane gathers its partner's record (entry 0 reads atom
+//     0's), so no exec-mask change guards a load; partner types are
+//     gathered only for chunks that hold a vdW entry;
+//   - the coulomb term is evaluated on every lane and kept by a select (zero
+//     for an entry not in te or a pair beyond the cut-off, whose table node
+//     is clamped into the table); the vdW term runs only where an entry
+//     carries the vdW flag;
+//   - one table read per term: the node's coefficients {c0, c1, c2}
+//     (computed in FP64 from cm3d's table), c0 + p (c1 + p c2);
+//   - the vdW and coulomb F/r of a pair are added before the force update;
+//   - an orthorhombic box takes the diagonal of h * 2^-32.
+// ---------------------------------------------------------------------------
+// record / type of sorted atom j (j < NB_MAX_ATOMS: a 32-bit byte offset)
+static __device__ __forceinline__ uint4 recf_at(const uint4 *rec, uint32_t j) {
+  return *(const uint4 *)((const char *)rec + (j << 4));
+}
+static __device__ __forceinline__ int typef_at(const int *t, uint32_t j) {
+  return *(const int *)((const char *)t + (j << 2));
+}
+
+template <bool TLDS, bool ORTHO, int G>
+static __device__ __forceinline__ void mix_fonly_run(const MixArgs &A, int bid, const float4 *fc,
+                                                     const TPairF *stp) {
+#pragma clang fp contract(fast)
+  static_assert(G == 16 || G == NB_WAVE, "16 or 64 lanes per atom");
+  constexpr int APW = NB_WAVE / G;
+  const int lane = lane_id(), gl = lane % G, sub = lane / G;
+  const int nw = A.nblocks * 4;
+  const float4 *cc = fc + A.fc_coff;
+  const int kmax = A.ntable - 3;
+  const float pc0 = A.rmins_coul2 * A.dx2coul;
+  for (int s0 = (bid * 4 + (threadIdx.x >> 6)) * APW; s0 < A.n; s0 += nw * APW) {
+    const int s = s0 + sub;
+    const bool live = APW == 1 || s < A.n;
+    const int sa = live ? s : s0;  // a dead group reads its wave's first atom
+    const uint4 ri = A.rec[sa];
+    const float qi = __uint_as_float(ri.w);
+    const int ti = A.stype[sa];
+    const int nb = live ? A.nnb[sa] : 0;
+    const uint32_t *lst = A.nbr + (size_t)sa * A.cap + gl;
+    const TPairF *tprow = (TLDS ? stp : A.tpf) + ti * A.ntypes;
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    // entries two chunks ahead of the evaluation, partner records and types
+    // one chunk ahead
+    uint32_t ec = lst[0], en = lst[G];
+    ec = gl < nb ? ec : 0u;
+    en = G + gl < nb ? en : 0u;
+    uint4 rjc = recf_at(A.rec, ec & NB_IDX_MASK);
+    int tjc = 0;
+    if (__any((ec >> 28) & NB_FLAG_VT)) tjc = typef_at(A.stype, ec & NB_IDX_MASK);
+    int nbw = nb;
+    if constexpr (APW == 4)
+      nbw = max(max(__builtin_amdgcn_readlane(nb, 0), __builtin_amdgcn_readlane(nb, 16)),
+                max(__builtin_amdgcn_readlane(nb, 32), __builtin_amdgcn_readlane(nb, 48)));
+#pragma unroll 2
+    for (int k0 = 0; k0 < nbw; k0 += G) {
+      uint32_t e2 = lst[k0 + 2 * G];
+      e2 = k0 + 2 * G + gl < nb ? e2 : 0u;
+      const uint4 rjn = recf_at(A.rec, en & NB_IDX_MASK);
+      // partner types only for chunks holding vdW entries (the gathers, one
+      // cache line a lane, bound this kernel)
+      int tjn = 0;
+      if (__any((en >> 28) & NB_FLAG_VT)) tjn = typef_at(A.stype, en & NB_IDX_MASK);
+      const unsigned fl = ec >> 28;
+      const float ux = (float)(int)(ri.x - rjc.x);
+      const float uy = (float)(int)(ri.y - rjc.y);
+      const float uz = (float)(int)(ri.z - rjc.z);
+      float dx, dy, dz;
+      if (ORTHO) {
+        dx = ux * A.hs[0];
+        dy = uy * A.hs[4];
+        dz = uz * A.hs[8];
+      } else {
+        dx = ux * A.hs[0] + uy * A.hs[1] + uz * A.hs[2];
+        dy = ux * A.hs[3] + uy * A.hs[4] + uz * A.hs[5];
+        dz = ux * A.hs[6] + uy * A.hs[7] + uz * A.hs[8];
+      }
+      const float d2 = dx * dx + dy * dy + dz * dz;
+      float fr;  // F/r of the pair
+      {
+        float p = d2 * A.dx2coul - pc0;
+        int kk = (int)p;
+        const bool lo = kk < 1;
+        p = lo ? 1.f : p;
+        kk = lo ? 1 : min(kk, kmax);
+        p -= (float)kk;
+        const float4 T = cc[kk];
+        const bool on = (fl & NB_FLAG_ET) && d2 < A.rmaxl_coul2;
+        const float f = T.x + p * (T.y + p * T.z);
+        fr = on ? qi * __uint_as_float(rjc.w) * f : 0.f;
+      }
+      if (fl & NB_FLAG_VT) {
+        const TPairF P = tprow[tjc];
+        if (d2 < P.rmaxl2) {
+          float p = (d2 - P.rmins2) * P.dx2;
+          int kk = (int)p;
+          if (kk < 1) { kk = 1; p = 1.f; }
+          kk = min(kk, kmax);
+          p -= (float)kk;
+          const float4 T = fc[P.in * A.ntable + kk];
+          fr += T.x + p * (T.y + p * T.z);
+        }
+      }
+      fx -= dx * fr;
+      fy -= dy * fr;
+      fz -= dz * fr;
+      ec = en;
+      en = e2;
+      rjc = rjn;
+      tjc = tjn;
+    }
+    const long long X = group_sum_ll<G>(to_fix(fx)), Y = group_sum_ll<G>(to_fix(fy));
+    const long long Z = group_sum_ll<G>(to_fix(fz));
+    if (gl == 0 && live) {
+      const int ai = A.perm[sa], n = A.n;
+      A.fl[ai] = (double)X / NB_FX_SCALE;
+      A.fl[n + ai] = (double)Y / NB_FX_SCALE;
+      A.fl[2 * n + ai] = (double)Z / NB_FX_SCALE;
+    }
+  }
+}
+
+template <bool TLDS, int G>
+static __device__ __forceinline__ void mix_fonly_box(const MixArgs &A, int bid, const float4 *fc,
+                                                     const TPairF *stp) {
+  if (A.ortho) mix_fonly_run<TLDS, true, G>(A, bid, fc, stp);
+  else mix_fonly_run<TLDS, false, G>(A, bid, fc, stp);
+}
+
+static __device__ __forceinline__ void mix_fonly_body(const MixArgs &A, int bid) {
+  extern __shared__ float4 tabs4[];
+  __shared__ TPairF stp[NB_TPF_LDS];
+  const int nt2 = A.ntypes * A.ntypes;
+  const bool lds = A.fc_lds && nt2 <= NB_TPF_LDS;
+  if (lds) {
+    const int nn = A.fc_lds / (int)sizeof(float4);
+    for (int t = threadIdx.x; t < nn; t += blockDim.x) tabs4[t] = A.fcf[t];
+    for (int t = threadIdx.x; t < nt2; t += blockDim.x) stp[t] = A.tpf[t];
+  }
+  __syncthreads();
+  if (lds) {
+    if (A.g16) mix_fonly_box<true, 16>(A, bid, tabs4, stp);
+    else mix_fonly_box<true, NB_WAVE>(A, bid, tabs4, stp);
+  } else {
+    if (A.g16) mix_fonly_box<false, 16>(A, bid, A.fcf, stp);
+    else mix_fonly_box<false, NB_WAVE>(A, bid, A.fcf, stp);
+  }
+}
+
 #define k_mix_0t_body pair_mixed_body<0, true>
 #define k_mix_1t_body pair_mixed_body<1, true>
-#define k_mix_1f_body pair_mixed_body<1, false>
+#define k_mix_1f_body mix_fonly_body
 NB_KERNEL_PAIR(k_mix_0t, MixArgs, 256)
 NB_KERNEL_PAIR(k_mix_1t, MixArgs, 256)
 NB_KERNEL_PAIR(k_mix_1f, MixArgs, 256)
@@ -255,13 +416,18 @@ int nb_launch_pair_mixed(nb_ctx *c, int iflg, const Box &b, double *d_fs, double
   M.rmins_coul2 = (float)c->rmins_coul2; M.dx2coul = (float)c->dx2coul;
   M.rmaxl_coul2 = (float)c->rmaxl_coul2;
   M.fl = d_fl; M.fs = d_fs;
+  M.fcf = c->d_fcoeff; M.fc_lds = c->fcoef_lds_f;
+  M.fc_coff = c->ctab_off / 3;
+  M.ortho = b.ortho;
+  const float s32 = 2.3283064365386963e-10f;  // 2^-32
+  for (int k = 0; k < 9; k++) M.hs[k] = M.h[k] * s32;
   const int g = c->npair_blocks, lds = c->tab_lds_bytes_f;
   if (iflg == 0)
     NB_GO(c, K_MIX_0T, k_mix_0t, g, 256, lds, M, s);
   else if (d_fs)
     NB_GO(c, K_MIX_1T, k_mix_1t, g, 256, lds, M, s);
   else
-    NB_GO(c, K_MIX_1F, k_mix_1f, g, 256, lds, M, s);
+    NB_GO(c, K_MIX_1F, k_mix_1f, g, 256, c->fcoef_lds_f, M, s);
   if (!c->rec) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
