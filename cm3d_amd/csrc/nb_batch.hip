@@ -225,7 +225,6 @@ extern "C" int nb_batch_step(nb_batch *b, int nsteps, void *stream, int use_grap
   if (!b || nsteps < 0) { nb_set_error("nb_batch_step: bad argument"); return NB_ERR_ARG; }
   hipStream_t s = stream ? (hipStream_t)stream : b->stream;
   BCHK(hipSetDevice(b->dev));
-  if (nsteps == 0) return NB_OK;
   if (stale(b)) {
     // a member was regrown, re-gridded or re-thermostatted since recording
     BCHK(hipDeviceSynchronize());
@@ -243,6 +242,7 @@ extern "C" int nb_batch_step(nb_batch *b, int nsteps, void *stream, int use_grap
     b->graph = g;
     b->have_graph = 1;
   }
+  if (nsteps == 0) return NB_OK;  // prepared only
   launch_all(b->pre, s);  // snapshot + clear the sticky overflow flags
   for (int k = 0; k < nsteps; k++) {
     if (use_graph) BCHK(hipGraphLaunch(b->gexec, s));

@@ -34,7 +34,7 @@ struct LaunchRec;
 #define NB_FLAG_ES 8u  // coulomb se  (rcute_resp+skin)
 #define NB_IDX_MASK 0x0FFFFFFFu
 #define NB_LIST_PAD 256  // slots after the last row: the forces-only pair kernels read
-                         // up to two chunks of 64 entries past a row's end unguarded
+                         // up to 63 entries past a row's end unguarded
 #define NB_MAX_ATOMS (1 << 27)  // also keeps a 32-byte record's byte offset in 32 bits
 #ifndef NB_EW_SYM
 #define NB_EW_SYM 16     // values of m = |kc| per structure-factor lane (2 k vectors each)

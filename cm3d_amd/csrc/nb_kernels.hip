@@ -669,10 +669,12 @@ static __device__ __forceinline__ void pair64_body(const PairArgs &A, int bid) {
 // lists, no energies.  The same pairs, minimum image, r^2, cut-off tests and
 // table node as pair64_run (cm3d's operation order, no contraction), in
 // fewer instructions per pair:
-//   - a row's entries past its end read the slots that follow it (the
-//     neighbour array ends in NB_LIST_PAD spare slots) and a select zeroes
-//     them; every lane gathers its partner's record (entry 0 reads atom
-//     0's), so no exec-mask change guards a load; partner types are gathered
+//   - a chunk of entries is loaded when it starts inside the row (a
+//     wave-uniform test); its entries past the row's end read the slots
+//     that follow (the neighbour array ends in NB_LIST_PAD spare slots) and
+//     a select zeroes them; every lane gathers its partner's record (entry
+//     0 reads atom 0's), so no exec-mask change guards a load; partner
+//     types are gathered
 //     only for chunks that hold a vdW entry.  The gathers bound the kernel:
 //     a lane's record is a cache line of its own, and the L1 serves about
 //     one line a cycle;
@@ -701,7 +703,10 @@ static __device__ __forceinline__ void pair64_fonly_run(const PairArgs &A, int b
   const int nw = A.nblocks * 4;
   const double *cc = fc + A.ctab_off;
   const int kmax = A.ntable - 2;
-  for (int s = bid * 4 + (threadIdx.x >> 6); s < A.n; s += nw) {
+  // the wave's atom (and so its row length) is wave-uniform: scalar loads
+  // and branches
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  for (int s = bid * 4 + wv; s < A.n; s += nw) {
     const double4 ri = A.rec[s];
     const int ti = A.stype[s];
     const int nb = A.nnb[s];
@@ -709,16 +714,23 @@ static __device__ __forceinline__ void pair64_fonly_run(const PairArgs &A, int b
     double fx = 0., fy = 0., fz = 0.;
     // entries two chunks ahead of the evaluation, partner records and
     // types one chunk ahead
-    uint32_t ec = lst[0], en = lst[NB_WAVE];
+    // a chunk that starts past the row's end is not loaded (a wave-uniform
+    // branch); entries of a loaded chunk past the row's end are zeroed
+    uint32_t ec = lst[0], en = 0u;
     ec = lane < nb ? ec : 0u;
-    en = NB_WAVE + lane < nb ? en : 0u;
+    if (NB_WAVE < nb) {
+      en = lst[NB_WAVE];
+      en = NB_WAVE + lane < nb ? en : 0u;
+    }
     double4 rjc = rec64_at(A.rec, ec & NB_IDX_MASK);
     int tjc = 0;
     if (__any((ec >> 28) & NB_FLAG_VT)) tjc = type_at(A.stype, ec & NB_IDX_MASK);
 #pragma unroll 2
     for (int k0 = 0; k0 < nb; k0 += NB_WAVE) {
-      uint32_t e2 = lst[k0 + 2 * NB_WAVE];
-      e2 = k0 + 2 * NB_WAVE + lane < nb ? e2 : 0u;
+      // loaded raw; the select that zeroes entries past the row's end is
+      // applied a chunk later, when the load has long completed
+      uint32_t e2 = 0u;
+      if (k0 + 2 * NB_WAVE < nb) e2 = lst[k0 + 2 * NB_WAVE];
       const double4 rjn = rec64_at(A.rec, en & NB_IDX_MASK);
       // partner types only for chunks holding vdW entries (the gathers, one
       // cache line a lane, bound this kernel)
@@ -776,7 +788,7 @@ static __device__ __forceinline__ void pair64_fonly_run(const PairArgs &A, int b
         }
       }
       ec = en;
-      en = e2;
+      en = k0 + 2 * NB_WAVE + lane < nb ? e2 : 0u;
       rjc = rjn;
       tjc = tjn;
     }

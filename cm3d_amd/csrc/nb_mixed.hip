@@ -241,11 +241,13 @@ static __device__ __forceinline__ void pair_mixed_body(const MixArgs &A, int bid
 // Forces-only mixed kernel of the MD step (k_mix_1f): fl over the t / te
 // lists.  Same pairs and FP32 arithmetic as pair_mixed_run, in fewer
 // instructions per pair:
-//   - a row's entries past its end read the slots that follow it (the
-//     neighbour array ends in NB_LIST_PAD spare slots) and a select zeroes
-//     them; every lane gathers its partner's record (entry 0 reads atom
-//     0's), so no exec-mask change guards a load; partner types are
-//     gathered only for chunks that hold a vdW entry;
+//   - a chunk of entries is loaded when it starts inside the wave's longest
+//     row (a wave-uniform test); its entries past a row's end read the
+//     slots that follow (the neighbour array ends in NB_LIST_PAD spare
+//     slots) and a select zeroes them; every lane gathers its partner's
+//     record (entry 0 reads atom 0's), so no exec-mask change guards a
+//     load; partner types are gathered only for chunks that hold a vdW
+//     entry;
 //   - the coulomb term is evaluated on every lane and kept by a select (zero
 //     for an entry not in te or a pair beyond the cut-off, whose table node
 //     is clamped into the table); the vdW term runs only where an entry
@@ -274,7 +276,8 @@ static __device__ __forceinline__ void mix_fonly_run(const MixArgs &A, int bid, 
   const float4 *cc = fc + A.fc_coff;
   const int kmax = A.ntable - 3;
   const float pc0 = A.rmins_coul2 * A.dx2coul;
-  for (int s0 = (bid * 4 + (threadIdx.x >> 6)) * APW; s0 < A.n; s0 += nw * APW) {
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform
+  for (int s0 = (bid * 4 + wv) * APW; s0 < A.n; s0 += nw * APW) {
     const int s = s0 + sub;
     const bool live = APW == 1 || s < A.n;
     const int sa = live ? s : s0;  // a dead group reads its wave's first atom
@@ -287,20 +290,27 @@ static __device__ __forceinline__ void mix_fonly_run(const MixArgs &A, int bid, 
     float fx = 0.f, fy = 0.f, fz = 0.f;
     // entries two chunks ahead of the evaluation, partner records and types
     // one chunk ahead
-    uint32_t ec = lst[0], en = lst[G];
-    ec = gl < nb ? ec : 0u;
-    en = G + gl < nb ? en : 0u;
-    uint4 rjc = recf_at(A.rec, ec & NB_IDX_MASK);
-    int tjc = 0;
-    if (__any((ec >> 28) & NB_FLAG_VT)) tjc = typef_at(A.stype, ec & NB_IDX_MASK);
     int nbw = nb;
     if constexpr (APW == 4)
       nbw = max(max(__builtin_amdgcn_readlane(nb, 0), __builtin_amdgcn_readlane(nb, 16)),
                 max(__builtin_amdgcn_readlane(nb, 32), __builtin_amdgcn_readlane(nb, 48)));
+    // a chunk that starts past the longest row of the wave is not loaded (a
+    // wave-uniform branch); entries past a row's end are zeroed
+    uint32_t ec = lst[0], en = 0u;
+    ec = gl < nb ? ec : 0u;
+    if (G < nbw) {
+      en = lst[G];
+      en = G + gl < nb ? en : 0u;
+    }
+    uint4 rjc = recf_at(A.rec, ec & NB_IDX_MASK);
+    int tjc = 0;
+    if (__any((ec >> 28) & NB_FLAG_VT)) tjc = typef_at(A.stype, ec & NB_IDX_MASK);
 #pragma unroll 2
     for (int k0 = 0; k0 < nbw; k0 += G) {
-      uint32_t e2 = lst[k0 + 2 * G];
-      e2 = k0 + 2 * G + gl < nb ? e2 : 0u;
+      // loaded raw; the select that zeroes entries past the row's end is
+      // applied a chunk later, when the load has long completed
+      uint32_t e2 = 0u;
+      if (k0 + 2 * G < nbw) e2 = lst[k0 + 2 * G];
       const uint4 rjn = recf_at(A.rec, en & NB_IDX_MASK);
       // partner types only for chunks holding vdW entries (the gathers, one
       // cache line a lane, bound this kernel)
@@ -350,7 +360,7 @@ static __device__ __forceinline__ void mix_fonly_run(const MixArgs &A, int bid, 
       fy -= dy * fr;
       fz -= dz * fr;
       ec = en;
-      en = e2;
+      en = k0 + 2 * G + gl < nb ? e2 : 0u;
       rjc = rjn;
       tjc = tjn;
     }
