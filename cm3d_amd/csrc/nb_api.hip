@@ -199,6 +199,10 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
     return fail(NB_ERR_HIP);
   }
   c->natoms = n; c->ntypes = p->ntypes; c->ninter = p->ninter; c->ntable = p->ntable;
+  c->ibits = 1;
+  while ((1 << c->ibits) < n) c->ibits++;
+  c->imask = (1u << c->ibits) - 1u;
+  c->etype = c->ibits < 28 && p->ntypes <= (1 << (28 - c->ibits));
   c->iperd = p->iperd; c->prec = p->precision; c->skin = p->skin;
   c->kappa = p->kappa; c->ewald_self = p->ewald_self; c->ewald_bkg = p->ewald_bkg;
   c->dx2coul = p->dx2coul; c->rmins_coul2 = p->rmins_coul2; c->rmaxs_coul2 = p->rmaxs_coul2;
@@ -987,7 +991,7 @@ extern "C" int nb_get_pairs(nb_ctx *c, int which, int *ip, int *jp, int cap, int
     for (int k = 0; k < nnb[s]; k++) {
       uint32_t e = nbr[(size_t)s * c->cap + k];
       if (!((e >> 28) & bit[which])) continue;
-      int b = perm[e & NB_IDX_MASK];
+      int b = perm[e & c->imask];
       if (a < b) pr.emplace_back(a, b);
     }
   }

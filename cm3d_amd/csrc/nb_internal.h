@@ -32,7 +32,7 @@ struct LaunchRec;
 #define NB_FLAG_VS 2u  // vdW s list  (rmaxs+skin)
 #define NB_FLAG_ET 4u  // coulomb te  (rcute_max+skin)
 #define NB_FLAG_ES 8u  // coulomb se  (rcute_resp+skin)
-#define NB_IDX_MASK 0x0FFFFFFFu
+#define NB_IDX_MASK 0x0FFFFFFFu  // index and type bits of a list entry
 #define NB_LIST_PAD 256  // slots after the last row: the forces-only pair kernels read
                          // up to 63 entries past a row's end unguarded
 #define NB_MAX_ATOMS (1 << 27)  // also keeps a 32-byte record's byte offset in 32 bits
@@ -265,6 +265,11 @@ struct nb_ctx {
   uint32_t *d_nbr;
   int *d_nnb;
   int cap;
+  // a list entry: 4 flag bits, then (when ntypes fits) the partner's type
+  // above its sorted index: [flags:4 | type : 28 - ibits | index : ibits]
+  int ibits;                   // bits of the sorted index (enough for natoms)
+  int etype;                   // entries carry the partner's type
+  uint32_t imask;              // (1 << ibits) - 1
   int *d_flags;                // [0] overflow max count of the last build,
                                // [1] rebuild flag, [2] overflow max since the
                                // host last cleared it (sticky over a batch

@@ -220,6 +220,7 @@ struct ListArgs {
   int *nnb, *flags;
   int4 *hcnt;
   const int *cond;
+  int ibits, etype;  // entry layout: the partner's type above ibits index bits (when etype)
 };
 
 static __device__ __forceinline__ int wrapc(int v, int n) {
@@ -394,12 +395,13 @@ static __device__ __forceinline__ void k_list_body(const ListArgs &A, int bid) {
     const bool isv = (fl & NB_FLAG_VT) != 0 || (fl != 0 && !rowv);
     const unsigned long long mv = __ballot(isv), mc = __ballot(fl != 0 && !isv);
     if (fl) {
+      const uint32_t ent = (uint32_t)s2c | (A.etype ? (uint32_t)tjc << A.ibits : 0u) | (fl << 28);
       if (isv) {
         const int slot = nv + mbcnt(mv);
-        if (slot < A.cap) out[slot] = (uint32_t)s2c | (fl << 28);
+        if (slot < A.cap) out[slot] = ent;
       } else {
         const int slot = A.cap - 1 - (nc + mbcnt(mc));
-        if (slot >= 0) out[slot] = (uint32_t)s2c | (fl << 28);
+        if (slot >= 0) out[slot] = ent;
       }
       if (ai < ajc) {
         hs += (fl & NB_FLAG_VS) != 0;
@@ -523,7 +525,7 @@ static __device__ __forceinline__ void pair64_run(const PairArgs &A, int bid, co
     double4 rjc = make_double4(0., 0., 0., 0.);
     int tjc = 0;
     if ((ec >> 28) & (fv | fe)) {
-      const int j = (int)(ec & NB_IDX_MASK);
+      const int j = (int)(ec & A.imask);
       rjc = A.rec[j];
       tjc = A.stype[j];
     }
@@ -534,7 +536,7 @@ static __device__ __forceinline__ void pair64_run(const PairArgs &A, int bid, co
       double4 rjn = make_double4(0., 0., 0., 0.);
       int tjn = 0;
       if ((en >> 28) & (fv | fe)) {
-        const int j = (int)(en & NB_IDX_MASK);
+        const int j = (int)(en & A.imask);
         rjn = A.rec[j];
         tjn = A.stype[j];
       }
@@ -722,20 +724,21 @@ static __device__ __forceinline__ void pair64_fonly_run(const PairArgs &A, int b
       en = lst[NB_WAVE];
       en = NB_WAVE + lane < nb ? en : 0u;
     }
-    double4 rjc = rec64_at(A.rec, ec & NB_IDX_MASK);
+    const uint32_t im = A.imask;
+    double4 rjc = rec64_at(A.rec, ec & im);
     int tjc = 0;
-    if (__any((ec >> 28) & NB_FLAG_VT)) tjc = type_at(A.stype, ec & NB_IDX_MASK);
+    if (!A.etype && __any((ec >> 28) & NB_FLAG_VT)) tjc = type_at(A.stype, ec & im);
 #pragma unroll 2
     for (int k0 = 0; k0 < nb; k0 += NB_WAVE) {
       // loaded raw; the select that zeroes entries past the row's end is
       // applied a chunk later, when the load has long completed
       uint32_t e2 = 0u;
       if (k0 + 2 * NB_WAVE < nb) e2 = lst[k0 + 2 * NB_WAVE];
-      const double4 rjn = rec64_at(A.rec, en & NB_IDX_MASK);
-      // partner types only for chunks holding vdW entries (the gathers, one
-      // cache line a lane, bound this kernel)
+      const double4 rjn = rec64_at(A.rec, en & im);
+      // partner types from the entry, else gathered only for chunks holding
+      // vdW entries (the gathers, one cache line a lane, bound this kernel)
       int tjn = 0;
-      if (__any((en >> 28) & NB_FLAG_VT)) tjn = type_at(A.stype, en & NB_IDX_MASK);
+      if (!A.etype && __any((en >> 28) & NB_FLAG_VT)) tjn = type_at(A.stype, en & im);
       const unsigned fl = ec >> 28;
       double dx = ri.x - rjc.x, dy = ri.y - rjc.y, dz = ri.z - rjc.z;
       if (ORTHO) {
@@ -751,7 +754,8 @@ static __device__ __forceinline__ void pair64_fonly_run(const PairArgs &A, int b
       }
       const double d2 = dx * dx + dy * dy + dz * dz;
       if (fl & NB_FLAG_VT) {
-        const int in = TLDS ? smap[ti * A.ntypes + tjc] : A.map[ti * A.ntypes + tjc];
+        const int tj = A.etype ? (int)((ec & NB_IDX_MASK) >> A.ibits) : tjc;
+        const int in = TLDS ? smap[ti * A.ntypes + tj] : A.map[ti * A.ntypes + tj];
         const InterPar &P = TLDS ? sip[in] : A.ip[in];
         if (d2 < P.rmaxl2) {
           double p = (d2 - P.rmins2) * P.dx2;
@@ -1359,6 +1363,7 @@ static int launch_build_impl(nb_ctx *c, const Box &b, hipStream_t s, const int *
   A.xoff = c->d_xoff; A.xlist = c->d_xlist; A.pcut = (const double2 *)c->d_pcut;
   A.rmaxs_skin2_e = c->rmaxs_skin2_e; A.rmaxl_skin2_e = c->rmaxl_skin2_e;
   A.b = b; A.nbr = c->d_nbr; A.nnb = c->d_nnb; A.flags = c->d_flags;
+  A.ibits = c->ibits; A.etype = c->etype;
   A.hcnt = (int4 *)c->d_hcnt; A.cond = cond;
   NB_GO(c, K_LIST, k_list, blocks(n, 4), 256, 0, A, s);
   if (!c->rec) LCHK(hipGetLastError());
@@ -1438,6 +1443,7 @@ int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_f
       P.map = c->d_map; P.rec = (const double4 *)c->d_rec64; P.ip = c->d_ip;
       P.vtab3 = c->d_vtab3; P.ctab3 = c->d_ctab3;
       P.fcoef = c->d_fcoef; P.fc_lds = c->fcoef_lds;
+      P.imask = c->imask; P.ibits = c->ibits; P.etype = c->etype;
       P.rmins_coul2 = c->rmins_coul2; P.dx2coul = c->dx2coul; P.rmaxl_coul2 = c->rmaxl_coul2;
       P.clast = c->clast;
       P.b = b; P.fl = d_fl; P.fs = d_fs; P.epart = c->d_epair;

@@ -29,6 +29,8 @@ struct PairArgs {
   const double *vtab3, *ctab3;
   const double *fcoef;  // forces-only kernel: force-column coefficients (vdW tables, then coulomb)
   int fc_lds;           // ... staged in LDS (bytes > 0) or read from HBM
+  uint32_t imask;       // list entry: sorted index bits
+  int ibits, etype;     // ... and the partner's type above them (when etype)
   double rmins_coul2, dx2coul, rmaxl_coul2, clast;
   Box b;
   double *fl, *fs;

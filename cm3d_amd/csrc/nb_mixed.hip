@@ -53,6 +53,8 @@ struct MixArgs {
   int fc_coff;        // node offset of the coulomb table
   int ortho;          // h is diagonal
   float hs[9];        // h * 2^-32: maps a fixed-point difference to a displacement
+  uint32_t imask;     // list entry: sorted index bits
+  int ibits, etype;   // ... and the partner's type above them (when etype)
 };
 
 static __device__ __forceinline__ float interp3f(const float *t, int k, int c, float p) {
@@ -112,7 +114,7 @@ static __device__ __forceinline__ void pair_mixed_run(const MixArgs &A, int bid,
     uint4 rjc = make_uint4(0u, 0u, 0u, 0u);
     int tjc = 0;
     if ((ec >> 28) & (fv | fe)) {
-      const int j = (int)(ec & NB_IDX_MASK);
+      const int j = (int)(ec & A.imask);
       rjc = A.rec[j];
       tjc = A.stype[j];
     }
@@ -128,7 +130,7 @@ static __device__ __forceinline__ void pair_mixed_run(const MixArgs &A, int bid,
       uint4 rjn = make_uint4(0u, 0u, 0u, 0u);
       int tjn = 0;
       if ((en >> 28) & (fv | fe)) {
-        const int j = (int)(en & NB_IDX_MASK);
+        const int j = (int)(en & A.imask);
         rjn = A.rec[j];
         tjn = A.stype[j];
       }
@@ -302,20 +304,21 @@ static __device__ __forceinline__ void mix_fonly_run(const MixArgs &A, int bid, 
       en = lst[G];
       en = G + gl < nb ? en : 0u;
     }
-    uint4 rjc = recf_at(A.rec, ec & NB_IDX_MASK);
+    const uint32_t im = A.imask;
+    uint4 rjc = recf_at(A.rec, ec & im);
     int tjc = 0;
-    if (__any((ec >> 28) & NB_FLAG_VT)) tjc = typef_at(A.stype, ec & NB_IDX_MASK);
+    if (!A.etype && __any((ec >> 28) & NB_FLAG_VT)) tjc = typef_at(A.stype, ec & im);
 #pragma unroll 2
     for (int k0 = 0; k0 < nbw; k0 += G) {
       // loaded raw; the select that zeroes entries past the row's end is
       // applied a chunk later, when the load has long completed
       uint32_t e2 = 0u;
       if (k0 + 2 * G < nbw) e2 = lst[k0 + 2 * G];
-      const uint4 rjn = recf_at(A.rec, en & NB_IDX_MASK);
-      // partner types only for chunks holding vdW entries (the gathers, one
-      // cache line a lane, bound this kernel)
+      const uint4 rjn = recf_at(A.rec, en & im);
+      // partner types from the entry, else gathered only for chunks holding
+      // vdW entries (the gathers, one cache line a lane, bound this kernel)
       int tjn = 0;
-      if (__any((en >> 28) & NB_FLAG_VT)) tjn = typef_at(A.stype, en & NB_IDX_MASK);
+      if (!A.etype && __any((en >> 28) & NB_FLAG_VT)) tjn = typef_at(A.stype, en & im);
       const unsigned fl = ec >> 28;
       const float ux = (float)(int)(ri.x - rjc.x);
       const float uy = (float)(int)(ri.y - rjc.y);
@@ -345,7 +348,8 @@ static __device__ __forceinline__ void mix_fonly_run(const MixArgs &A, int bid, 
         fr = on ? qi * __uint_as_float(rjc.w) * f : 0.f;
       }
       if (fl & NB_FLAG_VT) {
-        const TPairF P = tprow[tjc];
+        const int tj = A.etype ? (int)((ec & NB_IDX_MASK) >> A.ibits) : tjc;
+        const TPairF P = tprow[tj];
         if (d2 < P.rmaxl2) {
           float p = (d2 - P.rmins2) * P.dx2;
           int kk = (int)p;
@@ -429,6 +433,7 @@ int nb_launch_pair_mixed(nb_ctx *c, int iflg, const Box &b, double *d_fs, double
   M.fcf = c->d_fcoeff; M.fc_lds = c->fcoef_lds_f;
   M.fc_coff = c->ctab_off / 3;
   M.ortho = b.ortho;
+  M.imask = c->imask; M.ibits = c->ibits; M.etype = c->etype;
   const float s32 = 2.3283064365386963e-10f;  // 2^-32
   for (int k = 0; k < 9; k++) M.hs[k] = M.h[k] * s32;
   const int g = c->npair_blocks, lds = c->tab_lds_bytes_f;
