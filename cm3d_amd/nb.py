@@ -544,7 +544,9 @@ class NBBatch:
 
     def step(self, nsteps=1, stream=None, graph=True):
         """One tick of nsteps steps; returns when it is done (systems whose
-        list build overflowed are redone with more slots)."""
+        list build overflowed are redone with more slots).  nsteps=0 only
+        re-records / captures the step, so that this can happen on the
+        caller's thread between ticks run elsewhere."""
         check(lib().nb_batch_step(self._h, int(nsteps), C.c_void_p(stream) if stream else None,
                                   int(bool(graph))), "nb_batch_step")
 

@@ -246,6 +246,10 @@ class Server:
                     dirty = False
                 k = min(self.tick_steps, min(j.nsteps - j.done for j in active))
                 t = time.perf_counter()
+                # re-recording and graph capture happen here, between ticks:
+                # a capture forbids the legacy-stream copies of the
+                # admissions that overlap the tick on this thread
+                batch.step(0, None, self.graph)
                 fut = tick_pool.submit(batch.step, k, None, self.graph)
                 while not fut.done():
                     if not admit():  # admissions overlap the running tick

@@ -282,7 +282,11 @@ void nb_batch_destroy(nb_batch *b);
  * system that did is restored, given the slots (x1.25), rebuilt and stepped
  * again alone, which is bitwise what the batch computes for it -- no system
  * continues on truncated lists.  A batch whose members were regrown or
- * re-thermostatted since it recorded its step records it again. */
+ * re-thermostatted since it recorded its step records it again.
+ * nsteps 0 only prepares the next tick: records the step again if a member
+ * changed and, with use_graph, captures its graph, so that a caller can do
+ * both on its own thread (a capture forbids legacy-stream work, such as
+ * nb_create's copies, on other threads) and run ticks elsewhere. */
 int nb_batch_step(nb_batch *b, int nsteps, void *stream, int use_graph);
 /* Number of kernel launches one batched step issues. */
 int nb_batch_launches(nb_batch *b, int *nkernels);
