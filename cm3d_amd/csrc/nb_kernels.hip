@@ -204,14 +204,14 @@ NB_KERNEL_PAIR(k_pack64, Pack64Args, 256)
 // ngbr_liste do (null map, exclusion, minimum image, strict
 // r^2 < (rc+skin)^2, s nested in t), and the survivors are compacted with a
 // ballot + mbcnt prefix into the atom's slots.  Both directions of every pair
-// are stored; the per-atom half-pair counts (i < j) go to hcnt.
+// are stored; the per-atom half-pair counts (each pair once) go to hcnt.
 // ---------------------------------------------------------------------------
 #define NB_PCUT_LDS 1024  // type pairs whose list cutoffs are staged in LDS
 struct ListArgs {
   int n, ntypes, cap, ncx, ncy, ncz, rx, ry, rz, pcut_lds, prune;
   double rprune2, wcx, wcy, wcz;  // pruning bound^2, cell widths (A) of an orthorhombic box
   const double4 *rec;
-  const int *stype, *perm, *scell, *cstart, *ccount;
+  const int *stype, *perm, *iperm, *scell, *cstart, *ccount;
   const int *xoff, *xlist;
   const double2 *pcut;  // {(rmaxl+skin)^2, (rmaxs+skin)^2} per (ti,tj); -1 for null
   double rmaxs_skin2_e, rmaxl_skin2_e;
@@ -242,13 +242,19 @@ static __device__ __forceinline__ void k_list_body(const ListArgs &A, int bid) {
   const int ti = A.stype[s], ai = A.perm[s];
   const int c = A.scell[s];
   const int cz = c % A.ncz, cy = (c / A.ncz) % A.ncy, cx = c / (A.ncy * A.ncz);
-  // this atom's exclusions: the first four as wave-uniform values
+  // this atom's exclusions (kind bits 30-31 over an atom index), the first
+  // four as wave-uniform values with the partner's sorted index, so that a
+  // candidate is tested by its sorted index and no atom index is gathered
   const int x0 = A.xoff[ai], x1 = A.xoff[ai + 1], nx = x1 - x0;
   const unsigned none = 0xFFFFFFFFu;  // index bits 0x3FFFFFFF match no atom
-  const unsigned ex0 = nx > 0 ? (unsigned)A.xlist[x0] : none;
-  const unsigned ex1 = nx > 1 ? (unsigned)A.xlist[x0 + 1] : none;
-  const unsigned ex2 = nx > 2 ? (unsigned)A.xlist[x0 + 2] : none;
-  const unsigned ex3 = nx > 3 ? (unsigned)A.xlist[x0 + 3] : none;
+  auto xsorted = [&](int e) {
+    const unsigned v = (unsigned)A.xlist[e];
+    return (v & 0xC0000000u) | (unsigned)A.iperm[v & 0x3FFFFFFFu];
+  };
+  const unsigned ex0 = nx > 0 ? xsorted(x0) : none;
+  const unsigned ex1 = nx > 1 ? xsorted(x0 + 1) : none;
+  const unsigned ex2 = nx > 2 ? xsorted(x0 + 2) : none;
+  const unsigned ex3 = nx > 3 ? xsorted(x0 + 3) : none;
   const bool qi = ri.w != 0.0;
   // candidate runs: lane l < ncol takes column l (offsets ox, oy) and
   // finds its <= 2 slot runs
@@ -334,7 +340,7 @@ static __device__ __forceinline__ void k_list_body(const ListArgs &A, int bid) {
   int col = 0, colend = lc[0];
   int cb = 0;
   double4 rjc = make_double4(0., 0., 0., 0.);
-  int tjc = 0, ajc = -1, s2c = 0;
+  int tjc = 0, s2c = 0;
   bool okc = false;
   {
     const int c = cb + lane;
@@ -344,12 +350,12 @@ static __device__ __forceinline__ void k_list_body(const ListArgs &A, int bid) {
       const int loc = c - cstart, r0 = lc[2 * NB_WAVE + col];
       s2c = loc < r0 ? lc[NB_WAVE + col] + loc : lc[3 * NB_WAVE + col] + (loc - r0);
       okc = s2c != s;
-      if (okc) { rjc = A.rec[s2c]; tjc = A.stype[s2c]; ajc = A.perm[s2c]; }
+      if (okc) { rjc = A.rec[s2c]; tjc = A.stype[s2c]; }
     }
   }
   for (; cb < total; cb += NB_WAVE) {
     double4 rjn = make_double4(0., 0., 0., 0.);
-    int tjn = 0, ajn = -1, s2n = 0;
+    int tjn = 0, s2n = 0;
     bool okn = false;
     {
       const int c = cb + NB_WAVE + lane;
@@ -359,7 +365,7 @@ static __device__ __forceinline__ void k_list_body(const ListArgs &A, int bid) {
         const int loc = c - cstart, r0 = lc[2 * NB_WAVE + col];
         s2n = loc < r0 ? lc[NB_WAVE + col] + loc : lc[3 * NB_WAVE + col] + (loc - r0);
         okn = s2n != s;
-        if (okn) { rjn = A.rec[s2n]; tjn = A.stype[s2n]; ajn = A.perm[s2n]; }
+        if (okn) { rjn = A.rec[s2n]; tjn = A.stype[s2n]; }
       }
     }
     unsigned fl = 0;
@@ -367,14 +373,14 @@ static __device__ __forceinline__ void k_list_body(const ListArgs &A, int bid) {
       double dx = ri.x - rjc.x, dy = ri.y - rjc.y, dz = ri.z - rjc.z;
       nb_minimg_box(dx, dy, dz, A.b);
       const double d2 = dx * dx + dy * dy + dz * dz;
-      const unsigned uj = (unsigned)ajc;
+      const unsigned uj = (unsigned)s2c;
       unsigned ex = 0;
       if ((ex0 & 0x3FFFFFFFu) == uj) ex |= ex0 >> 30;
       if ((ex1 & 0x3FFFFFFFu) == uj) ex |= ex1 >> 30;
       if ((ex2 & 0x3FFFFFFFu) == uj) ex |= ex2 >> 30;
       if ((ex3 & 0x3FFFFFFFu) == uj) ex |= ex3 >> 30;
       for (int e = x0 + 4; e < x1; e++) {
-        const unsigned v = (unsigned)A.xlist[e];
+        const unsigned v = xsorted(e);
         if ((v & 0x3FFFFFFFu) == uj) ex |= v >> 30;
       }
       const double2 cut = A.pcut_lds ? pc[ti * A.ntypes + tjc] : pcut[tjc];
@@ -403,7 +409,9 @@ static __device__ __forceinline__ void k_list_body(const ListArgs &A, int bid) {
         const int slot = A.cap - 1 - (nc + mbcnt(mc));
         if (slot >= 0) out[slot] = ent;
       }
-      if (ai < ajc) {
+      // each unordered pair once (the filters are symmetric in i and j,
+      // so counting by sorted index gives cm3d's i < j totals)
+      if (s < s2c) {
         hs += (fl & NB_FLAG_VS) != 0;
         ht += (fl & NB_FLAG_VT) != 0;
         hse += (fl & NB_FLAG_ES) != 0;
@@ -412,7 +420,7 @@ static __device__ __forceinline__ void k_list_body(const ListArgs &A, int bid) {
     }
     nv += __popcll(mv);
     nc += __popcll(mc);
-    rjc = rjn; tjc = tjn; ajc = ajn; s2c = s2n; okc = okn;
+    rjc = rjn; tjc = tjn; s2c = s2n; okc = okn;
   }
   const int count = nv + nc;
   // close the row: the Coulomb-only entries move down behind the vdW ones,
@@ -1359,6 +1367,7 @@ static int launch_build_impl(nb_ctx *c, const Box &b, hipStream_t s, const int *
   A.rprune2 = c->rprune2;
   A.wcx = b.h[0] / c->ncx; A.wcy = b.h[4] / c->ncy; A.wcz = b.h[8] / c->ncz;
   A.rec = (const double4 *)c->d_rec64; A.stype = c->d_stype; A.perm = c->d_perm;
+  A.iperm = c->d_iperm;
   A.scell = c->d_cell; A.cstart = c->d_cstart; A.ccount = c->d_ccount;
   A.xoff = c->d_xoff; A.xlist = c->d_xlist; A.pcut = (const double2 *)c->d_pcut;
   A.rmaxs_skin2_e = c->rmaxs_skin2_e; A.rmaxl_skin2_e = c->rmaxl_skin2_e;
