@@ -358,15 +358,55 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
       const int km = (m > 0 && -m >= r.kc_lo && -m <= r.kc_hi) ? r.k_lo - m - r.kc_lo : -1;
       ent.push_back(make_int4(kp, km, m, 0));
     }
-    for (int m0 = r.mlo; m0 <= r.mhi; m0 += NB_EW_SYM) {
-      EwSym y = {r.k_lo, r.kc_lo, r.kc_hi, r.ka, r.kb, m0, std::min(NB_EW_SYM, r.mhi - m0 + 1), 0};
+  }
+  // structure-factor lanes.  When every row is short (<= 8 or <= 4 values
+  // of m, the small k spheres of small systems), a lane takes 2 or 4 rows of
+  // one ka plane with consecutive kb, m from 0 (a row whose kc range starts
+  // above m = 0 only leaves its first slots unstored): the rows share the
+  // c-axis recurrence and each row's phase follows from the previous row's
+  // by one step of e^{i theta_b}, so the lane's sine / cosine per charge is
+  // spread over 2 or 4 rows.  Otherwise each lane is a run of <=
+  // NB_EW_SYM values of m of one row.  One row count for the whole system
+  // keeps the lanes in as few 64-lane groups as possible.  Fewer lanes also
+  // mean fewer workgroups, which a small system alone on the GPU needs to
+  // fill it, so only a context that shares the GPU (a batch member) or has
+  // NB_SFAC_ROWS_MIN_Q charges or more takes several rows a lane.
+  {
+    int mmax = 0;
+    for (int g = 0; g < c->nrow; g++) mmax = std::max(mmax, row[g].mhi + 1);
+    const bool full = p->sharing > 1 || ncharge >= NB_SFAC_ROWS_MIN_Q;
+    const int nr = !full ? 1 : (mmax <= NB_EW_SYM / 4 ? 4 : (mmax <= NB_EW_SYM / 2 ? 2 : 1));
+    for (int g = 0; g < c->nrow;) {
+      EwSym y;
+      memset(&y, 0, sizeof(y));
+      y.ka = row[g].ka; y.kb = row[g].kb; y.nr = nr;
+      for (int q = 0; q < 4; q++) { y.k_lo[q] = 0; y.kc_lo[q] = 1; y.kc_hi[q] = 0; }
+      if (nr == 1) {
+        const EwRow &r = row[g];
+        y.k_lo[0] = r.k_lo; y.kc_lo[0] = r.kc_lo; y.kc_hi[0] = r.kc_hi;
+        for (int m0 = r.mlo; m0 <= r.mhi; m0 += NB_EW_SYM) {
+          y.m0 = m0;
+          y.cnt = std::min(NB_EW_SYM, r.mhi - m0 + 1);
+          sym.push_back(y);
+        }
+        g++;
+        continue;
+      }
+      // rows g, g + 1, ... while they continue the plane by one kb
+      int q = 0;
+      for (; q < nr && g + q < c->nrow && (q == 0 || row[g + q].chain); q++) {
+        const EwRow &r = row[g + q];
+        y.k_lo[q] = r.k_lo; y.kc_lo[q] = r.kc_lo; y.kc_hi[q] = r.kc_hi;
+        y.cnt = std::max(y.cnt, r.mhi + 1);
+      }
       sym.push_back(y);
+      g += q;
     }
   }
   const int nent = (int)ent.size();
-  // structure factor: one lane per run of <= NB_EW_SYM values of m, longest
-  // first (a wave's trip count is its first lane's), then by m0 (lanes of
-  // one wave start their phases alike); a workgroup of 4 waves takes 64
+  // structure factor: lanes (above) longest first (a wave's trip count is
+  // its first lane's), then by m0 (lanes of one wave start their phases
+  // alike); a workgroup of 4 waves takes 64
   // lanes and one chunk of charges (>= NB_SFAC_MIN_Q); chunks sized for ~NB_SFAC_WG
   // workgroups, ~16 a CU, which the long charge loops need to hide FP64
   // latency; a system sharing the GPU with others gets a proportionally

@@ -61,6 +61,9 @@ struct LaunchRec;
 #ifndef NB_SFAC_PAIR
 #define NB_SFAC_PAIR 1   // structure factor: two charges per trip of the charge loop
 #endif
+#ifndef NB_SFAC_ROWS_MIN_Q
+#define NB_SFAC_ROWS_MIN_Q 16384  // charges from which a lone system's structure-factor lanes take several rows
+#endif
 #ifndef NB_SFAC_MIN_Q
 #define NB_SFAC_MIN_Q 64  // fewest charges in a structure-factor chunk
 #endif
@@ -112,8 +115,9 @@ struct EwRow {
   int e0;            // its first force-pass entry (one per m)
   int chain;         // the previous row is (ka, kb - 1): its phase follows by one kb step
 };
-struct EwSym {       // a structure-factor lane: <= NB_EW_SYM values of m from m0 in one row
-  int k_lo, kc_lo, kc_hi, ka, kb, m0, cnt, pad;
+struct EwSym {       // a structure-factor lane: nr rows (ka, kb .. kb + nr - 1), cnt values of m from m0
+  int ka, kb, m0, cnt, nr, pad0, pad1, pad2;
+  int k_lo[4], kc_lo[4], kc_hi[4];  // per row: k index of kc_lo, its kc range
 };
 
 // cm3d's -DLINUX anint: (int)(x +- 0.5), truncating (md_macros.h:31-52).

@@ -868,11 +868,13 @@ NB_KERNEL_PAIR_W(k_pair64_1q, PairArgs, 256, 4)
 //   [-1/2,1/2) (exact for e^{2 pi i k.s} with integer k), stored as 2 pi s,
 //   the charge times sqrt(4 pi) (md_ewald.c:220-224), e^{2 pi i s_c} and
 //   e^{2 pi i NB_EW_SYM s_c}.
-// k_ewsfac: lanes = runs of <= NB_EW_SYM values of m in one row (sorted longest
-//   first, so a wave's trip count is its first lane's); a workgroup walks one
-//   chunk of charges and each lane sums q w e^{i m theta_c} split into its
-//   cosine and sine parts in registers -- no cross-lane reduction -- then
-//   the 4 waves combine and write the chunk's partial S(k) for +m and -m.
+// k_ewsfac: lanes = runs of <= NB_EW_SYM values of m in one row, or, in a
+//   system whose rows are all short, 2 / 4 rows of consecutive kb with <= 8
+//   / 4 values of m each (sorted longest first, so a wave's trip count is
+//   its first lane's); a workgroup walks one chunk of charges and each lane sums
+//   q w e^{i m theta_c} split into its cosine and sine parts in registers --
+//   no cross-lane reduction -- then the 4 waves combine and write the
+//   chunk's partial S(k) for +m and -m.
 // k_ewkred: per k, the chunk partials summed in fixed order, then preg, prep,
 //   |S|^2 and the energy / virial terms (md_ewald.c:247-302).
 // k_ewforce: lanes = charges; a workgroup walks one range of rows in k
@@ -953,6 +955,42 @@ static __device__ __forceinline__ void sfac_charge(const double *cq, double wr, 
     }
   }
 }
+// One charge against a lane of R rows (ka, kb + r) x M values of m from 0:
+// w_r = w e^{i r theta_b} by steps of e^{i theta_b} (eb), z^m from 1 by the
+// three-term recurrence, shared by the R rows.  Accumulator r * M + m.
+template <int R, int M>
+static __device__ __forceinline__ void sfac_charge_rows(const double *cq, const double *eb,
+                                                        double wr, double wi,
+                                                        double (&ar)[NB_EW_SYM], double (&ai)[NB_EW_SYM],
+                                                        double (&br)[NB_EW_SYM], double (&bi)[NB_EW_SYM]) {
+  const double zr = cq[4], zi = cq[5];
+  double wrr[R], wir[R];
+  wrr[0] = wr;
+  wir[0] = wi;
+#pragma unroll
+  for (int r = 1; r < R; r++) {
+    wrr[r] = wrr[r - 1] * eb[0] - wir[r - 1] * eb[1];
+    wir[r] = wir[r - 1] * eb[0] + wrr[r - 1] * eb[1];
+  }
+  // z^0 = 1 and z^-1 = conj z
+  double c = 1., s = 0., cp = zr, sp = -zi;
+  const double t2 = 2. * zr;
+#pragma unroll
+  for (int m = 0; m < M; m++) {
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      ar[r * M + m] = __builtin_fma(wrr[r], c, ar[r * M + m]);
+      ai[r * M + m] = __builtin_fma(wir[r], c, ai[r * M + m]);
+      br[r * M + m] = __builtin_fma(wrr[r], s, br[r * M + m]);
+      bi[r * M + m] = __builtin_fma(wir[r], s, bi[r * M + m]);
+    }
+    if (m + 1 < M) {
+      const double cn = __builtin_fma(t2, c, -cp), sn = __builtin_fma(t2, s, -sp);
+      cp = c; sp = s;
+      c = cn; s = sn;
+    }
+  }
+}
 // w = q e^{i (ka theta_a + kb theta_b)} for charge t of the tile, from one
 // sine / cosine
 static __device__ __forceinline__ void sfac_w(const double *lds, int t, double ka, double kb,
@@ -963,13 +1001,15 @@ static __device__ __forceinline__ void sfac_w(const double *lds, int t, double k
   *wr = cq[3] * cs;
   *wi = cq[3] * sn;
 }
-template <int M>
-static __device__ __forceinline__ void sfac_run(const EwSfacArgs &A, double *lds, int chunk,
-                                                const EwSym &y, bool live) {
+template <int R, int M>
+static __device__ __forceinline__ void sfac_run(const EwSfacArgs &A, double *lds, double *ldsb,
+                                                int chunk, const EwSym &y, bool live) {
+  static_assert(R * M <= NB_EW_SYM, "a lane holds NB_EW_SYM (row, m) accumulators");
+  constexpr int RM = R * M;
   const int lane = lane_id(), w = threadIdx.x >> 6;
   double ar[NB_EW_SYM], ai[NB_EW_SYM], br[NB_EW_SYM], bi[NB_EW_SYM];
 #pragma unroll
-  for (int m = 0; m < M; m++) { ar[m] = 0.; ai[m] = 0.; br[m] = 0.; bi[m] = 0.; }
+  for (int m = 0; m < RM; m++) { ar[m] = 0.; ai[m] = 0.; br[m] = 0.; bi[m] = 0.; }
   const double ka = y.ka, kb = y.kb;
   const int mb = y.m0 % NB_EW_SYM, m16 = y.m0 / NB_EW_SYM;
   const int c0 = chunk * A.qc, c1 = min(A.nq, c0 + A.qc);
@@ -982,6 +1022,7 @@ static __device__ __forceinline__ void sfac_run(const EwSfacArgs &A, double *lds
       double *d = lds + 8 * t;
       d[0] = r.x; d[1] = r.y; d[2] = r.z; d[3] = r.w;
       d[4] = st.x; d[5] = st.y; d[6] = st.z; d[7] = st.w;
+      if constexpr (R > 1) nb_sincos(r.y, &ldsb[2 * t + 1], &ldsb[2 * t]);  // e^{i theta_b}
     }
     __syncthreads();
     int t = w;
@@ -990,13 +1031,19 @@ static __device__ __forceinline__ void sfac_run(const EwSfacArgs &A, double *lds
     for (; t + 4 < nt; t += 8) {  // two charges per trip for ILP
       sfac_w(lds, t, ka, kb, &wr, &wi);
       sfac_w(lds, t + 4, ka, kb, &wr2, &wi2);
-      sfac_charge<M>(lds + 8 * t, wr, wi, y.m0, m16, mb, ar, ai, br, bi);
-      sfac_charge<M>(lds + 8 * (t + 4), wr2, wi2, y.m0, m16, mb, ar, ai, br, bi);
+      if constexpr (R == 1) {
+        sfac_charge<M>(lds + 8 * t, wr, wi, y.m0, m16, mb, ar, ai, br, bi);
+        sfac_charge<M>(lds + 8 * (t + 4), wr2, wi2, y.m0, m16, mb, ar, ai, br, bi);
+      } else {
+        sfac_charge_rows<R, M>(lds + 8 * t, ldsb + 2 * t, wr, wi, ar, ai, br, bi);
+        sfac_charge_rows<R, M>(lds + 8 * (t + 4), ldsb + 2 * (t + 4), wr2, wi2, ar, ai, br, bi);
+      }
     }
 #endif
     for (; t < nt; t += 4) {
       sfac_w(lds, t, ka, kb, &wr, &wi);
-      sfac_charge<M>(lds + 8 * t, wr, wi, y.m0, m16, mb, ar, ai, br, bi);
+      if constexpr (R == 1) sfac_charge<M>(lds + 8 * t, wr, wi, y.m0, m16, mb, ar, ai, br, bi);
+      else sfac_charge_rows<R, M>(lds + 8 * t, ldsb + 2 * t, wr, wi, ar, ai, br, bi);
     }
   }
   // ((w3 + w2) + w1) + w0, one wave's sums at a time through the same LDS
@@ -1006,7 +1053,7 @@ static __device__ __forceinline__ void sfac_run(const EwSfacArgs &A, double *lds
     __syncthreads();
     if (w == src) {
 #pragma unroll
-      for (int m = 0; m < M; m++) {
+      for (int m = 0; m < RM; m++) {
         if (src < 3) {
           ar[m] += red[4 * m][lane]; ai[m] += red[4 * m + 1][lane];
           br[m] += red[4 * m + 2][lane]; bi[m] += red[4 * m + 3][lane];
@@ -1020,15 +1067,19 @@ static __device__ __forceinline__ void sfac_run(const EwSfacArgs &A, double *lds
   if (w == 0 && live) {
     double2 *out = A.part + (size_t)chunk * A.ktot;
 #pragma unroll
-    for (int m = 0; m < M; m++) {
-      if (m < y.cnt) {
-        const double sar = ar[m] + red[4 * m][lane], sai = ai[m] + red[4 * m + 1][lane];
-        const double sbr = br[m] + red[4 * m + 2][lane], sbi = bi[m] + red[4 * m + 3][lane];
-        const int mm = y.m0 + m;
-        if (mm >= y.kc_lo && mm <= y.kc_hi)
-          out[y.k_lo + mm - y.kc_lo] = make_double2(sar - sbi, sai + sbr);
-        if (mm > 0 && -mm >= y.kc_lo && -mm <= y.kc_hi)
-          out[y.k_lo - mm - y.kc_lo] = make_double2(sar + sbi, sai - sbr);
+    for (int r = 0; r < R; r++) {
+#pragma unroll
+      for (int m = 0; m < M; m++) {
+        const int q = r * M + m;
+        if (m < y.cnt) {
+          const double sar = ar[q] + red[4 * q][lane], sai = ai[q] + red[4 * q + 1][lane];
+          const double sbr = br[q] + red[4 * q + 2][lane], sbi = bi[q] + red[4 * q + 3][lane];
+          const int mm = y.m0 + m;
+          if (mm >= y.kc_lo[r] && mm <= y.kc_hi[r])
+            out[y.k_lo[r] + mm - y.kc_lo[r]] = make_double2(sar - sbi, sai + sbr);
+          if (mm > 0 && -mm >= y.kc_lo[r] && -mm <= y.kc_hi[r])
+            out[y.k_lo[r] - mm - y.kc_lo[r]] = make_double2(sar + sbi, sai - sbr);
+        }
       }
     }
   }
@@ -1038,24 +1089,32 @@ static __device__ __forceinline__ void k_ewsfac_body(const EwSfacArgs &A, int bi
   // staged through LDS in tiles and each wave takes every 4th charge; the 4
   // waves' sums are then combined in LDS in a fixed order
   __shared__ double lds[EW_SFAC_LDS];
+  __shared__ double ldsb[2 * EW_QT];  // e^{i theta_b} of the tile's charges (lanes of several rows)
   static_assert(8 * EW_QT <= EW_SFAC_LDS && 4 * NB_EW_SYM * NB_WAVE <= EW_SFAC_LDS,
                 "charge tile or reduction exceeds the LDS buffer");
-  static_assert(NB_EW_SYM >= 2 && NB_EW_SYM <= 16, "the dispatch below covers runs of 1..16 values of m");
+  static_assert(NB_EW_SYM == 16, "the dispatch below covers lanes of 16 (row, m) slots");
   const int lane = lane_id();
   const int sw = bid % A.nsw, chunk = bid / A.nsw;
   const int g = sw * NB_WAVE + lane;
-  EwSym y = {0, 1, 0, 0, 0, 0, 0, 0};  // no k: nothing stored
+  EwSym y;
+  y.ka = y.kb = y.m0 = y.cnt = 0;
+  y.nr = 1;
+  for (int q = 0; q < 4; q++) { y.k_lo[q] = 0; y.kc_lo[q] = 1; y.kc_hi[q] = 0; }  // no k: nothing stored
   const bool live = g < A.nsym;
   if (live) y = A.sym[g];
-  // lane 0 (always a real lane) has the longest run; the 4 waves of the block
-  // share the lanes, so the trip count is uniform over the block
-  switch (__builtin_amdgcn_readfirstlane(y.cnt)) {
-#define EW_SFAC_CASE(M) \
-  case M: sfac_run<M>(A, lds, chunk, y, live); break;
-    EW_SFAC_CASE(1) EW_SFAC_CASE(2) EW_SFAC_CASE(3) EW_SFAC_CASE(4)
-    EW_SFAC_CASE(5) EW_SFAC_CASE(6) EW_SFAC_CASE(7) EW_SFAC_CASE(8)
-    EW_SFAC_CASE(9) EW_SFAC_CASE(10) EW_SFAC_CASE(11) EW_SFAC_CASE(12)
-    EW_SFAC_CASE(13) EW_SFAC_CASE(14) EW_SFAC_CASE(15) EW_SFAC_CASE(16)
+  // lane 0 (always a real lane) has the system's row count and the group's
+  // longest run; the 4 waves of the block share the lanes, so the trip
+  // counts are uniform over the block
+  switch (__builtin_amdgcn_readfirstlane(y.nr * 32 + y.cnt)) {
+#define EW_SFAC_CASE(R, M) \
+  case R * 32 + M: sfac_run<R, M>(A, lds, ldsb, chunk, y, live); break;
+    EW_SFAC_CASE(1, 1) EW_SFAC_CASE(1, 2) EW_SFAC_CASE(1, 3) EW_SFAC_CASE(1, 4)
+    EW_SFAC_CASE(1, 5) EW_SFAC_CASE(1, 6) EW_SFAC_CASE(1, 7) EW_SFAC_CASE(1, 8)
+    EW_SFAC_CASE(1, 9) EW_SFAC_CASE(1, 10) EW_SFAC_CASE(1, 11) EW_SFAC_CASE(1, 12)
+    EW_SFAC_CASE(1, 13) EW_SFAC_CASE(1, 14) EW_SFAC_CASE(1, 15) EW_SFAC_CASE(1, 16)
+    EW_SFAC_CASE(2, 1) EW_SFAC_CASE(2, 2) EW_SFAC_CASE(2, 3) EW_SFAC_CASE(2, 4)
+    EW_SFAC_CASE(2, 5) EW_SFAC_CASE(2, 6) EW_SFAC_CASE(2, 7) EW_SFAC_CASE(2, 8)
+    EW_SFAC_CASE(4, 1) EW_SFAC_CASE(4, 2) EW_SFAC_CASE(4, 3) EW_SFAC_CASE(4, 4)
 #undef EW_SFAC_CASE
     default: break;
   }

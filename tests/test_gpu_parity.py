@@ -234,6 +234,29 @@ def test_sharing_changes_only_the_work_split():
     assert _rel(out[1][2]["vk_ewald"], out[0][2]["vk_ewald"]) < 1e-13
 
 
+@pytest.mark.parametrize("name", ["mix600", "mix600_tri"])
+def test_multirow_structure_factor_lanes(name):
+    """With every k row short (kmax 7: <= 8 values of |kc|) a context that
+    shares the GPU gives each structure-factor lane two rows of consecutive
+    kb, the second row's phase stepped from the first's by e^{i theta_b};
+    the reciprocal energy, virial and forces stay within the FP64 bar of
+    cm3d's, and within rounding of the one-row lanes of a lone context."""
+    d = load_fixture(name)
+    from cm3d_amd import nb
+    out = []
+    for share in (1, 16):
+        c = nb.NBContext(d, precision="fp64", sharing=share)
+        c.build_lists(d["pos"], d["hmat"])
+        out.append(c.force(d["pos"], d["hmat"], iflg=1))
+        c.close()
+    fs, fl, e = out[1]
+    assert rel_rms(fl, d["fxl"]) < FP64_TOL, rel_rms(fl, d["fxl"])
+    assert _rel(e["vk_ewald"], d["vk_ewald"][0]) < FP64_TOL
+    assert np.allclose(e["vp_ewald_tensor"], d["vp_ewald_tensor"], rtol=1e-10, atol=1e-8)
+    assert rel_rms(fl, out[0][1]) < 1e-13
+    assert _rel(e["vk_ewald"], out[0][2]["vk_ewald"]) < 1e-13
+
+
 @pytest.mark.parametrize("sharing", [1, 16])
 @pytest.mark.parametrize("natoms", [1001, 1002, 1003])
 def test_mixed_short_lists_partial_groups(natoms, sharing):
