@@ -9,6 +9,8 @@
 #include <mutex>
 #include <vector>
 
+#include <cstdlib>
+
 #include "nb_internal.h"
 
 static thread_local char g_err[512];
@@ -203,6 +205,11 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
   while ((1 << c->ibits) < n) c->ibits++;
   c->imask = (1u << c->ibits) - 1u;
   c->etype = c->ibits < 28 && p->ntypes <= (1 << (28 - c->ibits));
+  // CM3D_NB_ENTRY_TYPE=0 keeps the types out of the entries, as when they
+  // do not fit, so that the kernels' gather of partner types can be tested
+  // on small systems
+  if (const char *ev = getenv("CM3D_NB_ENTRY_TYPE"))
+    if (ev[0] == '0') c->etype = 0;
   c->iperd = p->iperd; c->prec = p->precision; c->skin = p->skin;
   c->kappa = p->kappa; c->ewald_self = p->ewald_self; c->ewald_bkg = p->ewald_bkg;
   c->dx2coul = p->dx2coul; c->rmins_coul2 = p->rmins_coul2; c->rmaxs_coul2 = p->rmaxs_coul2;

@@ -1,3 +1,4 @@
+import ctypes as C
 import glob
 import os
 import sys
@@ -37,3 +38,36 @@ def rel_rms(a, b, floor=1.0):
     b = np.asarray(b, dtype=np.float64)
     den = max(float(np.sqrt(np.mean(b * b))), floor)
     return float(np.sqrt(np.mean((a - b) ** 2)) / den)
+
+
+H2D, D2H = 1, 2  # hipMemcpyHostToDevice, hipMemcpyDeviceToHost
+
+
+def hip_runtime():
+    """The HIP runtime the library itself runs on (device buffers for the
+    GPU tests; no second runtime in the process)."""
+    from cm3d_amd import nb
+    nb.lib()
+    h = C.CDLL("libamdhip64.so")
+    h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    h.hipFree.argtypes = [C.c_void_p]
+    return h
+
+
+class DevBuf:
+    """A device copy of a float64 host array."""
+
+    def __init__(self, h, host):
+        self.h, self.n = h, host.nbytes
+        self.p = C.c_void_p()
+        assert h.hipMalloc(C.byref(self.p), self.n) == 0
+        assert h.hipMemcpy(self.p, host.ctypes.data, self.n, H2D) == 0
+
+    def get(self):
+        out = np.empty(self.n // 8)
+        assert self.h.hipMemcpy(out.ctypes.data, self.p, self.n, D2H) == 0
+        return out
+
+    def free(self):
+        self.h.hipFree(self.p)

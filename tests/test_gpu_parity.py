@@ -234,6 +234,39 @@ def test_sharing_changes_only_the_work_split():
     assert _rel(out[1][2]["vk_ewald"], out[0][2]["vk_ewald"]) < 1e-13
 
 
+@pytest.mark.parametrize("name", ["water512", "mix600", "ref_abutane"])
+def test_partner_types_gathered_when_entries_lack_them(name, monkeypatch):
+    """List entries carry the partner's type above its index when the type
+    count fits the spare bits; otherwise the forces-only kernels of the MD
+    step gather the type.  Forcing the second layout on small systems
+    (CM3D_NB_ENTRY_TYPE=0) gives the same lists and bitwise the same
+    forces-only forces (nb_force_device without energies) in both
+    precisions."""
+    from conftest import DevBuf, hip_runtime
+    from cm3d_amd import nb
+    d = load_fixture(name)
+    h = hip_runtime()
+    pos = np.ascontiguousarray(d["pos"], dtype=np.float64)
+    for prec in ("fp64", "mixed"):
+        got = []
+        for flag in ("1", "0"):
+            monkeypatch.setenv("CM3D_NB_ENTRY_TYPE", flag)
+            c = nb.NBContext(d, precision=prec)
+            c.build_lists(d["pos"], d["hmat"])
+            pairs = [c.get_pairs(t) for t, _, _ in LIST_KEYS]
+            d_pos, fl = DevBuf(h, pos), DevBuf(h, np.full_like(pos, np.nan))
+            c.force_device(1, d_pos.p.value, d["hmat"], 0, fl.p.value)
+            got.append((pairs, fl.get()))
+            d_pos.free()
+            fl.free()
+            c.close()
+        monkeypatch.delenv("CM3D_NB_ENTRY_TYPE")
+        for (i0, j0), (i1, j1) in zip(got[0][0], got[1][0]):
+            assert np.array_equal(i0, i1) and np.array_equal(j0, j1), (name, prec)
+        assert np.isfinite(got[0][1]).all()
+        assert np.array_equal(got[0][1], got[1][1]), (name, prec)
+
+
 @pytest.mark.parametrize("name", ["mix600", "mix600_tri"])
 def test_multirow_structure_factor_lanes(name):
     """With every k row short (kmax 7: <= 8 values of |kc|) a context that

@@ -3,45 +3,12 @@ device allocation) against nb_force_device, through the C ABI: forces and
 every energy field bit for bit, on two calls in a row, the second on a
 smaller system, in both precisions and at both RESPA levels; a system whose
 scratch exceeds the static area is refused with an error on the host."""
-import ctypes as C
-
 import numpy as np
 import pytest
 
-from conftest import load_fixture
+from conftest import DevBuf as _Dev, hip_runtime as _hip, load_fixture
 
 pytestmark = pytest.mark.gpu
-
-H2D, D2H = 1, 2  # hipMemcpyHostToDevice, hipMemcpyDeviceToHost
-
-
-def _hip():
-    """The HIP runtime the library itself runs on (device buffers for the
-    test; no second runtime in the process)."""
-    from cm3d_amd import nb
-    nb.lib()
-    h = C.CDLL("libamdhip64.so")
-    h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
-    h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
-    h.hipFree.argtypes = [C.c_void_p]
-    return h
-
-
-class _Dev:
-    def __init__(self, h, host):
-        self.h, self.n = h, host.nbytes
-        self.p = C.c_void_p()
-        assert h.hipMalloc(C.byref(self.p), self.n) == 0
-        assert h.hipMemcpy(self.p, host.ctypes.data, self.n, H2D) == 0
-
-    def get(self):
-        out = np.empty(self.n // 8)
-        assert self.h.hipMemcpy(out.ctypes.data, self.p, self.n, D2H) == 0
-        return out
-
-    def free(self):
-        self.h.hipFree(self.p)
-
 
 def _both(d, prec, iflg):
     from cm3d_amd import nb
