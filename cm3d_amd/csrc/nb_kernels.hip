@@ -26,6 +26,14 @@ static __device__ __forceinline__ int mbcnt(unsigned long long m) {
                                    __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
 }
 
+// record of sorted atom j (j < NB_MAX_ATOMS: a 32-bit byte offset)
+static __device__ __forceinline__ double4 rec64_at(const double4 *rec, uint32_t j) {
+  return *(const double4 *)((const char *)rec + (j << 5));
+}
+static __device__ __forceinline__ int type_at(const int *t, uint32_t j) {
+  return *(const int *)((const char *)t + (j << 2));
+}
+
 // Every list-build kernel takes `cond`: when non-null and *cond == 0 it
 // returns at once, so a device-side rebuild test can gate a captured build.
 #define NB_SKIP(cond) \
@@ -310,7 +318,7 @@ static __device__ __forceinline__ void k_list_body(const ListArgs &A, int bid) {
   bool rowv = false;
   for (int tj = 0; tj < A.ntypes; tj++)
     rowv |= (A.pcut_lds ? pc[ti * A.ntypes + tj].x : pcut[tj].x) > 0.;
-  int nv = 0, nc = 0, hs = 0, ht = 0, hse = 0, hte = 0;
+  int nv = 0, nc = 0;
   uint32_t *out = A.nbr + (size_t)s * A.cap;
   // The candidates are the concatenation, column by column, of each
   // column's runs; 64 consecutive candidates make a chunk whatever runs
@@ -335,39 +343,32 @@ static __device__ __forceinline__ void k_list_body(const ListArgs &A, int bid) {
   __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
   __builtin_amdgcn_wave_barrier();
   // a lane's cursor: col, the column of its next candidate, and colend,
-  // that column's end; it only moves forward (it stops at lane 63's entry,
-  // which is the total)
+  // that column's end; it only moves forward.  A lane past the total keeps
+  // its cursor on the last candidate (colrun[63] is the total, so the
+  // cursor stops by lane 63's entry) and gathers that candidate's record,
+  // unused: every lane gathers, no exec-mask change guards the loads.
   int col = 0, colend = lc[0];
-  int cb = 0;
-  double4 rjc = make_double4(0., 0., 0., 0.);
-  int tjc = 0, s2c = 0;
-  bool okc = false;
-  {
-    const int c = cb + lane;
-    if (c < total) {
-      while (colend <= c) colend = lc[++col];
-      const int cstart = col > 0 ? lc[col - 1] : 0;
-      const int loc = c - cstart, r0 = lc[2 * NB_WAVE + col];
-      s2c = loc < r0 ? lc[NB_WAVE + col] + loc : lc[3 * NB_WAVE + col] + (loc - r0);
-      okc = s2c != s;
-      if (okc) { rjc = A.rec[s2c]; tjc = A.stype[s2c]; }
-    }
-  }
-  for (; cb < total; cb += NB_WAVE) {
-    double4 rjn = make_double4(0., 0., 0., 0.);
-    int tjn = 0, s2n = 0;
-    bool okn = false;
-    {
-      const int c = cb + NB_WAVE + lane;
-      if (c < total) {
-        while (colend <= c) colend = lc[++col];
-        const int cstart = col > 0 ? lc[col - 1] : 0;
-        const int loc = c - cstart, r0 = lc[2 * NB_WAVE + col];
-        s2n = loc < r0 ? lc[NB_WAVE + col] + loc : lc[3 * NB_WAVE + col] + (loc - r0);
-        okn = s2n != s;
-        if (okn) { rjn = A.rec[s2n]; tjn = A.stype[s2n]; }
-      }
-    }
+  auto cand = [&](int c) {
+    const int cc = min(c, total - 1);
+    while (colend <= cc) colend = lc[++col];
+    const int cstart = col > 0 ? lc[col - 1] : 0;
+    const int loc = cc - cstart, r0 = lc[2 * NB_WAVE + col];
+    return loc < r0 ? lc[NB_WAVE + col] + loc : lc[3 * NB_WAVE + col] + (loc - r0);
+  };
+  int s2c = cand(lane);
+  bool okc = lane < total && s2c != s;
+  double4 rjc = rec64_at(A.rec, (uint32_t)s2c);
+  int tjc = type_at(A.stype, (uint32_t)s2c);
+  // half pairs (each once, by sorted index) counted from ballots: wave-wide
+  // totals, no per-lane sums
+  int hs = 0, ht = 0, hse = 0, hte = 0;
+#pragma unroll 2
+  for (int cb = 0; cb < total; cb += NB_WAVE) {
+    const int cn = cb + NB_WAVE + lane;
+    const int s2n = cand(cn);
+    const bool okn = cn < total && s2n != s;
+    const double4 rjn = rec64_at(A.rec, (uint32_t)s2n);
+    const int tjn = type_at(A.stype, (uint32_t)s2n);
     unsigned fl = 0;
     if (okc) {
       double dx = ri.x - rjc.x, dy = ri.y - rjc.y, dz = ri.z - rjc.z;
@@ -375,13 +376,20 @@ static __device__ __forceinline__ void k_list_body(const ListArgs &A, int bid) {
       const double d2 = dx * dx + dy * dy + dz * dz;
       const unsigned uj = (unsigned)s2c;
       unsigned ex = 0;
-      if ((ex0 & 0x3FFFFFFFu) == uj) ex |= ex0 >> 30;
-      if ((ex1 & 0x3FFFFFFFu) == uj) ex |= ex1 >> 30;
-      if ((ex2 & 0x3FFFFFFFu) == uj) ex |= ex2 >> 30;
-      if ((ex3 & 0x3FFFFFFFu) == uj) ex |= ex3 >> 30;
-      for (int e = x0 + 4; e < x1; e++) {
-        const unsigned v = xsorted(e);
-        if ((v & 0x3FFFFFFFu) == uj) ex |= v >> 30;
+      // only as many tests as the atom has exclusions (wave-uniform)
+      if (nx > 0) {
+        if ((ex0 & 0x3FFFFFFFu) == uj) ex |= ex0 >> 30;
+        if (nx > 1) {
+          if ((ex1 & 0x3FFFFFFFu) == uj) ex |= ex1 >> 30;
+          if (nx > 2) {
+            if ((ex2 & 0x3FFFFFFFu) == uj) ex |= ex2 >> 30;
+            if ((ex3 & 0x3FFFFFFFu) == uj) ex |= ex3 >> 30;
+            for (int e = x0 + 4; e < x1; e++) {
+              const unsigned v = xsorted(e);
+              if ((v & 0x3FFFFFFFu) == uj) ex |= v >> 30;
+            }
+          }
+        }
       }
       const double2 cut = A.pcut_lds ? pc[ti * A.ntypes + tjc] : pcut[tjc];
       if (!(ex & 1u) && d2 < cut.x) {  // null pairs carry -1: never true
@@ -409,15 +417,14 @@ static __device__ __forceinline__ void k_list_body(const ListArgs &A, int bid) {
         const int slot = A.cap - 1 - (nc + mbcnt(mc));
         if (slot >= 0) out[slot] = ent;
       }
-      // each unordered pair once (the filters are symmetric in i and j,
-      // so counting by sorted index gives cm3d's i < j totals)
-      if (s < s2c) {
-        hs += (fl & NB_FLAG_VS) != 0;
-        ht += (fl & NB_FLAG_VT) != 0;
-        hse += (fl & NB_FLAG_ES) != 0;
-        hte += (fl & NB_FLAG_ET) != 0;
-      }
     }
+    // each unordered pair once (the filters are symmetric in i and j, so
+    // counting by sorted index gives cm3d's i < j totals)
+    const unsigned hf = s < s2c ? fl : 0u;
+    hs += __popcll(__ballot((hf & NB_FLAG_VS) != 0));
+    ht += __popcll(__ballot((hf & NB_FLAG_VT) != 0));
+    hse += __popcll(__ballot((hf & NB_FLAG_ES) != 0));
+    hte += __popcll(__ballot((hf & NB_FLAG_ET) != 0));
     nv += __popcll(mv);
     nc += __popcll(mc);
     rjc = rjn; tjc = tjn; s2c = s2n; okc = okn;
@@ -439,10 +446,6 @@ static __device__ __forceinline__ void k_list_body(const ListArgs &A, int bid) {
       if (i < nc) out[nv + i] = v;
     }
   }
-  hs = nb_wave_sum(hs);
-  ht = nb_wave_sum(ht);
-  hse = nb_wave_sum(hse);
-  hte = nb_wave_sum(hte);
   if (lane == 0) {
     A.nnb[s] = count < A.cap ? count : A.cap;
     if (count > A.cap) {
@@ -697,14 +700,6 @@ static __device__ __forceinline__ void pair64_body(const PairArgs &A, int bid) {
 //     (c0 + p (c1 + p c2): cm3d's quadratic interpolation to rounding);
 //   - an orthorhombic box takes the diagonal minimum image.
 // ---------------------------------------------------------------------------
-// record of sorted atom j (j < NB_MAX_ATOMS: a 32-bit byte offset)
-static __device__ __forceinline__ double4 rec64_at(const double4 *rec, uint32_t j) {
-  return *(const double4 *)((const char *)rec + (j << 5));
-}
-static __device__ __forceinline__ int type_at(const int *t, uint32_t j) {
-  return *(const int *)((const char *)t + (j << 2));
-}
-
 template <bool TLDS, bool ORTHO>
 static __device__ __forceinline__ void pair64_fonly_run(const PairArgs &A, int bid,
                                                         const double *fc, const InterPar *sip,
