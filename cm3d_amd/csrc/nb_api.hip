@@ -383,6 +383,7 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
     for (int g = 0; g < c->nrow; g++) mmax = std::max(mmax, row[g].mhi + 1);
     const bool full = p->sharing > 1 || ncharge >= NB_SFAC_ROWS_MIN_Q;
     const int nr = !full ? 1 : (mmax <= NB_EW_SYM / 4 ? 4 : (mmax <= NB_EW_SYM / 2 ? 2 : 1));
+    c->sfac_rows = nr;
     for (int g = 0; g < c->nrow;) {
       EwSym y;
       memset(&y, 0, sizeof(y));

@@ -290,6 +290,7 @@ struct nb_ctx {
   int crx, cry, crz;           // cell reach of the list stencil per axis (0..3)
   double *d_chg, *d_stp, *d_spart, *d_kd, *d_fpart;
   int nchunk, qc, nsw;         // structure factor: charge chunks, waves of lanes
+  int sfac_rows;               // ... rows per lane (1: k_ewsfac; 2 or 4: k_ewsfacr)
   int nrange, nqb, ntile;      // force: tile ranges, charge workgroups, tiles
   double *d_epair, *d_eew, *d_eec, *d_eout;
   int npair_blocks, new_blocks, nfin_blocks;

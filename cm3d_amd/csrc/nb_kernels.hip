@@ -1079,34 +1079,57 @@ static __device__ __forceinline__ void sfac_run(const EwSfacArgs &A, double *lds
     }
   }
 }
-static __device__ __forceinline__ void k_ewsfac_body(const EwSfacArgs &A, int bid) {
-  // block = 4 waves on the same 64 lanes; the block's chunk of charges is
-  // staged through LDS in tiles and each wave takes every 4th charge; the 4
-  // waves' sums are then combined in LDS in a fixed order
-  __shared__ double lds[EW_SFAC_LDS];
-  __shared__ double ldsb[2 * EW_QT];  // e^{i theta_b} of the tile's charges (lanes of several rows)
-  static_assert(8 * EW_QT <= EW_SFAC_LDS && 4 * NB_EW_SYM * NB_WAVE <= EW_SFAC_LDS,
-                "charge tile or reduction exceeds the LDS buffer");
-  static_assert(NB_EW_SYM == 16, "the dispatch below covers lanes of 16 (row, m) slots");
+// block = 4 waves on the same 64 lanes; the block's chunk of charges is
+// staged through LDS in tiles and each wave takes every 4th charge; the 4
+// waves' sums are then combined in LDS in a fixed order.  Lane 0 (always a
+// real lane) has the group's longest run; the 4 waves of the block share the
+// lanes, so the trip counts are uniform over the block.  Lanes of one row
+// (k_ewsfac) and of several rows (k_ewsfacr) are separate kernels, so that
+// the one-row kernel keeps its smaller register and LDS footprint (its
+// waves then leave room on a SIMD for the list build and the pair kernel on
+// the MD step's other branch).
+static __device__ __forceinline__ EwSym sfac_lane(const EwSfacArgs &A, int bid, int *chunk,
+                                                  bool *live) {
   const int lane = lane_id();
-  const int sw = bid % A.nsw, chunk = bid / A.nsw;
+  const int sw = bid % A.nsw;
+  *chunk = bid / A.nsw;
   const int g = sw * NB_WAVE + lane;
   EwSym y;
   y.ka = y.kb = y.m0 = y.cnt = 0;
   y.nr = 1;
   for (int q = 0; q < 4; q++) { y.k_lo[q] = 0; y.kc_lo[q] = 1; y.kc_hi[q] = 0; }  // no k: nothing stored
-  const bool live = g < A.nsym;
-  if (live) y = A.sym[g];
-  // lane 0 (always a real lane) has the system's row count and the group's
-  // longest run; the 4 waves of the block share the lanes, so the trip
-  // counts are uniform over the block
+  *live = g < A.nsym;
+  if (*live) y = A.sym[g];
+  return y;
+}
+static __device__ __forceinline__ void k_ewsfac_body(const EwSfacArgs &A, int bid) {
+  __shared__ double lds[EW_SFAC_LDS];
+  static_assert(8 * EW_QT <= EW_SFAC_LDS && 4 * NB_EW_SYM * NB_WAVE <= EW_SFAC_LDS,
+                "charge tile or reduction exceeds the LDS buffer");
+  static_assert(NB_EW_SYM == 16, "the dispatch below covers runs of 1..16 values of m");
+  int chunk;
+  bool live;
+  const EwSym y = sfac_lane(A, bid, &chunk, &live);
+  switch (__builtin_amdgcn_readfirstlane(y.cnt)) {
+#define EW_SFAC_CASE(M) \
+  case M: sfac_run<1, M>(A, lds, nullptr, chunk, y, live); break;
+    EW_SFAC_CASE(1) EW_SFAC_CASE(2) EW_SFAC_CASE(3) EW_SFAC_CASE(4)
+    EW_SFAC_CASE(5) EW_SFAC_CASE(6) EW_SFAC_CASE(7) EW_SFAC_CASE(8)
+    EW_SFAC_CASE(9) EW_SFAC_CASE(10) EW_SFAC_CASE(11) EW_SFAC_CASE(12)
+    EW_SFAC_CASE(13) EW_SFAC_CASE(14) EW_SFAC_CASE(15) EW_SFAC_CASE(16)
+#undef EW_SFAC_CASE
+    default: break;
+  }
+}
+static __device__ __forceinline__ void k_ewsfacr_body(const EwSfacArgs &A, int bid) {
+  __shared__ double lds[EW_SFAC_LDS];
+  __shared__ double ldsb[2 * EW_QT];  // e^{i theta_b} of the tile's charges
+  int chunk;
+  bool live;
+  const EwSym y = sfac_lane(A, bid, &chunk, &live);
   switch (__builtin_amdgcn_readfirstlane(y.nr * 32 + y.cnt)) {
 #define EW_SFAC_CASE(R, M) \
   case R * 32 + M: sfac_run<R, M>(A, lds, ldsb, chunk, y, live); break;
-    EW_SFAC_CASE(1, 1) EW_SFAC_CASE(1, 2) EW_SFAC_CASE(1, 3) EW_SFAC_CASE(1, 4)
-    EW_SFAC_CASE(1, 5) EW_SFAC_CASE(1, 6) EW_SFAC_CASE(1, 7) EW_SFAC_CASE(1, 8)
-    EW_SFAC_CASE(1, 9) EW_SFAC_CASE(1, 10) EW_SFAC_CASE(1, 11) EW_SFAC_CASE(1, 12)
-    EW_SFAC_CASE(1, 13) EW_SFAC_CASE(1, 14) EW_SFAC_CASE(1, 15) EW_SFAC_CASE(1, 16)
     EW_SFAC_CASE(2, 1) EW_SFAC_CASE(2, 2) EW_SFAC_CASE(2, 3) EW_SFAC_CASE(2, 4)
     EW_SFAC_CASE(2, 5) EW_SFAC_CASE(2, 6) EW_SFAC_CASE(2, 7) EW_SFAC_CASE(2, 8)
     EW_SFAC_CASE(4, 1) EW_SFAC_CASE(4, 2) EW_SFAC_CASE(4, 3) EW_SFAC_CASE(4, 4)
@@ -1118,6 +1141,7 @@ static __device__ __forceinline__ void k_ewsfac_body(const EwSfacArgs &A, int bi
 #define NB_SFAC_WPE 1
 #endif
 NB_KERNEL_PAIR_W(k_ewsfac, EwSfacArgs, 256, NB_SFAC_WPE)
+NB_KERNEL_PAIR_W(k_ewsfacr, EwSfacArgs, 256, NB_SFAC_WPE)
 
 // per k: 2 preg S(k), the amplitude the force pass needs
 struct KData { double cr, ci; };
@@ -1538,7 +1562,12 @@ int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_f
   nb_time_mark(c, 3, s);
   EwSfacArgs sf = {c->d_sym, (const double4 *)c->d_chg, (const double4 *)c->d_stp, nq, c->qc,
                    c->nsym, c->nsw, c->ktot, (double2 *)c->d_spart};
-  NB_GO(c, K_EWSFAC, k_ewsfac, rp && ew ? c->nsw * c->nchunk : 0, 256, 0, sf, s);
+  // both kernels are always recorded, the unused one with no workgroups,
+  // so that systems of either lane kind record the same step (a batch
+  // launches each kernel once for the systems that give it work)
+  const int nsfac = rp && ew ? c->nsw * c->nchunk : 0;
+  NB_GO(c, K_EWSFAC, k_ewsfac, c->sfac_rows == 1 ? nsfac : 0, 256, 0, sf, s);
+  NB_GO(c, K_EWSFACR, k_ewsfacr, c->sfac_rows > 1 ? nsfac : 0, 256, 0, sf, s);
   nb_time_mark(c, 4, s);
   double det = b.h[0] * (b.h[4] * b.h[8] - b.h[5] * b.h[7]) +
                b.h[1] * (b.h[5] * b.h[6] - b.h[3] * b.h[8]) +
@@ -1594,6 +1623,7 @@ static struct RegisterKernels {
     g_batch_fn[K_EWSFAC] = k_ewsfac_bl;
     g_batch_fn[K_EWKRED] = k_ewkred_bl;
     g_batch_fn[K_EWFORCE] = k_ewforce_bl;
+    g_batch_fn[K_EWSFACR] = k_ewsfacr_bl;
     g_batch_fn[K_EWFIN] = k_ewfin_bl;
   }
 } g_register_kernels;
