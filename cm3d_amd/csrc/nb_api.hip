@@ -123,7 +123,7 @@ extern "C" void nb_destroy(nb_ctx *c) {
   hipDeviceSynchronize();
   void *ptrs[] = {c->d_itype, c->d_map, c->d_ip, c->d_vtab3, c->d_ctab3, c->d_vtab3f,
                   c->d_ctab3f, c->d_qch, c->d_xoff, c->d_xlist, c->d_eoff, c->d_elist,
-                  c->d_qidx, c->d_qpos, c->d_row, c->d_sym, c->d_ent, c->d_trow, c->d_rtile, c->d_fpart, c->d_kabc, c->d_pos, c->d_posb, c->d_fs, c->d_fl,
+                  c->d_qidx, c->d_qpos, c->d_row, c->d_sym, c->d_rowm, c->d_sfmtab, c->d_ent, c->d_trow, c->d_rtile, c->d_fpart, c->d_kabc, c->d_pos, c->d_posb, c->d_fs, c->d_fl,
                   c->d_perm, c->d_iperm, c->d_cell, c->d_ccount, c->d_cstart, c->d_cfill, c->d_cwork,
                   c->d_hcnt, c->d_pcut,
                   c->d_tmp, c->d_rec64, c->d_stype, c->d_nbr, c->d_nnb, c->d_flags,
@@ -439,6 +439,69 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
     c->qc = std::max(1, (ncharge + nch - 1) / nch);
     c->nchunk = std::max(1, (ncharge + c->qc - 1) / c->qc);
   }
+  // The structure factor as matrix products on the FP64 matrix cores
+  // (k_ewsfacm), for systems whose lanes hold one row: per ka plane it is
+  // [Re w; Im w] (rows x charges) times [cos m theta_c, sin m theta_c]
+  // (charges x m), with w = q e^{i (ka theta_a + kb theta_b)} from per-charge
+  // phase tables, so neither the per-lane sine / cosine nor the per-entry
+  // recurrence of the VALU kernel remains.  Rows in tiles of 8; a wave
+  // takes NB_SFM_R tiles and every charge of its chunk.  CM3D_NB_SFAC_MFMA=0 / 1 turns it off / on for any system it
+  // can serve (for tests and A/B timing).
+  std::vector<EwRowM> rowm;
+  {
+    int mmax = -1, kamin = 0, kamax = 0, kbmax = 0;
+    for (const EwRow &r : row) {
+      mmax = std::max(mmax, r.mhi);
+      kamin = std::min(kamin, r.ka); kamax = std::max(kamax, r.ka);
+      kbmax = std::max(kbmax, std::abs(r.kb));
+    }
+    const bool can = c->sfac_rows == 1 && c->nrow > 0 && ncharge > 0 && kamin >= 0 &&
+                     mmax < 8 * NB_SFM_NT && kamax < NB_SFM_TAB && kbmax < NB_SFM_TAB;
+    bool use = can && (long long)ncharge * c->ktot >= NB_SFM_MIN_WORK;
+    if (const char *ev = getenv("CM3D_NB_SFAC_MFMA")) use = can && ev[0] != '0';
+    c->sfm = use;
+    if (use) {
+      // tiles: runs of <= 8 rows, each row the previous one's (ka, kb + 1),
+      // so that a tile's 8 rows read one a-table entry and 8 consecutive
+      // b-table entries of a charge (no LDS bank conflicts); tiles sorted by
+      // their longest m range, longest first
+      std::vector<std::vector<int>> tl;
+      for (int g = 0; g < c->nrow; g++) {
+        if (tl.empty() || !row[g].chain || tl.back().size() == 8) tl.emplace_back();
+        tl.back().push_back(g);
+      }
+      std::vector<int> tmhi(tl.size());
+      for (size_t t = 0; t < tl.size(); t++) {
+        tmhi[t] = -1;
+        for (int g : tl[t]) tmhi[t] = std::max(tmhi[t], row[g].mhi);
+      }
+      std::vector<int> ord(tl.size());
+      for (size_t t = 0; t < tl.size(); t++) ord[t] = (int)t;
+      std::stable_sort(ord.begin(), ord.end(),
+                       [&](int a, int b) { return (tmhi[a] >> 3) > (tmhi[b] >> 3); });
+      const int ntile = (int)tl.size();
+      const int tpw = NB_SFM_R * 4;  // row tiles a workgroup
+      c->sfm_nrwg = (ntile + tpw - 1) / tpw;
+      rowm.assign((size_t)c->sfm_nrwg * tpw * 8, EwRowM{0, 0, 0, 1, 0, -1, -1, 0});  // no k
+      for (int t = 0; t < ntile; t++) {
+        const std::vector<int> &rs = tl[ord[t]];
+        for (size_t q = 0; q < rs.size(); q++) {
+          const EwRow &r = row[rs[q]];
+          rowm[(size_t)t * 8 + q] = EwRowM{r.ka, r.kb, r.k_lo, r.kc_lo, r.kc_hi, r.mhi,
+                                           tmhi[ord[t]], 0};
+        }
+      }
+      c->sfm_la = kamax + 1;
+      c->sfm_lb = kbmax + 1;
+      c->sfm_lz = 8 * ((mmax + 8) / 8);
+      // chunks of whole staged tiles for ~NB_SFM_WG / share workgroups
+      const int nq16 = (ncharge + NB_SFM_NQ - 1) / NB_SFM_NQ;
+      int nch = std::max(1, std::max(1, NB_SFM_WG / share) / c->sfm_nrwg);
+      nch = std::min(nch, std::max(1, nq16 / 4));  // at least 4 tiles a chunk
+      c->qc = NB_SFM_NQ * ((nq16 + nch - 1) / nch);
+      c->nchunk = std::max(1, (ncharge + c->qc - 1) / c->qc);
+    }
+  }
   // force: NB_EWF_Q charges per lane, a workgroup per (256 NB_EWF_Q
   // charges, tile range); tiles are runs of whole rows small enough for LDS
   // and for enough ranges to give ~NB_EWF_WG workgroups (each range's
@@ -496,6 +559,8 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
   if ((rc = upload(c, &c->d_qpos, qpos.data(), qpos.size()))) return fail(rc);
   if ((rc = upload(c, &c->d_row, row.data(), row.size()))) return fail(rc);
   if ((rc = upload(c, &c->d_sym, sym.data(), sym.size()))) return fail(rc);
+  if (c->sfm && (rc = upload(c, &c->d_rowm, rowm.data(), rowm.size()))) return fail(rc);
+  if (c->sfm && (rc = dalloc(c, &c->d_sfmtab, nb_sfm_table_doubles(c)))) return fail(rc);
   if ((rc = upload(c, &c->d_ent, ent.data(), ent.size()))) return fail(rc);
   if ((rc = upload(c, &c->d_trow, trow.data(), trow.size()))) return fail(rc);
   if ((rc = upload(c, &c->d_rtile, rtile.data(), rtile.size()))) return fail(rc);
@@ -904,19 +969,20 @@ extern "C" int nb_force_device(nb_ctx *c, int iflg, const double *d_pxyz, const 
 // columns) carved from the static scratch area instead of the context's own
 // buffers; allocates nothing.  Every kernel overwrites the scratch it reads
 // before reading it, so nothing in it has to start at zero.
-static size_t scratch_layout(const nb_ctx *c, size_t off[10]) {
+static size_t scratch_layout(const nb_ctx *c, size_t off[11]) {
   const size_t nq = std::max(c->ncharge, 1), kt = std::max(c->ktot, 1);
-  const size_t bytes[9] = {32 * nq, 32 * nq, 16 * (size_t)c->nchunk * kt, nb_kdata_bytes() * kt,
-                           24 * (size_t)c->nrange * nq,
-                           8 * (size_t)c->npair_blocks * NB_NE_PAIRQ,
-                           8 * (size_t)c->nfin_blocks * NB_NE_ECORR,
-                           8 * (size_t)c->new_blocks * NB_NE_EW, 8 * 64};
+  const size_t bytes[10] = {32 * nq, 32 * nq, 16 * (size_t)c->nchunk * kt, nb_kdata_bytes() * kt,
+                            24 * (size_t)c->nrange * nq,
+                            8 * (size_t)c->npair_blocks * NB_NE_PAIRQ,
+                            8 * (size_t)c->nfin_blocks * NB_NE_ECORR,
+                            8 * (size_t)c->new_blocks * NB_NE_EW, 8 * 64,
+                            c->sfm ? 8 * nb_sfm_table_doubles(c) : 0};
   size_t o = 0;
-  for (int k = 0; k < 9; k++) {
+  for (int k = 0; k < 10; k++) {
     off[k] = o;
     o += (bytes[k] + 255) & ~(size_t)255;
   }
-  off[9] = o;
+  off[10] = o;
   return o;
 }
 
@@ -931,7 +997,7 @@ extern "C" int nb_force_device_noalloc(nb_ctx *c, int iflg, const double *d_pxyz
     nb_set_error("nb_force_device_noalloc: iflg must be 0 or 1");
     return NB_ERR_ARG;
   }
-  size_t off[10];
+  size_t off[11];
   const size_t need = scratch_layout(c, off);
   if (need > NB_SCRATCH_BYTES) {
     nb_set_error("nb_force_device_noalloc: this system needs %zu bytes of scratch, the static "
@@ -960,6 +1026,7 @@ extern "C" int nb_force_device_noalloc(nb_ctx *c, int iflg, const double *d_pxyz
   v.d_eec = (double *)(base + off[6]);
   v.d_eew = (double *)(base + off[7]);
   v.d_eout = (double *)(base + off[8]);
+  if (c->sfm) v.d_sfmtab = (double *)(base + off[9]);
   v.timing = 0;
   v.rec = nullptr;
   Box b = make_box(hmat);

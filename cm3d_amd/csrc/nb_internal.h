@@ -73,6 +73,18 @@ struct LaunchRec;
 #ifndef NB_SFAC_WORK
 #define NB_SFAC_WORK 65536  // the same for a structure-factor workgroup
 #endif
+#ifndef NB_SFM_NT
+#define NB_SFM_NT 5  // MFMA structure factor: m tiles of 8 a row tile may need (m < 40)
+#endif
+#define NB_SFM_NQ 8     // ... charges per staged tile (2 matrix-core K steps of 4)
+#define NB_SFM_R 4      // ... row tiles (8 rows) per wave
+#define NB_SFM_TAB 40   // ... |ka|, |kb| below this (phase-table length)
+#ifndef NB_SFM_WG
+#define NB_SFM_WG 1024  // ... workgroups it aims for
+#endif
+#ifndef NB_SFM_MIN_WORK
+#define NB_SFM_MIN_WORK (1LL << 30)  // ... charges x k vectors from which it is used
+#endif
 #ifndef NB_MIX_G16_MAX
 #define NB_MIX_G16_MAX 256  // mixed pair kernel: 16 lanes per atom below this mean list length
 #endif
@@ -115,6 +127,17 @@ struct EwRow {
   int e0;            // its first force-pass entry (one per m)
   int chain;         // the previous row is (ka, kb - 1): its phase follows by one kb step
 };
+// A row of the MFMA structure factor, in its tile order
+struct EwRowM { int ka, kb, k_lo, kc_lo, kc_hi, mhi, tmhi, pad; };  // tmhi: its tile's longest m
+// Doubles per charge of the MFMA structure factor's tables: q e^{i ka theta_a}
+// as {re, -im, im, re} for ka = 0..la-1, e^{i kb theta_b} for kb = -(lb-1)
+// .. lb-1, e^{i m theta_c} for m = 0..lz-1; odd in 16-byte units so that
+// consecutive charges start in different LDS banks
+__host__ __device__ static inline int nb_sfm_stride(int la, int lb, int lz) {
+  int s = 4 * la + 2 * (2 * lb - 1) + 2 * lz;
+  if ((s / 2) % 2 == 0) s += 2;
+  return s;
+}
 struct EwSym {       // a structure-factor lane: nr rows (ka, kb .. kb + nr - 1), cnt values of m from m0
   int ka, kb, m0, cnt, nr, pad0, pad1, pad2;
   int k_lo[4], kc_lo[4], kc_hi[4];  // per row: k index of kc_lo, its kc range
@@ -291,6 +314,11 @@ struct nb_ctx {
   double *d_chg, *d_stp, *d_spart, *d_kd, *d_fpart;
   int nchunk, qc, nsw;         // structure factor: charge chunks, waves of lanes
   int sfac_rows;               // ... rows per lane (1: k_ewsfac; 2 or 4: k_ewsfacr)
+  int sfm;                     // ... as matrix products (k_ewsfacm) instead
+  int sfm_la, sfm_lb, sfm_lz;  // ... its phase-table lengths
+  int sfm_nrwg;                // ... workgroups per chunk
+  EwRowM *d_rowm;              // ... its rows, sorted, in tiles of 8
+  double *d_sfmtab;            // ... per-charge phase tables (charges padded to whole tiles)
   int nrange, nqb, ntile;      // force: tile ranges, charge workgroups, tiles
   double *d_epair, *d_eew, *d_eec, *d_eout;
   int npair_blocks, new_blocks, nfin_blocks;
@@ -317,6 +345,13 @@ struct nb_ctx {
   // when set, launch sites append to this list instead of launching
   std::vector<LaunchRec> *rec;
 };
+
+// Doubles of the MFMA structure factor's per-charge tables (charges padded
+// to whole staged tiles)
+static inline size_t nb_sfm_table_doubles(const nb_ctx *c) {
+  const size_t nqp = NB_SFM_NQ * (((size_t)c->ncharge + NB_SFM_NQ - 1) / NB_SFM_NQ);
+  return nqp * (size_t)nb_sfm_stride(c->sfm_la, c->sfm_lb, c->sfm_lz);
+}
 #define NB_TIME_REC 2048
 #define NB_TIME_EV 8  // start | pack | pair | charge records | structure factor |
                       // k reduction | reciprocal force | ecorr + forces + reductions (end)

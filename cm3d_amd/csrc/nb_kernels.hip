@@ -1143,6 +1143,189 @@ static __device__ __forceinline__ void k_ewsfacr_body(const EwSfacArgs &A, int b
 NB_KERNEL_PAIR_W(k_ewsfac, EwSfacArgs, 256, NB_SFAC_WPE)
 NB_KERNEL_PAIR_W(k_ewsfacr, EwSfacArgs, 256, NB_SFAC_WPE)
 
+// k_ewsfacm: the structure factor on the FP64 matrix cores.  The MFMA
+// v_mfma_f64_16x16x4f64 adds A[16][4] B[4][16] to D[16][16]; lane l holds
+// A[l % 16][l / 16], B[l / 16][l % 16], and D[l / 16 + 4 r][l % 16] as the
+// r-th double of the result.  Here the K index is a charge (4 a step), A's
+// row i = c 8 + g is Re (c = 0) or Im (c = 1) of w = q e^{i (ka theta_a +
+// kb theta_b)} for row g of a tile of 8 k rows, and B's column j = s 8 + u
+// is cos (s = 0) or sin (s = 1) of m theta_c for m = 8 b + u of m tile b,
+// so D[c 8 + g][s 8 + u] is Re / Im of A_m (s = 0) or B_m (s = 1) of
+// k_ewsfac for 8 rows x 8 values of m.  The phases come from per-charge
+// tables (k_ewsfmtab): q e^{i ka theta_a} stored {re, -im, im, re} (lane c
+// reads the pair whose dot product with e^{i kb theta_b} is Re or Im w),
+// e^{i kb theta_b}, e^{i m theta_c}; a tile of NB_SFM_NQ charges' tables
+// is copied HBM -> LDS by direct loads (no registers) into one of two
+// buffers while the products of the previous tile run.  A workgroup takes
+// NB_SFM_R x 4 row tiles (runs of consecutive kb in one ka plane, sorted
+// by m range, longest first: a tile's rows read one a-table entry and 8
+// consecutive b-table entries) and one chunk of charges; each wave owns
+// NB_SFM_R tiles and every charge of the chunk, so the chunk's partial
+// S(k) leaves the registers with no cross-wave sum.
+struct EwSfmArgs {
+  const EwRowM *rowm; const double *tab; int nq, qc, nrwg, nchunk, la, lb, lz, ktot;
+  double2 *part;
+};
+// k_ewsfmtab: one table entry per thread, from one sine / cosine of the
+// entry's multiple of the charge's phase (padding charges past nq: zero
+// charge, unit phases); SFM_NE_MAX threads a charge, the ones past its
+// entries idle
+#define SFM_NE_MAX (NB_SFM_TAB + 2 * NB_SFM_TAB - 1 + 8 * NB_SFM_NT)
+struct EwSfmTabArgs {
+  const double4 *chg; int nq, nqp, la, lb, lz; double *tab;
+};
+static __device__ __forceinline__ void k_ewsfmtab_body(const EwSfmTabArgs &A, int bid) {
+  const int sc = nb_sfm_stride(A.la, A.lb, A.lz);
+  const unsigned idx = (unsigned)bid * 256u + threadIdx.x;
+  const int t = (int)(idx / (unsigned)SFM_NE_MAX), e = (int)(idx % (unsigned)SFM_NE_MAX);
+  if (t >= A.nqp || e >= A.la + 2 * A.lb - 1 + A.lz) return;
+  const double4 r = t < A.nq ? A.chg[t] : make_double4(0., 0., 0., 0.);
+  double *d = A.tab + (size_t)t * sc;
+  // entry e: the a table (ka = e), the b table (kb = e - la - (lb - 1)), the
+  // c table (m = e - la - (2 lb - 1)); one sine / cosine for all three
+  const int eb = e - A.la, ec = eb - (2 * A.lb - 1);
+  const int ax = e < A.la ? 0 : (ec < 0 ? 1 : 2);
+  const int n = ax == 0 ? e : (ax == 1 ? eb - (A.lb - 1) : ec);
+  const double th = ax == 0 ? r.x : (ax == 1 ? r.y : r.z);
+  double sn, cs;
+  nb_sincos(n * th, &sn, &cs);
+  if (ax == 0) {
+    const double q = t < A.nq ? r.w : 0.;
+    *(double4 *)(d + 4 * e) = make_double4(q * cs, -(q * sn), q * sn, q * cs);
+  } else {
+    *(double2 *)(d + (ax == 1 ? 4 * A.la + 2 * eb : 4 * A.la + 2 * (2 * A.lb - 1) + 2 * ec)) =
+        make_double2(cs, sn);
+  }
+}
+NB_KERNEL_PAIR(k_ewsfmtab, EwSfmTabArgs, 256)
+
+typedef double sfm_d4 __attribute__((ext_vector_type(4)));
+#define SFM_SC_MAX (4 * NB_SFM_TAB + 2 * (2 * NB_SFM_TAB - 1) + 2 * 8 * NB_SFM_NT + 2)
+typedef __attribute__((address_space(3))) void *sfm_lptr;
+// tile tl of the chunk (tables of charges c0 + NB_SFM_NQ tl ..) into an LDS
+// buffer: 1 KB per wave-load, 16 B a lane, waited for by the caller
+static __device__ __forceinline__ void sfm_load(const EwSfmArgs &A, double *dst, int c0, int tl,
+                                                int sc) {
+  const int bytes = NB_SFM_NQ * sc * 8, lane = lane_id();
+  const char *src = (const char *)(A.tab + (size_t)(c0 + tl * NB_SFM_NQ) * sc);
+  for (int q = threadIdx.x >> 6; q * 1024 < bytes; q += 4) {
+    const int off = q * 1024 + lane * 16;
+    if (off < bytes)
+      __builtin_amdgcn_global_load_lds(src + off, (sfm_lptr)((char *)dst + q * 1024), 16, 0, 0);
+  }
+}
+// the products of one staged tile (NB_SFM_NQ / 4 matrix-core K steps)
+template <int NT>
+static __device__ __forceinline__ void sfm_tile(const double *tab, int sc, const int *aoff,
+                                                const int *boff, int zoff,
+                                                sfm_d4 (&acc)[NB_SFM_R][NT > 0 ? NT : 1]) {
+#pragma clang fp contract(fast)
+  const int kq = lane_id() >> 4;
+#pragma unroll 1
+  for (int ks = 0; ks < NB_SFM_NQ / 4; ks++) {
+    const double *cb = tab + (ks * 4 + kq) * sc;
+    double y[NT];
+#pragma unroll
+    for (int b = 0; b < NT; b++) y[b] = cb[zoff + 16 * b];
+#pragma unroll
+    for (int r = 0; r < NB_SFM_R; r++) {
+      const double2 ta = *(const double2 *)(cb + aoff[r]);
+      const double2 tw = *(const double2 *)(cb + boff[r]);
+      const double x = ta.x * tw.x + ta.y * tw.y;
+#pragma unroll
+      for (int b = 0; b < NT; b++)
+        acc[r][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y[b], acc[r][b], 0, 0, 0);
+    }
+  }
+}
+template <int NT>
+static __device__ __forceinline__ void sfm_run(const EwSfmArgs &A, double *const (&buf)[2],
+                                               int chunk, int tile0) {
+  constexpr int R = NB_SFM_R;
+  const int sc = nb_sfm_stride(A.la, A.lb, A.lz);
+  const int lane = lane_id(), kq = lane >> 4, i = lane & 15;
+  const int tb = 4 * A.la, tz = tb + 2 * (2 * A.lb - 1);
+  int aoff[R], boff[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const EwRowM e = A.rowm[(tile0 + r) * 8 + (i & 7)];
+    aoff[r] = 4 * e.ka + 2 * (i >> 3);
+    boff[r] = tb + 2 * (e.kb + A.lb - 1);
+  }
+  const int zoff = tz + 2 * (i & 7) + (i >> 3);
+  sfm_d4 acc[R][NT > 0 ? NT : 1];
+#pragma unroll
+  for (int r = 0; r < R; r++)
+#pragma unroll
+    for (int b = 0; b < NT; b++) acc[r][b] = sfm_d4{0., 0., 0., 0.};
+  const int c0 = chunk * A.qc, nc = min(A.nq, c0 + A.qc) - c0;
+  const int ntl = (nc + NB_SFM_NQ - 1) / NB_SFM_NQ;  // tiles (the tables are padded to whole ones)
+  sfm_load(A, buf[0], c0, 0, sc);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  // two tiles a trip so that each buffer is one LDS object at every access
+  for (int tl = 0; tl < ntl; tl += 2) {
+    if (tl + 1 < ntl) sfm_load(A, buf[1], c0, tl + 1, sc);
+    if constexpr (NT > 0) sfm_tile<NT>(buf[0], sc, aoff, boff, zoff, acc);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (tl + 1 >= ntl) break;
+    if (tl + 2 < ntl) sfm_load(A, buf[0], c0, tl + 2, sc);
+    if constexpr (NT > 0) sfm_tile<NT>(buf[1], sc, aoff, boff, zoff, acc);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+  }
+  if constexpr (NT > 0) {
+    // lane: m = 8 b + (i & 7); s = i >> 3 (0: A_m, 1: B_m, the other in
+    // lane ^ 8); rows kq and kq + 4 of each tile (doubles h and 2 + h:
+    // Re and Im)
+    double2 *out = A.part + (size_t)chunk * A.ktot;
+    const int s = i >> 3;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const EwRowM e = A.rowm[(tile0 + r) * 8 + kq + 4 * h];
+#pragma unroll
+        for (int b = 0; b < NT; b++) {
+          const double xr = acc[r][b][h], xi = acc[r][b][2 + h];
+          const double yr = __shfl_xor(xr, 8), yi = __shfl_xor(xi, 8);
+          const int m = 8 * b + (i & 7);
+          if (s == 0) {
+            if (m >= e.kc_lo && m <= e.kc_hi)
+              out[e.k_lo + m - e.kc_lo] = make_double2(xr - yi, xi + yr);
+          } else if (m > 0 && -m >= e.kc_lo && -m <= e.kc_hi) {
+            out[e.k_lo - m - e.kc_lo] = make_double2(yr + xi, yi - xr);
+          }
+        }
+      }
+    }
+  }
+}
+static __device__ __forceinline__ void k_ewsfacm_body(const EwSfmArgs &A, int bid) {
+  __shared__ double sfm_b0[NB_SFM_NQ * SFM_SC_MAX], sfm_b1[NB_SFM_NQ * SFM_SC_MAX];
+  double *const buf[2] = {sfm_b0, sfm_b1};
+  // blocks are dealt to the 8 XCDs in turn: the nrwg workgroups of a chunk
+  // go to one XCD, so its tables are read from that XCD's L2
+  const int xcd = bid & 7, j = bid >> 3;
+  const int chunk = (j / A.nrwg) * 8 + xcd, rwg = j % A.nrwg;
+  if (chunk >= A.nchunk) return;
+  const int tile0 = (rwg * 4 + (threadIdx.x >> 6)) * NB_SFM_R;
+  // tiles are sorted by their longest m range: the wave's first is its longest
+  const int mhi = A.rowm[tile0 * 8].tmhi;
+  switch (__builtin_amdgcn_readfirstlane((mhi + 8) >> 3)) {
+    case 1: sfm_run<1>(A, buf, chunk, tile0); break;
+    case 2: sfm_run<2>(A, buf, chunk, tile0); break;
+    case 3: sfm_run<3>(A, buf, chunk, tile0); break;
+    case 4: sfm_run<4>(A, buf, chunk, tile0); break;
+    case 5: sfm_run<5>(A, buf, chunk, tile0); break;
+    default: sfm_run<0>(A, buf, chunk, tile0); break;  // loads with the others
+  }
+}
+static_assert(NB_SFM_NT == 5, "k_ewsfacm dispatches m-tile counts 1..5");
+static_assert(NB_SFM_NQ % 4 == 0, "k_ewsfacm: whole K steps of 4 charges a tile");
+NB_KERNEL_PAIR_W(k_ewsfacm, EwSfmArgs, 256, 2)
+
 // per k: 2 preg S(k), the amplitude the force pass needs
 struct KData { double cr, ci; };
 
@@ -1562,12 +1745,22 @@ int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_f
   nb_time_mark(c, 3, s);
   EwSfacArgs sf = {c->d_sym, (const double4 *)c->d_chg, (const double4 *)c->d_stp, nq, c->qc,
                    c->nsym, c->nsw, c->ktot, (double2 *)c->d_spart};
-  // both kernels are always recorded, the unused one with no workgroups,
-  // so that systems of either lane kind record the same step (a batch
-  // launches each kernel once for the systems that give it work)
-  const int nsfac = rp && ew ? c->nsw * c->nchunk : 0;
+  // the structure-factor kernels are always recorded, the unused ones with
+  // no workgroups, so that systems of any lane kind record the same step (a
+  // batch launches each kernel once for the systems that give it work)
+  const int nsfac = rp && ew && !c->sfm ? c->nsw * c->nchunk : 0;
   NB_GO(c, K_EWSFAC, k_ewsfac, c->sfac_rows == 1 ? nsfac : 0, 256, 0, sf, s);
   NB_GO(c, K_EWSFACR, k_ewsfacr, c->sfac_rows > 1 ? nsfac : 0, 256, 0, sf, s);
+  const int nqp = NB_SFM_NQ * ((nq + NB_SFM_NQ - 1) / NB_SFM_NQ);
+  EwSfmTabArgs st = {(const double4 *)c->d_chg, nq, nqp, c->sfm_la, c->sfm_lb, c->sfm_lz,
+                     c->d_sfmtab};
+  const long long nte = (long long)nqp * SFM_NE_MAX;
+  NB_GO(c, K_EWSFMTAB, k_ewsfmtab, rp && ew && c->sfm ? (int)((nte + 255) / 256) : 0, 256, 0,
+        st, s);
+  EwSfmArgs sm = {c->d_rowm, c->d_sfmtab, nq, c->qc, c->sfm_nrwg, c->nchunk, c->sfm_la,
+                  c->sfm_lb, c->sfm_lz, c->ktot, (double2 *)c->d_spart};
+  NB_GO(c, K_EWSFACM, k_ewsfacm, rp && ew && c->sfm ? 8 * c->sfm_nrwg * ((c->nchunk + 7) / 8) : 0,
+        256, 0, sm, s);
   nb_time_mark(c, 4, s);
   double det = b.h[0] * (b.h[4] * b.h[8] - b.h[5] * b.h[7]) +
                b.h[1] * (b.h[5] * b.h[6] - b.h[3] * b.h[8]) +
@@ -1624,6 +1817,8 @@ static struct RegisterKernels {
     g_batch_fn[K_EWKRED] = k_ewkred_bl;
     g_batch_fn[K_EWFORCE] = k_ewforce_bl;
     g_batch_fn[K_EWSFACR] = k_ewsfacr_bl;
+    g_batch_fn[K_EWSFACM] = k_ewsfacm_bl;
+    g_batch_fn[K_EWSFMTAB] = k_ewsfmtab_bl;
     g_batch_fn[K_EWFIN] = k_ewfin_bl;
   }
 } g_register_kernels;

@@ -27,12 +27,17 @@ def _run_alone(s, prec, nsteps, graph):
     return out
 
 
+@pytest.mark.parametrize("mfma", [None, "1"])
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
-def test_batch_equals_alone(prec):
+def test_batch_equals_alone(prec, mfma, monkeypatch):
+    """mfma "1": every member's structure factor on the matrix cores (its
+    batched table and product kernels)."""
     from cm3d_amd import nb
+    if mfma:
+        monkeypatch.setenv("CM3D_NB_SFAC_MFMA", mfma)
     nsteps = 12
     syss = _systems()
-    print(f"[batch] {prec}: sizes {[s['natoms'] for s in syss]}", flush=True)
+    print(f"[batch] {prec} mfma={mfma}: sizes {[s['natoms'] for s in syss]}", flush=True)
     alone = [_run_alone(s, prec, nsteps, graph=False) for s in syss]
     ctxs, mds = [], []
     for s in syss:

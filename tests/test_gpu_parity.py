@@ -99,11 +99,16 @@ def _irregular_klist(d):
     return e
 
 
+@pytest.mark.parametrize("mfma", [None, "1"])
 @pytest.mark.parametrize("name", ["water512", "water512_tri"])
-def test_fp64_irregular_k_rows_match_oracle(name):
+def test_fp64_irregular_k_rows_match_oracle(name, mfma, monkeypatch):
+    """mfma "1": the structure factor on the matrix cores, whose row tiles
+    are runs of consecutive kb (broken here by the missing (0, 1) row)."""
     from oracle import oracle as O
+    if mfma:
+        monkeypatch.setenv("CM3D_NB_SFAC_MFMA", mfma)
     d = _irregular_klist(load_fixture(name))
-    print(f"[irregular k] {name}: {int(d['ktot'])} k", flush=True)
+    print(f"[irregular k] {name} mfma={mfma}: {int(d['ktot'])} k", flush=True)
     L = O.build_lists(d, d["pos"], d["hmat"])
     _, ofl, oe = O.force(d, L, d["pos"], d["hmat"], 1)
     c = _ctx(d)
@@ -288,6 +293,56 @@ def test_multirow_structure_factor_lanes(name):
     assert np.allclose(e["vp_ewald_tensor"], d["vp_ewald_tensor"], rtol=1e-10, atol=1e-8)
     assert rel_rms(fl, out[0][1]) < 1e-13
     assert _rel(e["vk_ewald"], out[0][2]["vk_ewald"]) < 1e-13
+
+
+@pytest.mark.parametrize("name", ["water512", "water512_tri", "mix600", "mix600_tri",
+                                  "ref_abutane"])
+def test_matrix_core_structure_factor(name, monkeypatch):
+    """The structure factor as FP64 matrix-core products (k_ewsfacm, used
+    by default for large k spheres and charge counts) forced on the
+    fixtures (CM3D_NB_SFAC_MFMA=1): the reciprocal energy, virial and
+    forces stay within the FP64 bar of cm3d's, and within rounding of the
+    VALU kernel's (CM3D_NB_SFAC_MFMA=0), in both precisions."""
+    d = load_fixture(name)
+    from cm3d_amd import nb
+    for prec in ("fp64", "mixed"):
+        out = {}
+        for flag in ("1", "0"):
+            monkeypatch.setenv("CM3D_NB_SFAC_MFMA", flag)
+            c = nb.NBContext(d, precision=prec)
+            c.build_lists(d["pos"], d["hmat"])
+            out[flag] = c.force(d["pos"], d["hmat"], iflg=1)
+            c.close()
+        monkeypatch.delenv("CM3D_NB_SFAC_MFMA")
+        fs, fl, e = out["1"]
+        assert np.isfinite(fl).all()
+        assert rel_rms(fl, out["0"][1]) < 1e-13, (name, prec, rel_rms(fl, out["0"][1]))
+        assert _rel(e["vk_ewald"], out["0"][2]["vk_ewald"]) < 1e-13
+        assert np.allclose(e["vp_ewald_tensor"], out["0"][2]["vp_ewald_tensor"], rtol=1e-12,
+                           atol=1e-9)
+        if prec == "fp64":
+            assert rel_rms(fl, d["fxl"]) < FP64_TOL, rel_rms(fl, d["fxl"])
+            assert _rel(e["vk_ewald"], d["vk_ewald"][0]) < FP64_TOL
+
+
+def test_matrix_core_structure_factor_full_size(monkeypatch):
+    """The 98,304-atom water of kmax 32 (68,532 k vectors), where the
+    matrix-core structure factor is the default: the same forces and
+    reciprocal energy as the VALU kernel to rounding."""
+    from cm3d_amd import nb, systems as S
+    s = S.water(nside=32, rc=12.0, alpha=0.30, kmax=32)
+    out = {}
+    for flag in (None, "0"):
+        if flag:
+            monkeypatch.setenv("CM3D_NB_SFAC_MFMA", flag)
+        c = nb.NBContext(s, precision="mixed")
+        c.build_lists(s["pos"], s["hmat"])
+        out[flag] = c.force(s["pos"], s["hmat"], iflg=1)
+        c.close()
+    fl, e = out[None][1], out[None][2]
+    assert np.isfinite(fl).all()
+    assert rel_rms(fl, out["0"][1]) < 1e-13, rel_rms(fl, out["0"][1])
+    assert _rel(e["vk_ewald"], out["0"][2]["vk_ewald"]) < 1e-13
 
 
 @pytest.mark.parametrize("sharing", [1, 16])

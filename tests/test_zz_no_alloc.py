@@ -1,7 +1,8 @@
 """nb_force_device_noalloc (scratch in the library's static per-GPU area, no
 device allocation) against nb_force_device, through the C ABI: forces and
 every energy field bit for bit, on two calls in a row, the second on a
-smaller system, in both precisions and at both RESPA levels; a system whose
+smaller system, in both precisions and at both RESPA levels, with either
+structure-factor kernel; a system whose
 scratch exceeds the static area is refused with an error on the host."""
 import numpy as np
 import pytest
@@ -31,11 +32,16 @@ def _both(d, prec, iflg):
     return out
 
 
+@pytest.mark.parametrize("mfma", [None, "1"])
 @pytest.mark.parametrize("iflg", [1, 0])
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
-def test_noalloc_equals_force_device(prec, iflg):
+def test_noalloc_equals_force_device(prec, iflg, mfma, monkeypatch):
+    # mfma "1": the structure factor on the matrix cores, its phase tables in
+    # the static area too
+    if mfma:
+        monkeypatch.setenv("CM3D_NB_SFAC_MFMA", mfma)
     for name in ("water512", "mix600"):  # the second call on a smaller system
-        print(f"[noalloc] {name} {prec} iflg={iflg}", flush=True)
+        print(f"[noalloc] {name} {prec} iflg={iflg} mfma={mfma}", flush=True)
         d = load_fixture(name)
         (fs0, fl0, e0), (fs1, fl1, e1) = _both(d, prec, iflg)
         assert np.array_equal(fs0, fs1), name
