@@ -123,7 +123,7 @@ extern "C" void nb_destroy(nb_ctx *c) {
   hipDeviceSynchronize();
   void *ptrs[] = {c->d_itype, c->d_map, c->d_ip, c->d_vtab3, c->d_ctab3, c->d_vtab3f,
                   c->d_ctab3f, c->d_qch, c->d_xoff, c->d_xlist, c->d_eoff, c->d_elist,
-                  c->d_qidx, c->d_qpos, c->d_row, c->d_sym, c->d_rowm, c->d_sfmtab, c->d_ent, c->d_trow, c->d_rtile, c->d_fpart, c->d_kabc, c->d_pos, c->d_posb, c->d_fs, c->d_fl,
+                  c->d_qidx, c->d_qpos, c->d_row, c->d_sym, c->d_rowm, c->d_sfmw, c->d_sfmtab, c->d_ent, c->d_trow, c->d_rtile, c->d_fpart, c->d_kabc, c->d_pos, c->d_posb, c->d_fs, c->d_fl,
                   c->d_perm, c->d_iperm, c->d_cell, c->d_ccount, c->d_cstart, c->d_cfill, c->d_cwork,
                   c->d_hcnt, c->d_pcut,
                   c->d_tmp, c->d_rec64, c->d_stype, c->d_nbr, c->d_nnb, c->d_flags,
@@ -448,6 +448,7 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
   // takes NB_SFM_R tiles and every charge of its chunk.  CM3D_NB_SFAC_MFMA=0 / 1 turns it off / on for any system it
   // can serve (for tests and A/B timing).
   std::vector<EwRowM> rowm;
+  std::vector<EwSfmWave> sfmw;
   {
     int mmax = -1, kamin = 0, kamax = 0, kbmax = 0;
     for (const EwRow &r : row) {
@@ -479,10 +480,19 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
       for (size_t t = 0; t < tl.size(); t++) ord[t] = (int)t;
       std::stable_sort(ord.begin(), ord.end(),
                        [&](int a, int b) { return (tmhi[a] >> 3) > (tmhi[b] >> 3); });
+      // waves: NB_SFM_R tiles each, fewer where the first (longest) tile
+      // needs 4 or 5 m tiles (the accumulators then fit the same
+      // registers); the empty tile past the last real one fills the rest
       const int ntile = (int)tl.size();
-      const int tpw = NB_SFM_R * 4;  // row tiles a workgroup
-      c->sfm_nrwg = (ntile + tpw - 1) / tpw;
-      rowm.assign((size_t)c->sfm_nrwg * tpw * 8, EwRowM{0, 0, 0, 1, 0, -1, -1, 0});  // no k
+      for (int t = 0; t < ntile;) {
+        EwSfmWave w = {{ntile, ntile, ntile, ntile}, (tmhi[ord[t]] + 8) >> 3, 0, 0, 0};
+        w.nr = nb_sfm_wave_tiles(w.nt);
+        for (int q = 0; q < w.nr && t < ntile; q++, t++) w.t[q] = t;
+        sfmw.push_back(w);
+      }
+      while (sfmw.size() % 4) sfmw.push_back(EwSfmWave{{ntile, ntile, ntile, ntile}, 0, NB_SFM_R, 0, 0});
+      c->sfm_nrwg = (int)sfmw.size() / 4;
+      rowm.assign((size_t)(ntile + 1) * 8, EwRowM{0, 0, 0, 1, 0, -1, -1, 0});  // no k
       for (int t = 0; t < ntile; t++) {
         const std::vector<int> &rs = tl[ord[t]];
         for (size_t q = 0; q < rs.size(); q++) {
@@ -560,6 +570,7 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
   if ((rc = upload(c, &c->d_row, row.data(), row.size()))) return fail(rc);
   if ((rc = upload(c, &c->d_sym, sym.data(), sym.size()))) return fail(rc);
   if (c->sfm && (rc = upload(c, &c->d_rowm, rowm.data(), rowm.size()))) return fail(rc);
+  if (c->sfm && (rc = upload(c, &c->d_sfmw, sfmw.data(), sfmw.size()))) return fail(rc);
   if (c->sfm && (rc = dalloc(c, &c->d_sfmtab, nb_sfm_table_doubles(c)))) return fail(rc);
   if ((rc = upload(c, &c->d_ent, ent.data(), ent.size()))) return fail(rc);
   if ((rc = upload(c, &c->d_trow, trow.data(), trow.size()))) return fail(rc);

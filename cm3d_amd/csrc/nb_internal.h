@@ -76,8 +76,11 @@ struct LaunchRec;
 #ifndef NB_SFM_NT
 #define NB_SFM_NT 5  // MFMA structure factor: m tiles of 8 a row tile may need (m < 40)
 #endif
-#define NB_SFM_NQ 8     // ... charges per staged tile (2 matrix-core K steps of 4)
-#define NB_SFM_R 4      // ... row tiles (8 rows) per wave
+#define NB_SFM_NQ 16    // ... charges per staged tile (4 matrix-core K steps of 4)
+#define NB_SFM_R 4      // ... row tiles (8 rows) per wave (fewer for waves of 4 or 5 m tiles)
+// row tiles a wave of nt m tiles takes: the accumulators and two K steps'
+// operands stay in 256 registers
+__host__ __device__ static constexpr int nb_sfm_wave_tiles(int nt) { return nt >= 5 ? 2 : (nt == 4 ? 3 : 4); }
 #define NB_SFM_TAB 40   // ... |ka|, |kb| below this (phase-table length)
 #ifndef NB_SFM_WG
 #define NB_SFM_WG 1024  // ... workgroups it aims for
@@ -129,15 +132,21 @@ struct EwRow {
 };
 // A row of the MFMA structure factor, in its tile order
 struct EwRowM { int ka, kb, k_lo, kc_lo, kc_hi, mhi, tmhi, pad; };  // tmhi: its tile's longest m
-// Doubles per charge of the MFMA structure factor's tables: q e^{i ka theta_a}
-// as {re, -im, im, re} for ka = 0..la-1, e^{i kb theta_b} for kb = -(lb-1)
-// .. lb-1, e^{i m theta_c} for m = 0..lz-1; odd in 16-byte units so that
-// consecutive charges start in different LDS banks
+// Per-charge tables of the MFMA structure factor, in doubles: q e^{i ka
+// theta_a} as {im, re, -im} for ka = 0..la-1 (padded to an even count),
+// then e^{i kb theta_b} for kb = 0..lb-1, then e^{i m theta_c} for m =
+// 0..lz-1; the stride odd in 16-byte units so that consecutive charges
+// start in different LDS banks
+__host__ __device__ static inline int nb_sfm_tb(int la) { return (3 * la + 1) & ~1; }
+__host__ __device__ static inline int nb_sfm_tz(int la, int lb) { return nb_sfm_tb(la) + 2 * lb; }
 __host__ __device__ static inline int nb_sfm_stride(int la, int lb, int lz) {
-  int s = 4 * la + 2 * (2 * lb - 1) + 2 * lz;
+  int s = nb_sfm_tz(la, lb) + 2 * lz;
   if ((s / 2) % 2 == 0) s += 2;
   return s;
 }
+// A wave of the MFMA structure factor: its row tiles (an empty one past
+// the last real tile where it has fewer) and their m-tile count
+struct EwSfmWave { int t[4]; int nt, nr, pad0, pad1; };
 struct EwSym {       // a structure-factor lane: nr rows (ka, kb .. kb + nr - 1), cnt values of m from m0
   int ka, kb, m0, cnt, nr, pad0, pad1, pad2;
   int k_lo[4], kc_lo[4], kc_hi[4];  // per row: k index of kc_lo, its kc range
@@ -318,6 +327,7 @@ struct nb_ctx {
   int sfm_la, sfm_lb, sfm_lz;  // ... its phase-table lengths
   int sfm_nrwg;                // ... workgroups per chunk
   EwRowM *d_rowm;              // ... its rows, sorted, in tiles of 8
+  EwSfmWave *d_sfmw;           // ... its waves' tiles (4 a workgroup)
   double *d_sfmtab;            // ... per-charge phase tables (charges padded to whole tiles)
   int nrange, nqb, ntile;      // force: tile ranges, charge workgroups, tiles
   double *d_epair, *d_eew, *d_eec, *d_eout;

@@ -1152,25 +1152,28 @@ NB_KERNEL_PAIR_W(k_ewsfacr, EwSfacArgs, 256, NB_SFAC_WPE)
 // is cos (s = 0) or sin (s = 1) of m theta_c for m = 8 b + u of m tile b,
 // so D[c 8 + g][s 8 + u] is Re / Im of A_m (s = 0) or B_m (s = 1) of
 // k_ewsfac for 8 rows x 8 values of m.  The phases come from per-charge
-// tables (k_ewsfmtab): q e^{i ka theta_a} stored {re, -im, im, re} (lane c
-// reads the pair whose dot product with e^{i kb theta_b} is Re or Im w),
-// e^{i kb theta_b}, e^{i m theta_c}; a tile of NB_SFM_NQ charges' tables
-// is copied HBM -> LDS by direct loads (no registers) into one of two
-// buffers while the products of the previous tile run.  A workgroup takes
-// NB_SFM_R x 4 row tiles (runs of consecutive kb in one ka plane, sorted
-// by m range, longest first: a tile's rows read one a-table entry and 8
-// consecutive b-table entries) and one chunk of charges; each wave owns
-// NB_SFM_R tiles and every charge of the chunk, so the chunk's partial
-// S(k) leaves the registers with no cross-wave sum.
+// tables (k_ewsfmtab): q e^{i ka theta_a} stored {im, re, -im}, so that
+// lane c reads at 1 - c the pair (p, q) with w's Re / Im = p Re t + q Im t
+// for t = e^{i kb theta_b} (stored for |kb|, conjugated by the row's sign),
+// and e^{i m theta_c}.  A tile of NB_SFM_NQ charges' tables is copied HBM
+// -> LDS by direct loads (no registers) into one of two buffers while the
+// products of the previous tile run, and within a tile the operands of the
+// next K step are loaded while the current one's products issue.  A
+// workgroup takes 4 waves' row tiles (runs of consecutive kb in one ka
+// plane, sorted by m range, longest first: a tile's rows read one a-table
+// entry and 8 consecutive b-table entries) and one chunk of charges; each
+// wave owns its tiles and every charge of the chunk, so the chunk's
+// partial S(k) leaves the registers with no cross-wave sum.
 struct EwSfmArgs {
-  const EwRowM *rowm; const double *tab; int nq, qc, nrwg, nchunk, la, lb, lz, ktot;
+  const EwRowM *rowm; const EwSfmWave *wave; const double *tab;
+  int nq, qc, nrwg, nchunk, la, lb, lz, ktot;
   double2 *part;
 };
 // k_ewsfmtab: one table entry per thread, from one sine / cosine of the
 // entry's multiple of the charge's phase (padding charges past nq: zero
 // charge, unit phases); SFM_NE_MAX threads a charge, the ones past its
 // entries idle
-#define SFM_NE_MAX (NB_SFM_TAB + 2 * NB_SFM_TAB - 1 + 8 * NB_SFM_NT)
+#define SFM_NE_MAX (2 * NB_SFM_TAB + 8 * NB_SFM_NT)
 struct EwSfmTabArgs {
   const double4 *chg; int nq, nqp, la, lb, lz; double *tab;
 };
@@ -1178,29 +1181,28 @@ static __device__ __forceinline__ void k_ewsfmtab_body(const EwSfmTabArgs &A, in
   const int sc = nb_sfm_stride(A.la, A.lb, A.lz);
   const unsigned idx = (unsigned)bid * 256u + threadIdx.x;
   const int t = (int)(idx / (unsigned)SFM_NE_MAX), e = (int)(idx % (unsigned)SFM_NE_MAX);
-  if (t >= A.nqp || e >= A.la + 2 * A.lb - 1 + A.lz) return;
+  if (t >= A.nqp || e >= A.la + A.lb + A.lz) return;
   const double4 r = t < A.nq ? A.chg[t] : make_double4(0., 0., 0., 0.);
   double *d = A.tab + (size_t)t * sc;
-  // entry e: the a table (ka = e), the b table (kb = e - la - (lb - 1)), the
-  // c table (m = e - la - (2 lb - 1)); one sine / cosine for all three
-  const int eb = e - A.la, ec = eb - (2 * A.lb - 1);
-  const int ax = e < A.la ? 0 : (ec < 0 ? 1 : 2);
-  const int n = ax == 0 ? e : (ax == 1 ? eb - (A.lb - 1) : ec);
+  // entry e: the a table (ka = e), the b table (kb = e - la), the c table
+  // (m = e - la - lb); one sine / cosine for all three
+  const int ax = e < A.la ? 0 : (e < A.la + A.lb ? 1 : 2);
+  const int n = ax == 0 ? e : (ax == 1 ? e - A.la : e - A.la - A.lb);
   const double th = ax == 0 ? r.x : (ax == 1 ? r.y : r.z);
   double sn, cs;
   nb_sincos(n * th, &sn, &cs);
   if (ax == 0) {
     const double q = t < A.nq ? r.w : 0.;
-    *(double4 *)(d + 4 * e) = make_double4(q * cs, -(q * sn), q * sn, q * cs);
+    d[3 * n] = q * sn; d[3 * n + 1] = q * cs; d[3 * n + 2] = -(q * sn);
   } else {
-    *(double2 *)(d + (ax == 1 ? 4 * A.la + 2 * eb : 4 * A.la + 2 * (2 * A.lb - 1) + 2 * ec)) =
+    *(double2 *)(d + (ax == 1 ? nb_sfm_tb(A.la) : nb_sfm_tz(A.la, A.lb)) + 2 * n) =
         make_double2(cs, sn);
   }
 }
 NB_KERNEL_PAIR(k_ewsfmtab, EwSfmTabArgs, 256)
 
 typedef double sfm_d4 __attribute__((ext_vector_type(4)));
-#define SFM_SC_MAX (4 * NB_SFM_TAB + 2 * (2 * NB_SFM_TAB - 1) + 2 * 8 * NB_SFM_NT + 2)
+#define SFM_SC_MAX (3 * NB_SFM_TAB + 2 * NB_SFM_TAB + 2 * 8 * NB_SFM_NT + 4)
 typedef __attribute__((address_space(3))) void *sfm_lptr;
 // tile tl of the chunk (tables of charges c0 + NB_SFM_NQ tl ..) into an LDS
 // buffer: 1 KB per wave-load, 16 B a lane, waited for by the caller
@@ -1214,46 +1216,73 @@ static __device__ __forceinline__ void sfm_load(const EwSfmArgs &A, double *dst,
       __builtin_amdgcn_global_load_lds(src + off, (sfm_lptr)((char *)dst + q * 1024), 16, 0, 0);
   }
 }
-// the products of one staged tile (NB_SFM_NQ / 4 matrix-core K steps)
-template <int NT>
-static __device__ __forceinline__ void sfm_tile(const double *tab, int sc, const int *aoff,
-                                                const int *boff, int zoff,
-                                                sfm_d4 (&acc)[NB_SFM_R][NT > 0 ? NT : 1]) {
-#pragma clang fp contract(fast)
-  const int kq = lane_id() >> 4;
-#pragma unroll 1
-  for (int ks = 0; ks < NB_SFM_NQ / 4; ks++) {
-    const double *cb = tab + (ks * 4 + kq) * sc;
-    double y[NT];
+// the operands of one K step: per row tile the a-table pair and the b-table
+// entry, per m tile the cos / sin
+template <int NT, int R>
+struct SfmOps { double y[NT]; double p[R], q[R]; double2 t[R]; };
+template <int NT, int R>
+static __device__ __forceinline__ void sfm_ops(SfmOps<NT, R> &o, const double *cb,
+                                               const int *aoff, const int *boff, int zoff) {
 #pragma unroll
-    for (int b = 0; b < NT; b++) y[b] = cb[zoff + 16 * b];
-#pragma unroll
-    for (int r = 0; r < NB_SFM_R; r++) {
-      const double2 ta = *(const double2 *)(cb + aoff[r]);
-      const double2 tw = *(const double2 *)(cb + boff[r]);
-      const double x = ta.x * tw.x + ta.y * tw.y;
-#pragma unroll
-      for (int b = 0; b < NT; b++)
-        acc[r][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y[b], acc[r][b], 0, 0, 0);
-    }
-  }
-}
-template <int NT>
-static __device__ __forceinline__ void sfm_run(const EwSfmArgs &A, double *const (&buf)[2],
-                                               int chunk, int tile0) {
-  constexpr int R = NB_SFM_R;
-  const int sc = nb_sfm_stride(A.la, A.lb, A.lz);
-  const int lane = lane_id(), kq = lane >> 4, i = lane & 15;
-  const int tb = 4 * A.la, tz = tb + 2 * (2 * A.lb - 1);
-  int aoff[R], boff[R];
+  for (int b = 0; b < NT; b++) o.y[b] = cb[zoff + 16 * b];
 #pragma unroll
   for (int r = 0; r < R; r++) {
-    const EwRowM e = A.rowm[(tile0 + r) * 8 + (i & 7)];
-    aoff[r] = 4 * e.ka + 2 * (i >> 3);
-    boff[r] = tb + 2 * (e.kb + A.lb - 1);
+    o.p[r] = cb[aoff[r]];
+    o.q[r] = cb[aoff[r] + 1];
+    o.t[r] = *(const double2 *)(cb + boff[r]);
   }
-  const int zoff = tz + 2 * (i & 7) + (i >> 3);
-  sfm_d4 acc[R][NT > 0 ? NT : 1];
+}
+// the products of one K step from its operands
+template <int NT, int R>
+static __device__ __forceinline__ void sfm_step(const SfmOps<NT, R> &o, const double *sg,
+                                                sfm_d4 (&acc)[R][NT]) {
+#pragma clang fp contract(fast)
+  double x[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) x[r] = o.p[r] * o.t[r].x + (o.q[r] * sg[r]) * o.t[r].y;
+#pragma unroll
+  for (int r = 0; r < R; r++)
+#pragma unroll
+    for (int b = 0; b < NT; b++)
+      acc[r][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[r], o.y[b], acc[r][b], 0, 0, 0);
+}
+// the products of one staged tile (NB_SFM_NQ / 4 matrix-core K steps), two
+// operand sets in turn: each step's loads are issued before the previous
+// step's products
+template <int NT, int R>
+static __device__ __forceinline__ void sfm_tile(const double *tab, int sc, const int *aoff,
+                                                const int *boff, const double *sg, int zoff,
+                                                sfm_d4 (&acc)[R][NT]) {
+  constexpr int KS = NB_SFM_NQ / 4;
+  static_assert(KS % 2 == 0, "k_ewsfacm: K steps in pairs");
+  const int kq = lane_id() >> 4;
+  SfmOps<NT, R> o0, o1;
+  sfm_ops<NT, R>(o0, tab + kq * sc, aoff, boff, zoff);
+#pragma unroll 1
+  for (int ks = 0; ks < KS; ks += 2) {
+    sfm_ops<NT, R>(o1, tab + ((ks + 1) * 4 + kq) * sc, aoff, boff, zoff);
+    sfm_step<NT, R>(o0, sg, acc);
+    if (ks + 2 < KS) sfm_ops<NT, R>(o0, tab + ((ks + 2) * 4 + kq) * sc, aoff, boff, zoff);
+    sfm_step<NT, R>(o1, sg, acc);
+  }
+}
+template <int NT, int R>
+static __device__ __forceinline__ void sfm_run(const EwSfmArgs &A, double *const (&buf)[2],
+                                               int chunk, const EwSfmWave &wv) {
+  const int sc = nb_sfm_stride(A.la, A.lb, A.lz);
+  const int lane = lane_id(), kq = lane >> 4, i = lane & 15;
+  const int tb = nb_sfm_tb(A.la);
+  int aoff[R], boff[R];
+  double sg[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const EwRowM e = A.rowm[wv.t[r] * 8 + (i & 7)];
+    aoff[r] = 3 * e.ka + 1 - (i >> 3);
+    boff[r] = tb + 2 * abs(e.kb);
+    sg[r] = e.kb < 0 ? -1. : 1.;
+  }
+  const int zoff = nb_sfm_tz(A.la, A.lb) + 2 * (i & 7) + (i >> 3);
+  sfm_d4 acc[R][NT];
 #pragma unroll
   for (int r = 0; r < R; r++)
 #pragma unroll
@@ -1266,12 +1295,12 @@ static __device__ __forceinline__ void sfm_run(const EwSfmArgs &A, double *const
   // two tiles a trip so that each buffer is one LDS object at every access
   for (int tl = 0; tl < ntl; tl += 2) {
     if (tl + 1 < ntl) sfm_load(A, buf[1], c0, tl + 1, sc);
-    if constexpr (NT > 0) sfm_tile<NT>(buf[0], sc, aoff, boff, zoff, acc);
+    if constexpr (NT > 0) sfm_tile<NT, R>(buf[0], sc, aoff, boff, sg, zoff, acc);
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (tl + 1 >= ntl) break;
     if (tl + 2 < ntl) sfm_load(A, buf[0], c0, tl + 2, sc);
-    if constexpr (NT > 0) sfm_tile<NT>(buf[1], sc, aoff, boff, zoff, acc);
+    if constexpr (NT > 0) sfm_tile<NT, R>(buf[1], sc, aoff, boff, sg, zoff, acc);
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
   }
@@ -1285,7 +1314,7 @@ static __device__ __forceinline__ void sfm_run(const EwSfmArgs &A, double *const
     for (int r = 0; r < R; r++) {
 #pragma unroll
       for (int h = 0; h < 2; h++) {
-        const EwRowM e = A.rowm[(tile0 + r) * 8 + kq + 4 * h];
+        const EwRowM e = A.rowm[wv.t[r] * 8 + kq + 4 * h];
 #pragma unroll
         for (int b = 0; b < NT; b++) {
           const double xr = acc[r][b][h], xi = acc[r][b][2 + h];
@@ -1310,16 +1339,14 @@ static __device__ __forceinline__ void k_ewsfacm_body(const EwSfmArgs &A, int bi
   const int xcd = bid & 7, j = bid >> 3;
   const int chunk = (j / A.nrwg) * 8 + xcd, rwg = j % A.nrwg;
   if (chunk >= A.nchunk) return;
-  const int tile0 = (rwg * 4 + (threadIdx.x >> 6)) * NB_SFM_R;
-  // tiles are sorted by their longest m range: the wave's first is its longest
-  const int mhi = A.rowm[tile0 * 8].tmhi;
-  switch (__builtin_amdgcn_readfirstlane((mhi + 8) >> 3)) {
-    case 1: sfm_run<1>(A, buf, chunk, tile0); break;
-    case 2: sfm_run<2>(A, buf, chunk, tile0); break;
-    case 3: sfm_run<3>(A, buf, chunk, tile0); break;
-    case 4: sfm_run<4>(A, buf, chunk, tile0); break;
-    case 5: sfm_run<5>(A, buf, chunk, tile0); break;
-    default: sfm_run<0>(A, buf, chunk, tile0); break;  // loads with the others
+  const EwSfmWave wv = A.wave[rwg * 4 + (threadIdx.x >> 6)];
+  switch (__builtin_amdgcn_readfirstlane(wv.nt)) {
+    case 1: sfm_run<1, NB_SFM_R>(A, buf, chunk, wv); break;
+    case 2: sfm_run<2, NB_SFM_R>(A, buf, chunk, wv); break;
+    case 3: sfm_run<3, NB_SFM_R>(A, buf, chunk, wv); break;
+    case 4: sfm_run<4, nb_sfm_wave_tiles(4)>(A, buf, chunk, wv); break;
+    case 5: sfm_run<5, nb_sfm_wave_tiles(5)>(A, buf, chunk, wv); break;
+    default: sfm_run<0, 1>(A, buf, chunk, wv); break;  // loads with the others
   }
 }
 static_assert(NB_SFM_NT == 5, "k_ewsfacm dispatches m-tile counts 1..5");
@@ -1757,8 +1784,8 @@ int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_f
   const long long nte = (long long)nqp * SFM_NE_MAX;
   NB_GO(c, K_EWSFMTAB, k_ewsfmtab, rp && ew && c->sfm ? (int)((nte + 255) / 256) : 0, 256, 0,
         st, s);
-  EwSfmArgs sm = {c->d_rowm, c->d_sfmtab, nq, c->qc, c->sfm_nrwg, c->nchunk, c->sfm_la,
-                  c->sfm_lb, c->sfm_lz, c->ktot, (double2 *)c->d_spart};
+  EwSfmArgs sm = {c->d_rowm, c->d_sfmw, c->d_sfmtab, nq, c->qc, c->sfm_nrwg, c->nchunk,
+                  c->sfm_la, c->sfm_lb, c->sfm_lz, c->ktot, (double2 *)c->d_spart};
   NB_GO(c, K_EWSFACM, k_ewsfacm, rp && ew && c->sfm ? 8 * c->sfm_nrwg * ((c->nchunk + 7) / 8) : 0,
         256, 0, sm, s);
   nb_time_mark(c, 4, s);
