@@ -123,7 +123,7 @@ extern "C" void nb_destroy(nb_ctx *c) {
   hipDeviceSynchronize();
   void *ptrs[] = {c->d_itype, c->d_map, c->d_ip, c->d_vtab3, c->d_ctab3, c->d_vtab3f,
                   c->d_ctab3f, c->d_qch, c->d_xoff, c->d_xlist, c->d_eoff, c->d_elist,
-                  c->d_qidx, c->d_qpos, c->d_row, c->d_sym, c->d_rowm, c->d_sfmw, c->d_sfmtab, c->d_ent, c->d_trow, c->d_rtile, c->d_fpart, c->d_kabc, c->d_pos, c->d_posb, c->d_fs, c->d_fl,
+                  c->d_qidx, c->d_qpos, c->d_row, c->d_sym, c->d_rowm, c->d_sfmw, c->d_sfmtab, c->d_ftile, c->d_ftent, c->d_frtile, c->d_ent, c->d_trow, c->d_rtile, c->d_fpart, c->d_kabc, c->d_pos, c->d_posb, c->d_fs, c->d_fl,
                   c->d_perm, c->d_iperm, c->d_cell, c->d_ccount, c->d_cstart, c->d_cfill, c->d_cwork,
                   c->d_hcnt, c->d_pcut,
                   c->d_tmp, c->d_rec64, c->d_stype, c->d_nbr, c->d_nnb, c->d_flags,
@@ -552,6 +552,86 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
     rtile.push_back(c->ntile);
     c->nrange = (int)rtile.size() - 1;
   }
+  // The reciprocal force as matrix products (k_ewforcem), for contexts
+  // that share the GPU and systems of NB_EFM_MIN_WORK charge x k products
+  // or more (CM3D_NB_EWF_MFMA=0 / 1 turns it off / on for any system it can
+  // serve; the structure factor's setting when only that one is given):
+  // per 16-row tile and 16
+  // charges, P and Q of k_ewforce for every (row, charge) from four
+  // products per 2 values of m, the rows as 4 runs of 4 consecutive kb
+  // (run q's row r is tile row q + 4 r, so that a lane's 4 rows of the
+  // result are one run: one sine / cosine per run and charge)
+  std::vector<EwFTile> ftile;
+  std::vector<int4> ftent;
+  std::vector<int> frtile(1, 0);
+  {
+    int mmax = -1;
+    for (const EwRow &r : row) mmax = std::max(mmax, r.mhi);
+    const bool can = c->nrow > 0 && ncharge > 0 && mmax < 40;
+    bool use = can && (share > 1 || (long long)ncharge * c->ktot >= NB_EFM_MIN_WORK);
+    if (const char *ev = getenv("CM3D_NB_SFAC_MFMA")) use = can && ev[0] != '0';
+    if (const char *ev = getenv("CM3D_NB_EWF_MFMA")) use = can && ev[0] != '0';
+    c->efm = use;
+    if (use) {
+      std::vector<std::vector<int>> runs;
+      for (int g = 0; g < c->nrow; g++) {
+        if (runs.empty() || !row[g].chain || runs.back().size() == 4) runs.emplace_back();
+        runs.back().push_back(g);
+      }
+      std::vector<int> rmhi(runs.size()), ord(runs.size());
+      for (size_t q = 0; q < runs.size(); q++) {
+        rmhi[q] = -1;
+        for (int g : runs[q]) rmhi[q] = std::max(rmhi[q], row[g].mhi);
+        ord[q] = (int)q;
+      }
+      std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return rmhi[a] > rmhi[b]; });
+      std::vector<long long> work;
+      for (size_t q0 = 0; q0 < ord.size(); q0 += 4) {
+        EwFTile t;
+        memset(&t, 0, sizeof(t));
+        int mt0 = 1 << 30, mt1 = -1;
+        for (int q = 0; q < 4 && q0 + q < ord.size(); q++) {
+          const std::vector<int> &rs = runs[ord[q0 + q]];
+          t.ka[q] = row[rs[0]].ka; t.kb0[q] = row[rs[0]].kb;
+          for (int g : rs) { mt0 = std::min(mt0, row[g].mlo); mt1 = std::max(mt1, row[g].mhi); }
+        }
+        t.mt0 = mt0;
+        t.nks = (mt1 - mt0 + 2) / 2;
+        t.ent_off = (int)ftent.size();
+        for (int mm = 0; mm < 2 * t.nks; mm++) {
+          const int m = mt0 + mm;
+          for (int i = 0; i < 16; i++) {
+            int4 e = make_int4(-1, -1, m, 0);
+            const int q = i % 4, rr = i / 4;
+            if (q0 + q < ord.size() && rr < (int)runs[ord[q0 + q]].size()) {
+              const EwRow &r = row[runs[ord[q0 + q]][rr]];
+              if (m >= r.mlo && m <= r.mhi) {
+                e.x = (m >= r.kc_lo && m <= r.kc_hi) ? r.k_lo + m - r.kc_lo : -1;
+                e.y = (m > 0 && -m >= r.kc_lo && -m <= r.kc_hi) ? r.k_lo - m - r.kc_lo : -1;
+              }
+            }
+            ftent.push_back(e);
+          }
+        }
+        ftile.push_back(t);
+        work.push_back(t.nks);
+      }
+      // charge blocks of 4 waves x NB_EFM_CT x 16 charges; ranges of tiles
+      // of about equal work for ~NB_EWF_WG / share workgroups in all
+      c->efm_ncb = (ncharge + 64 * NB_EFM_CT - 1) / (64 * NB_EFM_CT);
+      const int nt = (int)ftile.size();
+      const int nr = std::max(1, std::min(nt, std::max(1, ewf_wg / c->efm_ncb)));
+      long long tot = 0, cum = 0;
+      for (long long w : work) tot += w;
+      for (int t = 0; t + 1 < nt; t++) {
+        cum += work[t];
+        const int r = (int)frtile.size();
+        if (r < nr && cum * nr >= (long long)r * tot) frtile.push_back(t + 1);
+      }
+      frtile.push_back(nt);
+      c->nrange = (int)frtile.size() - 1;
+    }
+  }
 
   // device uploads
   if ((rc = upload(c, &c->d_itype, p->itype, n))) return fail(rc);
@@ -571,6 +651,9 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
   if ((rc = upload(c, &c->d_sym, sym.data(), sym.size()))) return fail(rc);
   if (c->sfm && (rc = upload(c, &c->d_rowm, rowm.data(), rowm.size()))) return fail(rc);
   if (c->sfm && (rc = upload(c, &c->d_sfmw, sfmw.data(), sfmw.size()))) return fail(rc);
+  if (c->efm && (rc = upload(c, &c->d_ftile, ftile.data(), ftile.size()))) return fail(rc);
+  if (c->efm && (rc = upload(c, &c->d_ftent, ftent.data(), ftent.size()))) return fail(rc);
+  if (c->efm && (rc = upload(c, &c->d_frtile, frtile.data(), frtile.size()))) return fail(rc);
   if (c->sfm && (rc = dalloc(c, &c->d_sfmtab, nb_sfm_table_doubles(c)))) return fail(rc);
   if ((rc = upload(c, &c->d_ent, ent.data(), ent.size()))) return fail(rc);
   if ((rc = upload(c, &c->d_trow, trow.data(), trow.size()))) return fail(rc);

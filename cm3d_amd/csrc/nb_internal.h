@@ -144,6 +144,16 @@ __host__ __device__ static inline int nb_sfm_stride(int la, int lb, int lz) {
   if ((s / 2) % 2 == 0) s += 2;
   return s;
 }
+// A 16-row tile of the MFMA reciprocal force pass: 4 runs of up to 4
+// consecutive kb in one ka plane (run q's row r is tile row q + 4 r), its
+// m range from mt0 in nks steps of 2, its coefficient entries at ent_off
+// (2 nks x 16, {k of +m, k of -m, m, 0}, -1: absent)
+struct EwFTile { int ent_off, mt0, nks, pad; int ka[4], kb0[4]; };
+#define NB_EFM_CT 3    // MFMA force pass: charge tiles (16 charges) per wave
+#ifndef NB_EFM_MIN_WORK
+#define NB_EFM_MIN_WORK (1LL << 24)  // ... charges x k vectors from which a lone system uses it
+#endif
+#define NB_EFM_ESTR 9  // ... doubles per staged coefficient entry (8 + 1 against bank conflicts)
 // A wave of the MFMA structure factor: its row tiles (an empty one past
 // the last real tile where it has fewer) and their m-tile count
 struct EwSfmWave { int t[4]; int nt, nr, pad0, pad1; };
@@ -329,6 +339,10 @@ struct nb_ctx {
   EwRowM *d_rowm;              // ... its rows, sorted, in tiles of 8
   EwSfmWave *d_sfmw;           // ... its waves' tiles (4 a workgroup)
   double *d_sfmtab;            // ... per-charge phase tables (charges padded to whole tiles)
+  int efm, efm_ncb;            // reciprocal force as matrix products (k_ewforcem), charge blocks
+  EwFTile *d_ftile;            // ... its row tiles
+  int4 *d_ftent;               // ... their coefficient entries
+  int *d_frtile;               // ... ranges (tile bounds)
   int nrange, nqb, ntile;      // force: tile ranges, charge workgroups, tiles
   double *d_epair, *d_eew, *d_eec, *d_eout;
   int npair_blocks, new_blocks, nfin_blocks;

@@ -1544,6 +1544,138 @@ static __device__ __forceinline__ void k_ewforce_body(const EwForceArgs &A, int 
 #endif
 NB_KERNEL_PAIR_W(k_ewforce, EwForceArgs, 256, NB_EWF_WPE)
 
+// k_ewforcem: the reciprocal force pass on the FP64 matrix cores (see
+// k_ewsfacm for the v_mfma_f64_16x16x4f64 lane layout).  For a tile of 16
+// k rows and 16 charges, P and Q of k_ewforce for every (row, charge) are
+// four products D = A B over the K index (m, c / s), 2 values of m a step:
+// A[row][(m, c)] is the row's coefficient of cos (c = 0) or sin (c = 1) of
+// m theta_c in Re P (Er, -Di), Im P (Ei, Dr), Re Q (m Dr, -m Ei) or Im Q
+// (m Di, m Er), staged in LDS once per tile for the workgroup; B[(m, c)]
+// [charge] is cos / sin m theta_c, which each lane steps in registers by
+// z^2 (z = e^{i theta_c}).  The result gives a lane 4 rows of one run of
+// consecutive kb for one charge, so w = e^{i (ka theta_a + kb theta_b)}
+// takes one sine / cosine per run and steps of e^{i theta_b}, and the lane
+// adds A Im(w P) + Gc Im(w Q) for its rows to its charge's force; the 4
+// lane groups' sums are combined at the end into the range's partial.
+struct EwFmArgs {
+  const EwFTile *tile; const int4 *tent; const int *rtile; const KData *kd;
+  const double4 *chg, *stp; int nq, ncb; double *fpart; Box b;
+};
+#define EFM_MS 40  // values of m a tile may span (the host keeps m < 40)
+static __device__ __forceinline__ void k_ewforcem_body(const EwFmArgs &A, int bid) {
+#pragma clang fp contract(fast)
+  constexpr int CT = NB_EFM_CT, ES = NB_EFM_ESTR;
+  __shared__ double kl[EFM_MS * 16 * ES];
+  const int r = bid / A.ncb, cb = bid % A.ncb;
+  const int lane = lane_id(), kq = lane >> 4, j16 = lane & 15, w = threadIdx.x >> 6;
+  double ta[CT], tb[CT], tcz[CT], zr[CT], zi[CT], z2r[CT], z2i[CT], ebr[CT], ebi[CT];
+  double fx[CT], fy[CT], fz[CT];
+#pragma unroll
+  for (int ct = 0; ct < CT; ct++) {
+    const int c = ((cb * 4 + w) * CT + ct) * 16 + j16;
+    double4 q4 = make_double4(0., 0., 0., 0.), st = make_double4(1., 0., 1., 0.);
+    if (c < A.nq) { q4 = A.chg[c]; st = A.stp[c]; }
+    ta[ct] = q4.x; tb[ct] = q4.y; tcz[ct] = q4.z;
+    zr[ct] = st.x; zi[ct] = st.y;
+    z2r[ct] = st.x * st.x - st.y * st.y;
+    z2i[ct] = 2. * st.x * st.y;
+    nb_sincos(q4.y, &ebi[ct], &ebr[ct]);
+    fx[ct] = fy[ct] = fz[ct] = 0.;
+  }
+  const double tpi = 2. * M_PI, *hi = A.b.hi;
+  const double gcx = tpi * hi[6], gcy = tpi * hi[7], gcz = tpi * hi[8];
+  const int t0 = A.rtile[r], t1 = A.rtile[r + 1];
+  for (int t = t0; t < t1; t++) {
+    const int ent_off = A.tile[t].ent_off, mt0 = A.tile[t].mt0, nks = A.tile[t].nks;
+    // this lane's run: kq (a per-lane index, read from memory directly)
+    const double ka = A.tile[t].ka[kq], kb0 = A.tile[t].kb0[kq];
+    const int ne = 2 * nks * 16;
+    __syncthreads();
+    for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+      const int4 en = A.tent[ent_off + e];
+      const KData p = A.kd[en.x >= 0 ? en.x : 0], q = A.kd[en.y >= 0 ? en.y : 0];
+      const double pr = en.x >= 0 ? p.cr : 0., pi = en.x >= 0 ? p.ci : 0.;
+      const double mr = en.y >= 0 ? q.cr : 0., mi = en.y >= 0 ? q.ci : 0.;
+      const double er = pr + mr, ei = -(pi + mi), dr = pr - mr, di = mi - pi, fm = en.z;
+      double *d = kl + e * ES;  // e = mm * 16 + row
+      d[0] = er; d[1] = -di; d[2] = ei; d[3] = dr;
+      d[4] = fm * dr; d[5] = -(fm * ei); d[6] = fm * di; d[7] = fm * er;
+    }
+    __syncthreads();
+    // z^m for m = mt0 + (kq >> 1): from 1, z, z^2 when mt0 <= 1, else a sine / cosine
+    double cr[CT], ci[CT];
+    const int m0 = mt0 + (kq >> 1);
+#pragma unroll
+    for (int ct = 0; ct < CT; ct++) {
+      if (mt0 <= 1) {
+        cr[ct] = m0 == 0 ? 1. : (m0 == 1 ? zr[ct] : z2r[ct]);
+        ci[ct] = m0 == 0 ? 0. : (m0 == 1 ? zi[ct] : z2i[ct]);
+      } else {
+        nb_sincos(m0 * tcz[ct], &ci[ct], &cr[ct]);
+      }
+    }
+    sfm_d4 acc[CT][4];
+#pragma unroll
+    for (int ct = 0; ct < CT; ct++)
+#pragma unroll
+      for (int k = 0; k < 4; k++) acc[ct][k] = sfm_d4{0., 0., 0., 0.};
+    const double *ab = kl + ((kq >> 1) * 16 + j16) * ES + (kq & 1);
+    const bool sn = kq & 1;
+#pragma unroll 1
+    for (int ks = 0; ks < nks; ks++) {
+      const double *a = ab + ks * 32 * ES;
+      const double a0 = a[0], a1 = a[2], a2 = a[4], a3 = a[6];
+#pragma unroll
+      for (int ct = 0; ct < CT; ct++) {
+        const double bv = sn ? ci[ct] : cr[ct];
+        acc[ct][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bv, acc[ct][0], 0, 0, 0);
+        acc[ct][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bv, acc[ct][1], 0, 0, 0);
+        acc[ct][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, bv, acc[ct][2], 0, 0, 0);
+        acc[ct][3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a3, bv, acc[ct][3], 0, 0, 0);
+        const double nr = cr[ct] * z2r[ct] - ci[ct] * z2i[ct];
+        ci[ct] = cr[ct] * z2i[ct] + ci[ct] * z2r[ct];
+        cr[ct] = nr;
+      }
+    }
+    // this lane's rows: run kq, rows rr = 0..3 (kb = kb0 + rr), result
+    // double rr of each product
+#pragma unroll
+    for (int ct = 0; ct < CT; ct++) {
+      double wr, wi;
+      nb_sincos(ka * ta[ct] + kb0 * tb[ct], &wi, &wr);
+#pragma unroll
+      for (int rr = 0; rr < 4; rr++) {
+        const double kb = kb0 + rr;
+        const double s0 = wr * acc[ct][1][rr] + wi * acc[ct][0][rr];  // Im(w P)
+        const double s1 = wr * acc[ct][3][rr] + wi * acc[ct][2][rr];  // Im(w Q)
+        const double ax = tpi * (ka * hi[0] + kb * hi[3]);
+        const double ay = tpi * (ka * hi[1] + kb * hi[4]);
+        const double az = tpi * (ka * hi[2] + kb * hi[5]);
+        fx[ct] += ax * s0 + gcx * s1;
+        fy[ct] += ay * s0 + gcy * s1;
+        fz[ct] += az * s0 + gcz * s1;
+        const double w2 = wr * ebr[ct] - wi * ebi[ct];
+        wi = wr * ebi[ct] + wi * ebr[ct];
+        wr = w2;
+      }
+    }
+  }
+  // the 4 lane groups hold the same 16 charges' sums over different rows
+#pragma unroll
+  for (int ct = 0; ct < CT; ct++) {
+    fx[ct] += __shfl_xor(fx[ct], 16); fy[ct] += __shfl_xor(fy[ct], 16); fz[ct] += __shfl_xor(fz[ct], 16);
+    fx[ct] += __shfl_xor(fx[ct], 32); fy[ct] += __shfl_xor(fy[ct], 32); fz[ct] += __shfl_xor(fz[ct], 32);
+    const int c = ((cb * 4 + w) * CT + ct) * 16 + j16;
+    if (kq == 0 && c < A.nq) {
+      double *o = A.fpart + (size_t)(3 * r) * A.nq + c;
+      o[0] = fx[ct];
+      o[A.nq] = fy[ct];
+      o[2 * (size_t)A.nq] = fz[ct];
+    }
+  }
+}
+NB_KERNEL_PAIR_W(k_ewforcem, EwFmArgs, 256, 2)
+
 // k_ewfin, a workgroup per 64 charges (lane = charge): the four waves sum
 // interleaved quarters of the range partials (ranges w, w+4, ...), combined
 // in LDS in a fixed order; wave 0 then does ecorr (md_ewald.c:385-453) over
@@ -1801,7 +1933,12 @@ int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_f
   EwForceArgs ef = {c->d_row, c->d_ent, c->d_rtile, c->d_trow, (const KData *)c->d_kd,
                     (const double4 *)c->d_chg, (const double4 *)c->d_stp, nq, c->nqb, c->d_fpart,
                     b};
-  NB_GO(c, K_EWFORCE, k_ewforce, rp && ewf ? c->nqb * c->nrange : 0, 256, 0, ef, s);
+  NB_GO(c, K_EWFORCE, k_ewforce, rp && ewf && !c->efm ? c->nqb * c->nrange : 0, 256, 0, ef, s);
+  EwFmArgs em = {c->d_ftile, c->d_ftent, c->d_frtile, (const KData *)c->d_kd,
+                 (const double4 *)c->d_chg, (const double4 *)c->d_stp, nq, c->efm_ncb, c->d_fpart,
+                 b};
+  NB_GO(c, K_EWFORCEM, k_ewforcem, rp && ewf && c->efm ? c->efm_ncb * c->nrange : 0, 256, 0, em,
+        s);
   nb_time_mark(c, 6, s);
   EwFinArgs fa = {c->d_fpart, (const double4 *)c->d_chg, c->d_qidx, nq, n, c->nrange, ewf, ec,
                   c->d_eoff, c->d_elist, c->d_pos, c->d_qch, c->kappa, d_fl, c->nfin_blocks,
@@ -1846,6 +1983,7 @@ static struct RegisterKernels {
     g_batch_fn[K_EWSFACR] = k_ewsfacr_bl;
     g_batch_fn[K_EWSFACM] = k_ewsfacm_bl;
     g_batch_fn[K_EWSFMTAB] = k_ewsfmtab_bl;
+    g_batch_fn[K_EWFORCEM] = k_ewforcem_bl;
     g_batch_fn[K_EWFIN] = k_ewfin_bl;
   }
 } g_register_kernels;

@@ -298,11 +298,12 @@ def test_multirow_structure_factor_lanes(name):
 @pytest.mark.parametrize("name", ["water512", "water512_tri", "mix600", "mix600_tri",
                                   "ref_abutane"])
 def test_matrix_core_structure_factor(name, monkeypatch):
-    """The structure factor as FP64 matrix-core products (k_ewsfacm, used
-    by default for large k spheres and charge counts) forced on the
-    fixtures (CM3D_NB_SFAC_MFMA=1): the reciprocal energy, virial and
-    forces stay within the FP64 bar of cm3d's, and within rounding of the
-    VALU kernel's (CM3D_NB_SFAC_MFMA=0), in both precisions."""
+    """The reciprocal sum as FP64 matrix-core products (k_ewsfacm and
+    k_ewforcem, used by default for large k spheres and charge counts)
+    forced on the fixtures (CM3D_NB_SFAC_MFMA=1, which the force pass
+    follows): the reciprocal energy, virial and forces stay within the FP64
+    bar of cm3d's, and within rounding of the VALU kernels'
+    (CM3D_NB_SFAC_MFMA=0), in both precisions."""
     d = load_fixture(name)
     from cm3d_amd import nb
     for prec in ("fp64", "mixed"):
@@ -323,6 +324,29 @@ def test_matrix_core_structure_factor(name, monkeypatch):
         if prec == "fp64":
             assert rel_rms(fl, d["fxl"]) < FP64_TOL, rel_rms(fl, d["fxl"])
             assert _rel(e["vk_ewald"], d["vk_ewald"][0]) < FP64_TOL
+
+
+@pytest.mark.parametrize("sfac,ewf", [("1", "0"), ("0", "1")])
+@pytest.mark.parametrize("name", ["water512", "water512_tri", "ref_abutane", "irregular"])
+def test_matrix_core_reciprocal_kernels_each(name, sfac, ewf, monkeypatch):
+    """Each matrix-core reciprocal kernel on its own (CM3D_NB_SFAC_MFMA,
+    CM3D_NB_EWF_MFMA) against both VALU kernels: forces and reciprocal
+    energy to rounding, FP64.  "irregular": water512 with the cut k rows
+    (rows from |kc| = 3 up, so that force tiles start above m = 1)."""
+    d = _irregular_klist(load_fixture("water512")) if name == "irregular" else load_fixture(name)
+    from cm3d_amd import nb
+    out = {}
+    for flags in ((sfac, ewf), ("0", "0")):
+        monkeypatch.setenv("CM3D_NB_SFAC_MFMA", flags[0])
+        monkeypatch.setenv("CM3D_NB_EWF_MFMA", flags[1])
+        c = nb.NBContext(d, precision="fp64")
+        c.build_lists(d["pos"], d["hmat"])
+        out[flags] = c.force(d["pos"], d["hmat"], iflg=1)
+        c.close()
+    a, b = out[(sfac, ewf)], out[("0", "0")]
+    assert np.isfinite(a[1]).all()
+    assert rel_rms(a[1], b[1]) < 1e-13, (name, sfac, ewf, rel_rms(a[1], b[1]))
+    assert _rel(a[2]["vk_ewald"], b[2]["vk_ewald"]) < 1e-13
 
 
 def test_matrix_core_structure_factor_full_size(monkeypatch):
