@@ -288,20 +288,39 @@ def run_c3(nb, S, a, torch):
         drift[name] = _drift([m], [s], a.c3_drift_steps, lambda k, m=m: m.step(k, graph=True))
     npairs, cnt_rc = pairs_within_rc(nb, s, pos_eq)
     counts = ctx.list_counts()
+    info = ctx.recip_info()
     md.close(); md64.close(); ctx.close(); ctx64.close()
     return s, dict(n=n, wall=wall, wall64=wall64, lat=lat, rebuilds=nreb, tm=tm, tm64=tm64,
-                   ferr=ferr, drift=drift, npairs=npairs, cnt_rc=cnt_rc, counts=counts)
+                   ferr=ferr, drift=drift, npairs=npairs, cnt_rc=cnt_rc, counts=counts,
+                   info=info)
 
 
-def ewald_roofline(name, tm_ms, flop):
+def ewald_roofline(name, tm_ms, flop, mfma_insts=0):
+    """A reciprocal-sum kernel against the FP64 peak (78.6 TFLOP/s, vector
+    and matrix cores alike on MI355X).  achieved: the algorithmic FLOP per
+    (k vector, charge) -- 10 structure factor, 16 force pass, cm3d's
+    recursion and sums -- times K x Q, over the launch time.  The kernels
+    pair kc = +m and -m and (on the matrix cores) share each charge's
+    cos / sin of m theta_c across rows, so they execute fewer FP64
+    operations than that count; a matrix-core kernel's executed FLOP (2048
+    a v_mfma_f64_16x16x4f64, the count the library reports) are given
+    beside it."""
     tf = flop / (tm_ms * 1e-3) / 1e12
-    return {"kernel": name, "bound": "valu-fp64", "achieved": round(tf, 3),
-            "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(tf / FP64_PEAK_TFLOPS, 5),
-            "ms_per_launch": round(tm_ms, 5), "flop_per_launch": flop,
-            "flop_basis": "algorithmic: per (k vector, charge) 10 FLOP structure factor, 16 FLOP "
-                          "force pass (cm3d's c-axis recursion); the kernels serve kc = +m and -m "
-                          "with one phase step and execute fewer operations -- valu_issue_frac "
-                          "is their executed VALU wave-instructions against the issue peak"}
+    out = {"kernel": name, "bound": "mfma-fp64" if mfma_insts else "valu-fp64",
+           "achieved": round(tf, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+           "frac": round(tf / FP64_PEAK_TFLOPS, 5), "ms_per_launch": round(tm_ms, 5),
+           "flop_per_launch": flop,
+           "flop_basis": "algorithmic: per (k vector, charge) 10 FLOP structure factor, 16 FLOP "
+                         "force pass (cm3d's c-axis recursion); the kernels serve kc = +m and -m "
+                         "with one phase step (and, on the matrix cores, one cos/sin of m "
+                         "theta_c per charge for all rows), so they execute fewer operations: "
+                         "executed_* (matrix cores) or valu_issue_frac (VALU) give that work "
+                         "against the peak"}
+    if mfma_insts:
+        ex = 2048.0 * mfma_insts / (tm_ms * 1e-3) / 1e12
+        out.update(mfma_insts_per_launch=int(mfma_insts), executed_tflops=round(ex, 3),
+                   executed_frac=round(ex / FP64_PEAK_TFLOPS, 5))
+    return out
 
 
 def run_c2(nb, S, a, torch):
@@ -489,15 +508,20 @@ def main():
     tm, tm64 = r["tm"], r["tm64"]
     # the dominant kernel of the mixed step: the longest phase of force(...,1)
     kq = float(ktot) * nq
-    phases = {"k_ewsfac": (tm["sfac"], EW_SFAC_FLOP * kq),
-              "k_ewforce": (tm["ewforce"], EW_FORCE_FLOP * kq)}
+    info = r["info"]
+    # the structure-factor phase on the matrix cores is k_ewsfmtab (phase
+    # tables) + k_ewsfacm
+    ksf = "k_ewsfacm" if info["sfac_mfma"] else "k_ewsfac"
+    kfo = "k_ewforcem" if info["force_mfma"] else "k_ewforce"
+    phases = {ksf: (tm["sfac"], EW_SFAC_FLOP * kq, info["sfac_mfma_insts"]),
+              kfo: (tm["ewforce"], EW_FORCE_FLOP * kq, info["force_mfma_insts"])}
     dom = max(phases, key=lambda k: phases[k][0])
     roof = ewald_roofline(dom, *phases[dom])
     pj = _profile_json(f"{dom}_water32768_traffic.json")
     roof["traffic"] = round(float(pj["traffic_bytes_per_launch"]), 0) if pj else None
     roof["traffic_source"] = (f"profiles/ewald_water32768_counters.md (rocprofv3 --pmc "
                               f"FETCH_SIZE / WRITE_SIZE, copy-calibrated)") if pj else None
-    other = "k_ewforce" if dom == "k_ewsfac" else "k_ewsfac"
+    other = kfo if dom == ksf else ksf
     roof_o = ewald_roofline(other, *phases[other])
     for rf in (roof, roof_o):
         vj = _profile_json(f"{rf['kernel']}_water32768_valu.json")

@@ -510,6 +510,15 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
       nch = std::min(nch, std::max(1, nq16 / 4));  // at least 4 tiles a chunk
       c->qc = NB_SFM_NQ * ((nq16 + nch - 1) / nch);
       c->nchunk = std::max(1, (ncharge + c->qc - 1) / c->qc);
+      // matrix-core instructions a launch issues: each wave R x NT a K step
+      // of 4 charges, for every 16-charge tile of its chunk
+      long long per_step = 0, steps = 0;
+      for (const EwSfmWave &w : sfmw) per_step += (long long)w.nr * w.nt;
+      for (int ch = 0; ch < c->nchunk; ch++) {
+        const int nc = std::min(ncharge, (ch + 1) * c->qc) - ch * c->qc;
+        steps += 4LL * ((nc + NB_SFM_NQ - 1) / NB_SFM_NQ);
+      }
+      c->sfm_mfma = per_step * steps;
     }
   }
   // force: NB_EWF_Q charges per lane, a workgroup per (256 NB_EWF_Q
@@ -630,6 +639,8 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
       }
       frtile.push_back(nt);
       c->nrange = (int)frtile.size() - 1;
+      // each wave: NB_EFM_CT charge tiles x 4 products a K step of every tile
+      c->efm_mfma = (long long)c->efm_ncb * 4 * NB_EFM_CT * 4 * tot;
     }
   }
 
@@ -1159,6 +1170,15 @@ extern "C" int nb_energy_only(nb_ctx *c, const double *pxyz, const double hmat[9
   HCHK(hipMemcpyAsync(E, c->d_eout, sizeof(E), hipMemcpyDeviceToHost, c->stream));
   HCHK(hipStreamSynchronize(c->stream));
   nb_fill_energy(c, 1, E, hmat, e, true);
+  return NB_OK;
+}
+
+extern "C" int nb_recip_info(nb_ctx *c, long long out[4]) {
+  if (!c || !out) { nb_set_error("nb_recip_info: null argument"); return NB_ERR_ARG; }
+  out[0] = (c->sfm ? 1 : 0) | (c->efm ? 2 : 0);
+  out[1] = c->sfm ? c->sfm_mfma : 0;
+  out[2] = c->efm ? c->efm_mfma : 0;
+  out[3] = 0;
   return NB_OK;
 }
 

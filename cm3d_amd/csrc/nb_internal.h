@@ -340,6 +340,7 @@ struct nb_ctx {
   EwSfmWave *d_sfmw;           // ... its waves' tiles (4 a workgroup)
   double *d_sfmtab;            // ... per-charge phase tables (charges padded to whole tiles)
   int efm, efm_ncb;            // reciprocal force as matrix products (k_ewforcem), charge blocks
+  long long sfm_mfma, efm_mfma;  // matrix-core instructions a structure-factor / force launch issues
   EwFTile *d_ftile;            // ... its row tiles
   int4 *d_ftent;               // ... their coefficient entries
   int *d_frtile;               // ... ranges (tile bounds)

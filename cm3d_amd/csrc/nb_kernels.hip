@@ -1554,9 +1554,10 @@ NB_KERNEL_PAIR_W(k_ewforce, EwForceArgs, 256, NB_EWF_WPE)
 // [charge] is cos / sin m theta_c, which each lane steps in registers by
 // z^2 (z = e^{i theta_c}).  The result gives a lane 4 rows of one run of
 // consecutive kb for one charge, so w = e^{i (ka theta_a + kb theta_b)}
-// takes one sine / cosine per run and steps of e^{i theta_b}, and the lane
-// adds A Im(w P) + Gc Im(w Q) for its rows to its charge's force; the 4
-// lane groups' sums are combined at the end into the range's partial.
+// takes one sine / cosine per run and steps of e^{i theta_b}; the lane sums
+// ka Im(w P), kb Im(w P) and Im(w Q) over its rows for its charge, and the
+// force A Im(w P) + Gc Im(w Q) (A = 2 pi (ka h^-T_a + kb h^-T_b)) follows
+// from the 4 lane groups' combined sums at the end.
 struct EwFmArgs {
   const EwFTile *tile; const int4 *tent; const int *rtile; const KData *kd;
   const double4 *chg, *stp; int nq, ncb; double *fpart; Box b;
@@ -1568,19 +1569,27 @@ static __device__ __forceinline__ void k_ewforcem_body(const EwFmArgs &A, int bi
   __shared__ double kl[EFM_MS * 16 * ES];
   const int r = bid / A.ncb, cb = bid % A.ncb;
   const int lane = lane_id(), kq = lane >> 4, j16 = lane & 15, w = threadIdx.x >> 6;
-  double ta[CT], tb[CT], tcz[CT], zr[CT], zi[CT], z2r[CT], z2i[CT], ebr[CT], ebi[CT];
-  double fx[CT], fy[CT], fz[CT];
+  // a lane's B operand is cos (kq even) or sin (kq odd) of m theta_c; it
+  // keeps that one first: (x, y) = (Re, Im) or (Im, Re) of z^m, stepped by
+  // z^2 as x' = x Re z^2 + y u, y' = y Re z^2 - x u with u = -Im z^2 or
+  // +Im z^2
+  const bool sn = kq & 1;
+  double ta[CT], tb[CT], zr[CT], zi[CT], z2r[CT], cu[CT], ebr[CT], ebi[CT];
+  // per charge: sum over rows of ka Im(w P), kb Im(w P) and Im(w Q); the
+  // force is 2 pi (h^-T_a sum ka.. + h^-T_b sum kb..) + Gc sum Im(w Q)
+  double sa[CT], sb[CT], s1q[CT];
 #pragma unroll
   for (int ct = 0; ct < CT; ct++) {
     const int c = ((cb * 4 + w) * CT + ct) * 16 + j16;
     double4 q4 = make_double4(0., 0., 0., 0.), st = make_double4(1., 0., 1., 0.);
     if (c < A.nq) { q4 = A.chg[c]; st = A.stp[c]; }
-    ta[ct] = q4.x; tb[ct] = q4.y; tcz[ct] = q4.z;
+    ta[ct] = q4.x; tb[ct] = q4.y;
     zr[ct] = st.x; zi[ct] = st.y;
     z2r[ct] = st.x * st.x - st.y * st.y;
-    z2i[ct] = 2. * st.x * st.y;
+    const double z2i = 2. * st.x * st.y;
+    cu[ct] = sn ? z2i : -z2i;
     nb_sincos(q4.y, &ebi[ct], &ebr[ct]);
-    fx[ct] = fy[ct] = fz[ct] = 0.;
+    sa[ct] = sb[ct] = s1q[ct] = 0.;
   }
   const double tpi = 2. * M_PI, *hi = A.b.hi;
   const double gcx = tpi * hi[6], gcy = tpi * hi[7], gcz = tpi * hi[8];
@@ -1602,75 +1611,94 @@ static __device__ __forceinline__ void k_ewforcem_body(const EwFmArgs &A, int bi
       d[4] = fm * dr; d[5] = -(fm * ei); d[6] = fm * di; d[7] = fm * er;
     }
     __syncthreads();
-    // z^m for m = mt0 + (kq >> 1): from 1, z, z^2 when mt0 <= 1, else a sine / cosine
-    double cr[CT], ci[CT];
+    // z^m for m = mt0 + (kq >> 1): from 1, z, z^2 when mt0 <= 1, else a
+    // sine / cosine of the charge's theta_c
+    double cx[CT], cy[CT];
     const int m0 = mt0 + (kq >> 1);
 #pragma unroll
     for (int ct = 0; ct < CT; ct++) {
+      double cr, ci;
       if (mt0 <= 1) {
-        cr[ct] = m0 == 0 ? 1. : (m0 == 1 ? zr[ct] : z2r[ct]);
-        ci[ct] = m0 == 0 ? 0. : (m0 == 1 ? zi[ct] : z2i[ct]);
+        const double z2i = sn ? cu[ct] : -cu[ct];
+        cr = m0 == 0 ? 1. : (m0 == 1 ? zr[ct] : z2r[ct]);
+        ci = m0 == 0 ? 0. : (m0 == 1 ? zi[ct] : z2i);
       } else {
-        nb_sincos(m0 * tcz[ct], &ci[ct], &cr[ct]);
+        const int c = ((cb * 4 + w) * CT + ct) * 16 + j16;
+        nb_sincos(c < A.nq ? m0 * A.chg[c].z : 0., &ci, &cr);
+      }
+      cx[ct] = sn ? ci : cr;
+      cy[ct] = sn ? cr : ci;
+    }
+    const double *ab = kl + ((kq >> 1) * 16 + j16) * ES + (kq & 1);
+    sfm_d4 acc[CT][4];
+    {  // the first K step starts the sums (no zeroing)
+      const double a0 = ab[0], a1 = ab[2], a2 = ab[4], a3 = ab[6];
+      const sfm_d4 zero = {0., 0., 0., 0.};
+#pragma unroll
+      for (int ct = 0; ct < CT; ct++) {
+        const double bv = cx[ct];
+        acc[ct][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bv, zero, 0, 0, 0);
+        acc[ct][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bv, zero, 0, 0, 0);
+        acc[ct][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, bv, zero, 0, 0, 0);
+        acc[ct][3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a3, bv, zero, 0, 0, 0);
+        const double nx = cx[ct] * z2r[ct] + cy[ct] * cu[ct];
+        cy[ct] = cy[ct] * z2r[ct] - cx[ct] * cu[ct];
+        cx[ct] = nx;
       }
     }
-    sfm_d4 acc[CT][4];
-#pragma unroll
-    for (int ct = 0; ct < CT; ct++)
-#pragma unroll
-      for (int k = 0; k < 4; k++) acc[ct][k] = sfm_d4{0., 0., 0., 0.};
-    const double *ab = kl + ((kq >> 1) * 16 + j16) * ES + (kq & 1);
-    const bool sn = kq & 1;
 #pragma unroll 1
-    for (int ks = 0; ks < nks; ks++) {
+    for (int ks = 1; ks < nks; ks++) {
       const double *a = ab + ks * 32 * ES;
       const double a0 = a[0], a1 = a[2], a2 = a[4], a3 = a[6];
 #pragma unroll
       for (int ct = 0; ct < CT; ct++) {
-        const double bv = sn ? ci[ct] : cr[ct];
+        const double bv = cx[ct];
         acc[ct][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bv, acc[ct][0], 0, 0, 0);
         acc[ct][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bv, acc[ct][1], 0, 0, 0);
         acc[ct][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, bv, acc[ct][2], 0, 0, 0);
         acc[ct][3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a3, bv, acc[ct][3], 0, 0, 0);
-        const double nr = cr[ct] * z2r[ct] - ci[ct] * z2i[ct];
-        ci[ct] = cr[ct] * z2i[ct] + ci[ct] * z2r[ct];
-        cr[ct] = nr;
+        const double nx = cx[ct] * z2r[ct] + cy[ct] * cu[ct];
+        cy[ct] = cy[ct] * z2r[ct] - cx[ct] * cu[ct];
+        cx[ct] = nx;
       }
     }
     // this lane's rows: run kq, rows rr = 0..3 (kb = kb0 + rr), result
-    // double rr of each product
+    // double rr of each product; w of each charge from one sine / cosine,
+    // then steps of e^{i theta_b}
+    double wr[CT], wi[CT];
+#pragma unroll
+    for (int ct = 0; ct < CT; ct++) nb_sincos(ka * ta[ct] + kb0 * tb[ct], &wi[ct], &wr[ct]);
 #pragma unroll
     for (int ct = 0; ct < CT; ct++) {
-      double wr, wi;
-      nb_sincos(ka * ta[ct] + kb0 * tb[ct], &wi, &wr);
+      double p0 = 0., pb = 0.;  // sum of Im(w P), of kb Im(w P) over the run
 #pragma unroll
       for (int rr = 0; rr < 4; rr++) {
-        const double kb = kb0 + rr;
-        const double s0 = wr * acc[ct][1][rr] + wi * acc[ct][0][rr];  // Im(w P)
-        const double s1 = wr * acc[ct][3][rr] + wi * acc[ct][2][rr];  // Im(w Q)
-        const double ax = tpi * (ka * hi[0] + kb * hi[3]);
-        const double ay = tpi * (ka * hi[1] + kb * hi[4]);
-        const double az = tpi * (ka * hi[2] + kb * hi[5]);
-        fx[ct] += ax * s0 + gcx * s1;
-        fy[ct] += ay * s0 + gcy * s1;
-        fz[ct] += az * s0 + gcz * s1;
-        const double w2 = wr * ebr[ct] - wi * ebi[ct];
-        wi = wr * ebi[ct] + wi * ebr[ct];
-        wr = w2;
+        const double s0 = wr[ct] * acc[ct][1][rr] + wi[ct] * acc[ct][0][rr];  // Im(w P)
+        const double s1 = wr[ct] * acc[ct][3][rr] + wi[ct] * acc[ct][2][rr];  // Im(w Q)
+        p0 += s0;
+        pb += (kb0 + rr) * s0;
+        s1q[ct] += s1;
+        if (rr < 3) {
+          const double w2 = wr[ct] * ebr[ct] - wi[ct] * ebi[ct];
+          wi[ct] = wr[ct] * ebi[ct] + wi[ct] * ebr[ct];
+          wr[ct] = w2;
+        }
       }
+      sa[ct] += ka * p0;
+      sb[ct] += pb;
     }
   }
   // the 4 lane groups hold the same 16 charges' sums over different rows
 #pragma unroll
   for (int ct = 0; ct < CT; ct++) {
-    fx[ct] += __shfl_xor(fx[ct], 16); fy[ct] += __shfl_xor(fy[ct], 16); fz[ct] += __shfl_xor(fz[ct], 16);
-    fx[ct] += __shfl_xor(fx[ct], 32); fy[ct] += __shfl_xor(fy[ct], 32); fz[ct] += __shfl_xor(fz[ct], 32);
+    sa[ct] += __shfl_xor(sa[ct], 16); sb[ct] += __shfl_xor(sb[ct], 16); s1q[ct] += __shfl_xor(s1q[ct], 16);
+    sa[ct] += __shfl_xor(sa[ct], 32); sb[ct] += __shfl_xor(sb[ct], 32); s1q[ct] += __shfl_xor(s1q[ct], 32);
     const int c = ((cb * 4 + w) * CT + ct) * 16 + j16;
     if (kq == 0 && c < A.nq) {
       double *o = A.fpart + (size_t)(3 * r) * A.nq + c;
-      o[0] = fx[ct];
-      o[A.nq] = fy[ct];
-      o[2 * (size_t)A.nq] = fz[ct];
+      o[0] = tpi * (hi[0] * sa[ct] + hi[3] * sb[ct]) + gcx * s1q[ct];
+      o[A.nq] = tpi * (hi[1] * sa[ct] + hi[4] * sb[ct]) + gcy * s1q[ct];
+      o[2 * (size_t)A.nq] = tpi * (hi[2] * sa[ct] + hi[5] * sb[ct]) + gcz * s1q[ct];
     }
   }
 }

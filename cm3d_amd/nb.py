@@ -103,7 +103,7 @@ class NBMDEnergy(C.Structure):
 EXPORTS = ["nb_create", "nb_destroy", "nb_last_error", "nb_build_lists",
            "nb_need_rebuild", "nb_force", "nb_force_device", "nb_energy_only",
            "nb_build_lists_device", "nb_list_counts", "nb_get_pairs", "nb_timing",
-           "nb_footprint", "nb_list_reserve", "nb_force_device_noalloc",
+           "nb_footprint", "nb_recip_info", "nb_list_reserve", "nb_force_device_noalloc",
            "nb_md_create", "nb_md_destroy", "nb_md_set_state", "nb_md_get_state",
            "nb_md_init", "nb_md_step", "nb_md_energies", "nb_md_rebuilds", "nb_md_get_forces",
            "nb_md_set_thermostat", "nb_md_get_thermostat",
@@ -143,6 +143,7 @@ def lib():
     L.nb_energy_only.argtypes = [C.c_void_p, _dp, _dp, C.POINTER(NBEnergy)]
     L.nb_list_counts.argtypes = [C.c_void_p, _ip]
     L.nb_footprint.argtypes = [C.c_void_p, C.POINTER(C.c_size_t)]
+    L.nb_recip_info.argtypes = [C.c_void_p, C.POINTER(C.c_longlong)]
     L.nb_get_pairs.argtypes = [C.c_void_p, C.c_int, _ip, _ip, C.c_int, _ip]
     L.nb_timing.argtypes = [C.c_void_p, C.c_int, _dp]
     L.nb_md_create.argtypes = [C.c_void_p, C.POINTER(NBMDParams), C.POINTER(C.c_void_p)]
@@ -337,6 +338,15 @@ class NBContext:
         b = C.c_size_t(0)
         check(lib().nb_footprint(self._h, C.byref(b)), "nb_footprint")
         return int(b.value)
+
+    def recip_info(self):
+        """Which reciprocal kernels this context launches (nb_recip_info):
+        {"sfac_mfma", "force_mfma"} flags and the matrix-core instructions
+        one launch of each issues."""
+        o = (C.c_longlong * 4)()
+        check(lib().nb_recip_info(self._h, o), "nb_recip_info")
+        return {"sfac_mfma": bool(o[0] & 1), "force_mfma": bool(o[0] & 2),
+                "sfac_mfma_insts": int(o[1]), "force_mfma_insts": int(o[2])}
 
     def list_reserve(self, slots):
         """Neighbour slots per atom for the next build (nb_list_reserve);

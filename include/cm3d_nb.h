@@ -174,6 +174,15 @@ int nb_list_counts(nb_ctx *ctx, int counts[4]);
  * accounts with. */
 int nb_footprint(nb_ctx *ctx, size_t *bytes);
 
+/* How the context evaluates the reciprocal sum (no counterpart in cm3d:
+ * it reports which kernels a force call launches, for diagnostics and
+ * measurement).  out[0]: bit 0 set when the structure factor runs on the
+ * FP64 matrix cores, bit 1 when the reciprocal force pass does; out[1],
+ * out[2]: matrix-core instructions (v_mfma_f64_16x16x4f64, 2048 FLOP each)
+ * one launch of the structure factor / force pass issues (0 for the VALU
+ * kernels); out[3]: 0. */
+int nb_recip_info(nb_ctx *ctx, long long out[4]);
+
 /* Neighbour slots per atom for the next list build (rounded up to 64).  No
  * counterpart in cm3d, whose get_ngbr sizes its lists by counting with a
  * x1.2 margin (energy/control/ngbr_list.c:331-335); here slots are sized

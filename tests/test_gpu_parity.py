@@ -350,9 +350,10 @@ def test_matrix_core_reciprocal_kernels_each(name, sfac, ewf, monkeypatch):
 
 
 def test_matrix_core_structure_factor_full_size(monkeypatch):
-    """The 98,304-atom water of kmax 32 (68,532 k vectors), where the
-    matrix-core structure factor is the default: the same forces and
-    reciprocal energy as the VALU kernel to rounding."""
+    """The 98,304-atom water of kmax 32 (68,532 k vectors), where both
+    matrix-core reciprocal kernels are the default (nb_recip_info says so):
+    the same forces and reciprocal energy as the VALU kernels to
+    rounding."""
     from cm3d_amd import nb, systems as S
     s = S.water(nside=32, rc=12.0, alpha=0.30, kmax=32)
     out = {}
@@ -360,6 +361,13 @@ def test_matrix_core_structure_factor_full_size(monkeypatch):
         if flag:
             monkeypatch.setenv("CM3D_NB_SFAC_MFMA", flag)
         c = nb.NBContext(s, precision="mixed")
+        info = c.recip_info()
+        if flag is None:  # the default for this size: both passes on the matrix cores
+            assert info["sfac_mfma"] and info["force_mfma"], info
+            assert info["sfac_mfma_insts"] > 0 and info["force_mfma_insts"] > 0, info
+        else:
+            assert not info["sfac_mfma"] and not info["force_mfma"], info
+            assert info["sfac_mfma_insts"] == 0 and info["force_mfma_insts"] == 0, info
         c.build_lists(s["pos"], s["hmat"])
         out[flag] = c.force(s["pos"], s["hmat"], iflg=1)
         c.close()
