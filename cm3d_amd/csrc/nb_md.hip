@@ -762,7 +762,13 @@ extern "C" int nb_md_create(nb_ctx *c, const nb_md_params *p, nb_md **out) {
   memset(md, 0, sizeof(*md));
   md->c = c; md->n = n; md->nmol = nm; md->nbond = p->nbond; md->nbend = p->nbend;
   md->maxa = maxa;
-  md->fork = 1;
+  // the structure factor on the matrix cores keeps every SIMD's registers
+  // (two 242-register waves) for most of the step: a list build and pair
+  // kernel beside it wait for slots and slow it down, so then the two
+  // branches run one after the other (98,304-atom water: 2.49 ms a step
+  // against 2.57 forked; 12,288 atoms with the VALU passes: 0.112 forked
+  // against 0.121)
+  md->fork = c->sfm ? 0 : 1;
   md->ninnra = p->nres_intra > 0 ? p->nres_intra : 1;
   // md_step.c:73-78 with ninter = ninrra = 1
   md->dt = p->dt;

@@ -56,6 +56,34 @@ def test_md_trajectory_matches_cm3d_step(name, graph):
     ctx.close()
 
 
+@pytest.mark.parametrize("graph", [False, True])
+@pytest.mark.parametrize("name", ["water512", "water512_tri"])
+def test_md_trajectory_matrix_core_reciprocal(name, graph, monkeypatch):
+    """The same trajectories with both reciprocal passes forced onto the
+    matrix cores, as large systems run them: the step then launches the
+    list build, pair kernel and reciprocal sum on one stream instead of
+    forking the reciprocal sum onto the context's second stream."""
+    from cm3d_amd import nb
+    d = _system(name)
+    print(f"[md] {name} matrix-core reciprocal graph={graph}", flush=True)
+    monkeypatch.setenv("CM3D_NB_SFAC_MFMA", "1")
+    ctx = nb.NBContext(d, precision="fp64")
+    monkeypatch.delenv("CM3D_NB_SFAC_MFMA")
+    assert ctx.recip_info()["sfac_mfma"] and ctx.recip_info()["force_mfma"]
+    md = nb.NBMD(ctx, d, float(d["dt"][0]), nres_intra=int(d["ninnra"][0]))
+    md.set_state(d["traj_pos0"], d["traj_vel0"], d["hmat"])
+    md.init()
+    for s in range(1, int(d["traj_nsteps"][0]) + 1):
+        md.step(1, graph=graph)
+        pos, vel = md.get_state()
+        assert np.abs(pos - d[f"traj_pos{s}"]).max() < 1e-9, (name, s)
+        assert rel_rms(vel, d[f"traj_vel{s}"]) < 1e-9, (name, s)
+    e = md.energies()
+    assert abs(e["poter"] - d[f"traj_poter{s}"][0]) <= 1e-9 * max(1.0, abs(d[f"traj_poter{s}"][0]))
+    md.close()
+    ctx.close()
+
+
 def _long_run(sysd, prec, nsteps, graph, nres_intra=4):
     from cm3d_amd import nb
     ctx = nb.NBContext(sysd, precision=prec)
