@@ -123,7 +123,7 @@ extern "C" void nb_destroy(nb_ctx *c) {
   hipDeviceSynchronize();
   void *ptrs[] = {c->d_itype, c->d_map, c->d_ip, c->d_vtab3, c->d_ctab3, c->d_vtab3f,
                   c->d_ctab3f, c->d_qch, c->d_xoff, c->d_xlist, c->d_eoff, c->d_elist,
-                  c->d_qidx, c->d_qpos, c->d_row, c->d_sym, c->d_rowm, c->d_sfmw, c->d_sfmtab, c->d_ftile, c->d_ftent, c->d_frtile, c->d_ent, c->d_trow, c->d_rtile, c->d_fpart, c->d_kabc, c->d_pos, c->d_posb, c->d_fs, c->d_fl,
+                  c->d_qidx, c->d_qpos, c->d_row, c->d_sym, c->d_rowm, c->d_sfmw, c->d_sfmtab, c->d_ftile, c->d_ftent, c->d_fimg, c->d_frtile, c->d_ent, c->d_trow, c->d_rtile, c->d_fpart, c->d_kabc, c->d_pos, c->d_posb, c->d_fs, c->d_fl,
                   c->d_perm, c->d_iperm, c->d_cell, c->d_ccount, c->d_cstart, c->d_cfill, c->d_cwork,
                   c->d_hcnt, c->d_pcut,
                   c->d_tmp, c->d_rec64, c->d_stype, c->d_nbr, c->d_nnb, c->d_flags,
@@ -576,7 +576,7 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
   {
     int mmax = -1;
     for (const EwRow &r : row) mmax = std::max(mmax, r.mhi);
-    const bool can = c->nrow > 0 && ncharge > 0 && mmax < 40;
+    const bool can = c->nrow > 0 && ncharge > 0 && mmax < NB_EFM_MS;
     bool use = can && (share > 1 || (long long)ncharge * c->ktot >= NB_EFM_MIN_WORK);
     if (const char *ev = getenv("CM3D_NB_SFAC_MFMA")) use = can && ev[0] != '0';
     if (const char *ev = getenv("CM3D_NB_EWF_MFMA")) use = can && ev[0] != '0';
@@ -639,6 +639,7 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
       }
       frtile.push_back(nt);
       c->nrange = (int)frtile.size() - 1;
+      c->efm_nent = (int)ftent.size();
       // each wave: NB_EFM_CT charge tiles x 4 products a K step of every tile
       c->efm_mfma = (long long)c->efm_ncb * 4 * NB_EFM_CT * 4 * tot;
     }
@@ -665,6 +666,7 @@ extern "C" int nb_create(const nb_params *p, nb_ctx **out) {
   if (c->efm && (rc = upload(c, &c->d_ftile, ftile.data(), ftile.size()))) return fail(rc);
   if (c->efm && (rc = upload(c, &c->d_ftent, ftent.data(), ftent.size()))) return fail(rc);
   if (c->efm && (rc = upload(c, &c->d_frtile, frtile.data(), frtile.size()))) return fail(rc);
+  if (c->efm && (rc = dalloc(c, &c->d_fimg, 8 * (size_t)c->efm_nent))) return fail(rc);
   if (c->sfm && (rc = dalloc(c, &c->d_sfmtab, nb_sfm_table_doubles(c)))) return fail(rc);
   if ((rc = upload(c, &c->d_ent, ent.data(), ent.size()))) return fail(rc);
   if ((rc = upload(c, &c->d_trow, trow.data(), trow.size()))) return fail(rc);
@@ -1074,20 +1076,21 @@ extern "C" int nb_force_device(nb_ctx *c, int iflg, const double *d_pxyz, const 
 // columns) carved from the static scratch area instead of the context's own
 // buffers; allocates nothing.  Every kernel overwrites the scratch it reads
 // before reading it, so nothing in it has to start at zero.
-static size_t scratch_layout(const nb_ctx *c, size_t off[11]) {
+static size_t scratch_layout(const nb_ctx *c, size_t off[12]) {
   const size_t nq = std::max(c->ncharge, 1), kt = std::max(c->ktot, 1);
-  const size_t bytes[10] = {32 * nq, 32 * nq, 16 * (size_t)c->nchunk * kt, nb_kdata_bytes() * kt,
+  const size_t bytes[11] = {32 * nq, 32 * nq, 16 * (size_t)c->nchunk * kt, nb_kdata_bytes() * kt,
                             24 * (size_t)c->nrange * nq,
                             8 * (size_t)c->npair_blocks * NB_NE_PAIRQ,
                             8 * (size_t)c->nfin_blocks * NB_NE_ECORR,
                             8 * (size_t)c->new_blocks * NB_NE_EW, 8 * 64,
-                            c->sfm ? 8 * nb_sfm_table_doubles(c) : 0};
+                            c->sfm ? 8 * nb_sfm_table_doubles(c) : 0,
+                            c->efm ? 64 * (size_t)c->efm_nent : 0};
   size_t o = 0;
-  for (int k = 0; k < 10; k++) {
+  for (int k = 0; k < 11; k++) {
     off[k] = o;
     o += (bytes[k] + 255) & ~(size_t)255;
   }
-  off[10] = o;
+  off[11] = o;
   return o;
 }
 
@@ -1102,7 +1105,7 @@ extern "C" int nb_force_device_noalloc(nb_ctx *c, int iflg, const double *d_pxyz
     nb_set_error("nb_force_device_noalloc: iflg must be 0 or 1");
     return NB_ERR_ARG;
   }
-  size_t off[11];
+  size_t off[12];
   const size_t need = scratch_layout(c, off);
   if (need > NB_SCRATCH_BYTES) {
     nb_set_error("nb_force_device_noalloc: this system needs %zu bytes of scratch, the static "
@@ -1132,6 +1135,7 @@ extern "C" int nb_force_device_noalloc(nb_ctx *c, int iflg, const double *d_pxyz
   v.d_eew = (double *)(base + off[7]);
   v.d_eout = (double *)(base + off[8]);
   if (c->sfm) v.d_sfmtab = (double *)(base + off[9]);
+  if (c->efm) v.d_fimg = (double *)(base + off[10]);
   v.timing = 0;
   v.rec = nullptr;
   Box b = make_box(hmat);

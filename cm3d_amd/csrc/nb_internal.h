@@ -153,7 +153,7 @@ struct EwFTile { int ent_off, mt0, nks, pad; int ka[4], kb0[4]; };
 #ifndef NB_EFM_MIN_WORK
 #define NB_EFM_MIN_WORK (1LL << 24)  // ... charges x k vectors from which a lone system uses it
 #endif
-#define NB_EFM_ESTR 9  // ... doubles per staged coefficient entry (8 + 1 against bank conflicts)
+#define NB_EFM_MS 40   // ... values of m a tile may span (its image: 1 KB per value of m)
 // A wave of the MFMA structure factor: its row tiles (an empty one past
 // the last real tile where it has fewer) and their m-tile count
 struct EwSfmWave { int t[4]; int nt, nr, pad0, pad1; };
@@ -343,6 +343,8 @@ struct nb_ctx {
   long long sfm_mfma, efm_mfma;  // matrix-core instructions a structure-factor / force launch issues
   EwFTile *d_ftile;            // ... its row tiles
   int4 *d_ftent;               // ... their coefficient entries
+  int efm_nent;                // ... how many (2 nks x 16 a tile)
+  double *d_fimg;              // ... the tiles' coefficients as staged in LDS (8 per entry)
   int *d_frtile;               // ... ranges (tile bounds)
   int nrange, nqb, ntile;      // force: tile ranges, charge workgroups, tiles
   double *d_epair, *d_eew, *d_eec, *d_eout;

@@ -1544,161 +1544,273 @@ static __device__ __forceinline__ void k_ewforce_body(const EwForceArgs &A, int 
 #endif
 NB_KERNEL_PAIR_W(k_ewforce, EwForceArgs, 256, NB_EWF_WPE)
 
+// k_ewfimg: the force pass's coefficient image, a thread per (tile, m,
+// row) entry: the row's coefficients of cos / sin of m theta_c in Re P
+// (Er, -Di), Im P (Ei, Dr), Re Q (m Dr, -m Ei) and Im Q (m Di, m Er), from
+// S(k) of k = +m and -m, stored [m][coefficient][row] so that a tile is one
+// block of 2 nks KB that k_ewforcem copies into LDS as it is
+struct EwFimgArgs { const int4 *tent; const KData *kd; int nent; double *img; };
+static __device__ __forceinline__ void k_ewfimg_body(const EwFimgArgs &A, int bid) {
+  const int e = bid * 256 + threadIdx.x;
+  if (e >= A.nent) return;
+  const int4 en = A.tent[e];
+  const KData p = A.kd[en.x >= 0 ? en.x : 0], q = A.kd[en.y >= 0 ? en.y : 0];
+  const double pr = en.x >= 0 ? p.cr : 0., pi = en.x >= 0 ? p.ci : 0.;
+  const double mr = en.y >= 0 ? q.cr : 0., mi = en.y >= 0 ? q.ci : 0.;
+  const double er = pr + mr, ei = -(pi + mi), dr = pr - mr, di = mi - pi, fm = en.z;
+  double *d = A.img + (size_t)(e >> 4) * 128 + (e & 15);  // entry e = 16 m + row
+  d[0] = er; d[16] = -di; d[32] = ei; d[48] = dr;
+  d[64] = fm * dr; d[80] = -(fm * ei); d[96] = fm * di; d[112] = fm * er;
+}
+NB_KERNEL_PAIR(k_ewfimg, EwFimgArgs, 256)
+
 // k_ewforcem: the reciprocal force pass on the FP64 matrix cores (see
 // k_ewsfacm for the v_mfma_f64_16x16x4f64 lane layout).  For a tile of 16
 // k rows and 16 charges, P and Q of k_ewforce for every (row, charge) are
 // four products D = A B over the K index (m, c / s), 2 values of m a step:
 // A[row][(m, c)] is the row's coefficient of cos (c = 0) or sin (c = 1) of
-// m theta_c in Re P (Er, -Di), Im P (Ei, Dr), Re Q (m Dr, -m Ei) or Im Q
-// (m Di, m Er), staged in LDS once per tile for the workgroup; B[(m, c)]
-// [charge] is cos / sin m theta_c, which each lane steps in registers by
-// z^2 (z = e^{i theta_c}).  The result gives a lane 4 rows of one run of
-// consecutive kb for one charge, so w = e^{i (ka theta_a + kb theta_b)}
-// takes one sine / cosine per run and steps of e^{i theta_b}; the lane sums
-// ka Im(w P), kb Im(w P) and Im(w Q) over its rows for its charge, and the
-// force A Im(w P) + Gc Im(w Q) (A = 2 pi (ka h^-T_a + kb h^-T_b)) follows
-// from the 4 lane groups' combined sums at the end.
+// m theta_c in Re P, Im P, Re Q or Im Q (k_ewfimg's image of the tile,
+// copied HBM -> LDS by direct loads into one of two buffers while the
+// previous tile's products run); B[(m, c)][charge] is cos / sin m theta_c,
+// which each lane steps in registers by the three-term recurrence in steps
+// of 2 in m (z = e^{i theta_c} and z^2 start it).  The
+// result gives a lane 4 rows of one run of consecutive kb for one charge,
+// so w = e^{i (ka theta_a + kb theta_b)} takes one sine / cosine per run
+// and steps of e^{i theta_b}; the lane sums ka Im(w P), kb Im(w P) and
+// Im(w Q) over its rows for its charge, and the force A Im(w P) + Gc Im(w Q)
+// (A = 2 pi (ka h^-T_a + kb h^-T_b)) follows from the 4 lane groups'
+// combined sums at the end.
 struct EwFmArgs {
-  const EwFTile *tile; const int4 *tent; const int *rtile; const KData *kd;
+  const EwFTile *tile; const double *img; const int *rtile;
   const double4 *chg, *stp; int nq, ncb; double *fpart; Box b;
 };
-#define EFM_MS 40  // values of m a tile may span (the host keeps m < 40)
+// per charge tile of a lane: phases and steps, and the sums over rows of
+// ka Im(w P), kb Im(w P) and Im(w Q) (the force is 2 pi (h^-T_a sa +
+// h^-T_b sb) + Gc s1q)
+struct EfmState {
+  double ta[NB_EFM_CT], tb[NB_EFM_CT], zr[NB_EFM_CT], zi[NB_EFM_CT];
+  double tz[NB_EFM_CT], z2i[NB_EFM_CT];  // 2 Re z^2, Im z^2
+  double ebr[NB_EFM_CT], ebi[NB_EFM_CT];
+  double sa[NB_EFM_CT], sb[NB_EFM_CT], s1q[NB_EFM_CT];
+};
+// a tile's fields as a lane uses them (its run's ka, kb0), as loaded: the
+// uniform ones are moved to scalar registers (efm_uni) only where used, so
+// that nothing waits for the load where it is issued
+struct EfmDesc { int ent_off, mt0, nks, ka, kb0; };
+static __device__ __forceinline__ EfmDesc efm_desc(const EwFmArgs &A, int t, int kq) {
+  const EwFTile &T = A.tile[t];
+  return EfmDesc{T.ent_off, T.mt0, T.nks, T.ka[kq], T.kb0[kq]};
+}
+static __device__ __forceinline__ EfmDesc efm_uni(const EfmDesc &d) {
+  return EfmDesc{__builtin_amdgcn_readfirstlane(d.ent_off), __builtin_amdgcn_readfirstlane(d.mt0),
+                 __builtin_amdgcn_readfirstlane(d.nks), d.ka, d.kb0};
+}
+// a tile's image (2 nks KB from entry ent_off) into an LDS buffer: 1 KB per
+// wave-load, 16 B a lane, waited for by the caller
+static __device__ __forceinline__ void efm_load(const EwFmArgs &A, double *dst, int ent_off,
+                                                int nks) {
+  const char *src = (const char *)(A.img + (size_t)ent_off * 8);
+  const int lane = lane_id();
+  for (int q = threadIdx.x >> 6; q < 2 * nks; q += 4)
+    __builtin_amdgcn_global_load_lds(src + q * 1024 + lane * 16,
+                                     (sfm_lptr)((char *)dst + q * 1024), 16, 0, 0);
+}
+// the products and row sums of one staged tile (m from mt0 in nks steps of
+// 2; this lane's run starts at (ka, kb0))
+static __device__ __forceinline__ void efm_tile(const EwFmArgs &A, const double *kl, int mt0,
+                                                int nks, double ka, double kb0, int cw,
+                                                EfmState &S) {
+#pragma clang fp contract(fast)
+  constexpr int CT = NB_EFM_CT;
+  const int lane = lane_id(), kq = lane >> 4, j16 = lane & 15;
+  const bool sn = kq & 1;
+  // f(m) and f(m - 2) of this lane's function (cos or sin of m theta_c) for
+  // m = mt0 + (kq >> 1): from 1, z, z^2 when mt0 <= 1, else two sines /
+  // cosines of the charge's theta_c
+  double fa[CT], fb[CT];
+  const int m0 = mt0 + (kq >> 1);
+#pragma unroll
+  for (int ct = 0; ct < CT; ct++) {
+    const double z2r = 0.5 * S.tz[ct];
+    if (mt0 <= 1) {
+      if (sn) {
+        fa[ct] = m0 == 0 ? 0. : (m0 == 1 ? S.zi[ct] : S.z2i[ct]);
+        fb[ct] = m0 == 0 ? -S.z2i[ct] : (m0 == 1 ? -S.zi[ct] : 0.);
+      } else {
+        fa[ct] = m0 == 0 ? 1. : (m0 == 1 ? S.zr[ct] : z2r);
+        fb[ct] = m0 == 0 ? z2r : (m0 == 1 ? S.zr[ct] : 1.);
+      }
+    } else {
+      const int c = (cw * CT + ct) * 16 + j16;
+      const double th = c < A.nq ? A.chg[c].z : 0.;
+      double s0, c0, s2, c2;
+      nb_sincos(m0 * th, &s0, &c0);
+      nb_sincos((m0 - 2) * th, &s2, &c2);
+      fa[ct] = sn ? s0 : c0;
+      fb[ct] = sn ? s2 : c2;
+    }
+  }
+  // image [m][coefficient][row]: this lane's m is 2 ks + (kq >> 1), its
+  // coefficient 2 p + (kq & 1) for product p, its row j16
+  const double *ab = kl + ((kq >> 1) * 8 + (kq & 1)) * 16 + j16;
+  sfm_d4 acc[CT][4];
+  // B of step ks is f(m), stepped as f(m + 2) = 2 cos 2theta f(m) - f(m - 2)
+  // (one FMA): even steps read fa and step fb, odd steps the other way
+  {  // the first K step starts the sums (no zeroing)
+    const double a0 = ab[0], a1 = ab[32], a2 = ab[64], a3 = ab[96];
+    const sfm_d4 zero = {0., 0., 0., 0.};
+#pragma unroll
+    for (int ct = 0; ct < CT; ct++) {
+      acc[ct][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, fa[ct], zero, 0, 0, 0);
+      acc[ct][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, fa[ct], zero, 0, 0, 0);
+      acc[ct][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, fa[ct], zero, 0, 0, 0);
+      acc[ct][3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a3, fa[ct], zero, 0, 0, 0);
+      fb[ct] = S.tz[ct] * fa[ct] - fb[ct];
+    }
+  }
+  int ks = 1;
+#pragma unroll 1
+  for (; ks + 1 < nks; ks += 2) {
+    {
+      const double *a = ab + ks * 256;
+      const double a0 = a[0], a1 = a[32], a2 = a[64], a3 = a[96];
+#pragma unroll
+      for (int ct = 0; ct < CT; ct++) {
+        acc[ct][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, fb[ct], acc[ct][0], 0, 0, 0);
+        acc[ct][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, fb[ct], acc[ct][1], 0, 0, 0);
+        acc[ct][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, fb[ct], acc[ct][2], 0, 0, 0);
+        acc[ct][3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a3, fb[ct], acc[ct][3], 0, 0, 0);
+        fa[ct] = S.tz[ct] * fb[ct] - fa[ct];
+      }
+    }
+    {
+      const double *a = ab + (ks + 1) * 256;
+      const double a0 = a[0], a1 = a[32], a2 = a[64], a3 = a[96];
+#pragma unroll
+      for (int ct = 0; ct < CT; ct++) {
+        acc[ct][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, fa[ct], acc[ct][0], 0, 0, 0);
+        acc[ct][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, fa[ct], acc[ct][1], 0, 0, 0);
+        acc[ct][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, fa[ct], acc[ct][2], 0, 0, 0);
+        acc[ct][3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a3, fa[ct], acc[ct][3], 0, 0, 0);
+        fb[ct] = S.tz[ct] * fa[ct] - fb[ct];
+      }
+    }
+  }
+  if (ks < nks) {
+    const double *a = ab + ks * 256;
+    const double a0 = a[0], a1 = a[32], a2 = a[64], a3 = a[96];
+#pragma unroll
+    for (int ct = 0; ct < CT; ct++) {
+      acc[ct][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, fb[ct], acc[ct][0], 0, 0, 0);
+      acc[ct][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, fb[ct], acc[ct][1], 0, 0, 0);
+      acc[ct][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, fb[ct], acc[ct][2], 0, 0, 0);
+      acc[ct][3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a3, fb[ct], acc[ct][3], 0, 0, 0);
+    }
+  }
+  // this lane's rows: run kq, rows rr = 0..3 (kb = kb0 + rr), result
+  // double rr of each product; w of each charge from one sine / cosine,
+  // then steps of e^{i theta_b}
+  double wr[CT], wi[CT];
+#pragma unroll
+  for (int ct = 0; ct < CT; ct++) nb_sincos(ka * S.ta[ct] + kb0 * S.tb[ct], &wi[ct], &wr[ct]);
+#pragma unroll
+  for (int ct = 0; ct < CT; ct++) {
+    double p0 = 0., pb = 0.;  // sum of Im(w P), of kb Im(w P) over the run
+#pragma unroll
+    for (int rr = 0; rr < 4; rr++) {
+      const double s0 = wr[ct] * acc[ct][1][rr] + wi[ct] * acc[ct][0][rr];  // Im(w P)
+      const double s1 = wr[ct] * acc[ct][3][rr] + wi[ct] * acc[ct][2][rr];  // Im(w Q)
+      p0 += s0;
+      pb += (kb0 + rr) * s0;
+      S.s1q[ct] += s1;
+      if (rr < 3) {
+        const double w2 = wr[ct] * S.ebr[ct] - wi[ct] * S.ebi[ct];
+        wi[ct] = wr[ct] * S.ebi[ct] + wi[ct] * S.ebr[ct];
+        wr[ct] = w2;
+      }
+    }
+    S.sa[ct] += ka * p0;
+    S.sb[ct] += pb;
+  }
+}
 static __device__ __forceinline__ void k_ewforcem_body(const EwFmArgs &A, int bid) {
 #pragma clang fp contract(fast)
-  constexpr int CT = NB_EFM_CT, ES = NB_EFM_ESTR;
-  __shared__ double kl[EFM_MS * 16 * ES];
+  constexpr int CT = NB_EFM_CT;
+  __shared__ double efm_b0[NB_EFM_MS * 128], efm_b1[NB_EFM_MS * 128];
   const int r = bid / A.ncb, cb = bid % A.ncb;
   const int lane = lane_id(), kq = lane >> 4, j16 = lane & 15, w = threadIdx.x >> 6;
-  // a lane's B operand is cos (kq even) or sin (kq odd) of m theta_c; it
-  // keeps that one first: (x, y) = (Re, Im) or (Im, Re) of z^m, stepped by
-  // z^2 as x' = x Re z^2 + y u, y' = y Re z^2 - x u with u = -Im z^2 or
-  // +Im z^2
-  const bool sn = kq & 1;
-  double ta[CT], tb[CT], zr[CT], zi[CT], z2r[CT], cu[CT], ebr[CT], ebi[CT];
-  // per charge: sum over rows of ka Im(w P), kb Im(w P) and Im(w Q); the
-  // force is 2 pi (h^-T_a sum ka.. + h^-T_b sum kb..) + Gc sum Im(w Q)
-  double sa[CT], sb[CT], s1q[CT];
+  const int cw = cb * 4 + w;  // this wave's charge tiles: cw * CT .. + CT - 1
+  // a lane's B operand is cos (kq even) or sin (kq odd) of m theta_c
+  EfmState S;
 #pragma unroll
   for (int ct = 0; ct < CT; ct++) {
-    const int c = ((cb * 4 + w) * CT + ct) * 16 + j16;
+    const int c = (cw * CT + ct) * 16 + j16;
     double4 q4 = make_double4(0., 0., 0., 0.), st = make_double4(1., 0., 1., 0.);
     if (c < A.nq) { q4 = A.chg[c]; st = A.stp[c]; }
-    ta[ct] = q4.x; tb[ct] = q4.y;
-    zr[ct] = st.x; zi[ct] = st.y;
-    z2r[ct] = st.x * st.x - st.y * st.y;
-    const double z2i = 2. * st.x * st.y;
-    cu[ct] = sn ? z2i : -z2i;
-    nb_sincos(q4.y, &ebi[ct], &ebr[ct]);
-    sa[ct] = sb[ct] = s1q[ct] = 0.;
+    S.ta[ct] = q4.x; S.tb[ct] = q4.y;
+    S.zr[ct] = st.x; S.zi[ct] = st.y;
+    S.tz[ct] = 2. * (st.x * st.x - st.y * st.y);
+    S.z2i[ct] = 2. * st.x * st.y;
+    nb_sincos(q4.y, &S.ebi[ct], &S.ebr[ct]);
+    S.sa[ct] = S.sb[ct] = S.s1q[ct] = 0.;
   }
-  const double tpi = 2. * M_PI, *hi = A.b.hi;
-  const double gcx = tpi * hi[6], gcy = tpi * hi[7], gcz = tpi * hi[8];
+  // a tile's fields are read one tile ahead, so that the next tile's copy
+  // is issued at once and the products wait for no load
   const int t0 = A.rtile[r], t1 = A.rtile[r + 1];
-  for (int t = t0; t < t1; t++) {
-    const int ent_off = A.tile[t].ent_off, mt0 = A.tile[t].mt0, nks = A.tile[t].nks;
-    // this lane's run: kq (a per-lane index, read from memory directly)
-    const double ka = A.tile[t].ka[kq], kb0 = A.tile[t].kb0[kq];
-    const int ne = 2 * nks * 16;
+  EfmDesc cur = {0, 0, 0, 0, 0}, nxt = {0, 0, 0, 0, 0};
+  if (t0 < t1) {
+    cur = efm_uni(efm_desc(A, t0, kq));
+    efm_load(A, efm_b0, cur.ent_off, cur.nks);
+  }
+  if (t0 + 1 < t1) nxt = efm_desc(A, t0 + 1, kq);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  // two tiles a trip so that each buffer is one LDS object at every access
+  for (int t = t0; t < t1; t += 2) {
+    {
+      cur = efm_uni(cur);
+      if (t + 1 < t1) {
+        const EfmDesc u = efm_uni(nxt);
+        efm_load(A, efm_b1, u.ent_off, u.nks);
+      }
+      EfmDesc nn = {0, 0, 0, 0, 0};
+      if (t + 2 < t1) nn = efm_desc(A, t + 2, kq);
+      efm_tile(A, efm_b0, cur.mt0, cur.nks, cur.ka, cur.kb0, cw, S);
+      cur = nxt;
+      nxt = nn;
+    }
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    for (int e = threadIdx.x; e < ne; e += blockDim.x) {
-      const int4 en = A.tent[ent_off + e];
-      const KData p = A.kd[en.x >= 0 ? en.x : 0], q = A.kd[en.y >= 0 ? en.y : 0];
-      const double pr = en.x >= 0 ? p.cr : 0., pi = en.x >= 0 ? p.ci : 0.;
-      const double mr = en.y >= 0 ? q.cr : 0., mi = en.y >= 0 ? q.ci : 0.;
-      const double er = pr + mr, ei = -(pi + mi), dr = pr - mr, di = mi - pi, fm = en.z;
-      double *d = kl + e * ES;  // e = mm * 16 + row
-      d[0] = er; d[1] = -di; d[2] = ei; d[3] = dr;
-      d[4] = fm * dr; d[5] = -(fm * ei); d[6] = fm * di; d[7] = fm * er;
+    if (t + 1 >= t1) break;
+    {
+      cur = efm_uni(cur);
+      if (t + 2 < t1) {
+        const EfmDesc u = efm_uni(nxt);
+        efm_load(A, efm_b0, u.ent_off, u.nks);
+      }
+      EfmDesc nn = {0, 0, 0, 0, 0};
+      if (t + 3 < t1) nn = efm_desc(A, t + 3, kq);
+      efm_tile(A, efm_b1, cur.mt0, cur.nks, cur.ka, cur.kb0, cw, S);
+      cur = nxt;
+      nxt = nn;
     }
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    // z^m for m = mt0 + (kq >> 1): from 1, z, z^2 when mt0 <= 1, else a
-    // sine / cosine of the charge's theta_c
-    double cx[CT], cy[CT];
-    const int m0 = mt0 + (kq >> 1);
-#pragma unroll
-    for (int ct = 0; ct < CT; ct++) {
-      double cr, ci;
-      if (mt0 <= 1) {
-        const double z2i = sn ? cu[ct] : -cu[ct];
-        cr = m0 == 0 ? 1. : (m0 == 1 ? zr[ct] : z2r[ct]);
-        ci = m0 == 0 ? 0. : (m0 == 1 ? zi[ct] : z2i);
-      } else {
-        const int c = ((cb * 4 + w) * CT + ct) * 16 + j16;
-        nb_sincos(c < A.nq ? m0 * A.chg[c].z : 0., &ci, &cr);
-      }
-      cx[ct] = sn ? ci : cr;
-      cy[ct] = sn ? cr : ci;
-    }
-    const double *ab = kl + ((kq >> 1) * 16 + j16) * ES + (kq & 1);
-    sfm_d4 acc[CT][4];
-    {  // the first K step starts the sums (no zeroing)
-      const double a0 = ab[0], a1 = ab[2], a2 = ab[4], a3 = ab[6];
-      const sfm_d4 zero = {0., 0., 0., 0.};
-#pragma unroll
-      for (int ct = 0; ct < CT; ct++) {
-        const double bv = cx[ct];
-        acc[ct][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bv, zero, 0, 0, 0);
-        acc[ct][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bv, zero, 0, 0, 0);
-        acc[ct][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, bv, zero, 0, 0, 0);
-        acc[ct][3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a3, bv, zero, 0, 0, 0);
-        const double nx = cx[ct] * z2r[ct] + cy[ct] * cu[ct];
-        cy[ct] = cy[ct] * z2r[ct] - cx[ct] * cu[ct];
-        cx[ct] = nx;
-      }
-    }
-#pragma unroll 1
-    for (int ks = 1; ks < nks; ks++) {
-      const double *a = ab + ks * 32 * ES;
-      const double a0 = a[0], a1 = a[2], a2 = a[4], a3 = a[6];
-#pragma unroll
-      for (int ct = 0; ct < CT; ct++) {
-        const double bv = cx[ct];
-        acc[ct][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bv, acc[ct][0], 0, 0, 0);
-        acc[ct][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bv, acc[ct][1], 0, 0, 0);
-        acc[ct][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, bv, acc[ct][2], 0, 0, 0);
-        acc[ct][3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a3, bv, acc[ct][3], 0, 0, 0);
-        const double nx = cx[ct] * z2r[ct] + cy[ct] * cu[ct];
-        cy[ct] = cy[ct] * z2r[ct] - cx[ct] * cu[ct];
-        cx[ct] = nx;
-      }
-    }
-    // this lane's rows: run kq, rows rr = 0..3 (kb = kb0 + rr), result
-    // double rr of each product; w of each charge from one sine / cosine,
-    // then steps of e^{i theta_b}
-    double wr[CT], wi[CT];
-#pragma unroll
-    for (int ct = 0; ct < CT; ct++) nb_sincos(ka * ta[ct] + kb0 * tb[ct], &wi[ct], &wr[ct]);
-#pragma unroll
-    for (int ct = 0; ct < CT; ct++) {
-      double p0 = 0., pb = 0.;  // sum of Im(w P), of kb Im(w P) over the run
-#pragma unroll
-      for (int rr = 0; rr < 4; rr++) {
-        const double s0 = wr[ct] * acc[ct][1][rr] + wi[ct] * acc[ct][0][rr];  // Im(w P)
-        const double s1 = wr[ct] * acc[ct][3][rr] + wi[ct] * acc[ct][2][rr];  // Im(w Q)
-        p0 += s0;
-        pb += (kb0 + rr) * s0;
-        s1q[ct] += s1;
-        if (rr < 3) {
-          const double w2 = wr[ct] * ebr[ct] - wi[ct] * ebi[ct];
-          wi[ct] = wr[ct] * ebi[ct] + wi[ct] * ebr[ct];
-          wr[ct] = w2;
-        }
-      }
-      sa[ct] += ka * p0;
-      sb[ct] += pb;
-    }
   }
   // the 4 lane groups hold the same 16 charges' sums over different rows
+  const double tpi = 2. * M_PI, *hi = A.b.hi;
+  const double gcx = tpi * hi[6], gcy = tpi * hi[7], gcz = tpi * hi[8];
 #pragma unroll
   for (int ct = 0; ct < CT; ct++) {
-    sa[ct] += __shfl_xor(sa[ct], 16); sb[ct] += __shfl_xor(sb[ct], 16); s1q[ct] += __shfl_xor(s1q[ct], 16);
-    sa[ct] += __shfl_xor(sa[ct], 32); sb[ct] += __shfl_xor(sb[ct], 32); s1q[ct] += __shfl_xor(s1q[ct], 32);
-    const int c = ((cb * 4 + w) * CT + ct) * 16 + j16;
+    double sa = S.sa[ct], sb = S.sb[ct], s1q = S.s1q[ct];
+    sa += __shfl_xor(sa, 16); sb += __shfl_xor(sb, 16); s1q += __shfl_xor(s1q, 16);
+    sa += __shfl_xor(sa, 32); sb += __shfl_xor(sb, 32); s1q += __shfl_xor(s1q, 32);
+    const int c = (cw * CT + ct) * 16 + j16;
     if (kq == 0 && c < A.nq) {
       double *o = A.fpart + (size_t)(3 * r) * A.nq + c;
-      o[0] = tpi * (hi[0] * sa[ct] + hi[3] * sb[ct]) + gcx * s1q[ct];
-      o[A.nq] = tpi * (hi[1] * sa[ct] + hi[4] * sb[ct]) + gcy * s1q[ct];
-      o[2 * (size_t)A.nq] = tpi * (hi[2] * sa[ct] + hi[5] * sb[ct]) + gcz * s1q[ct];
+      o[0] = tpi * (hi[0] * sa + hi[3] * sb) + gcx * s1q;
+      o[A.nq] = tpi * (hi[1] * sa + hi[4] * sb) + gcy * s1q;
+      o[2 * (size_t)A.nq] = tpi * (hi[2] * sa + hi[5] * sb) + gcz * s1q;
     }
   }
 }
@@ -1962,9 +2074,10 @@ int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_f
                     (const double4 *)c->d_chg, (const double4 *)c->d_stp, nq, c->nqb, c->d_fpart,
                     b};
   NB_GO(c, K_EWFORCE, k_ewforce, rp && ewf && !c->efm ? c->nqb * c->nrange : 0, 256, 0, ef, s);
-  EwFmArgs em = {c->d_ftile, c->d_ftent, c->d_frtile, (const KData *)c->d_kd,
-                 (const double4 *)c->d_chg, (const double4 *)c->d_stp, nq, c->efm_ncb, c->d_fpart,
-                 b};
+  EwFimgArgs fi = {c->d_ftent, (const KData *)c->d_kd, c->efm_nent, c->d_fimg};
+  NB_GO(c, K_EWFIMG, k_ewfimg, rp && ewf && c->efm ? blocks(c->efm_nent, 256) : 0, 256, 0, fi, s);
+  EwFmArgs em = {c->d_ftile, c->d_fimg, c->d_frtile, (const double4 *)c->d_chg,
+                 (const double4 *)c->d_stp, nq, c->efm_ncb, c->d_fpart, b};
   NB_GO(c, K_EWFORCEM, k_ewforcem, rp && ewf && c->efm ? c->efm_ncb * c->nrange : 0, 256, 0, em,
         s);
   nb_time_mark(c, 6, s);
@@ -2012,6 +2125,7 @@ static struct RegisterKernels {
     g_batch_fn[K_EWSFACM] = k_ewsfacm_bl;
     g_batch_fn[K_EWSFMTAB] = k_ewsfmtab_bl;
     g_batch_fn[K_EWFORCEM] = k_ewforcem_bl;
+    g_batch_fn[K_EWFIMG] = k_ewfimg_bl;
     g_batch_fn[K_EWFIN] = k_ewfin_bl;
   }
 } g_register_kernels;

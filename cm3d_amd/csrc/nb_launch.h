@@ -17,7 +17,7 @@ enum KernelId {
   K_ZERO, K_CELL, K_SCAT, K_ORDER, K_PACK64, K_LIST, K_CHECK,
   K_PAIR64_0T, K_PAIR64_1T, K_PAIR64_1F, K_PAIR64_1N, K_PAIR64_1E, K_PAIR64_1Q, K_RED, K_EWCHG, K_EWSFAC, K_EWKRED,
   K_EWFORCE, K_EWFIN, K_PACKMIX, K_MIX_0T, K_MIX_1T, K_MIX_1F, K_MDINNER, K_MDKICK,
-  K_MDNHC, K_MDSAVE, K_OVFCHK, K_EWSFACR, K_EWSFACM, K_EWSFMTAB, K_EWFORCEM, K_COUNT
+  K_MDNHC, K_MDSAVE, K_OVFCHK, K_EWSFACR, K_EWSFACM, K_EWSFMTAB, K_EWFORCEM, K_EWFIMG, K_COUNT
 };
 
 struct PairArgs {
