@@ -2069,13 +2069,16 @@ int nb_launch_force(nb_ctx *c, int iflg, const Box &b, double *d_fs, double *d_f
   EwKredArgs kr = {c->d_kabc, c->ktot, (const double2 *)c->d_spart, c->nchunk, c->new_blocks,
                    b, 1. / det, falp2, (KData *)c->d_kd, c->d_eew};
   NB_GO(c, K_EWKRED, k_ewkred, rp && ew ? c->new_blocks : 0, 256, 0, kr, s);
+  // the matrix-core force pass's coefficient image (from the k data just
+  // reduced) is timed with the k reduction, so that the force phase is
+  // k_ewforcem's own
+  EwFimgArgs fi = {c->d_ftent, (const KData *)c->d_kd, c->efm_nent, c->d_fimg};
+  NB_GO(c, K_EWFIMG, k_ewfimg, rp && ewf && c->efm ? blocks(c->efm_nent, 256) : 0, 256, 0, fi, s);
   nb_time_mark(c, 5, s);
   EwForceArgs ef = {c->d_row, c->d_ent, c->d_rtile, c->d_trow, (const KData *)c->d_kd,
                     (const double4 *)c->d_chg, (const double4 *)c->d_stp, nq, c->nqb, c->d_fpart,
                     b};
   NB_GO(c, K_EWFORCE, k_ewforce, rp && ewf && !c->efm ? c->nqb * c->nrange : 0, 256, 0, ef, s);
-  EwFimgArgs fi = {c->d_ftent, (const KData *)c->d_kd, c->efm_nent, c->d_fimg};
-  NB_GO(c, K_EWFIMG, k_ewfimg, rp && ewf && c->efm ? blocks(c->efm_nent, 256) : 0, 256, 0, fi, s);
   EwFmArgs em = {c->d_ftile, c->d_fimg, c->d_frtile, (const double4 *)c->d_chg,
                  (const double4 *)c->d_stp, nq, c->efm_ncb, c->d_fpart, b};
   NB_GO(c, K_EWFORCEM, k_ewforcem, rp && ewf && c->efm ? c->efm_ncb * c->nrange : 0, 256, 0, em,

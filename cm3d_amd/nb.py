@@ -316,8 +316,9 @@ class NBContext:
 
     def timing_read(self):
         """Average ms per force call by phase: pack, pair, ewchg (charge
-        records), sfac (structure factor), kred (k reduction), ewforce
-        (reciprocal force), fin (ecorr + force sum + energy reductions),
+        records), sfac (structure factor), kred (k reduction, with the
+        matrix-core force pass's coefficient image), ewforce (reciprocal
+        force), fin (ecorr + force sum + energy reductions),
         ewald (ewchg .. fin), total."""
         out = np.zeros(9)
         check(lib().nb_timing(self._h, 2, out.ctypes.data_as(_dp)), "nb_timing")

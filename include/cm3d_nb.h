@@ -202,8 +202,9 @@ int nb_get_pairs(nb_ctx *ctx, int which, int *ip, int *jp, int cap, int *n);
  * stream the kernels run on (no counterpart in cm3d; diagnostics).
  * mode 1: start recording (up to 2048 calls), 0: stop, 2: read the averages
  * in ms: out = {calls, position pack, pair kernel(s), Ewald charge records,
- * structure factor, k reduction, reciprocal force, ecorr + force sum +
- * energy reductions, whole call}. */
+ * structure factor, k reduction (and the matrix-core force pass's
+ * coefficient image), reciprocal force, ecorr + force sum + energy
+ * reductions, whole call}. */
 #define NB_TIMING_N 9
 int nb_timing(nb_ctx *ctx, int mode, double out[NB_TIMING_N]);
 
